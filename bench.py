@@ -1,0 +1,151 @@
+"""Benchmark: RAAR/HIO iterations per second at 1024x1024 complex64 on one MI355X.
+
+Workload (single-frame GPU RAAR): a 1024^2 seeded object (smoothed-noise
+amplitude in [0.2,1] and phase in [-pi,pi], support radius 160 px), 1e8
+photons with Poisson noise, central beamstop of radius 8 px left
+unconstrained, RAAR at beta = 0.85.  A step is one iteration of the loop
+(column pass + row pass) with every input resident in HBM; shrink-wrap
+steps are timed separately (end-to-end field).
+
+Prints one JSON line.  --gpus above 1 is refused: this build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "HIO iters/s + HBM GB/s at 1024² c64; iters-to-R≤0.05 learned vs random start"
+N = 1024
+BYTES_PER_ITER = 80 * N * N          # algorithmic bytes of one iteration (c64/FP32)
+COL_BYTES = 20 * N * N               # column pass: read T1 8 + read I 4 + write T2 8 B/px
+ROW_BYTES = 36 * N * N               # row pass: read T2 8, g 8, S 4; write g 8, T1 8 B/px
+PEAK_GBS = 8000.0
+
+
+def make_inputs(cdx, ctx, seed=0):
+    obj = ctx.cplx()
+    ctx.sim_object(obj, 160, seed=seed)
+    I = ctx.real()
+    ctx.sim_intensity(obj, I, 1e8, seed=seed, poisson=True)
+    bs = ctx.real()
+    ctx.create_beamstop(bs, 8)
+    spt = ctx.real()
+    ctx.create_circle(spt, 0.2 * N)
+    F = ctx.cplx()
+    ctx.random_start(I, bs, F, seed=seed, mode=1)
+    g = ctx.cplx()
+    ctx.fft(F, g, cdx.INVERSE)
+    ctx.apply_norm(g, 1.0 / N)  # IFFT(F)/sqrt(N), as prepareIter leaves objectWave
+    ctx.sync()
+    return dict(F=F, g=g, I=I, bs=bs, spt=spt, obj=obj)
+
+
+def cpu_baseline(seconds=15.0):
+    """The oracle (FP32, OpenMP over rows/columns) on the host cores, on a
+    bounded sample: as many 1024^2 RAAR iterations as fit in `seconds`."""
+    from oracle.oracle import Oracle
+    o = Oracle("f32")
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    rs = np.random.default_rng(0)
+    obj = o.sim_object(N, 160, 0)
+    I = o.sim_intensity(obj, 1e8, 0, poisson=True)
+    bs = o.beamstop(N, N, 8)
+    spt = o.circle(N, N, 0.2 * N)
+    F = np.sqrt(I) * np.exp(2j * np.pi * rs.uniform(size=I.shape))
+    g = np.fft.ifft2(F) * N
+    kw = dict(beta=0.85)
+    o.phase_retrieve("1*RAAR", F, g, I, spt, 0.5, bs=bs, **kw)  # warm
+    n, t0 = 0, time.perf_counter()
+    while True:
+        out = o.phase_retrieve("2*RAAR", F, g, I, spt, 0.5, bs=bs, **kw)
+        F, g = out["F"], out["g"]
+        n += 2
+        el = time.perf_counter() - t0
+        if el > seconds:
+            break
+    return dict(value=n / el, unit="iters/s", cores=threads, kind="port",
+                sample=f"{n} RAAR iterations at 1024x1024 complex64 (FP32 oracle, OpenMP) in {el:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(sharding is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import cdixx_amd as cdx
+    ctx = cdx.Context(N, precision=cdx.F32)
+    ctx.set_threshold(0.5)
+    d = make_inputs(cdx, ctx)
+    p = cdx.Params.default(beta=0.85, useBS=1, freeBeamstop=1)
+
+    # warmup (untimed), then exactly `steps` iterations bracketed by syncs
+    ctx.time_iterations(p, cdx.RAAR, d["F"], d["g"], d["I"], d["spt"], max(args.warmup, 1), bs=d["bs"],
+                        use_graph=not args.no_graph, per_kernel=False)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ev_ms, col_ms, row_ms = ctx.time_iterations(p, cdx.RAAR, d["F"], d["g"], d["I"], d["spt"], args.steps,
+                                                bs=d["bs"], use_graph=not args.no_graph, per_kernel=True)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    ms_per_step = ev_ms / args.steps
+    iters_s = 1000.0 / ms_per_step
+
+    # end-to-end: 1000 RAAR iterations with shrink-wrap every 20 (reference form)
+    d2 = make_inputs(cdx, ctx, seed=1)
+    ctx.set_threshold(0.5)
+    pe = cdx.Params.default(beta=0.85, useBS=1, freeBeamstop=1, shrinkThreshold=0.1, recordR=1)
+    ctx.sync()
+    t1 = time.perf_counter()
+    nit, rt = ctx.phase_retrieve("50*(20*RAAR+1*shrinkWrap)", pe, d2["F"], d2["g"], d2["I"], d2["spt"],
+                                 bs=d2["bs"], rtrace_cap=1000)
+    ctx.sync()
+    e2e = time.perf_counter() - t1
+
+    dom = "row" if row_ms >= col_ms else "col"
+    t_dom = max(row_ms, col_ms)
+    achieved = (ROW_BYTES if dom == "row" else COL_BYTES) / (t_dom * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic_1024.json")
+    if os.path.exists(tfile):
+        traffic = json.load(open(tfile)).get(f"{dom}_pass_bytes")
+
+    out = {
+        "metric": METRIC, "value": round(iters_s, 1), "unit": "iters/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c64/f32",
+        "data": "synthetic (seeded object, Poisson 1e8 photons)",
+        "config": {"workload": "single-frame RAAR beta=0.85, 1024x1024 complex64, beamstop r=8, support r=160",
+                   "frame": [N, N], "algorithm": "RAAR", "graph": not args.no_graph},
+        "hbm_gbps_effective": round(iters_s * BYTES_PER_ITER / 1e9, 1),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": f"{dom} pass", "avg_ms": round(t_dom, 5),
+                     "col_pass_ms": round(col_ms, 5), "row_pass_ms": round(row_ms, 5)},
+        "wall_s_timed": round(wall, 4),
+        "end_to_end": {"schedule": "50*(20*RAAR+1*shrinkWrap)", "iterations": nit, "seconds": round(e2e, 4),
+                       "final_R": float(rt[-1]) if rt is not None and len(rt) else None},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,330 @@
+"""cdixx_amd -- MI355X-native CDI phase-retrieval hot path (Python host mirror).
+
+Thin ctypes layer over libcdixx_amd.so (include/cdx_abi.h).  It mirrors the
+reference's operator surface for this path (CDIxx include/cdi.hpp,
+include/cudaConfig.hpp): the same operations, argument meaning and failure
+behaviour, with an explicit context instead of hidden globals.  There is no
+CPU fallback: if the HIP library is missing, importing the module raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcdixx_amd.so")
+
+F32, F64 = 0, 1
+RAAR, ER, POSER, HIO, POSHIO, FHIO, SHRINKWRAP, XCORRELATION, KKT, TV = range(10)
+TOKENS = {"RAAR": RAAR, "ER": ER, "POSER": POSER, "HIO": HIO, "POSHIO": POSHIO, "FHIO": FHIO,
+          "shrinkWrap": SHRINKWRAP, "XCORRELATION": XCORRELATION, "KKT": KKT, "TV": TV}
+FORWARD, INVERSE = -1, 1
+STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_HIP", 3: "ERR_SCHEDULE", 4: "ERR_UNSUPPORTED",
+          5: "ERR_DATA", 6: "ERR_IO", 7: "ERR_CONFIG"}
+
+
+class CdxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run `python -m cdixx_amd.build` "
+                          "(there is no CPU fallback for this path)")
+    return ctypes.CDLL(LIB_PATH)
+
+
+lib = _load()
+
+
+class Params(ctypes.Structure):
+    """cdx_params (include/cdx_abi.h)."""
+    _fields_ = [("simCCDbit", ctypes.c_int), ("useBS", ctypes.c_int), ("freeBeamstop", ctypes.c_int),
+                ("noiseLevel", ctypes.c_int), ("rcolor", ctypes.c_int), ("exposure", ctypes.c_double),
+                ("beta", ctypes.c_double), ("relax", ctypes.c_double), ("shrinkThreshold", ctypes.c_double),
+                ("shrinkMode", ctypes.c_int), ("sigma0", ctypes.c_double), ("sigmaMin", ctypes.c_double),
+                ("sigmaDecay", ctypes.c_double), ("recordR", ctypes.c_int), ("useGraph", ctypes.c_int)]
+
+    @classmethod
+    def default(cls, **kw):
+        p = cls()
+        lib.cdx_params_default(ctypes.byref(p))
+        for k, v in kw.items():
+            if not hasattr(p, k):
+                raise AttributeError(k)
+            setattr(p, k, v)
+        return p
+
+
+_vp, _i, _d, _u64, _i64, _sz = (ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_uint64,
+                                ctypes.c_int64, ctypes.c_size_t)
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_SIGS = {
+    "cdx_last_error": ([], ctypes.c_char_p), "cdx_version": ([], ctypes.c_char_p),
+    "cdx_size_supported": ([_i], _i),
+    "cdx_malloc": ([ctypes.POINTER(_vp), _sz], _i), "cdx_free": ([_vp], _i),
+    "cdx_memcpy_h2d": ([_vp, _vp, _sz], _i), "cdx_memcpy_d2h": ([_vp, _vp, _sz], _i),
+    "cdx_memcpy_d2d": ([_vp, _vp, _sz], _i), "cdx_memset0": ([_vp, _sz], _i),
+    "cdx_device_synchronize": ([], _i),
+    "cdx_create": ([ctypes.POINTER(_vp), _i, _i, _i, _vp], _i), "cdx_destroy": ([_vp], _i),
+    "cdx_synchronize": ([_vp], _i), "cdx_stream": ([_vp], _vp),
+    "cdx_init_vars": ([_vp, _i, _d], _i), "cdx_set_threshold": ([_vp, _d], _i),
+    "cdx_get_threshold": ([_vp, _dp], _i), "cdx_set_beta": ([_vp, _d], _i), "cdx_set_relax": ([_vp, _d], _i),
+    "cdx_fft": ([_vp, _vp, _vp, _i], _i), "cdx_apply_mod_abs": ([_vp, _vp, _vp], _i),
+    "cdx_apply_mod": ([_vp, _vp, _vp, _vp, _i, _d], _i),
+    "cdx_apply_support": ([_vp, _vp, _vp, _i, _vp], _i), "cdx_apply_norm": ([_vp, _vp, _d], _i),
+    "cdx_apply_norm_real": ([_vp, _vp, _d], _i), "cdx_get_mod2": ([_vp, _vp, _vp, _d], _i),
+    "cdx_gauss_mult_freq": ([_vp, _vp, _vp, _d], _i), "cdx_gauss_conv": ([_vp, _vp, _vp, _d], _i),
+    "cdx_convert_fo": ([_vp, _vp, _vp, _i], _i), "cdx_create_support": ([_vp, _vp, _d], _i),
+    "cdx_create_beamstop": ([_vp, _vp, _d], _i), "cdx_create_circle": ([_vp, _vp, _d], _i),
+    "cdx_find_max": ([_vp, _vp, _i64, _dp], _i), "cdx_find_sum": ([_vp, _vp, _i64, _dp], _i),
+    "cdx_random_start": ([_vp, _vp, _vp, _u64, _i, _vp], _i),
+    "cdx_residual": ([_vp, _vp, _vp, _vp, _dp], _i), "cdx_rfactor": ([_vp, _vp, _vp, _vp, _dp], _i),
+    "cdx_params_default": ([ctypes.POINTER(Params)], None),
+    "cdx_phase_retrieve": ([_vp, ctypes.c_char_p, ctypes.POINTER(Params), _vp, _vp, _vp, _vp, _vp, _dp, _i, _ip],
+                           _i),
+    "cdx_expand_schedule": ([ctypes.c_char_p, _ip, _i], _i),
+    "cdx_time_iterations": ([_vp, ctypes.POINTER(Params), _i, _vp, _vp, _vp, _vp, _vp, _i, _i, _dp, _dp, _dp],
+                            _i),
+    "cdx_sim_object": ([_vp, _d, _u64, _d, _vp], _i),
+    "cdx_sim_intensity": ([_vp, _vp, _d, _u64, _i, _vp], _i),
+    "cdx_read_config_json": ([ctypes.c_char_p, ctypes.c_char_p, _i], _i),
+}
+for _name, (_args, _res) in _SIGS.items():
+    if hasattr(lib, _name):
+        _f = getattr(lib, _name)
+        _f.argtypes = _args
+        _f.restype = _res
+
+
+def check(rc):
+    if rc != 0:
+        raise CdxError(rc, lib.cdx_last_error().decode())
+    return rc
+
+
+def last_error():
+    return lib.cdx_last_error().decode()
+
+
+def expand_schedule(algorithm):
+    """AlgoParser expansion of an algorithm string (host only, no GPU needed)."""
+    n = lib.cdx_expand_schedule(algorithm.encode(), None, 0)
+    if n < 0:
+        raise CdxError(-n, f"bad algorithm string {algorithm!r}")
+    out = (ctypes.c_int * max(n, 1))()
+    lib.cdx_expand_schedule(algorithm.encode(), out, n)
+    return list(out[:n])
+
+
+def read_config(path):
+    """readConfig of a libconfig file -> dict of every field (host only)."""
+    import json
+    buf = ctypes.create_string_buffer(1 << 16)
+    check(lib.cdx_read_config_json(path.encode(), buf, len(buf)))
+    return json.loads(buf.value.decode())
+
+
+class DeviceBuffer:
+    """Raw device allocation owned by Python (cdx_malloc / cdx_free)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        p = _vp()
+        check(lib.cdx_malloc(ctypes.byref(p), self.nbytes))
+        self.ptr = p.value
+
+    @classmethod
+    def from_array(cls, a):
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        b.upload(a)
+        return b
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        check(lib.cdx_memcpy_h2d(self.ptr, a.ctypes.data, a.nbytes))
+
+    def download(self, dtype, shape):
+        out = np.empty(shape, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        check(lib.cdx_memcpy_d2h(out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+    def zero(self):
+        check(lib.cdx_memset0(self.ptr, self.nbytes))
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            lib.cdx_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """Explicit replacement of the reference's hidden state (image size, FFT
+    plan, cudaVars) -- one per thread."""
+
+    def __init__(self, rows, cols=None, precision=F32):
+        cols = rows if cols is None else cols
+        self.rows, self.cols, self.precision = rows, cols, precision
+        self.rdtype = np.float64 if precision == F64 else np.float32
+        self.cdtype = np.complex128 if precision == F64 else np.complex64
+        h = _vp()
+        check(lib.cdx_create(ctypes.byref(h), rows, cols, precision, None))
+        self.h = h.value
+
+    # buffers
+    @property
+    def shape(self):
+        return (self.rows, self.cols)
+
+    def real(self, a=None):
+        b = DeviceBuffer(self.rows * self.cols * np.dtype(self.rdtype).itemsize)
+        if a is not None:
+            b.upload(np.asarray(a, dtype=self.rdtype))
+        return b
+
+    def cplx(self, a=None):
+        b = DeviceBuffer(self.rows * self.cols * np.dtype(self.cdtype).itemsize)
+        if a is not None:
+            b.upload(np.asarray(a, dtype=self.cdtype))
+        return b
+
+    def get_real(self, b):
+        return b.download(self.rdtype, self.shape)
+
+    def get_cplx(self, b):
+        return b.download(self.cdtype, self.shape)
+
+    # state
+    def sync(self):
+        check(lib.cdx_synchronize(self.h))
+
+    def init_vars(self, rcolor=65535, scale=1.0):
+        check(lib.cdx_init_vars(self.h, rcolor, scale))
+
+    def set_threshold(self, t):
+        check(lib.cdx_set_threshold(self.h, t))
+
+    def threshold(self):
+        v = ctypes.c_double()
+        check(lib.cdx_get_threshold(self.h, ctypes.byref(v)))
+        return v.value
+
+    def set_beta(self, b):
+        check(lib.cdx_set_beta(self.h, b))
+
+    def set_relax(self, r):
+        check(lib.cdx_set_relax(self.h, r))
+
+    # free functions of the path
+    def fft(self, src, dst=None, direction=FORWARD):
+        dst = src if dst is None else dst
+        check(lib.cdx_fft(self.h, src.ptr, dst.ptr, direction))
+        return dst
+
+    def apply_mod_abs(self, F, I):
+        check(lib.cdx_apply_mod_abs(self.h, F.ptr, I.ptr))
+
+    def apply_mod(self, F, I, bs=None, noise_level=0, bsnorm=1.0):
+        check(lib.cdx_apply_mod(self.h, F.ptr, I.ptr, bs.ptr if bs else None, noise_level, bsnorm))
+
+    def apply_support(self, g, gp, algo, spt):
+        check(lib.cdx_apply_support(self.h, g.ptr, gp.ptr, algo, spt.ptr))
+
+    def apply_norm(self, d, f):
+        check(lib.cdx_apply_norm(self.h, d.ptr, f))
+
+    def get_mod2(self, out, x, norm=1.0):
+        check(lib.cdx_get_mod2(self.h, out.ptr, x.ptr, norm))
+
+    def gauss_mult_freq(self, src, dst, sigma):
+        check(lib.cdx_gauss_mult_freq(self.h, src.ptr, dst.ptr, sigma))
+
+    def gauss_conv(self, src, dst, sigma):
+        check(lib.cdx_gauss_conv(self.h, src.ptr, dst.ptr, sigma))
+
+    def convert_fo(self, d, out=None, is_complex=True):
+        check(lib.cdx_convert_fo(self.h, d.ptr, out.ptr if out else None, int(is_complex)))
+
+    def create_support(self, spt, oversampling_spt=2.0):
+        check(lib.cdx_create_support(self.h, spt.ptr, oversampling_spt))
+
+    def create_beamstop(self, bs, radius):
+        check(lib.cdx_create_beamstop(self.h, bs.ptr, radius))
+
+    def create_circle(self, m, radius):
+        check(lib.cdx_create_circle(self.h, m.ptr, radius))
+
+    def find_max(self, x, n=0):
+        v = ctypes.c_double()
+        check(lib.cdx_find_max(self.h, x.ptr, n, ctypes.byref(v)))
+        return v.value
+
+    def find_sum(self, x, n=0):
+        v = ctypes.c_double()
+        check(lib.cdx_find_sum(self.h, x.ptr, n, ctypes.byref(v)))
+        return v.value
+
+    def random_start(self, I, bs, F, seed=0, mode=0):
+        check(lib.cdx_random_start(self.h, I.ptr, bs.ptr if bs else None, seed, mode, F.ptr))
+
+    def residual(self, F, I, bs=None):
+        v = ctypes.c_double()
+        check(lib.cdx_residual(self.h, F.ptr, I.ptr, bs.ptr if bs else None, ctypes.byref(v)))
+        return v.value
+
+    def rfactor(self, F, I, bs=None):
+        v = ctypes.c_double()
+        check(lib.cdx_rfactor(self.h, F.ptr, I.ptr, bs.ptr if bs else None, ctypes.byref(v)))
+        return v.value
+
+    def phase_retrieve(self, algorithm, params, F, g, I, spt, bs=None, rtrace_cap=0):
+        """CDI::phaseRetrieve (src/post_gpu/cdi.cc:297-421). Returns (n_iter, rtrace)."""
+        rt = (ctypes.c_double * max(rtrace_cap, 1))()
+        n_iter = ctypes.c_int()
+        check(lib.cdx_phase_retrieve(self.h, algorithm.encode(), ctypes.byref(params), F.ptr, g.ptr, I.ptr,
+                                     bs.ptr if bs else None, spt.ptr, rt if rtrace_cap else None, rtrace_cap,
+                                     ctypes.byref(n_iter)))
+        return n_iter.value, np.array(rt[:rtrace_cap]) if rtrace_cap else None
+
+    def time_iterations(self, params, algo, F, g, I, spt, n_iter, bs=None, use_graph=True, per_kernel=True):
+        tot, cm, rm = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        check(lib.cdx_time_iterations(self.h, ctypes.byref(params), algo, F.ptr, g.ptr, I.ptr,
+                                      bs.ptr if bs else None, spt.ptr, n_iter, int(use_graph), ctypes.byref(tot),
+                                      ctypes.byref(cm) if per_kernel else None,
+                                      ctypes.byref(rm) if per_kernel else None))
+        return tot.value, (cm.value if per_kernel else None), (rm.value if per_kernel else None)
+
+    def sim_object(self, obj, radius, seed=0, smooth_sigma=3.0):
+        check(lib.cdx_sim_object(self.h, radius, seed, smooth_sigma, obj.ptr))
+
+    def sim_intensity(self, obj, I, photons, seed=0, poisson=True):
+        check(lib.cdx_sim_intensity(self.h, obj.ptr, photons, seed, int(poisson), I.ptr))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.cdx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

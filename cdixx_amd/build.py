@@ -1,0 +1,69 @@
+"""Build recipe for libcdixx_amd.so (HIP kernels for gfx950 + C ABI + host C++).
+
+Run as ``python -m cdixx_amd.build`` or through ``__graft_entry__.build()``.
+Objects go to cdixx_amd/_build/, the library to cdixx_amd/libcdixx_amd.so.
+"""
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+OUT = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libcdixx_amd.so")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+HIPCC = os.path.join(ROCM, "bin", "hipcc")
+
+# -packed-fp32-ops: keep the compiler from emitting packed FP32 VALU ops.
+HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC", "-Wall",
+             "-Wno-unused-function", "-Wno-unused-variable",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter",
+              f"-I{ROCM}/include", "-D__HIP_PLATFORM_AMD__"]
+
+HIP_SOURCES = ["kernels.hip", "abi.hip", "batch.hip", "unet.hip"]
+HOST_SOURCES = ["config.cpp", "cdi.cpp", "imgio.cpp"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("build failed:\n" + " ".join(cmd) + "\n" + r.stdout + r.stderr)
+    return r
+
+
+def _compile(src):
+    path = os.path.join(CSRC, src)
+    obj = os.path.join(OUT, src + ".o")
+    if src.endswith(".hip"):
+        cmd = [HIPCC] + HIP_FLAGS + ["-c", path, "-o", obj]
+    else:
+        cmd = ["g++"] + HOST_FLAGS + ["-c", path, "-o", obj]
+    _run(cmd)
+    return obj
+
+
+def build(verbose=True, jobs=None):
+    os.makedirs(OUT, exist_ok=True)
+    srcs = [s for s in HIP_SOURCES + HOST_SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    jobs = jobs or min(8, os.cpu_count() or 4)
+    with ThreadPoolExecutor(jobs) as ex:
+        objs = list(ex.map(_compile, srcs))
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
+    os.replace(tmp, LIB)
+    # the command-line driver (cdi_run equivalent)
+    cli_src = os.path.join(CSRC, "cdi_run.cpp")
+    if os.path.exists(cli_src):
+        _run(["g++"] + HOST_FLAGS + ["-o", os.path.join(HERE, "cdi_run"), cli_src,
+                                    f"-L{HERE}", "-lcdixx_amd", f"-Wl,-rpath,{HERE}",
+                                    "-Wl,-rpath,$ORIGIN"])
+    if verbose:
+        print(f"built {LIB}")
+    return LIB
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

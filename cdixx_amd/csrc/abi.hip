@@ -1,0 +1,772 @@
+// abi.hip -- C ABI of the CDI hot path (include/cdx_abi.h) on top of kernels.hip.
+#include "../../include/cdx_abi.h"
+#include "errors.hpp"
+#include "kernels.hpp"
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+#include <string>
+#include <vector>
+
+using namespace cdx;
+
+static thread_local std::string g_err;
+static int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+#define HIPCHK(expr)                                                                          \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess) return fail(CDX_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_));   \
+  } while (0)
+
+extern "C" const char* cdx_last_error(void) { return g_err.c_str(); }
+namespace cdx {
+void set_last_error(const char* msg) { g_err = msg ? msg : ""; }
+}
+extern "C" const char* cdx_version(void) { return "cdixx_amd 0.1 (gfx950)"; }
+extern "C" int cdx_size_supported(int n) { return line_supported(n); }
+
+// scal[] layout (doubles on the device)
+enum {
+  S_THR = 0,     // support threshold (cudaVars.threshold)
+  S_AVG = 1,     // shrink-wrap avg from the first call
+  S_LASTMAX = 2,
+  S_MAXBITS = 4, // reduce slot: max as bits
+  S_SUM = 5,     // reduce slot: sum
+  S_ACC = 6,     // [6],[7] generic accumulators
+  S_MMA = 8,     // [8],[9] min/max keys
+  S_MMP = 10,    // [10],[11]
+  S_TOT = 12,
+  S_COUNT = 16
+};
+
+struct cdx_ctx {
+  int rows, cols, prec;
+  size_t rsz, csz;  // bytes per real / complex element
+  int64_t n;
+  hipStream_t st;
+  bool own_stream;
+  void* tw_row;  // twiddles for line length cols
+  void* tw_col;  // twiddles for line length rows
+  void* T1;
+  void* T2;
+  void* Ipk;
+  void* tmpr;
+  double* scal;
+  double* partials;  // kMaxPartials per-workgroup sums
+  double* ring;
+  int ring_cap;
+  int rcolor;
+  double scale, beta, relax;
+};
+
+// twiddle tables: twf[k1*Q+q] = W^(q*k1), twi[n1*P+k1] = W^(n1*k1), W = exp(-2 pi i/N)
+static int make_twiddles(int n, int prec, void** dptr) {
+  int P, Q;
+  line_factors(n, &P, &Q);
+  std::vector<double> re(2 * (size_t)n), im(2 * (size_t)n);
+  for (int k1 = 0; k1 < P; k1++)
+    for (int q = 0; q < Q; q++) {
+      long e = ((long)q * k1) % n;
+      double a = -2.0 * M_PI * (double)e / n;
+      re[(size_t)k1 * Q + q] = cos(a);
+      im[(size_t)k1 * Q + q] = sin(a);
+    }
+  for (int n1 = 0; n1 < Q; n1++)
+    for (int k1 = 0; k1 < P; k1++) {
+      long e = ((long)n1 * k1) % n;
+      double a = -2.0 * M_PI * (double)e / n;
+      re[n + (size_t)n1 * P + k1] = cos(a);
+      im[n + (size_t)n1 * P + k1] = sin(a);
+    }
+  size_t es = prec == CDX_F64 ? 16 : 8;
+  std::vector<unsigned char> host(es * 2 * n);
+  for (size_t i = 0; i < 2 * (size_t)n; i++) {
+    if (prec == CDX_F64) {
+      double v[2] = {re[i], im[i]};
+      memcpy(&host[i * es], v, 16);
+    } else {
+      float v[2] = {(float)re[i], (float)im[i]};
+      memcpy(&host[i * es], v, 8);
+    }
+  }
+  HIPCHK(hipMalloc(dptr, host.size()));
+  HIPCHK(hipMemcpy(*dptr, host.data(), host.size(), hipMemcpyHostToDevice));
+  return CDX_OK;
+}
+
+extern "C" int cdx_malloc(void** ptr, size_t bytes) {
+  if (!ptr) return fail(CDX_ERR_ARG, "cdx_malloc: null out pointer");
+  HIPCHK(hipMalloc(ptr, bytes ? bytes : 1));
+  return CDX_OK;
+}
+extern "C" int cdx_free(void* ptr) {
+  if (ptr) HIPCHK(hipFree(ptr));
+  return CDX_OK;
+}
+extern "C" int cdx_memcpy_h2d(void* d, const void* s, size_t b) {
+  HIPCHK(hipMemcpy(d, s, b, hipMemcpyHostToDevice));
+  return CDX_OK;
+}
+extern "C" int cdx_memcpy_d2h(void* d, const void* s, size_t b) {
+  HIPCHK(hipMemcpy(d, s, b, hipMemcpyDeviceToHost));
+  return CDX_OK;
+}
+extern "C" int cdx_memcpy_d2d(void* d, const void* s, size_t b) {
+  HIPCHK(hipMemcpy(d, s, b, hipMemcpyDeviceToDevice));
+  return CDX_OK;
+}
+extern "C" int cdx_memset0(void* p, size_t b) {
+  HIPCHK(hipMemset(p, 0, b));
+  return CDX_OK;
+}
+extern "C" int cdx_device_synchronize(void) {
+  HIPCHK(hipDeviceSynchronize());
+  return CDX_OK;
+}
+
+extern "C" int cdx_create(cdx_ctx** out, int rows, int cols, int precision, void* stream) {
+  if (!out) return fail(CDX_ERR_ARG, "cdx_create: null out pointer");
+  *out = nullptr;
+  if (!line_supported(rows) || !line_supported(cols))
+    return fail(CDX_ERR_ARG, "cdx_create: unsupported size %dx%d (lines of 16..2048, 2^k or 3*2^k)", rows, cols);
+  if (precision != CDX_F32 && precision != CDX_F64) return fail(CDX_ERR_ARG, "cdx_create: bad precision %d", precision);
+  cdx_ctx* c = new cdx_ctx();
+  c->rows = rows;
+  c->cols = cols;
+  c->prec = precision;
+  c->rsz = precision == CDX_F64 ? 8 : 4;
+  c->csz = 2 * c->rsz;
+  c->n = (int64_t)rows * cols;
+  c->rcolor = 65535;
+  c->scale = 1;
+  c->beta = 1;
+  c->relax = 1.9;
+  int rc;
+  if (stream) {
+    c->st = (hipStream_t)stream;
+    c->own_stream = false;
+  } else {
+    if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+      delete c;
+      return fail(CDX_ERR_HIP, "cdx_create: hipStreamCreate failed");
+    }
+    c->own_stream = true;
+  }
+  if ((rc = make_twiddles(cols, precision, &c->tw_row)) || (rc = make_twiddles(rows, precision, &c->tw_col))) {
+    cdx_destroy(c);
+    return rc;
+  }
+  c->ring_cap = 8192;
+  if (hipMalloc(&c->T1, c->n * c->csz) != hipSuccess || hipMalloc(&c->T2, c->n * c->csz) != hipSuccess ||
+      hipMalloc(&c->Ipk, c->n * c->rsz) != hipSuccess || hipMalloc(&c->tmpr, c->n * c->rsz) != hipSuccess ||
+      hipMalloc((void**)&c->scal, S_COUNT * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&c->partials, kMaxPartials * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&c->ring, 2 * (size_t)c->ring_cap * sizeof(double)) != hipSuccess) {
+    cdx_destroy(c);
+    return fail(CDX_ERR_HIP, "cdx_create: out of device memory for %dx%d", rows, cols);
+  }
+  double init[S_COUNT] = {0};
+  init[S_THR] = 0.5;  // init_cuda_image, src/gpu/cdi.cu:20
+  if (hipMemcpy(c->scal, init, sizeof init, hipMemcpyHostToDevice) != hipSuccess) {
+    cdx_destroy(c);
+    return fail(CDX_ERR_HIP, "cdx_create: scalar upload failed");
+  }
+  *out = c;
+  return CDX_OK;
+}
+
+extern "C" int cdx_destroy(cdx_ctx* c) {
+  if (!c) return CDX_OK;
+  if (c->st) hipStreamSynchronize(c->st);
+  hipFree(c->tw_row);
+  hipFree(c->tw_col);
+  hipFree(c->T1);
+  hipFree(c->T2);
+  hipFree(c->Ipk);
+  hipFree(c->tmpr);
+  hipFree(c->scal);
+  hipFree(c->partials);
+  hipFree(c->ring);
+  if (c->own_stream && c->st) hipStreamDestroy(c->st);
+  delete c;
+  return CDX_OK;
+}
+
+#define CTX_CHECK(c)                                          \
+  do {                                                        \
+    if (!(c)) return fail(CDX_ERR_ARG, "%s: null context", __func__); \
+  } while (0)
+
+extern "C" int cdx_synchronize(cdx_ctx* c) {
+  CTX_CHECK(c);
+  HIPCHK(hipStreamSynchronize(c->st));
+  return CDX_OK;
+}
+extern "C" void* cdx_stream(cdx_ctx* c) { return c ? (void*)c->st : nullptr; }
+
+extern "C" int cdx_init_vars(cdx_ctx* c, int rcolor, double scale) {
+  CTX_CHECK(c);
+  if (rcolor != 0) c->rcolor = rcolor;
+  if (scale == scale) c->scale = scale;
+  return CDX_OK;
+}
+extern "C" int cdx_set_threshold(cdx_ctx* c, double thr) {
+  CTX_CHECK(c);
+  // stored at the precision of the support comparison
+  double v = c->prec == CDX_F64 ? thr : (double)(float)thr;
+  HIPCHK(hipMemcpyAsync(c->scal + S_THR, &v, sizeof v, hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return CDX_OK;
+}
+extern "C" int cdx_get_threshold(cdx_ctx* c, double* thr) {
+  CTX_CHECK(c);
+  if (!thr) return fail(CDX_ERR_ARG, "cdx_get_threshold: null out");
+  HIPCHK(hipMemcpyAsync(thr, c->scal + S_THR, sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return CDX_OK;
+}
+extern "C" int cdx_set_beta(cdx_ctx* c, double b) {
+  CTX_CHECK(c);
+  c->beta = b;
+  return CDX_OK;
+}
+extern "C" int cdx_set_relax(cdx_ctx* c, double r) {
+  CTX_CHECK(c);
+  c->relax = r;
+  return CDX_OK;
+}
+
+static ReduceSlots slots(cdx_ctx* c) {
+  return ReduceSlots{(void*)(c->scal + S_MAXBITS), c->partials, (int*)(c->scal + S_SUM)};
+}
+
+// ---------------------------------------------------------------------------
+// precision dispatch
+// ---------------------------------------------------------------------------
+template <class T>
+static int fft_t(cdx_ctx* c, const void* in, void* out, int dir) {
+  ProjArgs<T> pa{};
+  UpdArgs<T> ua{};
+  ReduceSlots rs{};
+  if (dir == CDX_FORWARD) {
+    HIPCHK(launch_row<T>(ROW_FWD, c->rows, c->cols, in, out, c->tw_row, nullptr, nullptr, ua, (T)1, nullptr, rs, c->st));
+    HIPCHK(launch_col<T>(COL_FWD, c->rows, c->cols, out, out, c->tw_col, nullptr, pa, 0, c->st));
+  } else {
+    HIPCHK(launch_col<T>(COL_INV, c->rows, c->cols, in, out, c->tw_col, nullptr, pa, 0, c->st));
+    HIPCHK(launch_row<T>(ROW_INV, c->rows, c->cols, out, out, c->tw_row, nullptr, nullptr, ua, (T)1, nullptr, rs, c->st));
+  }
+  return CDX_OK;
+}
+#define DISPATCH(c, fn, ...) ((c)->prec == CDX_F64 ? fn<double>(__VA_ARGS__) : fn<float>(__VA_ARGS__))
+
+extern "C" int cdx_fft(cdx_ctx* c, const void* in, void* out, int dir) {
+  CTX_CHECK(c);
+  if (!in || !out) return fail(CDX_ERR_ARG, "cdx_fft: null buffer");
+  if (dir != CDX_FORWARD && dir != CDX_INVERSE) return fail(CDX_ERR_ARG, "cdx_fft: bad direction %d", dir);
+  return DISPATCH(c, fft_t, c, in, out, dir);
+}
+
+extern "C" int cdx_apply_mod_abs(cdx_ctx* c, void* F, const void* I) {
+  CTX_CHECK(c);
+  if (!F || !I) return fail(CDX_ERR_ARG, "cdx_apply_mod_abs: null buffer");
+  if (c->prec == CDX_F64) HIPCHK(launch_mod_abs<double>(c->n, F, (const double*)I, c->st));
+  else HIPCHK(launch_mod_abs<float>(c->n, F, (const float*)I, c->st));
+  return CDX_OK;
+}
+extern "C" int cdx_apply_mod(cdx_ctx* c, void* F, const void* I, const void* bs, int noise, double bsnorm) {
+  CTX_CHECK(c);
+  if (!F || !I) return fail(CDX_ERR_ARG, "cdx_apply_mod: null buffer");
+  if (c->prec == CDX_F64)
+    HIPCHK(launch_mod_noise<double>(c->n, F, (const double*)I, (const double*)bs, noise, c->scale, c->rcolor, bsnorm, c->st));
+  else
+    HIPCHK(launch_mod_noise<float>(c->n, F, (const float*)I, (const float*)bs, noise, (float)c->scale,
+                                   (float)c->rcolor, (float)bsnorm, c->st));
+  return CDX_OK;
+}
+template <class T>
+static UpdArgs<T> upd_args(cdx_ctx* c, int algo, double beta, double relax) {
+  UpdArgs<T> ua;
+  ua.algo = algo;
+  ua.beta = (T)beta;
+  ua.relax = (T)relax;
+  ua.invN = (T)(1.0 / (double)c->n);
+  ua.scal = c->scal;
+  return ua;
+}
+extern "C" int cdx_apply_support(cdx_ctx* c, void* g, const void* gp, int algo, const void* spt) {
+  CTX_CHECK(c);
+  if (!g || !gp || !spt) return fail(CDX_ERR_ARG, "cdx_apply_support: null buffer");
+  if (c->prec == CDX_F64)
+    HIPCHK(launch_support<double>(c->n, g, gp, (const double*)spt, upd_args<double>(c, algo, c->beta, c->relax), c->st));
+  else
+    HIPCHK(launch_support<float>(c->n, g, gp, (const float*)spt, upd_args<float>(c, algo, c->beta, c->relax), c->st));
+  return CDX_OK;
+}
+extern "C" int cdx_apply_norm(cdx_ctx* c, void* d, double f) {
+  CTX_CHECK(c);
+  if (!d) return fail(CDX_ERR_ARG, "cdx_apply_norm: null buffer");
+  if (c->prec == CDX_F64) HIPCHK(launch_norm_c<double>(c->n, d, f, c->st));
+  else HIPCHK(launch_norm_c<float>(c->n, d, (float)f, c->st));
+  return CDX_OK;
+}
+extern "C" int cdx_apply_norm_real(cdx_ctx* c, void* d, double f) {
+  CTX_CHECK(c);
+  if (!d) return fail(CDX_ERR_ARG, "cdx_apply_norm_real: null buffer");
+  if (c->prec == CDX_F64) HIPCHK(launch_norm_r<double>(c->n, (double*)d, f, c->st));
+  else HIPCHK(launch_norm_r<float>(c->n, (float*)d, (float)f, c->st));
+  return CDX_OK;
+}
+extern "C" int cdx_get_mod2(cdx_ctx* c, void* out, const void* in, double norm) {
+  CTX_CHECK(c);
+  if (!out || !in) return fail(CDX_ERR_ARG, "cdx_get_mod2: null buffer");
+  if (c->prec == CDX_F64) HIPCHK(launch_mod2<double>(c->n, (double*)out, in, norm, c->st));
+  else HIPCHK(launch_mod2<float>(c->n, (float*)out, in, (float)norm, c->st));
+  return CDX_OK;
+}
+extern "C" int cdx_gauss_mult_freq(cdx_ctx* c, const void* in, void* out, double sigma) {
+  CTX_CHECK(c);
+  if (!in || !out) return fail(CDX_ERR_ARG, "cdx_gauss_mult_freq: null buffer");
+  if (c->prec == CDX_F64) HIPCHK(launch_gauss_mult_freq<double>(c->rows, c->cols, in, out, sigma, c->st));
+  else HIPCHK(launch_gauss_mult_freq<float>(c->rows, c->cols, in, out, (float)sigma, c->st));
+  return CDX_OK;
+}
+extern "C" int cdx_convert_fo(cdx_ctx* c, void* d, void* out, int is_complex) {
+  CTX_CHECK(c);
+  if (!d) return fail(CDX_ERR_ARG, "cdx_convert_fo: null buffer");
+  if (!out) out = d;
+  if (c->prec == CDX_F64) HIPCHK(launch_convert_fo<double>(c->rows, c->cols, d, out, is_complex, c->st));
+  else HIPCHK(launch_convert_fo<float>(c->rows, c->cols, d, out, is_complex, c->st));
+  return CDX_OK;
+}
+extern "C" int cdx_create_support(cdx_ctx* c, void* spt, double os) {
+  CTX_CHECK(c);
+  if (!spt) return fail(CDX_ERR_ARG, "cdx_create_support: null buffer");
+  if (!(os > 0)) return fail(CDX_ERR_ARG, "cdx_create_support: oversampling_spt must be > 0");
+  // rect re; re.startx = (oversampling_spt-1)/2*row/oversampling_spt-1 (cdi.cc:242-245)
+  int sx = (int)((os - 1) / 2 * c->rows / os - 1);
+  int sy = (int)((os - 1) / 2 * c->cols / os - 1);
+  int ex = c->rows - sx, ey = c->cols - sy;
+  if (c->prec == CDX_F64) HIPCHK(launch_rect_mask<double>(c->rows, c->cols, sx, sy, ex, ey, (double*)spt, c->st));
+  else HIPCHK(launch_rect_mask<float>(c->rows, c->cols, sx, sy, ex, ey, (float*)spt, c->st));
+  return CDX_OK;
+}
+extern "C" int cdx_create_beamstop(cdx_ctx* c, void* bs, double r) {
+  CTX_CHECK(c);
+  if (!bs) return fail(CDX_ERR_ARG, "cdx_create_beamstop: null buffer");
+  if (c->prec == CDX_F64) HIPCHK(launch_circle_mask<double>(c->rows, c->cols, (float)r, 1, (double*)bs, c->st));
+  else HIPCHK(launch_circle_mask<float>(c->rows, c->cols, (float)r, 1, (float*)bs, c->st));
+  return CDX_OK;
+}
+extern "C" int cdx_create_circle(cdx_ctx* c, void* m, double r) {
+  CTX_CHECK(c);
+  if (!m) return fail(CDX_ERR_ARG, "cdx_create_circle: null buffer");
+  if (c->prec == CDX_F64) HIPCHK(launch_circle_mask<double>(c->rows, c->cols, (float)r, 0, (double*)m, c->st));
+  else HIPCHK(launch_circle_mask<float>(c->rows, c->cols, (float)r, 0, (float*)m, c->st));
+  return CDX_OK;
+}
+
+static int read_reduce(cdx_ctx* c, double* mx, double* sm) {
+  double h[S_COUNT];
+  HIPCHK(hipMemcpyAsync(h, c->scal, sizeof h, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  if (mx) {
+    if (c->prec == CDX_F64) {
+      double v;
+      memcpy(&v, &h[S_MAXBITS], 8);
+      *mx = v;
+    } else {
+      float v;
+      memcpy(&v, &h[S_MAXBITS], 4);
+      *mx = v;
+    }
+  }
+  if (sm) {
+    int np;
+    memcpy(&np, &h[S_SUM], sizeof np);
+    if (np < 0 || np > kMaxPartials) return fail(CDX_ERR_HIP, "reduction: bad partial count %d", np);
+    std::vector<double> part(np);
+    HIPCHK(hipMemcpy(part.data(), c->partials, np * sizeof(double), hipMemcpyDeviceToHost));
+    double s = 0;
+    for (double v : part) s += v;
+    *sm = s;
+  }
+  // clear the max slot
+  double z = 0;
+  HIPCHK(hipMemcpyAsync(c->scal + S_MAXBITS, &z, sizeof z, hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return CDX_OK;
+}
+extern "C" int cdx_find_max(cdx_ctx* c, const void* x, int64_t n, double* out) {
+  CTX_CHECK(c);
+  if (!x || !out) return fail(CDX_ERR_ARG, "cdx_find_max: null buffer");
+  if (n <= 0) n = c->n;
+  if (c->prec == CDX_F64) HIPCHK(launch_reduce<double>(n, (const double*)x, slots(c), c->st));
+  else HIPCHK(launch_reduce<float>(n, (const float*)x, slots(c), c->st));
+  return read_reduce(c, out, nullptr);
+}
+extern "C" int cdx_find_sum(cdx_ctx* c, const void* x, int64_t n, double* out) {
+  CTX_CHECK(c);
+  if (!x || !out) return fail(CDX_ERR_ARG, "cdx_find_sum: null buffer");
+  if (n <= 0) n = c->n;
+  if (c->prec == CDX_F64) HIPCHK(launch_reduce<double>(n, (const double*)x, slots(c), c->st));
+  else HIPCHK(launch_reduce<float>(n, (const float*)x, slots(c), c->st));
+  return read_reduce(c, nullptr, out);
+}
+
+extern "C" int cdx_gauss_conv(cdx_ctx* c, const void* in, void* out, double sigma) {
+  CTX_CHECK(c);
+  if (!in || !out) return fail(CDX_ERR_ARG, "cdx_gauss_conv: null buffer");
+  if (!(sigma > 0)) return fail(CDX_ERR_ARG, "cdx_gauss_conv: sigma must be > 0");
+  if (in == out) return fail(CDX_ERR_ARG, "cdx_gauss_conv: in and out must differ (as applyConvolution's)");
+  if (c->prec == CDX_F64)
+    HIPCHK(launch_blur_real<double>(c->rows, c->cols, in, 0, (double*)c->tmpr, (double*)out, sigma, slots(c), c->st));
+  else
+    HIPCHK(launch_blur_real<float>(c->rows, c->cols, in, 0, (float*)c->tmpr, (float*)out, (float)sigma, slots(c), c->st));
+  return read_reduce(c, nullptr, nullptr);  // clears the max/sum slots the blur filled
+}
+
+extern "C" int cdx_random_start(cdx_ctx* c, const void* I, const void* bs, uint64_t seed, int mode, void* F) {
+  CTX_CHECK(c);
+  if (!I || !F) return fail(CDX_ERR_ARG, "cdx_random_start: null buffer");
+  double thr;
+  int rc = cdx_get_threshold(c, &thr);
+  if (rc) return rc;
+  if (c->prec == CDX_F64) HIPCHK(launch_start<double>(c->n, (const double*)I, (const double*)bs, thr, seed, mode, F, c->st));
+  else HIPCHK(launch_start<float>(c->n, (const float*)I, (const float*)bs, thr, seed, mode, F, c->st));
+  return CDX_OK;
+}
+
+static int read_acc(cdx_ctx* c, double* a, double* b) {
+  double h[2];
+  HIPCHK(hipMemcpyAsync(h, c->scal + S_ACC, sizeof h, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  if (a) *a = h[0];
+  if (b) *b = h[1];
+  return CDX_OK;
+}
+extern "C" int cdx_residual(cdx_ctx* c, const void* F, const void* I, const void* bs, double* out) {
+  CTX_CHECK(c);
+  if (!F || !I || !out) return fail(CDX_ERR_ARG, "cdx_residual: null buffer");
+  HIPCHK(launch_fill_f64(2, c->scal + S_ACC, 0.0, c->st));
+  if (c->prec == CDX_F64) HIPCHK(launch_residual<double>(c->n, F, (const double*)I, (const double*)bs, c->scal + S_ACC, c->st));
+  else HIPCHK(launch_residual<float>(c->n, F, (const float*)I, (const float*)bs, c->scal + S_ACC, c->st));
+  return read_acc(c, out, nullptr);
+}
+extern "C" int cdx_rfactor(cdx_ctx* c, const void* F, const void* I, const void* bs, double* out) {
+  CTX_CHECK(c);
+  if (!F || !I || !out) return fail(CDX_ERR_ARG, "cdx_rfactor: null buffer");
+  HIPCHK(launch_fill_f64(2, c->scal + S_ACC, 0.0, c->st));
+  if (c->prec == CDX_F64) {
+    HIPCHK(launch_pack_intensity<double>(c->n, (const double*)I, (const double*)bs, (double*)c->Ipk, c->st));
+    HIPCHK(launch_rfactor<double>(c->n, F, (const double*)c->Ipk, c->scal + S_ACC, c->st));
+  } else {
+    HIPCHK(launch_pack_intensity<float>(c->n, (const float*)I, (const float*)bs, (float*)c->Ipk, c->st));
+    HIPCHK(launch_rfactor<float>(c->n, F, (const float*)c->Ipk, c->scal + S_ACC, c->st));
+  }
+  double a, b;
+  int rc = read_acc(c, &a, &b);
+  if (rc) return rc;
+  *out = b > 0 ? a / b : 0;
+  return CDX_OK;
+}
+
+extern "C" void cdx_params_default(cdx_params* p) {
+  if (!p) return;
+  memset(p, 0, sizeof *p);
+  p->rcolor = 65535;
+  p->exposure = 1;
+  p->beta = 1;
+  p->relax = 1.9;
+  p->shrinkThreshold = 0;
+  p->shrinkMode = 0;
+  p->sigma0 = 3;
+  p->sigmaMin = 2;
+  p->sigmaDecay = 0.99;
+}
+
+// ---------------------------------------------------------------------------
+// the loop
+// ---------------------------------------------------------------------------
+namespace {
+template <class T>
+struct Loop {
+  cdx_ctx* c;
+  const cdx_params* p;
+  ProjArgs<T> pa;
+  bool f_pending;  // F holds the current Fourier estimate; T1 not yet valid
+  void* F;
+  void* g;
+  T* spt;
+
+  int setup(const void* I, const void* bs) {
+    const bool pack_bs = p->useBS && bs && (p->simCCDbit || p->freeBeamstop);
+    HIPCHK(launch_pack_intensity<T>(c->n, (const T*)I, pack_bs ? (const T*)bs : nullptr, (T*)c->Ipk, c->st));
+    pa.mode = p->simCCDbit ? PROJ_MOD_NOISE : (pack_bs ? PROJ_MOD_ABS_BS : PROJ_MOD_ABS);
+    pa.noiseLevel = p->noiseLevel;
+    pa.scale = (T)(1.0 / p->exposure);
+    pa.rcolor = (T)p->rcolor;
+    pa.rf = nullptr;
+    return CDX_OK;
+  }
+  int iterate(int algo, double* rf) {
+    pa.rf = rf;
+    if (f_pending)
+      HIPCHK(launch_col<T>(COL_PROJ_INV, c->rows, c->cols, F, c->T2, c->tw_col, (const T*)c->Ipk, pa, 0, c->st));
+    else
+      HIPCHK(launch_col<T>(COL_FWD_PROJ_INV, c->rows, c->cols, c->T1, c->T2, c->tw_col, (const T*)c->Ipk, pa, 0, c->st));
+    UpdArgs<T> ua = upd_args<T>(c, algo, p->beta, p->relax);
+    HIPCHK(launch_row<T>(ROW_INV_UPDATE_FWD, c->rows, c->cols, c->T2, c->T1, c->tw_row, g, spt, ua, (T)1, nullptr,
+                         ReduceSlots{}, c->st));
+    f_pending = false;
+    return CDX_OK;
+  }
+  int shrink_wrap(double sigma) {
+    ProjArgs<T> none{};
+    if (p->shrinkMode == 0) {
+      // applyGaussMult(patternWave, ., Real(row>>2)/gaussianSigma, 1) (cdi.cc:327)
+      T sf = (T)(c->rows >> 2) / (T)sigma;
+      if (f_pending)
+        HIPCHK(launch_col<T>(COL_GAUSS_INV, c->rows, c->cols, F, c->T2, c->tw_col, nullptr, none, (double)sf, c->st));
+      else
+        HIPCHK(launch_col<T>(COL_FWD_GAUSS_INV, c->rows, c->cols, c->T1, c->T2, c->tw_col, nullptr, none, (double)sf,
+                             c->st));
+      HIPCHK(launch_row<T>(ROW_INV_MOD2, c->rows, c->cols, c->T2, nullptr, c->tw_row, nullptr, nullptr, UpdArgs<T>{},
+                           (T)1, spt, slots(c), c->st));
+      HIPCHK(launch_threshold<T>(c->scal, slots(c), p->shrinkThreshold, c->rows, c->cols, 0, c->st));
+    } else {
+      HIPCHK(launch_blur_real<T>(c->rows, c->cols, g, 1, (T*)c->tmpr, spt, (T)sigma, slots(c), c->st));
+      HIPCHK(launch_threshold<T>(c->scal, slots(c), p->shrinkThreshold, c->rows, c->cols, 1, c->st));
+    }
+    return CDX_OK;
+  }
+  int finish() {
+    if (!f_pending) {
+      ProjArgs<T> none{};
+      HIPCHK(launch_col<T>(COL_FWD, c->rows, c->cols, c->T1, F, c->tw_col, nullptr, none, 0, c->st));
+      f_pending = true;
+    }
+    return CDX_OK;
+  }
+};
+}  // namespace
+
+extern "C" int cdx_expand_schedule(const char* algorithm, int* out, int cap);
+
+template <class T>
+static int phase_retrieve_t(cdx_ctx* c, const char* algorithm, const cdx_params* p, void* F, void* g, const void* I,
+                            const void* bs, void* spt, double* rtrace, int rcap, int* n_iter) {
+  int nt = cdx_expand_schedule(algorithm, nullptr, 0);
+  if (nt < 0) return fail(CDX_ERR_SCHEDULE, "cdx_phase_retrieve: bad algorithm string \"%s\"", algorithm);
+  std::vector<int> toks(nt);
+  cdx_expand_schedule(algorithm, toks.data(), nt);
+  for (int t : toks)
+    if (t == CDX_TV) return fail(CDX_ERR_UNSUPPORTED, "cdx_phase_retrieve: TV steps are outside this path");
+  Loop<T> L{c, p, {}, true, F, g, (T*)spt};
+  int rc = L.setup(I, bs);
+  if (rc) return rc;
+  const T rootN = (T)sqrt((double)c->n);
+  HIPCHK(launch_norm_c<T>(c->n, g, (T)1 / rootN, c->st));  // cdi.cc:313
+  const int nrec = p->recordR ? (rcap < c->ring_cap ? rcap : c->ring_cap) : 0;
+  if (nrec > 0) HIPCHK(launch_fill_f64(2 * (int64_t)nrec, c->ring, 0.0, c->st));
+
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  if (p->useGraph) HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+  double sigma = p->sigma0;
+  int iter = 0, nproj = 0;
+  for (int tok : toks) {
+    if (tok == CDX_SHRINKWRAP) {
+      if ((rc = L.shrink_wrap(sigma))) break;
+      if (sigma > p->sigmaMin) sigma *= p->sigmaDecay;
+    } else {
+      double* rf = nproj < nrec ? c->ring + 2 * nproj : nullptr;
+      if ((rc = L.iterate(tok, rf))) break;
+      nproj++;
+    }
+    iter++;
+  }
+  if (!rc) rc = L.finish();
+  if (p->useGraph) {
+    hipError_t e = hipStreamEndCapture(c->st, &graph);
+    if (!rc && e != hipSuccess) rc = fail(CDX_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    if (!rc) {
+      e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      if (e != hipSuccess) rc = fail(CDX_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    if (!rc) {
+      e = hipGraphLaunch(exec, c->st);
+      if (e != hipSuccess) rc = fail(CDX_ERR_HIP, "hipGraphLaunch: %s", hipGetErrorString(e));
+    }
+    if (exec) hipGraphExecDestroy(exec);
+    if (graph) hipGraphDestroy(graph);
+  }
+  if (rc) return rc;
+  HIPCHK(launch_norm_c<T>(c->n, g, rootN, c->st));  // cdi.cc:406
+  if (n_iter) *n_iter = iter;
+  if (nrec > 0 && rtrace) {
+    std::vector<double> h(2 * (size_t)nrec);
+    HIPCHK(hipMemcpyAsync(h.data(), c->ring, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    int m = nproj < nrec ? nproj : nrec;
+    for (int i = 0; i < m; i++) rtrace[i] = h[2 * i + 1] > 0 ? h[2 * i] / h[2 * i + 1] : 0;
+  }
+  return CDX_OK;
+}
+
+extern "C" int cdx_phase_retrieve(cdx_ctx* c, const char* algorithm, const cdx_params* p, void* F, void* g,
+                                  const void* I, const void* bs, void* spt, double* rtrace, int rcap, int* n_iter) {
+  CTX_CHECK(c);
+  if (!algorithm || !p || !F || !g || !I || !spt) return fail(CDX_ERR_ARG, "cdx_phase_retrieve: null argument");
+  if (!(p->exposure != 0)) return fail(CDX_ERR_ARG, "cdx_phase_retrieve: exposure must be non-zero");
+  return DISPATCH(c, phase_retrieve_t, c, algorithm, p, F, g, I, bs, spt, rtrace, rcap, n_iter);
+}
+
+// ---------------------------------------------------------------------------
+// timing
+// ---------------------------------------------------------------------------
+template <class T>
+static int time_t_(cdx_ctx* c, const cdx_params* p, int algo, void* F, void* g, const void* I, const void* bs,
+                   const void* spt, int n, int use_graph, double* ms_total, double* col_ms, double* row_ms) {
+  Loop<T> L{c, p, {}, true, F, g, (T*)spt};
+  int rc = L.setup(I, bs);
+  if (rc) return rc;
+  // first iteration leaves the F-pending state; it is not timed
+  if ((rc = L.iterate(algo, nullptr))) return rc;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  if (use_graph) {
+    HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < n && !rc; i++) rc = L.iterate(algo, nullptr);
+    hipError_t e = hipStreamEndCapture(c->st, &graph);
+    if (rc) return rc;
+    HIPCHK(e);
+    HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    HIPCHK(hipGraphLaunch(exec, c->st));  // warm the graph once
+  }
+  HIPCHK(hipStreamSynchronize(c->st));
+  HIPCHK(hipEventRecord(e0, c->st));
+  if (use_graph) {
+    HIPCHK(hipGraphLaunch(exec, c->st));
+  } else {
+    for (int i = 0; i < n && !rc; i++) rc = L.iterate(algo, nullptr);
+    if (rc) return rc;
+  }
+  HIPCHK(hipEventRecord(e1, c->st));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  if (ms_total) *ms_total = ms;
+  if (exec) hipGraphExecDestroy(exec);
+  if (graph) hipGraphDestroy(graph);
+  if (col_ms || row_ms) {
+    std::vector<hipEvent_t> ev(4 * (size_t)n);
+    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    UpdArgs<T> ua = upd_args<T>(c, algo, p->beta, p->relax);
+    for (int i = 0; i < n; i++) {
+      HIPCHK(hipEventRecord(ev[4 * i], c->st));
+      HIPCHK(launch_col<T>(COL_FWD_PROJ_INV, c->rows, c->cols, c->T1, c->T2, c->tw_col, (const T*)c->Ipk, L.pa, 0, c->st));
+      HIPCHK(hipEventRecord(ev[4 * i + 1], c->st));
+      HIPCHK(hipEventRecord(ev[4 * i + 2], c->st));
+      HIPCHK(launch_row<T>(ROW_INV_UPDATE_FWD, c->rows, c->cols, c->T2, c->T1, c->tw_row, g, (const T*)spt, ua, (T)1,
+                           nullptr, ReduceSlots{}, c->st));
+      HIPCHK(hipEventRecord(ev[4 * i + 3], c->st));
+    }
+    HIPCHK(hipStreamSynchronize(c->st));
+    double sc = 0, sr = 0;
+    for (int i = 0; i < n; i++) {
+      float a, b;
+      HIPCHK(hipEventElapsedTime(&a, ev[4 * i], ev[4 * i + 1]));
+      HIPCHK(hipEventElapsedTime(&b, ev[4 * i + 2], ev[4 * i + 3]));
+      sc += a;
+      sr += b;
+    }
+    for (auto& e : ev) hipEventDestroy(e);
+    if (col_ms) *col_ms = sc / n;
+    if (row_ms) *row_ms = sr / n;
+  }
+  // leave F consistent with the state
+  if ((rc = L.finish())) return rc;
+  HIPCHK(hipStreamSynchronize(c->st));
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  return CDX_OK;
+}
+
+extern "C" int cdx_time_iterations(cdx_ctx* c, const cdx_params* p, int algo, void* F, void* g, const void* I,
+                                   const void* bs, const void* spt, int n, int use_graph, double* ms_total,
+                                   double* col_ms, double* row_ms) {
+  CTX_CHECK(c);
+  if (!p || !F || !g || !I || !spt || n <= 0) return fail(CDX_ERR_ARG, "cdx_time_iterations: bad argument");
+  return DISPATCH(c, time_t_, c, p, algo, F, g, I, bs, spt, n, use_graph, ms_total, col_ms, row_ms);
+}
+
+// ---------------------------------------------------------------------------
+// synthetic forward model
+// ---------------------------------------------------------------------------
+__global__ void k_seed_minmax(unsigned long long* mm) {
+  mm[0] = ~0ull;
+  mm[1] = 0ull;
+  mm[2] = ~0ull;
+  mm[3] = 0ull;
+}
+
+extern "C" int cdx_sim_object(cdx_ctx* c, double radius, uint64_t seed, double sigma, void* obj) {
+  CTX_CHECK(c);
+  if (!obj) return fail(CDX_ERR_ARG, "cdx_sim_object: null buffer");
+  if (c->rows != c->cols) return fail(CDX_ERR_ARG, "cdx_sim_object: square contexts only");
+  const int n = c->rows;
+  double *a = nullptr, *ph = nullptr, *t = nullptr;
+  size_t bytes = (size_t)c->n * sizeof(double);
+  HIPCHK(hipMallocAsync((void**)&a, bytes, c->st));
+  HIPCHK(hipMallocAsync((void**)&ph, bytes, c->st));
+  HIPCHK(hipMallocAsync((void**)&t, bytes, c->st));
+  HIPCHK(launch_uniform_field(n, seed, 0, a, c->st));
+  HIPCHK(launch_uniform_field(n, seed, 1, ph, c->st));
+  HIPCHK(launch_smooth_periodic(n, a, t, sigma, 0, c->st));
+  HIPCHK(launch_smooth_periodic(n, t, a, sigma, 1, c->st));
+  HIPCHK(launch_smooth_periodic(n, ph, t, sigma, 0, c->st));
+  HIPCHK(launch_smooth_periodic(n, t, ph, sigma, 1, c->st));
+  hipLaunchKernelGGL(k_seed_minmax, dim3(1), dim3(1), 0, c->st, (unsigned long long*)(c->scal + S_MMA));
+  HIPCHK(hipGetLastError());
+  HIPCHK(launch_minmax(c->n, a, c->scal + S_MMA, c->st));
+  HIPCHK(launch_minmax(c->n, ph, c->scal + S_MMP, c->st));
+  if (c->prec == CDX_F64)
+    HIPCHK(launch_compose_object<double>(n, a, ph, c->scal + S_MMA, c->scal + S_MMP, (float)radius, obj, c->st));
+  else
+    HIPCHK(launch_compose_object<float>(n, a, ph, c->scal + S_MMA, c->scal + S_MMP, (float)radius, obj, c->st));
+  HIPCHK(hipFreeAsync(a, c->st));
+  HIPCHK(hipFreeAsync(ph, c->st));
+  HIPCHK(hipFreeAsync(t, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return CDX_OK;
+}
+
+extern "C" int cdx_sim_intensity(cdx_ctx* c, const void* obj, double photons, uint64_t seed, int poisson, void* I) {
+  CTX_CHECK(c);
+  if (!obj || !I) return fail(CDX_ERR_ARG, "cdx_sim_intensity: null buffer");
+  int rc = cdx_fft(c, obj, c->T2, CDX_FORWARD);
+  if (rc) return rc;
+  HIPCHK(launch_fill_f64(1, c->scal + S_TOT, 0.0, c->st));
+  if (c->prec == CDX_F64) {
+    HIPCHK(launch_sum_mod2<double>(c->n, c->T2, c->scal + S_TOT, c->st));
+    HIPCHK(launch_poisson_intensity<double>(c->n, c->T2, c->scal + S_TOT, photons, seed, poisson, (double*)I, c->st));
+  } else {
+    HIPCHK(launch_sum_mod2<float>(c->n, c->T2, c->scal + S_TOT, c->st));
+    HIPCHK(launch_poisson_intensity<float>(c->n, c->T2, c->scal + S_TOT, photons, seed, poisson, (float*)I, c->st));
+  }
+  HIPCHK(hipStreamSynchronize(c->st));
+  return CDX_OK;
+}

@@ -1,0 +1,1131 @@
+// kernels.hip -- CDNA4 kernels of the CDI phase-retrieval loop.
+//
+// One iteration of CDI::phaseRetrieve (CDIxx src/post_gpu/cdi.cc:349-354,381)
+// is projection -> IFFT -> 1/N -> support update -> FFT.  Here it is two
+// kernels that each stream the image once:
+//   column pass  (k_col, COL_FWD_PROJ_INV): forward column FFT of T1 = rowFFT(g),
+//                 modulus projection (+R-factor sums), inverse column FFT -> T2
+//   row pass     (k_row, ROW_INV_UPDATE_FWD): inverse row FFT of T2, 1/N,
+//                 HIO/RAAR/ER/... update against g and the support, store g,
+//                 forward row FFT -> T1
+// so F = colFFT(T1) is never stored between iterations.  Every other kernel
+// here serves setup, shrink-wrap, diagnostics or the standalone free
+// functions of the reference's cudaConfig.hpp.
+#include "fft_line.hpp"
+#include "kernels.hpp"
+#include <math.h>
+
+namespace cdx {
+
+// ---------------------------------------------------------------------------
+// line configurations
+// ---------------------------------------------------------------------------
+template <int N>
+struct LineCfg;
+#define CDX_LINE(n, p, q, tl)                         \
+  template <>                                         \
+  struct LineCfg<n> {                                 \
+    static constexpr int P = p, Q = q, TL = tl;       \
+  };
+CDX_LINE(16, 4, 4, 4)
+CDX_LINE(32, 4, 8, 4)
+CDX_LINE(64, 8, 8, 8)
+CDX_LINE(128, 8, 16, 8)
+CDX_LINE(256, 16, 16, 16)
+CDX_LINE(384, 16, 24, 8)
+CDX_LINE(512, 16, 32, 16)
+CDX_LINE(768, 32, 24, 8)
+CDX_LINE(1024, 32, 32, 32)
+CDX_LINE(2048, 32, 64, 32)
+#undef CDX_LINE
+
+int line_supported(int n) {
+  switch (n) {
+    case 16: case 32: case 64: case 128: case 256: case 384: case 512: case 768: case 1024: case 2048:
+      return 1;
+    default:
+      return 0;
+  }
+}
+void line_factors(int n, int* P, int* Q) {
+  switch (n) {
+#define F(n_) case n_: *P = LineCfg<n_>::P; *Q = LineCfg<n_>::Q; return;
+    F(16) F(32) F(64) F(128) F(256) F(384) F(512) F(768) F(1024) F(2048)
+#undef F
+    default: *P = *Q = 0;
+  }
+}
+
+// LDS pitch per line: at least the transpose footprint, == 8 (mod 32)
+// complex elements so neighbouring lines land on different banks.
+template <class T, int N>
+constexpr int lds_pitch() {
+  using L = Line<T, LineCfg<N>::P, LineCfg<N>::Q, LineCfg<N>::TL>;
+  int e = L::LDS_ELEMS;
+  return ((e + 23) / 32) * 32 + 8;
+}
+// columns per workgroup (coalescing: C*sizeof(cplx) >= 64 B)
+// (at least one full wavefront per workgroup for the shuffle reductions)
+template <class T, int N>
+constexpr int col_lines() {
+  constexpr int base = sizeof(T) == 4 ? 8 : 4;
+  constexpr int wave = 64 / LineCfg<N>::TL;
+  return base > wave ? base : wave;
+}
+// rows per workgroup
+template <class T, int N>
+constexpr int row_lines() {
+  constexpr int r = 128 / LineCfg<N>::TL;
+  return r < 1 ? 1 : r;
+}
+
+// ---------------------------------------------------------------------------
+// pointwise device helpers (reference semantics)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float dsqrt(float x) { return sqrtf(x); }
+__device__ __forceinline__ double dsqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float dhypot(float a, float b) { return hypotf(a, b); }
+__device__ __forceinline__ double dhypot(double a, double b) { return hypot(a, b); }
+
+// applyModAbs (cudaConfig.cu:827-841) with state == NULL
+template <class T>
+__device__ __forceinline__ cplx<T> mod_abs(cplx<T> f, T I) {
+  T mod = dhypot(f.x, f.y);
+  T rat = I > (T)0 ? dsqrt(I) : (T)0;
+  if (mod == (T)0) return {rat, (T)0};
+  rat /= mod;
+  return {f.x * rat, f.y * rat};
+}
+
+// applyMod, 5-argument form (cdi.cu:438-485) with bsnorm = 1; beamstop pixels
+// arrive as NaN intensities and are left unchanged.
+template <class T>
+__device__ __forceinline__ cplx<T> mod_noise(cplx<T> f, T I, int noise, T scale, T rcolor) {
+  if (isnan(I)) return f;
+  T mod2 = I > (T)0 ? I : (T)0;
+  T src2 = f.x * f.x + f.y * f.y;
+  T maximum = scale * (T)0.95;
+  if (mod2 >= maximum) mod2 = mod2 > src2 ? mod2 : src2;
+  T tol = dsqrt((T)noise + mod2 * rcolor / scale) * scale / rcolor;
+  T diff = mod2 - src2;
+  T val = mod2;
+  if (diff > tol) val -= tol;
+  else if (diff < -tol) val += tol;
+  if (src2 == (T)0) return {val, (T)0};
+  T s = dsqrt((val > (T)0 ? val : (T)0) / src2);
+  return {s * f.x, s * f.y};
+}
+
+template <class T>
+__device__ __forceinline__ cplx<T> project(cplx<T> f, T I, const ProjArgs<T>& pa) {
+  if (pa.mode == PROJ_MOD_NOISE) return mod_noise(f, I, pa.noiseLevel, pa.scale, pa.rcolor);
+  if (pa.mode == PROJ_MOD_ABS_BS && isnan(I)) return f;
+  return mod_abs(f, I);
+}
+
+// Apply*Support (cdi.cu:518-616) behind the applySupport dispatcher
+// (cdi.cc:25-36).  g = current object estimate, p = projected estimate g'.
+template <class T>
+__device__ __forceinline__ cplx<T> support_update(cplx<T> g, cplx<T> p, bool in, int algo, T beta, T relax) {
+  switch (algo) {
+    case 0:  // RAAR
+      if (in) return p;
+      return {beta * g.x + ((T)1 - (T)2 * beta) * p.x, beta * g.y + ((T)1 - (T)2 * beta) * p.y};
+    case 1:  // ER
+      if (in) return {g.x + relax * (p.x - g.x), g.y + relax * (p.y - g.y)};
+      return {(T)0, (T)0};
+    case 2:  // POSER
+      if (in && p.x > (T)0) return p;
+      return {(T)0, (T)0};
+    case 3:  // HIO
+      if (in) return p;
+      return {g.x - beta * p.x, g.y - beta * p.y};
+    case 4: {  // POSHIO
+      cplx<T> r = g;
+      if (p.x > (T)0 && in) r.x = g.x + relax * (p.x - g.x);
+      else r.x = g.x - beta * p.x;
+      r.y = g.y - beta * p.y;
+      return r;
+    }
+    default:  // FHIO, KKT, XCORRELATION, ...
+      if (in) return {g.x + relax * (p.x - g.x), g.y + relax * (p.y - g.y)};
+      return {g.x - (T)1.2 * p.x, g.y - (T)1.2 * p.y};
+  }
+}
+
+// ---------------------------------------------------------------------------
+// workgroup reductions
+// ---------------------------------------------------------------------------
+template <class V>
+__device__ __forceinline__ V wave_sum(V v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+template <class V>
+__device__ __forceinline__ V wave_max(V v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    V w = __shfl_xor(v, o, 64);
+    v = w > v ? w : v;
+  }
+  return v;
+}
+// sum of a per-thread double over the workgroup, added atomically to *dst
+__device__ __forceinline__ void block_atomic_sum(double v, double* dst, double* sh) {
+  v = wave_sum(v);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int nw = (blockDim.x + 63) >> 6;
+  __syncthreads();
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0;
+    for (int i = 0; i < nw; i++) s += sh[i];
+    atomicAdd(dst, s);
+  }
+}
+// workgroup sum stored at rs.partials[blockIdx.x] (fixed-order reduction)
+__device__ __forceinline__ void block_partial_sum(double v, const ReduceSlots& rs, double* sh) {
+  v = wave_sum(v);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int nw = (blockDim.x + 63) >> 6;
+  __syncthreads();
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0;
+    for (int i = 0; i < nw; i++) s += sh[i];
+    rs.partials[blockIdx.x] = s;
+    if (blockIdx.x == 0) *rs.nparts = (int)gridDim.x;
+  }
+}
+__device__ __forceinline__ void atomic_max_pos(void* bits, float v) {
+  atomicMax((unsigned int*)bits, __float_as_uint(v));
+}
+__device__ __forceinline__ void atomic_max_pos(void* bits, double v) {
+  atomicMax((unsigned long long*)bits, (unsigned long long)__double_as_longlong(v));
+}
+template <class T>
+__device__ __forceinline__ void block_atomic_max(T v, void* dst, double* sh) {
+  v = wave_max(v);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int nw = (blockDim.x + 63) >> 6;
+  __syncthreads();
+  if (l == 0) sh[w] = (double)v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0;
+    for (int i = 0; i < nw; i++) m = sh[i] > m ? sh[i] : m;
+    atomic_max_pos(dst, (T)m);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// column pass
+// ---------------------------------------------------------------------------
+template <class T, int N, int OP>
+__global__ void __launch_bounds__((col_lines<T, N>() * LineCfg<N>::TL))
+k_col(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out, int cols, const cplx<T>* __restrict__ tw,
+      const T* __restrict__ I, ProjArgs<T> pa, T gsig) {
+  constexpr int P = LineCfg<N>::P, Q = LineCfg<N>::Q, TL = LineCfg<N>::TL;
+  constexpr int C = col_lines<T, N>();
+  constexpr int PITCH = lds_pitch<T, N>();
+  using L = Line<T, P, Q, TL>;
+  extern __shared__ __align__(16) unsigned char smem[];
+  cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem);
+  const int c = threadIdx.x % C;
+  const int t = threadIdx.x / C;
+  const int col = blockIdx.x * C + c;
+  const bool ok = col < cols;
+  cplx<T>* my = lds + c * PITCH;
+  const cplx<T>* twf = tw;
+  const cplx<T>* twi = tw + N;
+  cplx<T> v[L::E];
+
+  constexpr bool from_f = (OP == COL_INV || OP == COL_PROJ_INV || OP == COL_GAUSS_INV);
+  if constexpr (from_f) {
+#pragma unroll
+    for (int i = 0; i < L::E; i++) {
+      const int r = L::dout_pos(i, t);
+      v[i] = ok ? in[(size_t)r * cols + col] : cplx<T>{(T)0, (T)0};
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < L::E; i++) {
+      const int r = L::din_pos(i, t);
+      v[i] = ok ? in[(size_t)r * cols + col] : cplx<T>{(T)0, (T)0};
+    }
+    L::forward(v, my, twf, t);
+  }
+
+  if constexpr (OP == COL_FWD) {
+    if (ok) {
+#pragma unroll
+      for (int i = 0; i < L::E; i++) out[(size_t)L::dout_pos(i, t) * cols + col] = v[i];
+    }
+    return;
+  }
+
+  if constexpr (OP == COL_FWD_PROJ_INV || OP == COL_PROJ_INV) {
+    double rnum = 0, rden = 0;
+#pragma unroll
+    for (int i = 0; i < L::E; i++) {
+      const int r = L::dout_pos(i, t);
+      const T Iv = ok ? I[(size_t)r * cols + col] : (T)NAN;
+      if (pa.rf && !isnan(Iv)) {
+        const T s = Iv > (T)0 ? dsqrt(Iv) : (T)0;
+        rnum += (double)fabs(dhypot(v[i].x, v[i].y) - s);
+        rden += (double)s;
+      }
+      if (ok) v[i] = project(v[i], Iv, pa);
+    }
+    if (pa.rf) {
+      __shared__ double sh[16];
+      block_atomic_sum(rnum, pa.rf, sh);
+      block_atomic_sum(rden, pa.rf + 1, sh);
+    }
+  }
+  if constexpr (OP == COL_FWD_GAUSS_INV || OP == COL_GAUSS_INV) {
+    // applyGaussMult(isFreq=1), cudaConfig.cu:380-395, wrapped coordinates
+    const int rows = N;
+    const T yrel = (T)(col >= cols / 2 ? col - cols : col);
+#pragma unroll
+    for (int i = 0; i < L::E; i++) {
+      const int r = L::dout_pos(i, t);
+      const T xrel = (T)(r >= rows / 2 ? r - rows : r);
+      const T f = (T)exp(-(xrel * xrel + yrel * yrel) / ((T)2 * gsig * gsig));
+      v[i].x *= f;
+      v[i].y *= f;
+    }
+  }
+
+  __syncthreads();  // LDS region reuse
+  L::inverse(v, my, twi, t);
+  if (ok) {
+#pragma unroll
+    for (int i = 0; i < L::E; i++) out[(size_t)L::din_pos(i, t) * cols + col] = v[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// row pass
+// ---------------------------------------------------------------------------
+template <class T, int N, int OP>
+__global__ void __launch_bounds__((row_lines<T, N>() * LineCfg<N>::TL))
+k_row(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out, int rows, const cplx<T>* __restrict__ tw,
+      cplx<T>* __restrict__ g, const T* __restrict__ spt, UpdArgs<T> ua, T scale, T* __restrict__ sout,
+      ReduceSlots rs) {
+  constexpr int P = LineCfg<N>::P, Q = LineCfg<N>::Q, TL = LineCfg<N>::TL;
+  constexpr int R = row_lines<T, N>();
+  constexpr int PITCH = lds_pitch<T, N>();
+  using L = Line<T, P, Q, TL>;
+  extern __shared__ __align__(16) unsigned char smem[];
+  cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem);
+  const int t = threadIdx.x % TL;
+  const int rr = threadIdx.x / TL;
+  const int row = blockIdx.x * R + rr;
+  const bool ok = row < rows;
+  cplx<T>* my = lds + rr * PITCH;
+  const cplx<T>* twf = tw;
+  const cplx<T>* twi = tw + N;
+  const size_t base = (size_t)row * N;
+  cplx<T> v[L::E];
+
+  if constexpr (OP == ROW_FWD) {
+#pragma unroll
+    for (int i = 0; i < L::E; i++) v[i] = ok ? in[base + L::din_pos(i, t)] : cplx<T>{(T)0, (T)0};
+    L::forward(v, my, twf, t);
+    if (ok) {
+#pragma unroll
+      for (int i = 0; i < L::E; i++) out[base + L::dout_pos(i, t)] = v[i];
+    }
+    return;
+  } else {
+#pragma unroll
+    for (int i = 0; i < L::E; i++) v[i] = ok ? in[base + L::dout_pos(i, t)] : cplx<T>{(T)0, (T)0};
+    L::inverse(v, my, twi, t);
+  }
+
+  if constexpr (OP == ROW_INV) {
+    if (ok) {
+#pragma unroll
+      for (int i = 0; i < L::E; i++) out[base + L::din_pos(i, t)] = {v[i].x * scale, v[i].y * scale};
+    }
+    return;
+  }
+
+  if constexpr (OP == ROW_INV_MOD2) {
+    T m = 0;
+    double s = 0;
+#pragma unroll
+    for (int i = 0; i < L::E; i++) {
+      const T a = v[i].x * v[i].x + v[i].y * v[i].y;
+      if (ok) sout[base + L::din_pos(i, t)] = a;
+      if (ok) {
+        m = a > m ? a : m;
+        s += (double)a;
+      }
+    }
+    __shared__ double sh[16];
+    block_atomic_max(m, rs.maxbits, sh);
+    block_partial_sum(s, rs, sh);
+    return;
+  }
+
+  if constexpr (OP == ROW_INV_UPDATE_FWD) {
+    const T thr = (T)ua.scal[0];
+#pragma unroll
+    for (int i = 0; i < L::E; i++) {
+      const size_t a = base + L::din_pos(i, t);
+      const cplx<T> p = {v[i].x * ua.invN, v[i].y * ua.invN};
+      const cplx<T> go = ok ? g[a] : cplx<T>{(T)0, (T)0};
+      const T sv = ok ? spt[a] : (T)0;
+      v[i] = support_update(go, p, sv > thr, ua.algo, ua.beta, ua.relax);
+      if (ok) g[a] = v[i];
+    }
+    __syncthreads();  // LDS region reuse
+    L::forward(v, my, twf, t);
+    if (ok) {
+#pragma unroll
+      for (int i = 0; i < L::E; i++) out[base + L::dout_pos(i, t)] = v[i];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+// raise the dynamic-LDS limit of a kernel above 64 KiB (once per kernel)
+template <class K>
+static void ensure_lds(K kern, size_t bytes) {
+  static size_t done = 0;
+  if (bytes > 64 * 1024 && bytes > done) {
+    (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    done = bytes;
+  }
+}
+
+template <class T, int N, int OP>
+static hipError_t col_launch_n(int cols, const void* in, void* out, const void* tw, const T* I,
+                               const ProjArgs<T>& pa, double gs, hipStream_t st) {
+  constexpr int C = col_lines<T, N>();
+  constexpr int TL = LineCfg<N>::TL;
+  const size_t lds = (size_t)C * lds_pitch<T, N>() * sizeof(cplx<T>);
+  auto kern = k_col<T, N, OP>;
+  ensure_lds(kern, lds);
+  dim3 grid((cols + C - 1) / C), block(C * TL);
+  hipLaunchKernelGGL(kern, grid, block, lds, st, (const cplx<T>*)in, (cplx<T>*)out, cols, (const cplx<T>*)tw, I,
+                     pa, (T)gs);
+  return hipGetLastError();
+}
+
+template <class T, int OP>
+static hipError_t col_launch_op(int rows, int cols, const void* in, void* out, const void* tw, const T* I,
+                                const ProjArgs<T>& pa, double gs, hipStream_t st) {
+  switch (rows) {
+#define CASE(n) case n: return col_launch_n<T, n, OP>(cols, in, out, tw, I, pa, gs, st);
+    CASE(16) CASE(32) CASE(64) CASE(128) CASE(256) CASE(384) CASE(512) CASE(768) CASE(1024) CASE(2048)
+#undef CASE
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <class T>
+hipError_t launch_col(int op, int rows, int cols, const void* in, void* out, const void* tw, const T* I,
+                      const ProjArgs<T>& pa, double gs, hipStream_t st) {
+  switch (op) {
+    case COL_FWD: return col_launch_op<T, COL_FWD>(rows, cols, in, out, tw, I, pa, gs, st);
+    case COL_INV: return col_launch_op<T, COL_INV>(rows, cols, in, out, tw, I, pa, gs, st);
+    case COL_FWD_PROJ_INV: return col_launch_op<T, COL_FWD_PROJ_INV>(rows, cols, in, out, tw, I, pa, gs, st);
+    case COL_PROJ_INV: return col_launch_op<T, COL_PROJ_INV>(rows, cols, in, out, tw, I, pa, gs, st);
+    case COL_FWD_GAUSS_INV: return col_launch_op<T, COL_FWD_GAUSS_INV>(rows, cols, in, out, tw, I, pa, gs, st);
+    case COL_GAUSS_INV: return col_launch_op<T, COL_GAUSS_INV>(rows, cols, in, out, tw, I, pa, gs, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <class T, int N, int OP>
+static hipError_t row_launch_n(int rows, const void* in, void* out, const void* tw, void* g, const T* spt,
+                               const UpdArgs<T>& ua, T scale, T* sout, const ReduceSlots& rs, hipStream_t st) {
+  constexpr int R = row_lines<T, N>();
+  constexpr int TL = LineCfg<N>::TL;
+  const size_t lds = (size_t)R * lds_pitch<T, N>() * sizeof(cplx<T>);
+  auto kern = k_row<T, N, OP>;
+  ensure_lds(kern, lds);
+  dim3 grid((rows + R - 1) / R), block(R * TL);
+  hipLaunchKernelGGL(kern, grid, block, lds, st, (const cplx<T>*)in, (cplx<T>*)out, rows, (const cplx<T>*)tw,
+                     (cplx<T>*)g, spt, ua, scale, sout, rs);
+  return hipGetLastError();
+}
+
+template <class T, int OP>
+static hipError_t row_launch_op(int rows, int cols, const void* in, void* out, const void* tw, void* g,
+                                const T* spt, const UpdArgs<T>& ua, T scale, T* sout, const ReduceSlots& rs,
+                                hipStream_t st) {
+  switch (cols) {
+#define CASE(n) case n: return row_launch_n<T, n, OP>(rows, in, out, tw, g, spt, ua, scale, sout, rs, st);
+    CASE(16) CASE(32) CASE(64) CASE(128) CASE(256) CASE(384) CASE(512) CASE(768) CASE(1024) CASE(2048)
+#undef CASE
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <class T>
+hipError_t launch_row(int op, int rows, int cols, const void* in, void* out, const void* tw, void* g,
+                      const T* spt, const UpdArgs<T>& ua, T scale, T* sout, const ReduceSlots& rs,
+                      hipStream_t st) {
+  switch (op) {
+    case ROW_FWD: return row_launch_op<T, ROW_FWD>(rows, cols, in, out, tw, g, spt, ua, scale, sout, rs, st);
+    case ROW_INV: return row_launch_op<T, ROW_INV>(rows, cols, in, out, tw, g, spt, ua, scale, sout, rs, st);
+    case ROW_INV_UPDATE_FWD:
+      return row_launch_op<T, ROW_INV_UPDATE_FWD>(rows, cols, in, out, tw, g, spt, ua, scale, sout, rs, st);
+    case ROW_INV_MOD2: return row_launch_op<T, ROW_INV_MOD2>(rows, cols, in, out, tw, g, spt, ua, scale, sout, rs, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template hipError_t launch_col<float>(int, int, int, const void*, void*, const void*, const float*,
+                                      const ProjArgs<float>&, double, hipStream_t);
+template hipError_t launch_col<double>(int, int, int, const void*, void*, const void*, const double*,
+                                       const ProjArgs<double>&, double, hipStream_t);
+template hipError_t launch_row<float>(int, int, int, const void*, void*, const void*, void*, const float*,
+                                      const UpdArgs<float>&, float, float*, const ReduceSlots&, hipStream_t);
+template hipError_t launch_row<double>(int, int, int, const void*, void*, const void*, void*, const double*,
+                                       const UpdArgs<double>&, double, double*, const ReduceSlots&, hipStream_t);
+
+// ---------------------------------------------------------------------------
+// elementwise kernels (grid-stride, 256 threads)
+// ---------------------------------------------------------------------------
+static inline dim3 ew_grid(int64_t n) {
+  int64_t b = (n + 255) / 256;
+  if (b > 65536) b = 65536;
+  if (b < 1) b = 1;
+  return dim3((unsigned)b);
+}
+#define EW_LOOP(n) for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (n); i += (int64_t)gridDim.x * blockDim.x)
+
+// beamstop folded into the intensities as a NaN sentinel (bs > 0.5)
+template <class T>
+__global__ void k_pack(int64_t n, const T* __restrict__ I, const T* __restrict__ bs, T* __restrict__ out) {
+  EW_LOOP(n) out[i] = (bs && bs[i] > (T)0.5) ? (T)NAN : I[i];
+}
+template <class T>
+hipError_t launch_pack_intensity(int64_t n, const T* I, const T* bs, T* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_pack<T>, ew_grid(n), dim3(256), 0, st, n, I, bs, out);
+  return hipGetLastError();
+}
+
+template <class T>
+__global__ void k_mod_abs(int64_t n, cplx<T>* F, const T* I) {
+  EW_LOOP(n) F[i] = mod_abs(F[i], I[i]);
+}
+template <class T>
+hipError_t launch_mod_abs(int64_t n, void* F, const T* I, hipStream_t st) {
+  hipLaunchKernelGGL(k_mod_abs<T>, ew_grid(n), dim3(256), 0, st, n, (cplx<T>*)F, I);
+  return hipGetLastError();
+}
+
+// applyMod with the explicit beamstop array and bsnorm (cdi.cu:438-485)
+template <class T>
+__global__ void k_mod_noise(int64_t n, cplx<T>* F, const T* I, const T* bs, int noise, T scale, T rcolor, T bsnorm) {
+  EW_LOOP(n) {
+    cplx<T> f = F[i];
+    if (bs && bs[i] > (T)0.5) {
+      if (bsnorm != (T)1) F[i] = {f.x * bsnorm, f.y * bsnorm};
+      continue;
+    }
+    if (bsnorm != (T)1) f = {f.x * bsnorm, f.y * bsnorm};
+    const T Iv = I[i];
+    F[i] = mod_noise(f, isnan(Iv) ? (T)0 : Iv, noise, scale, rcolor);
+  }
+}
+template <class T>
+hipError_t launch_mod_noise(int64_t n, void* F, const T* I, const T* bs, int noise, T scale, T rcolor, T bsnorm,
+                            hipStream_t st) {
+  hipLaunchKernelGGL(k_mod_noise<T>, ew_grid(n), dim3(256), 0, st, n, (cplx<T>*)F, I, bs, noise, scale, rcolor,
+                     bsnorm);
+  return hipGetLastError();
+}
+
+template <class T>
+__global__ void k_support(int64_t n, cplx<T>* g, const cplx<T>* gp, const T* spt, UpdArgs<T> ua) {
+  const T thr = (T)ua.scal[0];
+  EW_LOOP(n) g[i] = support_update(g[i], gp[i], spt[i] > thr, ua.algo, ua.beta, ua.relax);
+}
+template <class T>
+hipError_t launch_support(int64_t n, void* g, const void* gp, const T* spt, const UpdArgs<T>& ua, hipStream_t st) {
+  hipLaunchKernelGGL(k_support<T>, ew_grid(n), dim3(256), 0, st, n, (cplx<T>*)g, (const cplx<T>*)gp, spt, ua);
+  return hipGetLastError();
+}
+
+template <class T>
+__global__ void k_norm_c(int64_t n, cplx<T>* d, T f) {
+  EW_LOOP(n) d[i] = {d[i].x * f, d[i].y * f};
+}
+template <class T>
+hipError_t launch_norm_c(int64_t n, void* d, T f, hipStream_t st) {
+  hipLaunchKernelGGL(k_norm_c<T>, ew_grid(n), dim3(256), 0, st, n, (cplx<T>*)d, f);
+  return hipGetLastError();
+}
+template <class T>
+__global__ void k_norm_r(int64_t n, T* d, T f) {
+  EW_LOOP(n) d[i] *= f;
+}
+template <class T>
+hipError_t launch_norm_r(int64_t n, T* d, T f, hipStream_t st) {
+  hipLaunchKernelGGL(k_norm_r<T>, ew_grid(n), dim3(256), 0, st, n, d, f);
+  return hipGetLastError();
+}
+
+template <class T>
+__global__ void k_mod2(int64_t n, T* out, const cplx<T>* in, T norm) {
+  EW_LOOP(n) {
+    cplx<T> a = in[i];
+    out[i] = (a.x * a.x + a.y * a.y) * norm;
+  }
+}
+template <class T>
+hipError_t launch_mod2(int64_t n, T* out, const void* in, T norm, hipStream_t st) {
+  hipLaunchKernelGGL(k_mod2<T>, ew_grid(n), dim3(256), 0, st, n, out, (const cplx<T>*)in, norm);
+  return hipGetLastError();
+}
+
+template <class T>
+__global__ void k_gauss_freq(int rows, int cols, const cplx<T>* in, cplx<T>* out, T sigma) {
+  const int64_t n = (int64_t)rows * cols;
+  EW_LOOP(n) {
+    const int x = (int)(i / cols), y = (int)(i % cols);
+    const T xr = (T)(x >= rows / 2 ? x - rows : x);
+    const T yr = (T)(y >= cols / 2 ? y - cols : y);
+    const T f = (T)exp(-(xr * xr + yr * yr) / ((T)2 * sigma * sigma));
+    out[i] = {f * in[i].x, f * in[i].y};
+  }
+}
+template <class T>
+hipError_t launch_gauss_mult_freq(int rows, int cols, const void* in, void* out, T sigma, hipStream_t st) {
+  hipLaunchKernelGGL(k_gauss_freq<T>, ew_grid((int64_t)rows * cols), dim3(256), 0, st, rows, cols,
+                     (const cplx<T>*)in, (cplx<T>*)out, sigma);
+  return hipGetLastError();
+}
+
+// cudaConvertFO (cudaConfig.cu:202-216): quadrant swap; out may equal in
+template <class E>
+__global__ void k_convert_fo(int rows, int cols, const E* in, E* out) {
+  const int64_t half = (int64_t)rows * cols / 2;
+  EW_LOOP(half) {
+    const int x = (int)(i / cols), y = (int)(i % cols);
+    const int64_t ip = (int64_t)(x + (rows >> 1)) * cols + (y >= (cols >> 1) ? y - (cols >> 1) : y + (cols >> 1));
+    E a = in[i], b = in[ip];
+    out[i] = b;
+    out[ip] = a;
+  }
+}
+template <class T>
+hipError_t launch_convert_fo(int rows, int cols, const void* in, void* out, int cpx, hipStream_t st) {
+  const int64_t half = (int64_t)rows * cols / 2;
+  if (cpx)
+    hipLaunchKernelGGL(k_convert_fo<cplx<T>>, ew_grid(half), dim3(256), 0, st, rows, cols, (const cplx<T>*)in,
+                       (cplx<T>*)out);
+  else
+    hipLaunchKernelGGL(k_convert_fo<T>, ew_grid(half), dim3(256), 0, st, rows, cols, (const T*)in, (T*)out);
+  return hipGetLastError();
+}
+
+// createMask<rect> (cudaConfig.cu:1291-1303, rect::isInside :89-92)
+template <class T>
+__global__ void k_rect(int rows, int cols, int sx, int sy, int ex, int ey, T* out) {
+  const int64_t n = (int64_t)rows * cols;
+  EW_LOOP(n) {
+    const int x = (int)(i / cols), y = (int)(i % cols);
+    out[i] = (x >= sx && x < ex && y >= sy && y < ey) ? (T)1 : (T)0;
+  }
+}
+template <class T>
+hipError_t launch_rect_mask(int rows, int cols, int sx, int sy, int ex, int ey, T* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_rect<T>, ew_grid((int64_t)rows * cols), dim3(256), 0, st, rows, cols, sx, sy, ex, ey, out);
+  return hipGetLastError();
+}
+// createMask<C_circle> centred at (rows>>1, cols>>1) (cudaConfig.cu:93-98),
+// freq=1 applies the quadrant shift first (createBeamStop, misc.cc:191-195)
+template <class T>
+__global__ void k_circle(int rows, int cols, float radius, int freq, T* out) {
+  const int64_t n = (int64_t)rows * cols;
+  EW_LOOP(n) {
+    int x = (int)(i / cols), y = (int)(i % cols);
+    if (freq) {
+      x = x >= rows / 2 ? x - rows / 2 : x + rows / 2;
+      y = y >= cols / 2 ? y - cols / 2 : y + cols / 2;
+    }
+    const float dx = (float)(x - (rows >> 1)), dy = (float)(y - (cols >> 1));
+    out[i] = sqrtf(dx * dx + dy * dy) < radius ? (T)1 : (T)0;
+  }
+}
+template <class T>
+hipError_t launch_circle_mask(int rows, int cols, float radius, int freq, T* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_circle<T>, ew_grid((int64_t)rows * cols), dim3(256), 0, st, rows, cols, radius, freq, out);
+  return hipGetLastError();
+}
+
+// max (init 0, as max_op with Real(0) in cub_wrap.cu:88-92) and sum
+template <class T>
+__global__ void k_reduce(int64_t n, const T* x, ReduceSlots rs) {
+  T m = 0;
+  double s = 0;
+  EW_LOOP(n) {
+    const T v = x[i];
+    m = v > m ? v : m;
+    s += (double)v;
+  }
+  __shared__ double sh[16];
+  block_atomic_max(m, rs.maxbits, sh);
+  block_partial_sum(s, rs, sh);
+}
+template <class T>
+hipError_t launch_reduce(int64_t n, const T* x, const ReduceSlots& rs, hipStream_t st) {
+  dim3 grid = ew_grid(n);
+  if (grid.x > 1024) grid.x = 1024;  // partial-sum capacity
+  hipLaunchKernelGGL(k_reduce<T>, grid, dim3(256), 0, st, n, x, rs);
+  return hipGetLastError();
+}
+
+// Shrink-wrap threshold from the reduced max/sum, then clear the slots.
+// mode 0: cdi.cc:330-333   threshold = min(max, 5*avg)*shrinkThreshold,
+//         avg = 4*sum/N taken on the first call only (scal[1] holds it)
+// mode 1: holo.cc:141      threshold = max*shrinkThreshold
+// one wavefront; the partial sums are added in a fixed order
+template <class T>
+__global__ void __launch_bounds__(64) k_threshold(double* scal, ReduceSlots rs, double shrinkThreshold, int rows,
+                                                  int cols, int mode) {
+  const int np = *rs.nparts;
+  double s = 0;
+  for (int i = threadIdx.x; i < np; i += 64) s += rs.partials[i];
+  s = wave_sum(s);
+  if (threadIdx.x != 0) return;
+  T maxs;
+  if (sizeof(T) == 4) maxs = (T)__uint_as_float(*(unsigned int*)rs.maxbits);
+  else maxs = (T)__longlong_as_double(*(long long*)rs.maxbits);
+  if (mode == 0) {
+    T avg = (T)scal[1];
+    if (avg == (T)0) {
+      avg = (T)s * (T)4 / (T)rows / (T)cols;
+      scal[1] = (double)avg;
+    }
+    if (avg * (T)5 < maxs) maxs = avg * (T)5;
+  }
+  scal[0] = (double)(T)((double)maxs * shrinkThreshold);
+  scal[2] = (double)maxs;
+  if (sizeof(T) == 4) *(unsigned int*)rs.maxbits = 0u;
+  else *(unsigned long long*)rs.maxbits = 0ull;
+}
+template <class T>
+hipError_t launch_threshold(double* scal, const ReduceSlots& rs, double shrinkThreshold, int rows, int cols,
+                            int mode, hipStream_t st) {
+  hipLaunchKernelGGL(k_threshold<T>, dim3(1), dim3(64), 0, st, scal, rs, shrinkThreshold, rows, cols, mode);
+  return hipGetLastError();
+}
+
+// Real-space shrink-wrap blur (applyGaussConv, misc.cc:80-86 with createGauss
+// cudaConfig.cu:447-457 and applyConvolution :475-507): the 2D kernel
+// exp(-(dx^2+dy^2)/(2s^2))/(2 pi s^2) on a (2r+1)^2 window, r = floor(6s)>>1,
+// zero padding, is the product of two 1D kernels, so it runs as a row pass
+// and a column pass through LDS.
+template <class T, bool CPXIN>
+__global__ void k_blur_rows(int rows, int cols, const void* __restrict__ src, T* __restrict__ tmp, T sigma, int r) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  T* line = reinterpret_cast<T*>(smem);  // cols + 2r
+  T* w = line + cols + 2 * r;            // 2r+1 weights
+  const int x = blockIdx.x;
+  const double norm = 1.0 / (2.5066282746310002 * (double)sigma);  // 1/(sqrt(2 pi) s)
+  for (int j = threadIdx.x; j <= 2 * r; j += blockDim.x) {
+    const int d = j - r;
+    w[j] = (T)(norm * exp(-(double)(d * d) / (2.0 * (double)sigma * (double)sigma)));
+  }
+  for (int j = threadIdx.x; j < cols + 2 * r; j += blockDim.x) {
+    const int y = j - r;
+    T v = 0;
+    if (y >= 0 && y < cols) {
+      if constexpr (CPXIN) {
+        const cplx<T> a = ((const cplx<T>*)src)[(size_t)x * cols + y];
+        v = a.x * a.x + a.y * a.y;
+      } else {
+        v = ((const T*)src)[(size_t)x * cols + y];
+      }
+    }
+    line[j] = v;
+  }
+  __syncthreads();
+  for (int y = threadIdx.x; y < cols; y += blockDim.x) {
+    T s = 0;
+    for (int k = 0; k <= 2 * r; k++) s += line[y + k] * w[k];
+    tmp[(size_t)x * cols + y] = s;
+  }
+}
+// column pass: a workgroup owns 16 adjacent columns, staged through LDS
+template <class T>
+__global__ void k_blur_cols(int rows, int cols, const T* __restrict__ tmp, T* __restrict__ spt, T sigma, int r,
+                            ReduceSlots rs) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int CW = 16;
+  T* tile = reinterpret_cast<T*>(smem);  // (rows + 2r) x CW
+  T* w = tile + (rows + 2 * r) * CW;
+  const int c0 = blockIdx.x * CW;
+  const double norm = 1.0 / (2.5066282746310002 * (double)sigma);
+  for (int j = threadIdx.x; j <= 2 * r; j += blockDim.x) {
+    const int d = j - r;
+    w[j] = (T)(norm * exp(-(double)(d * d) / (2.0 * (double)sigma * (double)sigma)));
+  }
+  for (int j = threadIdx.x; j < (rows + 2 * r) * CW; j += blockDim.x) {
+    const int xx = j / CW - r, cc = j % CW;
+    const int y = c0 + cc;
+    tile[j] = (xx >= 0 && xx < rows && y < cols) ? tmp[(size_t)xx * cols + y] : (T)0;
+  }
+  __syncthreads();
+  T m = 0;
+  double sum = 0;
+  for (int j = threadIdx.x; j < rows * CW; j += blockDim.x) {
+    const int x = j / CW, cc = j % CW;
+    const int y = c0 + cc;
+    T s = 0;
+    for (int k = 0; k <= 2 * r; k++) s += tile[(x + k) * CW + cc] * w[k];
+    if (y < cols) {
+      spt[(size_t)x * cols + y] = s;
+      m = s > m ? s : m;
+      sum += (double)s;
+    }
+  }
+  __shared__ double sh[16];
+  block_atomic_max(m, rs.maxbits, sh);
+  block_partial_sum(sum, rs, sh);
+}
+// g complex (blur of |g|^2) when cpx_in, else a real image
+template <class T>
+hipError_t launch_blur_real(int rows, int cols, const void* g, int cpx_in, T* tmp, T* spt, T sigma,
+                            const ReduceSlots& rs, hipStream_t st) {
+  const int r = ((int)floor((double)sigma * 6)) >> 1;
+  size_t l1 = (size_t)(cols + 2 * r + 2 * r + 1) * sizeof(T);
+  if (cpx_in)
+    hipLaunchKernelGGL((k_blur_rows<T, true>), dim3(rows), dim3(256), l1, st, rows, cols, g, tmp, sigma, r);
+  else
+    hipLaunchKernelGGL((k_blur_rows<T, false>), dim3(rows), dim3(256), l1, st, rows, cols, g, tmp, sigma, r);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  size_t l2 = (size_t)((rows + 2 * r) * 16 + 2 * r + 1) * sizeof(T);
+  auto kern = k_blur_cols<T>;
+  ensure_lds(kern, l2);
+  hipLaunchKernelGGL(kern, dim3((cols + 15) / 16), dim3(256), l2, st, rows, cols, (const T*)tmp, spt, sigma, r, rs);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Philox4x64-10 (Salmon et al., SC'11) for random starts and the forward model
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void philox4x64(uint64_t c[4], uint64_t k0, uint64_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; r++) {
+    const uint64_t lo0 = 0xD2E7470EE14C6C93ULL * c[0], hi0 = __umul64hi(0xD2E7470EE14C6C93ULL, c[0]);
+    const uint64_t lo1 = 0xCA5A826395121157ULL * c[2], hi1 = __umul64hi(0xCA5A826395121157ULL, c[2]);
+    const uint64_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0;
+    c[1] = lo1;
+    c[2] = n2;
+    c[3] = lo0;
+    k0 += 0x9E3779B97F4A7C15ULL;
+    k1 += 0xBB67AE8584CAA73BULL;
+  }
+}
+__device__ __forceinline__ double philox_uniform(uint64_t seed, uint64_t stream, uint64_t idx) {
+  uint64_t c[4] = {idx, 0, 0, 0};
+  philox4x64(c, seed, stream);
+  return (double)(c[0] >> 11) * 0x1.0p-53;
+}
+
+// Start state (prepareIter, cdi.cc:222-223): createWaveFront(sqrt(I)) then
+// applyRandomPhase (cdi.cu:486-502), zero where bs > threshold.
+// mode 0: random sign (the reference's (U>0.5)*pi phase, evaluated as
+//         sincosf(pi) in float); mode 1: uniform phase in [-pi, pi).
+template <class T>
+__global__ void k_start(int64_t n, const T* I, const T* bs, double thr, uint64_t seed, int mode, cplx<T>* F) {
+  EW_LOOP(n) {
+    if (bs && (double)bs[i] > thr) {
+      F[i] = {(T)0, (T)0};
+      continue;
+    }
+    const T Iv = I[i];
+    const T m = Iv > (T)0 ? dsqrt(Iv) : (T)0;
+    const double u = philox_uniform(seed, 3, (uint64_t)i);
+    if (mode == 0) {
+      if (u > 0.5) F[i] = {-m, m * (T)(-8.742277657347586e-08)};
+      else F[i] = {m, (T)0};
+    } else {
+      const double ph = -kPi + 2.0 * kPi * u;
+      F[i] = {(T)(m * cos(ph)), (T)(m * sin(ph))};
+    }
+  }
+}
+template <class T>
+hipError_t launch_start(int64_t n, const T* I, const T* bs, double thr, uint64_t seed, int mode, void* F,
+                        hipStream_t st) {
+  hipLaunchKernelGGL(k_start<T>, ew_grid(n), dim3(256), 0, st, n, I, bs, thr, seed, mode, (cplx<T>*)F);
+  return hipGetLastError();
+}
+
+__global__ void k_fill_f64(int64_t n, double* p, double v) {
+  EW_LOOP(n) p[i] = v;
+}
+hipError_t launch_fill_f64(int64_t n, double* p, double v, hipStream_t st) {
+  hipLaunchKernelGGL(k_fill_f64, ew_grid(n), dim3(256), 0, st, n, p, v);
+  return hipGetLastError();
+}
+
+// Final residual of phaseRetrieve (cdi.cc:413-421): sum over I < 0.99 of
+// ((|F|^2 - I)*(1-bs))^2
+template <class T>
+__global__ void k_residual(int64_t n, const cplx<T>* F, const T* I, const T* bs, double* acc) {
+  double s = 0;
+  EW_LOOP(n) {
+    const cplx<T> f = F[i];
+    T d = I[i] < (T)0.99 ? (f.x * f.x + f.y * f.y) - I[i] : (T)0;
+    if (bs) d *= (T)1 - bs[i];
+    s += (double)(d * d);
+  }
+  __shared__ double sh[16];
+  block_atomic_sum(s, acc, sh);
+}
+template <class T>
+hipError_t launch_residual(int64_t n, const void* F, const T* I, const T* bs, double* acc, hipStream_t st) {
+  hipLaunchKernelGGL(k_residual<T>, ew_grid(n), dim3(256), 0, st, n, (const cplx<T>*)F, I, bs, acc);
+  return hipGetLastError();
+}
+
+// R-factor sums of an explicit F against the packed intensities
+template <class T>
+__global__ void k_rfactor(int64_t n, const cplx<T>* F, const T* Ipk, double* acc) {
+  double a = 0, b = 0;
+  EW_LOOP(n) {
+    const T Iv = Ipk[i];
+    if (isnan(Iv)) continue;
+    const T s = Iv > (T)0 ? dsqrt(Iv) : (T)0;
+    a += (double)fabs(dhypot(F[i].x, F[i].y) - s);
+    b += (double)s;
+  }
+  __shared__ double sh[16];
+  block_atomic_sum(a, acc, sh);
+  block_atomic_sum(b, acc + 1, sh);
+}
+template <class T>
+hipError_t launch_rfactor(int64_t n, const void* F, const T* Ipk, double* acc, hipStream_t st) {
+  hipLaunchKernelGGL(k_rfactor<T>, ew_grid(n), dim3(256), 0, st, n, (const cplx<T>*)F, Ipk, acc);
+  return hipGetLastError();
+}
+
+template <class T>
+__global__ void k_cast_c(int64_t n, const void* in, int in_f64, cplx<T>* out) {
+  EW_LOOP(n) {
+    if (in_f64) {
+      const cplx<double> a = ((const cplx<double>*)in)[i];
+      out[i] = {(T)a.x, (T)a.y};
+    } else {
+      const cplx<float> a = ((const cplx<float>*)in)[i];
+      out[i] = {(T)a.x, (T)a.y};
+    }
+  }
+}
+template <class T>
+hipError_t launch_cast_c(int64_t n, const void* in, int in_f64, void* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_cast_c<T>, ew_grid(n), dim3(256), 0, st, n, in, in_f64, (cplx<T>*)out);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// synthetic forward model (same streams and arithmetic as the CPU oracle)
+// ---------------------------------------------------------------------------
+__global__ void k_uniform_field(int64_t n, uint64_t seed, uint64_t stream, double* out) {
+  EW_LOOP(n) out[i] = philox_uniform(seed, stream, (uint64_t)i);
+}
+hipError_t launch_uniform_field(int n, uint64_t seed, uint64_t stream, double* out, hipStream_t st) {
+  const int64_t N = (int64_t)n * n;
+  hipLaunchKernelGGL(k_uniform_field, ew_grid(N), dim3(256), 0, st, N, seed, stream, out);
+  return hipGetLastError();
+}
+// periodic Gaussian smoothing along one axis, radius ceil(4 sigma), weights
+// normalised to sum 1 (accumulated in the same order as the oracle)
+__global__ void k_smooth(int n, const double* in, double* out, double sigma, int axis) {
+  const int r = (int)ceil(4 * sigma);
+  const int64_t N = (int64_t)n * n;
+  EW_LOOP(N) {
+    const int x = (int)(i / n), y = (int)(i % n);
+    double ws = 0;
+    for (int k = -r; k <= r; k++) ws += exp(-(double)k * k / (2 * sigma * sigma));
+    double s = 0;
+    for (int k = -r; k <= r; k++) {
+      const double w = exp(-(double)k * k / (2 * sigma * sigma)) / ws;
+      const int64_t j = axis ? (int64_t)(((x + k) % n + n) % n) * n + y : (int64_t)x * n + ((y + k) % n + n) % n;
+      s += w * in[j];
+    }
+    out[i] = s;
+  }
+}
+hipError_t launch_smooth_periodic(int n, const double* in, double* out, double sigma, int axis, hipStream_t st) {
+  const int64_t N = (int64_t)n * n;
+  hipLaunchKernelGGL(k_smooth, ew_grid(N), dim3(256), 0, st, n, in, out, sigma, axis);
+  return hipGetLastError();
+}
+// mm[0] = min, mm[1] = max; caller seeds mm with (+inf, -inf)
+__global__ void k_minmax(int64_t n, const double* x, double* mm) {
+  double lo = INFINITY, hi = -INFINITY;
+  EW_LOOP(n) {
+    lo = fmin(lo, x[i]);
+    hi = fmax(hi, x[i]);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    lo = fmin(lo, __shfl_xor(lo, o, 64));
+    hi = fmax(hi, __shfl_xor(hi, o, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    // order-preserving integer maps for atomics on doubles of any sign
+    long long bl = __double_as_longlong(lo), bh = __double_as_longlong(hi);
+    unsigned long long kl = bl < 0 ? ~(unsigned long long)bl : ((unsigned long long)bl | 0x8000000000000000ULL);
+    unsigned long long kh = bh < 0 ? ~(unsigned long long)bh : ((unsigned long long)bh | 0x8000000000000000ULL);
+    atomicMin((unsigned long long*)mm, kl);
+    atomicMax((unsigned long long*)mm + 1, kh);
+  }
+}
+hipError_t launch_minmax(int64_t n, const double* x, double* mm, hipStream_t st) {
+  hipLaunchKernelGGL(k_minmax, ew_grid(n), dim3(256), 0, st, n, x, mm);
+  return hipGetLastError();
+}
+__device__ __forceinline__ double unkey(unsigned long long k) {
+  const unsigned long long b = (k & 0x8000000000000000ULL) ? (k & 0x7FFFFFFFFFFFFFFFULL) : ~k;
+  return __longlong_as_double((long long)b);
+}
+template <class T>
+__global__ void k_compose(int n, const double* a, const double* ph, const double* mma, const double* mmp,
+                          float radius, cplx<T>* obj) {
+  const int64_t N = (int64_t)n * n;
+  const double amin = unkey(((const unsigned long long*)mma)[0]), amax = unkey(((const unsigned long long*)mma)[1]);
+  const double pmin = unkey(((const unsigned long long*)mmp)[0]), pmax = unkey(((const unsigned long long*)mmp)[1]);
+  const int c = n >> 1;
+  EW_LOOP(N) {
+    const int x = (int)(i / n), y = (int)(i % n);
+    const float dx = (float)(x - c), dy = (float)(y - c);
+    if (!(sqrtf(dx * dx + dy * dy) < radius)) {
+      obj[i] = {(T)0, (T)0};
+      continue;
+    }
+    const double amp = 0.2 + 0.8 * (a[i] - amin) / (amax - amin);
+    const double phs = -kPi + 2 * kPi * (ph[i] - pmin) / (pmax - pmin);
+    obj[i] = {(T)(amp * cos(phs)), (T)(amp * sin(phs))};
+  }
+}
+template <class T>
+hipError_t launch_compose_object(int n, const double* a, const double* ph, const double* mma, const double* mmp,
+                                 float radius, void* obj, hipStream_t st) {
+  hipLaunchKernelGGL(k_compose<T>, ew_grid((int64_t)n * n), dim3(256), 0, st, n, a, ph, mma, mmp, radius,
+                     (cplx<T>*)obj);
+  return hipGetLastError();
+}
+
+template <class T>
+__global__ void k_sum_mod2(int64_t n, const cplx<T>* F, double* acc) {
+  double s = 0;
+  EW_LOOP(n) s += (double)F[i].x * F[i].x + (double)F[i].y * F[i].y;
+  __shared__ double sh[16];
+  block_atomic_sum(s, acc, sh);
+}
+template <class T>
+hipError_t launch_sum_mod2(int64_t n, const void* F, double* acc, hipStream_t st) {
+  hipLaunchKernelGGL(k_sum_mod2<T>, ew_grid(n), dim3(256), 0, st, n, (const cplx<T>*)F, acc);
+  return hipGetLastError();
+}
+
+// Poisson(lam): multiplication method below 10, PTRS (Hoermann 1993) above;
+// uniforms from Philox stream 2, block counter (pixel, j).
+struct UStream {
+  uint64_t seed, idx, blk;
+  uint64_t buf[4];
+  int pos;
+  __device__ double next() {
+    if (pos == 4) {
+      buf[0] = idx; buf[1] = blk++; buf[2] = 0; buf[3] = 0;
+      philox4x64(buf, seed, 2);
+      pos = 0;
+    }
+    return (double)(buf[pos++] >> 11) * 0x1.0p-53;
+  }
+};
+__device__ double poisson_draw(double lam, uint64_t seed, uint64_t idx) {
+  UStream u{seed, idx, 0, {0, 0, 0, 0}, 4};
+  if (lam <= 0) return 0;
+  if (lam < 10) {
+    const double enlam = exp(-lam);
+    double prod = 1.0, X = 0;
+    for (;;) {
+      prod *= u.next();
+      if (prod > enlam) X += 1;
+      else return X;
+    }
+  }
+  const double slam = sqrt(lam), loglam = log(lam);
+  const double b = 0.931 + 2.53 * slam;
+  const double a = -0.059 + 0.02483 * b;
+  const double invalpha = 1.1239 + 1.1328 / (b - 3.4);
+  const double vr = 0.9277 - 3.6224 / (b - 2);
+  for (;;) {
+    const double U = u.next() - 0.5;
+    const double V = u.next();
+    const double us = 0.5 - fabs(U);
+    const double k = floor((2 * a / us + b) * U + lam + 0.43);
+    if (us >= 0.07 && V <= vr) return k;
+    if (k < 0 || (us < 0.013 && V > us)) continue;
+    if (log(V) + log(invalpha) - log(a / (us * us) + b) <= -lam + k * loglam - lgamma(k + 1)) return k;
+  }
+}
+template <class T>
+__global__ void k_poisson(int64_t n, const cplx<T>* F, const double* tot, double photons, uint64_t seed, int poisson,
+                          T* I) {
+  const double sc = *tot > 0 ? photons / *tot : 0;
+  EW_LOOP(n) {
+    const double lam = ((double)F[i].x * F[i].x + (double)F[i].y * F[i].y) * sc;
+    I[i] = poisson ? (T)poisson_draw(lam, seed, (uint64_t)i) : (T)lam;
+  }
+}
+template <class T>
+hipError_t launch_poisson_intensity(int64_t n, const void* F, const double* tot, double photons, uint64_t seed,
+                                    int poisson, T* I, hipStream_t st) {
+  hipLaunchKernelGGL(k_poisson<T>, ew_grid(n), dim3(256), 0, st, n, (const cplx<T>*)F, tot, photons, seed, poisson,
+                     I);
+  return hipGetLastError();
+}
+
+// explicit instantiations
+#define INST(T)                                                                                                  \
+  template hipError_t launch_pack_intensity<T>(int64_t, const T*, const T*, T*, hipStream_t);                    \
+  template hipError_t launch_mod_abs<T>(int64_t, void*, const T*, hipStream_t);                                  \
+  template hipError_t launch_mod_noise<T>(int64_t, void*, const T*, const T*, int, T, T, T, hipStream_t);        \
+  template hipError_t launch_support<T>(int64_t, void*, const void*, const T*, const UpdArgs<T>&, hipStream_t);  \
+  template hipError_t launch_norm_c<T>(int64_t, void*, T, hipStream_t);                                          \
+  template hipError_t launch_norm_r<T>(int64_t, T*, T, hipStream_t);                                             \
+  template hipError_t launch_mod2<T>(int64_t, T*, const void*, T, hipStream_t);                                  \
+  template hipError_t launch_gauss_mult_freq<T>(int, int, const void*, void*, T, hipStream_t);                   \
+  template hipError_t launch_convert_fo<T>(int, int, const void*, void*, int, hipStream_t);                      \
+  template hipError_t launch_rect_mask<T>(int, int, int, int, int, int, T*, hipStream_t);                        \
+  template hipError_t launch_circle_mask<T>(int, int, float, int, T*, hipStream_t);                              \
+  template hipError_t launch_reduce<T>(int64_t, const T*, const ReduceSlots&, hipStream_t);                      \
+  template hipError_t launch_threshold<T>(double*, const ReduceSlots&, double, int, int, int, hipStream_t);      \
+  template hipError_t launch_blur_real<T>(int, int, const void*, int, T*, T*, T, const ReduceSlots&, hipStream_t);    \
+  template hipError_t launch_start<T>(int64_t, const T*, const T*, double, uint64_t, int, void*, hipStream_t);   \
+  template hipError_t launch_residual<T>(int64_t, const void*, const T*, const T*, double*, hipStream_t);        \
+  template hipError_t launch_rfactor<T>(int64_t, const void*, const T*, double*, hipStream_t);                   \
+  template hipError_t launch_cast_c<T>(int64_t, const void*, int, void*, hipStream_t);                           \
+  template hipError_t launch_compose_object<T>(int, const double*, const double*, const double*, const double*, \
+                                               float, void*, hipStream_t);                                       \
+  template hipError_t launch_poisson_intensity<T>(int64_t, const void*, const double*, double, uint64_t, int, T*, \
+                                                  hipStream_t);                                                  \
+  template hipError_t launch_sum_mod2<T>(int64_t, const void*, double*, hipStream_t);
+INST(float)
+INST(double)
+#undef INST
+
+}  // namespace cdx

@@ -1,0 +1,115 @@
+// kernels.hpp -- launch interface of the CDI hot-path kernels (kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cdx {
+
+// Projection modes of the fused column pass.
+enum ProjMode : int {
+  PROJ_MOD_ABS = 0,     // applyModAbs, src/gpu/cudaConfig.cu:827-841
+  PROJ_MOD_NOISE = 1,   // applyMod, src/gpu/cdi.cu:438-485 (beamstop as NaN sentinel)
+  PROJ_MOD_ABS_BS = 2,  // applyModAbs, beamstop pixels left unconstrained
+};
+
+// Column-pass operations (one workgroup owns C whole columns).
+enum ColOp : int {
+  COL_FWD = 0,            // Din load -> forward -> Dout store
+  COL_INV = 1,            // Dout load -> inverse -> Din store
+  COL_FWD_PROJ_INV = 2,   // forward -> [R-factor] -> projection -> inverse
+  COL_PROJ_INV = 3,       // input already in Fourier space
+  COL_FWD_GAUSS_INV = 4,  // forward -> Gaussian multiply -> inverse (shrink-wrap)
+  COL_GAUSS_INV = 5,
+};
+
+// Row-pass operations (one workgroup owns R whole rows).
+enum RowOp : int {
+  ROW_FWD = 0,
+  ROW_INV = 1,            // optional scale
+  ROW_INV_UPDATE_FWD = 2, // inverse -> 1/N -> support update -> store g -> forward
+  ROW_INV_MOD2 = 3,       // inverse -> |.|^2 -> S, max and sum (shrink-wrap)
+};
+
+template <class T>
+struct ProjArgs {
+  int mode;
+  int noiseLevel;
+  T scale;   // cudaVars.scale = 1/exposure
+  T rcolor;  // cudaVars.rcolor
+  double* rf;  // R-factor accumulators (num, den) or nullptr
+};
+
+template <class T>
+struct UpdArgs {
+  int algo;
+  T beta, relax, invN;
+  const double* scal;  // scal[0] = support threshold
+};
+
+// Deterministic reductions: every workgroup of the producing kernel stores
+// its partial sum at partials[blockIdx.x] (workgroup 0 stores the count);
+// the consumer adds them in a fixed order.  The max is an atomic on the bit
+// pattern (order-independent for non-negative values).
+struct ReduceSlots {
+  void* maxbits;     // uint32 (float) or uint64 (double) bit pattern of the max
+  double* partials;  // one entry per workgroup (capacity kMaxPartials)
+  int* nparts;
+};
+constexpr int kMaxPartials = 4096;
+
+// twiddle tables for a line of length n: twf (n entries) then twi (n entries)
+int line_supported(int n);
+void line_factors(int n, int* P, int* Q);
+
+template <class T>
+hipError_t launch_col(int op, int rows, int cols, const void* in, void* out, const void* tw,
+                      const T* I, const ProjArgs<T>& pa, double gauss_sigma, hipStream_t st);
+template <class T>
+hipError_t launch_row(int op, int rows, int cols, const void* in, void* out, const void* tw,
+                      void* g, const T* spt, const UpdArgs<T>& ua, T scale, T* sout,
+                      const ReduceSlots& rs, hipStream_t st);
+
+// elementwise / small kernels
+template <class T> hipError_t launch_pack_intensity(int64_t n, const T* I, const T* bs, T* out, hipStream_t st);
+template <class T> hipError_t launch_mod_abs(int64_t n, void* F, const T* I, hipStream_t st);
+template <class T> hipError_t launch_mod_noise(int64_t n, void* F, const T* I, const T* bs, int noise, T scale,
+                                               T rcolor, T bsnorm, hipStream_t st);
+template <class T> hipError_t launch_support(int64_t n, void* g, const void* gp, const T* spt,
+                                             const UpdArgs<T>& ua, hipStream_t st);
+template <class T> hipError_t launch_norm_c(int64_t n, void* d, T f, hipStream_t st);
+template <class T> hipError_t launch_norm_r(int64_t n, T* d, T f, hipStream_t st);
+template <class T> hipError_t launch_mod2(int64_t n, T* out, const void* in, T norm, hipStream_t st);
+template <class T> hipError_t launch_gauss_mult_freq(int rows, int cols, const void* in, void* out, T sigma,
+                                                     hipStream_t st);
+template <class T> hipError_t launch_convert_fo(int rows, int cols, const void* in, void* out, int cpx,
+                                                hipStream_t st);
+template <class T> hipError_t launch_rect_mask(int rows, int cols, int sx, int sy, int ex, int ey, T* out,
+                                               hipStream_t st);
+template <class T> hipError_t launch_circle_mask(int rows, int cols, float radius, int freq, T* out,
+                                                 hipStream_t st);
+template <class T> hipError_t launch_reduce(int64_t n, const T* x, const ReduceSlots& rs, hipStream_t st);
+template <class T> hipError_t launch_threshold(double* scal, const ReduceSlots& rs, double shrinkThreshold,
+                                               int rows, int cols, int mode, hipStream_t st);
+template <class T> hipError_t launch_blur_real(int rows, int cols, const void* g, int cpx_in, T* tmp, T* spt,
+                                               T sigma, const ReduceSlots& rs, hipStream_t st);
+template <class T> hipError_t launch_start(int64_t n, const T* I, const T* bs, double thr, uint64_t seed,
+                                           int mode, void* F, hipStream_t st);
+hipError_t launch_fill_f64(int64_t n, double* p, double v, hipStream_t st);
+template <class T> hipError_t launch_residual(int64_t n, const void* F, const T* I, const T* bs, double* acc,
+                                              hipStream_t st);
+template <class T> hipError_t launch_rfactor(int64_t n, const void* F, const T* Ipk, double* acc,
+                                             hipStream_t st);
+template <class T> hipError_t launch_cast_c(int64_t n, const void* in, int in_f64, void* out, hipStream_t st);
+
+// synthetic forward model
+hipError_t launch_uniform_field(int n, uint64_t seed, uint64_t stream, double* out, hipStream_t st);
+hipError_t launch_smooth_periodic(int n, const double* in, double* out, double sigma, int axis,
+                                  hipStream_t st);
+hipError_t launch_minmax(int64_t n, const double* x, double* mm, hipStream_t st);
+template <class T> hipError_t launch_compose_object(int n, const double* a, const double* ph, const double* mma,
+                                                    const double* mmp, float radius, void* obj, hipStream_t st);
+template <class T> hipError_t launch_poisson_intensity(int64_t n, const void* F, const double* tot, double photons,
+                                                       uint64_t seed, int poisson, T* I, hipStream_t st);
+template <class T> hipError_t launch_sum_mod2(int64_t n, const void* F, double* acc, hipStream_t st);
+
+}  // namespace cdx

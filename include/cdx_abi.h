@@ -1,0 +1,177 @@
+/*
+ * cdx_abi.h -- C ABI of the MI355X CDI phase-retrieval path (libcdixx_amd.so).
+ *
+ * Every entry point returns CDX_OK (0) or a CDX_ERR_* status; the message of
+ * the last failure on the calling thread is cdx_last_error().  Nothing here
+ * aborts or exits (the reference aborts in gpuAssert, src/gpu/cudaConfig.cu:27-34,
+ * and exits on bad data, src/post_gpu/cdi.cc:97-101).
+ *
+ * The hidden globals of the reference (image size cuda_imgsz, the cuFFT
+ * plan, __constant__ cudaVars{rcolor, beta_HIO, scale, threshold}:
+ * src/gpu/cudaConfig.cu:14-18, src/gpu/cdi.cu:8-30) live in an explicit
+ * context.  One context per host thread; calls on one context are ordered
+ * on its stream.
+ *
+ * Device buffers are caller-owned raw pointers.  In a CDX_F32 context real
+ * arrays are float and complex arrays interleaved float pairs (the layout of
+ * the reference's complexFormat = _Complex float, include/format.hpp:131-132);
+ * in a CDX_F64 context they are double and double pairs.  Arrays are
+ * row-major, index = x*cols + y; Fourier-space arrays are in FFT order.
+ */
+#ifndef CDX_ABI_H
+#define CDX_ABI_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  CDX_OK = 0,
+  CDX_ERR_ARG = 1,         /* bad argument (null pointer, unsupported size, ...) */
+  CDX_ERR_HIP = 2,         /* HIP runtime error */
+  CDX_ERR_SCHEDULE = 3,    /* malformed algorithm string / unknown token */
+  CDX_ERR_UNSUPPORTED = 4, /* token outside this path (TV) */
+  CDX_ERR_DATA = 5,        /* NaN or all-zero pattern */
+  CDX_ERR_IO = 6,          /* file read/write failure */
+  CDX_ERR_CONFIG = 7       /* unreadable configuration file */
+};
+enum { CDX_F32 = 0, CDX_F64 = 1 };
+/* algorithm tokens: the values of enum Algorithm, include/format.hpp:22 */
+enum { CDX_RAAR = 0, CDX_ER, CDX_POSER, CDX_HIO, CDX_POSHIO, CDX_FHIO, CDX_SHRINKWRAP,
+       CDX_XCORRELATION, CDX_KKT, CDX_TV };
+enum { CDX_FORWARD = -1, CDX_INVERSE = 1 };
+
+typedef struct cdx_ctx cdx_ctx;
+
+const char* cdx_last_error(void);
+const char* cdx_version(void);
+/* 1 if a line length is supported by the FFT (16..2048, 2^k and 3*2^k) */
+int cdx_size_supported(int n);
+
+/* ---- device memory: replaces memMngr borrow/return and myMemcpy* ----
+ * include/cudaConfig.hpp:16-21,32-38 */
+int cdx_malloc(void** ptr, size_t bytes);
+int cdx_free(void* ptr);
+int cdx_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int cdx_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int cdx_memcpy_d2d(void* dst, const void* src, size_t bytes);
+int cdx_memset0(void* ptr, size_t bytes);
+int cdx_device_synchronize(void);
+
+/* ---- context: replaces resize_cuda_image + init_fft + init_cuda_image ----
+ * include/cudaConfig.hpp:22-23,45; src/gpu/cudaConfig.cu:69-74,137-160;
+ * src/gpu/cdi.cu:16-30.  stream may be NULL (the context creates one at
+ * default priority). */
+int cdx_create(cdx_ctx** out, int rows, int cols, int precision, void* stream);
+int cdx_destroy(cdx_ctx* ctx);
+int cdx_synchronize(cdx_ctx* ctx);
+void* cdx_stream(cdx_ctx* ctx);
+/* init_cuda_image(rcolor, scale), include/cudaConfig.hpp:23 */
+int cdx_init_vars(cdx_ctx* ctx, int rcolor, double scale);
+/* setThreshold, include/cudaConfig.hpp:28 (src/gpu/cdi.cu:242-244) */
+int cdx_set_threshold(cdx_ctx* ctx, double threshold);
+int cdx_get_threshold(cdx_ctx* ctx, double* threshold);
+/* vars.beta_HIO (fixed to 1 in the reference, src/gpu/cdi.cu:22) and the
+ * 1.9 relaxation literal of ER/FHIO/POSHIO (src/gpu/cdi.cu:534-616) */
+int cdx_set_beta(cdx_ctx* ctx, double beta);
+int cdx_set_relax(cdx_ctx* ctx, double relax);
+
+/* ---- free functions on the path ---- */
+/* myFFT / myIFFT, include/cudaConfig.hpp:46-47: unnormalised 2D C2C, in==out allowed */
+int cdx_fft(cdx_ctx* ctx, const void* in, void* out, int direction);
+/* applyModAbs(source, target), include/cudaConfig.hpp:111 */
+int cdx_apply_mod_abs(cdx_ctx* ctx, void* F, const void* I);
+/* applyMod(source, target, bs, noiseLevel, norm), include/cudaConfig.hpp:107 */
+int cdx_apply_mod(cdx_ctx* ctx, void* F, const void* I, const void* bs, int noiseLevel, double bsnorm);
+/* applySupport(gkp1, gkprime, algo, spt), include/cdi.hpp:43 */
+int cdx_apply_support(cdx_ctx* ctx, void* g, const void* gprime, int algo, const void* spt);
+/* applyNorm(complexFormat*, Real) and applyNorm(Real*, Real), include/cudaConfig.hpp:71,73 */
+int cdx_apply_norm(cdx_ctx* ctx, void* data, double factor);
+int cdx_apply_norm_real(cdx_ctx* ctx, void* data, double factor);
+/* getMod2(Real*, complexFormat*, norm), include/cudaConfig.hpp:93 */
+int cdx_get_mod2(cdx_ctx* ctx, void* out, const void* in, double norm);
+/* applyGaussMult(complex in, out, sigma, isFreq=1), include/cudaConfig.hpp:85 */
+int cdx_gauss_mult_freq(cdx_ctx* ctx, const void* in, void* out, double sigma);
+/* applyGaussConv(input, output, gaussMem, sigma), include/misc.hpp; real input */
+int cdx_gauss_conv(cdx_ctx* ctx, const void* in, void* out, double sigma);
+/* cudaConvertFO(data, out), include/cudaConfig.hpp:185; out may be NULL (in place) */
+int cdx_convert_fo(cdx_ctx* ctx, void* data, void* out, int is_complex);
+/* CDI::createSupport rectangle, src/post_gpu/cdi.cc:240-251 */
+int cdx_create_support(cdx_ctx* ctx, void* spt, double oversampling_spt);
+/* createBeamStop(row, column, beamStopSize), src/post_gpu/misc.cc:191-195 */
+int cdx_create_beamstop(cdx_ctx* ctx, void* bs, double radius);
+/* centred disk support of the given radius (createMask<C_circle>, isFreq=0) */
+int cdx_create_circle(cdx_ctx* ctx, void* mask, double radius);
+/* findMax / findSum, include/cub_wrap.hpp:3,6; blocking, result on the host */
+int cdx_find_max(cdx_ctx* ctx, const void* x, int64_t n, double* out);
+int cdx_find_sum(cdx_ctx* ctx, const void* x, int64_t n, double* out);
+/* createWaveFront(I) + applyRandomPhase, src/post_gpu/cdi.cc:222-223:
+ * mode 0 = random sign (the reference's start), 1 = uniform random phase.
+ * Pixels with bs > threshold start at 0. */
+int cdx_random_start(cdx_ctx* ctx, const void* I, const void* bs, uint64_t seed, int mode, void* F);
+/* residual of phaseRetrieve (src/post_gpu/cdi.cc:413-421); blocking */
+int cdx_residual(cdx_ctx* ctx, const void* F, const void* I, const void* bs, double* out);
+/* R = sum | |F| - sqrt(I) | / sum sqrt(I) over pixels outside the beamstop; blocking */
+int cdx_rfactor(cdx_ctx* ctx, const void* F, const void* I, const void* bs, double* out);
+
+/* ---- the loop: CDI::phaseRetrieve, src/post_gpu/cdi.cc:297-421 ---- */
+typedef struct cdx_params {
+  int simCCDbit;        /* 1: applyMod (noise-tolerant, beamstop-aware); 0: applyModAbs */
+  int useBS;            /* beamstop in use */
+  int freeBeamstop;     /* applyModAbs path leaves beamstop pixels unconstrained (new; default 0) */
+  int noiseLevel;
+  int rcolor;           /* 65535 in the reference (format.hpp:127) */
+  double exposure;      /* vars.scale = 1/exposure */
+  double beta;          /* vars.beta_HIO; reference 1 */
+  double relax;         /* ER/FHIO/POSHIO relaxation; reference 1.9 */
+  double shrinkThreshold;
+  int shrinkMode;       /* 0: frequency-domain (cdi.cc:326-341); 1: real-space (holo.cc:138-146) */
+  double sigma0;        /* 3 */
+  double sigmaMin;      /* 2 (freq) / 1.5 (real) */
+  double sigmaDecay;    /* 0.99 */
+  int recordR;          /* record the R-factor before every projection */
+  int useGraph;         /* capture the schedule in a hipGraph and launch it once */
+} cdx_params;
+void cdx_params_default(cdx_params* p);
+
+/* F: patternWave (in: start wave; out: FFT of the final object).
+ * g: objectWave (in: as prepareIter leaves it, IFFT(F)/sqrt(N); out: final
+ *    object times sqrt(N), as cdi.cc:313,406 leave it).
+ * I: patternData (FFT order); bs: beamstop or NULL; spt: support (in/out,
+ *    rewritten by shrink-wrap).  The context threshold is used and updated.
+ * rtrace (host, optional): R-factor before each projection, up to rtrace_cap.
+ * n_iter (optional): value of the reference's iteration counter at exit. */
+int cdx_phase_retrieve(cdx_ctx* ctx, const char* algorithm, const cdx_params* p, void* F, void* g,
+                       const void* I, const void* bs, void* spt, double* rtrace, int rtrace_cap, int* n_iter);
+
+/* ---- the schedule grammar: AlgoParser, src/independant/readConfig.cc:82-163 ---- */
+/* expands the schedule into tokens; returns the token count or -CDX_ERR_SCHEDULE */
+int cdx_expand_schedule(const char* algorithm, int* out, int cap);
+
+/* ---- configuration: readConfig::readConfig + print, include/readConfig.hpp:36-49 ----
+ * Parses a libconfig file and writes every readConfig field as one JSON
+ * object into buf (host only, no GPU needed). */
+int cdx_read_config_json(const char* path, char* buf, int cap);
+
+/* ---- measurement helpers (timing with HIP events on the context stream) ---- */
+/* Runs n_iter iterations of `algo` from the current state (F pending on the
+ * first call) without shrink-wrap.  ms_total: wall time of the n_iter
+ * iterations between two events.  If col_ms/row_ms are given, a second run
+ * brackets every launch with events and returns the mean duration of the
+ * column pass and of the row pass. */
+int cdx_time_iterations(cdx_ctx* ctx, const cdx_params* p, int algo, void* F, void* g, const void* I,
+                        const void* bs, const void* spt, int n_iter, int use_graph, double* ms_total,
+                        double* col_ms, double* row_ms);
+
+/* ---- synthetic forward model (Philox4x64-10 streams) ---- */
+/* n x n object: smoothed-noise amplitude in [0.2,1] and phase in [-pi,pi]
+ * inside a centred disk of the given radius.  Context must be n x n. */
+int cdx_sim_object(cdx_ctx* ctx, double radius, uint64_t seed, double smooth_sigma, void* obj);
+/* I = |FFT(obj)|^2 scaled to `photons` in total, Poisson sampled if `poisson` */
+int cdx_sim_intensity(cdx_ctx* ctx, const void* obj, double photons, uint64_t seed, int poisson, void* I);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

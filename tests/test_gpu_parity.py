@@ -1,0 +1,265 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Tolerances: FP32 results are compared with the FP64 oracle by relative L2;
+the bar for one iteration from an identical injected state is 1e-5 (FP32)
+and 1e-12 (FP64).  Integer/mask outputs are bit-exact.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+rng = np.random.default_rng(99)
+
+
+def crand(*shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+@pytest.fixture(scope="module")
+def cdx():
+    import cdixx_amd
+    return cdixx_amd
+
+
+TOL = {0: 2e-6, 1: 1e-13}  # FFT rel L2 by precision
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("shape", [(16, 16), (32, 64), (64, 64), (128, 256), (256, 256), (384, 384), (512, 512),
+                                   (768, 768), (1024, 1024), (2048, 2048), (256, 384), (1024, 512), (384, 16)])
+def test_fft_matches_numpy(cdx, shape, prec):
+    print(f"fft {shape} prec={prec}", flush=True)
+    ctx = cdx.Context(*shape, precision=prec)
+    x = crand(*shape)
+    d = ctx.cplx(x)
+    out = ctx.cplx()
+    ctx.fft(d, out, cdx.FORWARD)
+    assert rel_l2(ctx.get_cplx(out), np.fft.fft2(x)) < TOL[prec]
+    ctx.fft(out, out, cdx.INVERSE)  # in place
+    assert rel_l2(ctx.get_cplx(out), x * x.size) < TOL[prec]
+    ctx.close()
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_projections_and_updates(cdx, o64, prec):
+    n = 256
+    ctx = cdx.Context(n, precision=prec)
+    tol = 1e-6 if prec == 0 else 1e-14
+    F = crand(n, n) * 3
+    F[5, 5] = 0
+    I = np.abs(F) ** 2 * rng.uniform(0.3, 1.7, (n, n))
+    I[0, :20] = -1.0
+    dF, dI = ctx.cplx(F), ctx.real(I)
+    ctx.apply_mod_abs(dF, dI)
+    assert rel_l2(ctx.get_cplx(dF), o64.apply_mod_abs(F, I)) < tol
+    # applyMod with beamstop, noise and over-exposure
+    scale = 1 / 0.1
+    I2 = I.copy()
+    I2[100:110, :] = scale * 1.5
+    bs = o64.beamstop(n, n, 9)
+    ctx.init_vars(65535, scale)
+    dF.upload(F.astype(ctx.cdtype))
+    dI.upload(I2.astype(ctx.rdtype))
+    dbs = ctx.real(bs)
+    ctx.apply_mod(dF, dI, dbs, 24)
+    assert rel_l2(ctx.get_cplx(dF), o64.apply_mod(F, I2, bs, 24, scale, 65535)) < tol
+    # support updates, every token
+    g, gp = crand(n, n), crand(n, n)
+    spt = rng.uniform(0, 1, (n, n))
+    for algo in range(10):
+        for beta in (1.0, 0.9):
+            ctx.set_beta(beta)
+            ctx.set_threshold(0.5)
+            dg, dgp, ds = ctx.cplx(g), ctx.cplx(gp), ctx.real(spt)
+            ctx.apply_support(dg, dgp, algo, ds)
+            assert rel_l2(ctx.get_cplx(dg), o64.apply_support(g, gp, algo, spt, 0.5, beta)) < tol, (algo, beta)
+    ctx.close()
+
+
+def test_masks_and_small_ops(cdx, o64):
+    n = 256
+    ctx = cdx.Context(n)
+    s = ctx.real()
+    ctx.create_support(s, 2.0)
+    assert np.array_equal(ctx.get_real(s), o64.support_rect(n, n, 2.0))
+    ctx.create_beamstop(s, 8)
+    assert np.array_equal(ctx.get_real(s), o64.beamstop(n, n, 8))
+    ctx.create_circle(s, 40)
+    assert np.array_equal(ctx.get_real(s), o64.circle(n, n, 40))
+    x = crand(n, n)
+    d = ctx.cplx(x)
+    ctx.convert_fo(d)
+    assert np.array_equal(ctx.get_cplx(d), np.fft.fftshift(x.astype(np.complex64)))
+    r = rng.uniform(0, 5, (n, n)).astype(np.float32)
+    dr = ctx.real(r)
+    assert ctx.find_max(dr) == r.max()
+    assert abs(ctx.find_sum(dr) - r.astype(np.float64).sum()) < 1e-9 * r.sum()
+    out = ctx.real()
+    ctx.gauss_conv(dr, out, 2.2)
+    assert rel_l2(ctx.get_real(out), o64.gauss_conv_real(r, 2.2)) < 1e-6
+    ctx.close()
+
+
+def _injected(o64, n, rs_seed=0):
+    obj = o64.sim_object(n, n * 0.16, seed=1)
+    I = o64.sim_intensity(obj, 1e8, seed=1, poisson=False)
+    rs = np.random.default_rng(rs_seed)
+    F = np.sqrt(I) * np.exp(2j * np.pi * rs.uniform(size=I.shape))
+    g = np.fft.ifft2(F) * np.sqrt(n * n)        # as prepareIter leaves objectWave
+    g = g + 0.05 * np.abs(g).max() * (rs.standard_normal(g.shape) + 1j * rs.standard_normal(g.shape))
+    spt = rs.uniform(0, 1, I.shape)
+    return F, g, I, spt
+
+
+ALGOS = {"RAAR": 0, "ER": 1, "POSER": 2, "HIO": 3, "POSHIO": 4, "FHIO": 5}
+
+
+@pytest.mark.parametrize("n", [256, 384, 1024])
+@pytest.mark.parametrize("algo", list(ALGOS))
+@pytest.mark.parametrize("prec", [0, 1])
+def test_one_iteration_from_injected_state(cdx, o64, n, algo, prec):
+    """One step of CDI::phaseRetrieve from an identical state: FP32 within
+    1e-5 relative L2 of the FP64 oracle (FP64 within 1e-12)."""
+    print(f"one-iteration n={n} {algo} prec={prec}", flush=True)
+    F, g, I, spt = _injected(o64, n)
+    bs = o64.beamstop(n, n, 8)
+    for proj in ("abs", "noise", "absbs"):
+        kw = dict(beta=0.85, relax=1.9)
+        p = cdx.Params.default(beta=0.85)
+        if proj == "noise":
+            p.simCCDbit, p.useBS, p.noiseLevel, p.exposure = 1, 1, 24, 1e-6
+            kw.update(simCCDbit=1, useBS=1, noiseLevel=24, scale=1e6)
+        elif proj == "absbs":
+            p.useBS, p.freeBeamstop = 1, 1
+        ctx = cdx.Context(n, precision=prec)
+        ctx.set_threshold(0.5)
+        dF, dg, dI, ds, dbs = ctx.cplx(F), ctx.cplx(g), ctx.real(I), ctx.real(spt), ctx.real(bs)
+        ctx.phase_retrieve(f"1*{algo}", p, dF, dg, dI, ds, bs=dbs)
+        if proj == "absbs":
+            # oracle: the modulus projection leaves beamstop pixels unconstrained
+            Fo = o64.apply_mod_abs(F, I)
+            Fo[bs > 0.5] = F[bs > 0.5]
+            gp = np.fft.ifft2(Fo)
+            go = o64.apply_support(g / np.sqrt(n * n), gp, ALGOS[algo], spt, 0.5, 0.85)
+            ref = dict(F=np.fft.fft2(go), g=go * np.sqrt(n * n))
+        else:
+            ref = o64.phase_retrieve(f"1*{algo}", F, g, I, spt, 0.5, bs=bs, **kw)
+        tol = 1e-5 if prec == 0 else 1e-12
+        eg = rel_l2(ctx.get_cplx(dg), ref["g"])
+        eF = rel_l2(ctx.get_cplx(dF), ref["F"])
+        assert eg < tol and eF < tol, (proj, eg, eF)
+        ctx.close()
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_loop_with_shrinkwrap_matches_oracle(cdx, o64, mode):
+    """A schedule with shrink-wrap steps (frequency form, reference; real-space
+    form) in FP64 tracks the FP64 oracle; the support threshold matches."""
+    n = 128
+    F, g, I, spt0 = _injected(o64, n)
+    spt = o64.support_rect(n, n, 2.0)
+    alg = "3*(10*HIO+1*shrinkWrap)+5*ER"
+    kw = dict(beta=0.9, shrinkThreshold=0.1, shrinkMode=mode, sigmaMin=2.0 if mode == 0 else 1.5)
+    ref = o64.phase_retrieve(alg, F, g, I, spt, 0.5, rtrace_cap=35, **kw)
+    ctx = cdx.Context(n, precision=1)
+    ctx.set_threshold(0.5)
+    p = cdx.Params.default(beta=0.9, shrinkThreshold=0.1, shrinkMode=mode, sigmaMin=kw["sigmaMin"], recordR=1)
+    dF, dg, dI, ds = ctx.cplx(F), ctx.cplx(g), ctx.real(I), ctx.real(spt)
+    nit, rt = ctx.phase_retrieve(alg, p, dF, dg, dI, ds, rtrace_cap=35)
+    assert nit == ref["n_iter"] == 38
+    assert abs(ctx.threshold() - ref["threshold"]) <= 1e-9 * abs(ref["threshold"])
+    assert rel_l2(ctx.get_real(ds), ref["spt"]) < 1e-9
+    assert rel_l2(ctx.get_cplx(dg), ref["g"]) < 1e-9
+    assert rel_l2(ctx.get_cplx(dF), ref["F"]) < 1e-9
+    assert np.allclose(rt, ref["rtrace"], rtol=1e-9, atol=1e-12)
+    ctx.close()
+
+
+def test_graph_and_stream_launches_agree(cdx, o64):
+    n = 256
+    F, g, I, spt = _injected(o64, n)
+    outs = []
+    for graph in (0, 1):
+        ctx = cdx.Context(n)
+        ctx.set_threshold(0.5)
+        p = cdx.Params.default(beta=0.9, shrinkThreshold=0.05, useGraph=graph)
+        dF, dg, dI, ds = ctx.cplx(F), ctx.cplx(g), ctx.real(I), ctx.real(spt)
+        ctx.phase_retrieve("2*(20*RAAR+1*shrinkWrap)+5*ER", p, dF, dg, dI, ds)
+        outs.append((ctx.get_cplx(dF), ctx.get_cplx(dg), ctx.get_real(ds)))
+        ctx.close()
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
+
+
+def test_fp32_loop_tracks_fp64(cdx, o64):
+    """Over a short run the FP32 pipeline stays close to the FP64 oracle."""
+    n = 256
+    F, g, I, spt = _injected(o64, n)
+    spt = o64.circle(n, n, 0.2 * n)
+    ref = o64.phase_retrieve("50*RAAR", F, g, I, spt, 0.5, beta=0.85)
+    ctx = cdx.Context(n)
+    ctx.set_threshold(0.5)
+    dF, dg, dI, ds = ctx.cplx(F), ctx.cplx(g), ctx.real(I), ctx.real(spt)
+    ctx.phase_retrieve("50*RAAR", cdx.Params.default(beta=0.85), dF, dg, dI, ds)
+    assert rel_l2(ctx.get_cplx(dg), ref["g"]) < 1e-4
+    ctx.close()
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_forward_model_matches_oracle(cdx, o64, prec):
+    n = 256
+    ctx = cdx.Context(n, precision=prec)
+    obj = ctx.cplx()
+    ctx.sim_object(obj, 40, seed=7)
+    ref = o64.sim_object(n, 40, seed=7)
+    assert rel_l2(ctx.get_cplx(obj), ref) < (1e-6 if prec == 0 else 1e-12)
+    I = ctx.real()
+    ctx.sim_intensity(obj, I, 1e6, seed=7, poisson=False)
+    Iref = o64.sim_intensity(ref, 1e6, seed=7, poisson=False)
+    assert rel_l2(ctx.get_real(I), Iref) < (1e-5 if prec == 0 else 1e-11)
+    if prec == 1:
+        ctx.sim_intensity(obj, I, 1e6, seed=7, poisson=True)
+        Ip = ctx.get_real(I)
+        Ipr = o64.sim_intensity(ref, 1e6, seed=7, poisson=True)
+        assert (Ip == np.round(Ip)).all()
+        assert np.mean(Ip == Ipr) > 0.999
+    ctx.close()
+
+
+def test_random_start_and_diagnostics(cdx, o64):
+    n = 256
+    ctx = cdx.Context(n)
+    I = rng.uniform(0, 4, (n, n))
+    bs = o64.beamstop(n, n, 6)
+    dI, dbs, dF = ctx.real(I), ctx.real(bs), ctx.cplx()
+    ctx.random_start(dI, dbs, dF, seed=3, mode=0)
+    F = ctx.get_cplx(dF)
+    m = bs > 0.5
+    assert (F[m] == 0).all()
+    assert np.allclose(np.abs(F[~m]), np.sqrt(I[~m]), rtol=1e-6)
+    flips = np.array([o64.uniform(3, 3, i) > 0.5 for i in range(n * n)]).reshape(n, n)
+    assert np.array_equal((F.real < 0)[~m], flips[~m])
+    ctx.random_start(dI, None, dF, seed=3, mode=1)
+    F = ctx.get_cplx(dF)
+    assert np.allclose(np.abs(F), np.sqrt(I), rtol=1e-6)
+    Fx = crand(n, n)
+    dF.upload(Fx.astype(np.complex64))
+    assert abs(ctx.rfactor(dF, dI, dbs) - o64.rfactor(Fx, I, bs)) < 1e-6
+    assert abs(ctx.residual(dF, dI, dbs) / o64.residual(Fx, I, bs) - 1) < 1e-5
+    ctx.close()
+
+
+def test_errors_from_the_device_path(cdx):
+    ctx = cdx.Context(64)
+    with pytest.raises(cdx.CdxError) as e:
+        d = ctx.cplx()
+        ctx.phase_retrieve("5*TV", cdx.Params.default(), d, d, ctx.real(), ctx.real())
+    assert e.value.code == 4
+    with pytest.raises(cdx.CdxError) as e:
+        d = ctx.cplx()
+        ctx.phase_retrieve("5*NOPE", cdx.Params.default(), d, d, ctx.real(), ctx.real())
+    assert e.value.code == 3
+    ctx.close()

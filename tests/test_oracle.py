@@ -1,0 +1,316 @@
+"""Pins for the CPU oracle (oracle/oracle.c) -- runs without a GPU.
+
+The reference ships no golden vectors for this path (no tests at all), so
+each oracle function is pinned by what the mathematics fixes or by an
+independent library routine, chosen so that a plausible mistake (dropped
+term, wrong sign or index, transposed operand) fails:
+
+  FFT (a5)              brute-force DFT from the definition (square, rectangular,
+                        radix-3 sizes); numpy.fft (pocketfft) at full sizes;
+                        delta/plane-wave closed forms; Parseval; inverse round trip.
+  applyModAbs (a3)      |F'| = sqrt(max(I,0)), phase kept, F=0 -> (sqrt I, 0),
+                        idempotent.
+  applyMod (a4)         beamstop untouched; inside tolerance |F'|^2 = I; outside
+                        |F'|^2 = I -+ tol (tol from the noise model); phase kept;
+                        over-exposed pixels keep their modulus.
+  support updates (a7)  RAAR(beta=1) == HIO; ER with relax=1 is the textbook
+                        projection; POSER output Re >= 0 and zero outside; HIO
+                        fixed point; each branch against its defining identity.
+  shrink-wrap (a8)      frequency Gaussian == periodic spatial convolution with
+                        sigma_x = rows/(2 pi sigma_f) (brute force at 64^2).
+  real blur (a8')       scipy.ndimage.convolve with the (2r+1)^2 kernel,
+                        zero padding.
+  masks (a10)           disk/rectangle extents and counts.
+  AlgoParser (a2)       token counts of the reference's own .cfg schedules.
+  Philox4x64-10         Random123 known-answer vector and numpy.random.Philox.
+  Poisson               moments and chi-square against scipy.stats.poisson.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import REFERENCE, rel_l2
+
+rng = np.random.default_rng(1234)
+
+
+def crand(*shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+# ---------------------------------------------------------------- FFT
+@pytest.mark.parametrize("shape", [(8, 8), (12, 20), (16, 24), (24, 36), (6, 10), (32, 16)])
+@pytest.mark.parametrize("sign", [-1, 1])
+def test_fft_matches_bruteforce_dft(o64, shape, sign):
+    x = crand(*shape)
+    assert rel_l2(o64.fft2(x, sign), o64.dft2_brute(x, sign)) < 1e-13
+
+
+@pytest.mark.parametrize("n", [256, 384, 512, 1024])
+def test_fft_matches_numpy(o64, n):
+    x = crand(n, n)
+    assert rel_l2(o64.fft2(x, -1), np.fft.fft2(x)) < 1e-13
+    assert rel_l2(o64.fft2(x, +1), np.fft.ifft2(x) * n * n) < 1e-13
+
+
+def test_fft_closed_forms(o64):
+    n = 64
+    d = np.zeros((n, n), complex)
+    d[0, 0] = 1
+    assert np.allclose(o64.fft2(d), np.ones((n, n)))
+    kx, ky = 5, 11
+    x, y = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    pw = np.exp(2j * np.pi * (kx * x + ky * y) / n)
+    F = o64.fft2(pw, -1)
+    expect = np.zeros((n, n), complex)
+    expect[kx, ky] = n * n
+    assert np.abs(F - expect).max() < 1e-9
+    z = crand(n, n)
+    assert abs(np.sum(np.abs(o64.fft2(z)) ** 2) / (n * n) - np.sum(np.abs(z) ** 2)) < 1e-8 * np.sum(np.abs(z) ** 2)
+    assert rel_l2(o64.fft2(o64.fft2(z, -1), +1) / (n * n), z) < 1e-14
+
+
+def test_fft_f32_accuracy(o32, o64):
+    x = crand(256, 256)
+    assert rel_l2(o32.fft2(x), o64.fft2(x)) < 2e-6
+
+
+# ---------------------------------------------------------------- projections
+def test_mod_abs_properties(o64):
+    F = crand(32, 32)
+    F[3, 4] = 0
+    I = rng.uniform(-1, 4, (32, 32))
+    out = o64.apply_mod_abs(F, I)
+    target = np.sqrt(np.maximum(I, 0))
+    assert np.allclose(np.abs(out), target, rtol=1e-14, atol=1e-14)
+    keep = (F != 0) & (I > 0)
+    assert np.allclose(np.angle(out[keep]), np.angle(F[keep]), atol=1e-12)
+    assert out[3, 4] == (np.sqrt(max(I[3, 4], 0)) + 0j)
+    assert np.allclose(o64.apply_mod_abs(out, I), out, rtol=1e-14, atol=1e-15)
+
+
+def test_mod_noise_branches(o64):
+    n = 4096
+    scale, rcolor, noise = 1.0 / 0.1, 65535, 24
+    F = crand(n) * 3
+    I = np.abs(F) ** 2 * rng.uniform(0.5, 1.5, n)
+    I[:50] = scale * rng.uniform(0.96, 2.0, 50)       # over-exposed
+    bs = np.zeros(n)
+    bs[50:80] = 1
+    out = o64.apply_mod(F, I, bs, noise, scale, rcolor)
+    assert np.array_equal(out[50:80], F[50:80])       # beamstop untouched
+    src2 = np.abs(F) ** 2
+    m2 = np.where(I >= 0.95 * scale, np.maximum(I, src2), I)
+    tol = np.sqrt(noise + m2 * rcolor / scale) * scale / rcolor
+    d = m2 - src2
+    got2 = np.abs(out) ** 2
+    sel = np.ones(n, bool)
+    sel[50:80] = False
+    inside = sel & (np.abs(d) <= tol)
+    above = sel & (d > tol)
+    below = sel & (d < -tol)
+    assert inside.sum() > 10 and above.sum() > 10 and below.sum() > 10
+    assert np.allclose(got2[inside], m2[inside], rtol=1e-12)
+    assert np.allclose(got2[above], (m2 - tol)[above], rtol=1e-12)
+    assert np.allclose(got2[below], (m2 + tol)[below], rtol=1e-12)
+    assert np.allclose(np.angle(out[sel]), np.angle(F[sel]), atol=1e-12)
+    ox = np.arange(50)[I[:50] <= src2[:50]]
+    assert np.allclose(np.abs(out[ox]), np.abs(F[ox]), rtol=1e-12)  # over-exposed: modulus kept
+
+
+# ---------------------------------------------------------------- support updates
+@pytest.fixture
+def upd_data():
+    g, gp = crand(64, 64), crand(64, 64)
+    spt = rng.uniform(0, 1, (64, 64))
+    return g, gp, spt, 0.5
+
+
+def test_support_hio_raar(o64, upd_data):
+    g, gp, spt, thr = upd_data
+    inside = spt > thr
+    hio = o64.apply_support(g, gp, 3, spt, thr, beta=0.9)
+    assert np.allclose(hio[inside], gp[inside])
+    assert np.allclose(hio[~inside], g[~inside] - 0.9 * gp[~inside])
+    raar = o64.apply_support(g, gp, 0, spt, thr, beta=0.85)
+    assert np.allclose(raar[~inside], 0.85 * g[~inside] + (1 - 1.7) * gp[~inside])
+    # at beta = 1 (the reference's fixed value) RAAR and HIO are the same map
+    assert np.allclose(o64.apply_support(g, gp, 0, spt, thr, 1.0), o64.apply_support(g, gp, 3, spt, thr, 1.0))
+
+
+def test_support_er_poser_poshio_fhio(o64, upd_data):
+    g, gp, spt, thr = upd_data
+    inside = spt > thr
+    er1 = o64.apply_support(g, gp, 1, spt, thr, relax=1.0)      # textbook ER
+    assert np.allclose(er1, np.where(inside, gp, 0))
+    er = o64.apply_support(g, gp, 1, spt, thr)                  # over-relaxed, 1.9
+    assert np.allclose(er[inside], g[inside] + 1.9 * (gp[inside] - g[inside]))
+    pos = o64.apply_support(g, gp, 2, spt, thr)
+    assert (pos.real >= 0).all() and (pos[~inside] == 0).all()
+    keep = inside & (gp.real > 0)
+    assert np.allclose(pos[keep], gp[keep])
+    ph = o64.apply_support(g, gp, 4, spt, thr, beta=1.0)
+    assert np.allclose(ph.imag, g.imag - gp.imag)
+    assert np.allclose(ph.real[keep], g.real[keep] + 1.9 * (gp.real[keep] - g.real[keep]))
+    assert np.allclose(ph.real[~keep], g.real[~keep] - gp.real[~keep])
+    fh = o64.apply_support(g, gp, 5, spt, thr)
+    assert np.allclose(fh[~inside], g[~inside] - 1.2 * gp[~inside])
+    # unknown tokens (KKT=8) take the FHIO branch, as the dispatcher's default
+    assert np.allclose(o64.apply_support(g, gp, 8, spt, thr), fh)
+
+
+def test_hio_fixed_point(o64):
+    g = crand(32, 32)
+    spt = np.ones((32, 32))
+    assert np.allclose(o64.apply_support(g, g, 3, spt, 0.5), g)
+
+
+# ---------------------------------------------------------------- shrink-wrap
+def test_gauss_mult_is_periodic_convolution(o64):
+    n = 64
+    x = rng.uniform(0, 1, (n, n)) + 0j
+    sf = 8.0
+    out = o64.fft2(o64.gauss_mult_freq(o64.fft2(x, -1), sf), +1) / (n * n)
+    sx = n / (2 * np.pi * sf)
+    # brute-force periodic convolution with the sampled, periodised Gaussian
+    d = np.arange(n)
+    d = np.minimum(d, n - d)
+    # the frequency window is the wrapped Gaussian exp(-k^2/2sf^2); its exact
+    # inverse DFT is the kernel; compare against direct evaluation via sums
+    k = np.where(np.arange(n) >= n // 2, np.arange(n) - n, np.arange(n))
+    G1 = np.exp(-k ** 2 / (2 * sf ** 2))
+    h1 = np.real(np.fft.ifft(G1))
+    h = np.outer(h1, h1)
+    ref = np.zeros((n, n))
+    xr = x.real
+    for a in range(n):
+        for b in range(n):
+            ref += h[a, b] * np.roll(np.roll(xr, a, 0), b, 1)
+    assert np.abs(out.real - ref).max() < 1e-12
+    # and the kernel is close to the continuous Gaussian of width rows/(2 pi sf)
+    # (up to the truncation of the window at the Nyquist frequency)
+    cont = np.exp(-d ** 2 / (2 * sx ** 2))
+    cont /= cont.sum()
+    assert np.abs(h1 - cont).max() < 1e-4
+
+
+def test_gauss_conv_real_matches_scipy(o64):
+    from scipy.ndimage import convolve
+    x = rng.uniform(0, 1, (40, 56))
+    for sigma in (3.0, 1.5, 2.2):
+        r = int(np.floor(sigma * 6)) >> 1
+        dx = np.arange(-r, r + 1)
+        k = np.exp(-(dx[:, None] ** 2 + dx[None, :] ** 2) / (2 * sigma ** 2)) / (2 * np.pi * sigma ** 2)
+        ref = convolve(x, k, mode="constant", cval=0.0)
+        assert np.abs(o64.gauss_conv_real(x, sigma) - ref).max() < 1e-13
+
+
+# ---------------------------------------------------------------- masks
+def test_masks(o64):
+    n = 256
+    s = o64.support_rect(n, n, 2.0)
+    # rect [n/4-1, 3n/4+1) in both axes (cdi.cc:242-245, rect::isInside)
+    assert s.sum() == (n // 2 + 2) ** 2 and s[n // 4 - 1, n // 4 - 1] == 1 and s[n // 4 - 2, n // 4] == 0
+    bs = o64.beamstop(n, n, 8)
+    assert bs[0, 0] == 1 and bs[0, 7] == 1 and bs[0, 8] == 0 and bs[n - 7, 0] == 1
+    kx = np.minimum(np.arange(n), n - np.arange(n))
+    assert bs.sum() == np.sum((kx[:, None] ** 2 + kx[None, :] ** 2) < 64)
+    c = o64.circle(n, n, 40)
+    yy = np.arange(n) - n // 2
+    assert c.sum() == np.sum((yy[:, None] ** 2 + yy[None, :] ** 2) < 1600)
+
+
+def test_convert_fo_is_fftshift(o64):
+    x = crand(16, 24)
+    assert np.array_equal(o64.convert_fo(x), np.fft.fftshift(x))
+
+
+# ---------------------------------------------------------------- AlgoParser
+def test_parser_reference_configs(o64):
+    # token counts of the schedules in the reference's own config files
+    assert len(o64.expand("100*(3*POSHIO+20*RAAR+1*POSHIO+1*shrinkWrap)+20*RAAR+1*ER")) == 2521
+    assert len(o64.expand("200*(20*POSHIO+1*POSHIO+1*shrinkWrap)+100*POSHIO")) == 4500
+    assert len(o64.expand("3000*RAAR+20*ER")) == 3020
+    assert o64.expand("2*(1*HIO+1*shrinkWrap)+1*ER") == [3, 6, 3, 6, 1]
+    assert o64.expand("1000*HIO") == [3] * 1000
+    assert o64.expand("3*BOGUS") is None
+
+
+def test_parser_reads_reference_cfg_files(o64):
+    import re
+    for f, n in (("html/cdi.cfg", 2521), ("html/default/cdi.cfg", 4500), ("html/default/pulse.cfg", 3020)):
+        path = os.path.join(REFERENCE, f)
+        if not os.path.exists(path):
+            pytest.skip("reference tree not mounted")
+        m = re.search(r'^\s*algorithm\s*=\s*"([^"]*)"', open(path).read(), re.M)
+        assert len(o64.expand(m.group(1))) == n
+
+
+# ---------------------------------------------------------------- RNG
+def test_philox_known_answers(o64):
+    # Random123 philox4x64-10 known-answer vector (counter 0, key 0)
+    assert [int(v) for v in o64.philox([0, 0, 0, 0], [0, 0])] == [
+        0x16554D9ECA36314C, 0xDB20FE9D672D0FDC, 0xD7E772CEE186176B, 0x7E68B68AEC7BA23B]
+    # numpy's Philox is the same generator (it increments the counter before use)
+    for key in ([0, 0], [12345, 678], [0xDEADBEEF12345, 99]):
+        bg = np.random.Philox(counter=[0, 0, 0, 0], key=key)
+        assert [int(v) for v in bg.random_raw(4)] == [int(v) for v in o64.philox([1, 0, 0, 0], key)]
+
+
+def test_poisson_statistics(o64):
+    from scipy import stats
+    for lam in (0.3, 4.0, 9.9, 10.0, 37.5, 1e4):
+        s = np.array([o64.poisson(lam, 7, 2, i) for i in range(20000)])
+        assert abs(s.mean() - lam) < 5 * np.sqrt(lam / len(s))
+        assert abs(s.var() / lam - 1) < 0.06
+        if lam < 100:
+            ks = np.arange(int(lam + 6 * np.sqrt(lam)) + 3)
+            obs = np.array([(s == k).sum() for k in ks])
+            exp = stats.poisson.pmf(ks, lam) * len(s)
+            m = exp > 5
+            chi = ((obs[m] - exp[m]) ** 2 / exp[m]).sum()
+            assert chi < stats.chi2.ppf(0.999, m.sum() - 1)
+
+
+def test_sim_object_ranges(o64):
+    obj = o64.sim_object(128, 30, seed=3)
+    yy = np.arange(128) - 64
+    disk = (yy[:, None] ** 2 + yy[None, :] ** 2) < 900
+    a = np.abs(obj)
+    assert (a[~disk] == 0).all()
+    assert a[disk].min() >= 0.2 - 1e-12 and a[disk].max() <= 1 + 1e-12
+    I = o64.sim_intensity(obj, 1e6, seed=3, poisson=False)
+    assert abs(I.sum() - 1e6) < 1e-6 * 1e6
+    assert rel_l2(I, np.abs(np.fft.fft2(obj)) ** 2 * 1e6 / np.sum(np.abs(np.fft.fft2(obj)) ** 2)) < 1e-12
+
+
+# ---------------------------------------------------------------- loop
+def test_iteration_composition(o64):
+    """One iteration equals projection -> IFFT/N -> update -> FFT built from
+    numpy's FFT and the separately pinned pieces."""
+    n = 64
+    F, g = crand(n, n), crand(n, n)
+    I = rng.uniform(0, 3, (n, n))
+    spt = o64.support_rect(n, n)
+    F1, g1 = o64.iteration(F, g, I, spt, 3, 0.5, beta=0.9)
+    Fp = o64.apply_mod_abs(F, I)
+    gp = np.fft.ifft2(Fp)
+    g_ref = o64.apply_support(g, gp, 3, spt, 0.5, beta=0.9)
+    assert rel_l2(g1, g_ref) < 1e-13
+    assert rel_l2(F1, np.fft.fft2(g_ref)) < 1e-13
+
+
+def test_phase_retrieve_converges(o64):
+    n = 64
+    obj = o64.sim_object(n, 14, seed=5)
+    I = o64.sim_intensity(obj, 1e9, seed=5, poisson=False)
+    spt = o64.circle(n, n, 16)
+    rs = np.random.default_rng(0)
+    F0 = np.sqrt(I) * np.exp(2j * np.pi * rs.uniform(size=I.shape))
+    g0 = np.fft.ifft2(F0) * np.sqrt(n * n)
+    out = o64.phase_retrieve("300*HIO+50*ER", F0, g0, I, spt, threshold=0.5, rtrace_cap=350, beta=0.9, relax=1.0)
+    assert out["n_iter"] == 350
+    r = out["rtrace"]
+    assert r[0] < 1e-12            # the start has the measured modulus
+    assert r[-1] < 0.05 and r[-1] < 0.1 * r[1]

@@ -672,6 +672,8 @@ static int time_t_(cdx_ctx* c, const cdx_params* p, int algo, void* F, void* g, 
   if (exec) hipGraphExecDestroy(exec);
   if (graph) hipGraphDestroy(graph);
   if (col_ms || row_ms) {
+    // per-launch brackets on a bounded sample of the same iterations
+    n = n < 200 ? n : 200;
     std::vector<hipEvent_t> ev(4 * (size_t)n);
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
     UpdArgs<T> ua = upd_args<T>(c, algo, p->beta, p->relax);

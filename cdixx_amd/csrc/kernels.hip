@@ -761,12 +761,13 @@ __global__ void k_blur_rows(int rows, int cols, const void* __restrict__ src, T*
     tmp[(size_t)x * cols + y] = s;
   }
 }
-// column pass: a workgroup owns 16 adjacent columns, staged through LDS
+// column pass: a workgroup owns CW adjacent columns (64 B per row), staged
+// through LDS: (rows + 2r) * 64 B <= 133 KB for 2048 rows
 template <class T>
 __global__ void k_blur_cols(int rows, int cols, const T* __restrict__ tmp, T* __restrict__ spt, T sigma, int r,
                             ReduceSlots rs) {
   extern __shared__ __align__(16) unsigned char smem[];
-  constexpr int CW = 16;
+  constexpr int CW = 64 / sizeof(T);
   T* tile = reinterpret_cast<T*>(smem);  // (rows + 2r) x CW
   T* w = tile + (rows + 2 * r) * CW;
   const int c0 = blockIdx.x * CW;
@@ -810,10 +811,13 @@ hipError_t launch_blur_real(int rows, int cols, const void* g, int cpx_in, T* tm
     hipLaunchKernelGGL((k_blur_rows<T, false>), dim3(rows), dim3(256), l1, st, rows, cols, g, tmp, sigma, r);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  size_t l2 = (size_t)((rows + 2 * r) * 16 + 2 * r + 1) * sizeof(T);
+  constexpr int CW = 64 / sizeof(T);
+  size_t l2 = (size_t)((rows + 2 * r) * CW + 2 * r + 1) * sizeof(T);
+  if (l2 > 160 * 1024) return hipErrorInvalidValue;
   auto kern = k_blur_cols<T>;
   ensure_lds(kern, l2);
-  hipLaunchKernelGGL(kern, dim3((cols + 15) / 16), dim3(256), l2, st, rows, cols, (const T*)tmp, spt, sigma, r, rs);
+  hipLaunchKernelGGL(kern, dim3((cols + CW - 1) / CW), dim3(256), l2, st, rows, cols, (const T*)tmp, spt, sigma, r,
+                     rs);
   return hipGetLastError();
 }
 
@@ -1057,22 +1061,24 @@ struct UStream {
 };
 __device__ double poisson_draw(double lam, uint64_t seed, uint64_t idx) {
   UStream u{seed, idx, 0, {0, 0, 0, 0}, 4};
-  if (lam <= 0) return 0;
+  if (!(lam > 0) || !(lam < 1e300)) return 0;  // also NaN / inf
   if (lam < 10) {
     const double enlam = exp(-lam);
     double prod = 1.0, X = 0;
-    for (;;) {
+    for (int it = 0; it < 4096; it++) {
       prod *= u.next();
       if (prod > enlam) X += 1;
       else return X;
     }
+    return X;
   }
   const double slam = sqrt(lam), loglam = log(lam);
   const double b = 0.931 + 2.53 * slam;
   const double a = -0.059 + 0.02483 * b;
   const double invalpha = 1.1239 + 1.1328 / (b - 3.4);
   const double vr = 0.9277 - 3.6224 / (b - 2);
-  for (;;) {
+  // PTRS accepts ~90% of proposals; the cap only bounds the loop
+  for (int it = 0; it < 4096; it++) {
     const double U = u.next() - 0.5;
     const double V = u.next();
     const double us = 0.5 - fabs(U);
@@ -1081,6 +1087,7 @@ __device__ double poisson_draw(double lam, uint64_t seed, uint64_t idx) {
     if (k < 0 || (us < 0.013 && V > us)) continue;
     if (log(V) + log(invalpha) - log(a / (us * us) + b) <= -lam + k * loglam - lgamma(k + 1)) return k;
   }
+  return floor(lam + 0.5);
 }
 template <class T>
 __global__ void k_poisson(int64_t n, const cplx<T>* F, const double* tot, double photons, uint64_t seed, int poisson,

@@ -701,7 +701,7 @@ static double us_next(ustream* u) {
  * transformed rejection (Hoermann 1993) above. */
 int64_t orc_poisson(double lam, uint64_t seed, uint64_t stream, uint64_t idx) {
   ustream u = {seed, stream, idx, 0, {0, 0, 0, 0}, 4};
-  if (lam <= 0) return 0;
+  if (!(lam > 0) || !(lam < 1e300)) return 0;  /* also NaN / inf */
   if (lam < 10) {
     double enlam = exp(-lam), prod = 1.0;
     int64_t X = 0;

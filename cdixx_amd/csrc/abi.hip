@@ -772,3 +772,49 @@ extern "C" int cdx_sim_intensity(cdx_ctx* c, const void* obj, double photons, ui
   HIPCHK(hipStreamSynchronize(c->st));
   return CDX_OK;
 }
+
+// ---------------------------------------------------------------------------
+// setup operations of CDI::prepareIter / readObjectWave / checkAutoCorrelation
+// ---------------------------------------------------------------------------
+extern "C" int cdx_convert_fo_phase(cdx_ctx* c, void* d, double norm) {
+  CTX_CHECK(c);
+  if (!d) return fail(CDX_ERR_ARG, "cdx_convert_fo_phase: null buffer");
+  if (c->prec == CDX_F64) HIPCHK(launch_fo_phase<double>(c->rows, c->cols, d, norm, c->st));
+  else HIPCHK(launch_fo_phase<float>(c->rows, c->cols, d, (float)norm, c->st));
+  return CDX_OK;
+}
+extern "C" int cdx_mask_multiply(cdx_ctx* c, void* d, const void* m, int invert) {
+  CTX_CHECK(c);
+  if (!d || !m) return fail(CDX_ERR_ARG, "cdx_mask_multiply: null buffer");
+  if (c->prec == CDX_F64) HIPCHK(launch_mask_mul<double>(c->n, (double*)d, (const double*)m, invert, c->st));
+  else HIPCHK(launch_mask_mul<float>(c->n, (float*)d, (const float*)m, invert, c->st));
+  return CDX_OK;
+}
+extern "C" int cdx_poisson_noise_wo(cdx_ctx* c, void* d, double noiseLevel, uint64_t seed) {
+  CTX_CHECK(c);
+  if (!d) return fail(CDX_ERR_ARG, "cdx_poisson_noise_wo: null buffer");
+  if (c->prec == CDX_F64)
+    HIPCHK(launch_poisson_wo<double>(c->n, (double*)d, noiseLevel, c->scale, c->rcolor, seed, c->st));
+  else
+    HIPCHK(launch_poisson_wo<float>(c->n, (float*)d, noiseLevel, (float)c->scale, (float)c->rcolor, seed, c->st));
+  return CDX_OK;
+}
+extern "C" int cdx_extend_to_complex(cdx_ctx* c, const void* a, void* b) {
+  CTX_CHECK(c);
+  if (!a || !b) return fail(CDX_ERR_ARG, "cdx_extend_to_complex: null buffer");
+  if (c->prec == CDX_F64) HIPCHK(launch_extend<double>(c->n, (const double*)a, b, c->st));
+  else HIPCHK(launch_extend<float>(c->n, (const float*)a, b, c->st));
+  return CDX_OK;
+}
+extern "C" int cdx_create_wavefront(cdx_ctx* c, const void* I, const void* P, int orow, int ocol, void* out) {
+  CTX_CHECK(c);
+  if (!out) return fail(CDX_ERR_ARG, "cdx_create_wavefront: null buffer");
+  if (orow <= 0 || ocol <= 0 || orow > c->rows || ocol > c->cols)
+    return fail(CDX_ERR_ARG, "cdx_create_wavefront: object %dx%d does not fit the %dx%d frame", orow, ocol, c->rows,
+                c->cols);
+  if (c->prec == CDX_F64)
+    HIPCHK(launch_wavefront<double>(c->rows, c->cols, (const double*)I, (const double*)P, orow, ocol, out, c->st));
+  else
+    HIPCHK(launch_wavefront<float>(c->rows, c->cols, (const float*)I, (const float*)P, orow, ocol, out, c->st));
+  return CDX_OK;
+}

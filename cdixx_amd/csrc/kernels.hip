@@ -1106,8 +1106,96 @@ hipError_t launch_poisson_intensity(int64_t n, const void* F, const double* tot,
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// setup kernels of CDI::prepareIter / readObjectWave / checkAutoCorrelation
+// ---------------------------------------------------------------------------
+// convertFOPhase (cudaConfig.cu:861-870): multiply by (-1)^(x+y) * norm
+template <class T>
+__global__ void k_fo_phase(int rows, int cols, cplx<T>* d, T norm) {
+  const int64_t n = (int64_t)rows * cols;
+  EW_LOOP(n) {
+    const int x = (int)(i / cols), y = (int)(i % cols);
+    const T f = ((x + y) & 1) ? -norm : norm;
+    d[i] = {d[i].x * f, d[i].y * f};
+  }
+}
+template <class T>
+hipError_t launch_fo_phase(int rows, int cols, void* d, T norm, hipStream_t st) {
+  hipLaunchKernelGGL(k_fo_phase<T>, ew_grid((int64_t)rows * cols), dim3(256), 0, st, rows, cols, (cplx<T>*)d, norm);
+  return hipGetLastError();
+}
+// out = in * (invert ? 1 - m : m)   (linearConst + multiply, cdi.cc:185-187)
+template <class T>
+__global__ void k_mask_mul(int64_t n, T* d, const T* m, int invert) {
+  EW_LOOP(n) d[i] *= invert ? (T)1 - m[i] : m[i];
+}
+template <class T>
+hipError_t launch_mask_mul(int64_t n, T* d, const T* m, int invert, hipStream_t st) {
+  hipLaunchKernelGGL(k_mask_mul<T>, ew_grid(n), dim3(256), 0, st, n, d, m, invert);
+  return hipGetLastError();
+}
+// applyPoissonNoise_WO (cdi.cu:291-296): quantise to 1/rcolor steps of scale
+// and add zero-mean dark noise Poisson(noiseLevel) - noiseLevel
+template <class T>
+__global__ void k_poisson_wo(int64_t n, T* w, double noise, T scale, T rcolor, uint64_t seed) {
+  EW_LOOP(n) {
+    const double dark = poisson_draw(noise, seed, (uint64_t)i);
+    w[i] = scale * (T)((double)(int)(w[i] * rcolor / scale) + dark - noise) / rcolor;
+  }
+}
+template <class T>
+hipError_t launch_poisson_wo(int64_t n, T* w, double noise, T scale, T rcolor, uint64_t seed, hipStream_t st) {
+  hipLaunchKernelGGL(k_poisson_wo<T>, ew_grid(n), dim3(256), 0, st, n, w, noise, scale, rcolor, seed);
+  return hipGetLastError();
+}
+// extendToComplex (cudaConfig.cu:682-686): (a, 0)
+template <class T>
+__global__ void k_extend(int64_t n, const T* a, cplx<T>* b) {
+  EW_LOOP(n) b[i] = {a[i], (T)0};
+}
+template <class T>
+hipError_t launch_extend(int64_t n, const T* a, void* b, hipStream_t st) {
+  hipLaunchKernelGGL(k_extend<T>, ew_grid(n), dim3(256), 0, st, n, a, (cplx<T>*)b);
+  return hipGetLastError();
+}
+// createWaveFront, integer-size form (cudaConfig.cu:688-709): an objrow x
+// objcol image centred in the frame, amplitude sqrt(max(I,0)) (1 without I),
+// phase 2*pi*(P - 0.5) when a phase image is given
+template <class T>
+__global__ void k_wavefront(int rows, int cols, const T* I, const T* P, int orow, int ocol, cplx<T>* out) {
+  const int64_t n = (int64_t)rows * cols;
+  const int mx = (rows - orow) / 2, my = (cols - ocol) / 2;
+  EW_LOOP(n) {
+    const int x = (int)(i / cols), y = (int)(i % cols);
+    if (x < mx || x >= mx + orow || y < my || y >= my + ocol) {
+      out[i] = {(T)0, (T)0};
+      continue;
+    }
+    const int64_t t = (int64_t)(x - mx) * ocol + (y - my);
+    const T mod = I ? dsqrt(I[t] > (T)0 ? I[t] : (T)0) : (T)1;
+    if (P) {
+      const T ph = (P[t] - (T)0.5) * (T)2 * (T)kPi;
+      out[i] = {mod * (T)cos(ph), mod * (T)sin(ph)};
+    } else {
+      out[i] = {mod, (T)0};
+    }
+  }
+}
+template <class T>
+hipError_t launch_wavefront(int rows, int cols, const T* I, const T* P, int orow, int ocol, void* out,
+                            hipStream_t st) {
+  hipLaunchKernelGGL(k_wavefront<T>, ew_grid((int64_t)rows * cols), dim3(256), 0, st, rows, cols, I, P, orow, ocol,
+                     (cplx<T>*)out);
+  return hipGetLastError();
+}
+
 // explicit instantiations
 #define INST(T)                                                                                                  \
+  template hipError_t launch_fo_phase<T>(int, int, void*, T, hipStream_t);                                       \
+  template hipError_t launch_mask_mul<T>(int64_t, T*, const T*, int, hipStream_t);                               \
+  template hipError_t launch_poisson_wo<T>(int64_t, T*, double, T, T, uint64_t, hipStream_t);                    \
+  template hipError_t launch_extend<T>(int64_t, const T*, void*, hipStream_t);                                   \
+  template hipError_t launch_wavefront<T>(int, int, const T*, const T*, int, int, void*, hipStream_t);           \
   template hipError_t launch_pack_intensity<T>(int64_t, const T*, const T*, T*, hipStream_t);                    \
   template hipError_t launch_mod_abs<T>(int64_t, void*, const T*, hipStream_t);                                  \
   template hipError_t launch_mod_noise<T>(int64_t, void*, const T*, const T*, int, T, T, T, hipStream_t);        \

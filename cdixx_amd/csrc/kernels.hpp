@@ -101,6 +101,15 @@ template <class T> hipError_t launch_rfactor(int64_t n, const void* F, const T* 
                                              hipStream_t st);
 template <class T> hipError_t launch_cast_c(int64_t n, const void* in, int in_f64, void* out, hipStream_t st);
 
+// setup kernels (prepareIter / readObjectWave / checkAutoCorrelation)
+template <class T> hipError_t launch_fo_phase(int rows, int cols, void* d, T norm, hipStream_t st);
+template <class T> hipError_t launch_mask_mul(int64_t n, T* d, const T* m, int invert, hipStream_t st);
+template <class T> hipError_t launch_poisson_wo(int64_t n, T* w, double noise, T scale, T rcolor, uint64_t seed,
+                                                hipStream_t st);
+template <class T> hipError_t launch_extend(int64_t n, const T* a, void* b, hipStream_t st);
+template <class T> hipError_t launch_wavefront(int rows, int cols, const T* I, const T* P, int orow, int ocol,
+                                               void* out, hipStream_t st);
+
 // synthetic forward model
 hipError_t launch_uniform_field(int n, uint64_t seed, uint64_t stream, double* out, hipStream_t st);
 hipError_t launch_smooth_periodic(int n, const double* in, double* out, double sigma, int axis,

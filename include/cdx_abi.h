@@ -115,6 +115,29 @@ int cdx_residual(cdx_ctx* ctx, const void* F, const void* I, const void* bs, dou
 /* R = sum | |F| - sqrt(I) | / sum sqrt(I) over pixels outside the beamstop; blocking */
 int cdx_rfactor(cdx_ctx* ctx, const void* F, const void* I, const void* bs, double* out);
 
+/* ---- setup operations used by CDI::prepareIter / readObjectWave ---- */
+/* convertFOPhase(data, norm), include/cudaConfig.hpp:125: times (-1)^(x+y)*norm */
+int cdx_convert_fo_phase(cdx_ctx* ctx, void* data, double norm);
+/* multiply(data, data, mask) or with 1-mask (linearConst + multiply,
+ * src/post_gpu/cdi.cc:185-187); include/cudaConfig.hpp:118 */
+int cdx_mask_multiply(cdx_ctx* ctx, void* data, const void* mask, int invert);
+/* applyPoissonNoise_WO(wave, noiseLevel, state), include/cudaConfig.hpp:100:
+ * quantise and add dark Poisson noise (Philox stream 2 with this seed) */
+int cdx_poisson_noise_wo(cdx_ctx* ctx, void* data, double noiseLevel, uint64_t seed);
+/* extendToComplex(a, b), include/cudaConfig.hpp:70 */
+int cdx_extend_to_complex(cdx_ctx* ctx, const void* a, void* b);
+/* createWaveFront(I, phase, objectWave, row, col), include/cudaConfig.hpp:82
+ * (I and phase may be NULL) */
+int cdx_create_wavefront(cdx_ctx* ctx, const void* I, const void* phase, int obj_rows, int obj_cols, void* out);
+
+/* ---- the .bin image format: struct imageFile, include/imageFile.hpp:3-17;
+ * readImage_c .bin branch, src/independant/imgio.c:23-43; writeComplexImage /
+ * writeFloatImage, imgio.c:223-243.  Host memory; type 3 = FLOAT, 4 = FLOAT2,
+ * 5 = DOUBLE, 6 = DOUBLE2. ---- */
+int cdx_bin_read_header(const char* path, int* type, int* rows, int* cols);
+int cdx_bin_read(const char* path, void* host, size_t capacity);
+int cdx_bin_write(const char* path, int type, int rows, int cols, const void* host);
+
 /* ---- the loop: CDI::phaseRetrieve, src/post_gpu/cdi.cc:297-421 ---- */
 typedef struct cdx_params {
   int simCCDbit;        /* 1: applyMod (noise-tolerant, beamstop-aware); 0: applyModAbs */

@@ -93,6 +93,8 @@ def main():
     ctx.set_threshold(0.5)
     d = make_inputs(cdx, ctx)
     p = cdx.Params.default(beta=0.85, useBS=1, freeBeamstop=1)
+    tuning = ctx.autotune(reps=40)  # picks the lines-per-workgroup of both passes
+    tiling = ctx.tiling()
 
     # warmup (untimed), then exactly `steps` iterations bracketed by syncs
     ctx.time_iterations(p, cdx.RAAR, d["F"], d["g"], d["I"], d["spt"], max(args.warmup, 1), bs=d["bs"],
@@ -131,7 +133,9 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c64/f32",
         "data": "synthetic (seeded object, Poisson 1e8 photons)",
         "config": {"workload": "single-frame RAAR beta=0.85, 1024x1024 complex64, beamstop r=8, support r=160",
-                   "frame": [N, N], "algorithm": "RAAR", "graph": not args.no_graph},
+                   "frame": [N, N], "algorithm": "RAAR", "graph": not args.no_graph,
+                   "tiling": {"col_lines": tiling[0], "row_lines": tiling[1]},
+                   "tuner_us": {f"{'col' if p_ == 0 else 'row'}{l_}": round(us, 2) for p_, l_, us in tuning}},
         "hbm_gbps_effective": round(iters_s * BYTES_PER_ITER / 1e9, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_GBS, 4), "traffic": traffic,

@@ -31,7 +31,7 @@ class CdxError(RuntimeError):
 
 def _load():
     if not os.path.exists(LIB_PATH):
-        raise ImportError(f"{LIB_PATH} is not built; run `python -m cdixx_amd.build` "
+        raise ImportError(f"{LIB_PATH} is not built; run `python cdixx_amd/build.py` "
                           "(there is no CPU fallback for this path)")
     return ctypes.CDLL(LIB_PATH)
 
@@ -92,6 +92,14 @@ _SIGS = {
     "cdx_sim_object": ([_vp, _d, _u64, _d, _vp], _i),
     "cdx_sim_intensity": ([_vp, _vp, _d, _u64, _i, _vp], _i),
     "cdx_read_config_json": ([ctypes.c_char_p, ctypes.c_char_p, _i], _i),
+    "cdx_set_tiling": ([_vp, _i, _i], _i), "cdx_get_tiling": ([_vp, _ip, _ip], _i),
+    "cdx_autotune": ([_vp, _i, _dp, _i], _i),
+    "cdx_convert_fo_phase": ([_vp, _vp, _d], _i), "cdx_mask_multiply": ([_vp, _vp, _vp, _i], _i),
+    "cdx_poisson_noise_wo": ([_vp, _vp, _d, _u64], _i), "cdx_extend_to_complex": ([_vp, _vp, _vp], _i),
+    "cdx_create_wavefront": ([_vp, _vp, _vp, _i, _i, _vp], _i),
+    "cdx_bin_read_header": ([ctypes.c_char_p, _ip, _ip, _ip], _i),
+    "cdx_bin_read": ([ctypes.c_char_p, _vp, _sz], _i),
+    "cdx_bin_write": ([ctypes.c_char_p, _i, _i, _i, _vp], _i),
 }
 for _name, (_args, _res) in _SIGS.items():
     if hasattr(lib, _name):
@@ -201,10 +209,29 @@ class Context:
         return b
 
     def get_real(self, b):
+        self.sync()
         return b.download(self.rdtype, self.shape)
 
     def get_cplx(self, b):
+        self.sync()
         return b.download(self.cdtype, self.shape)
+
+    # tiling of the FFT passes
+    def set_tiling(self, col_lines=0, row_lines=0):
+        check(lib.cdx_set_tiling(self.h, col_lines, row_lines))
+
+    def tiling(self):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        check(lib.cdx_get_tiling(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def autotune(self, reps=50):
+        """Times every valid tiling of both passes and keeps the fastest.
+        Returns [(pass, lines, us_per_launch), ...] (pass 0 = column, 1 = row)."""
+        rep = (ctypes.c_double * 96)()
+        check(lib.cdx_autotune(self.h, reps, rep, 96))
+        out = [(int(rep[i]), int(rep[i + 1]), rep[i + 2]) for i in range(0, 96, 3) if rep[i + 1] > 0]
+        return out
 
     # state
     def sync(self):

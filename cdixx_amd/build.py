@@ -1,6 +1,7 @@
 """Build recipe for libcdixx_amd.so (HIP kernels for gfx950 + C ABI + host C++).
 
-Run as ``python -m cdixx_amd.build`` or through ``__graft_entry__.build()``.
+Run as ``python cdixx_amd/build.py`` or through ``__graft_entry__.build()`` (not
+through the package, whose import needs the built library).
 Objects go to cdixx_amd/_build/, the library to cdixx_amd/libcdixx_amd.so.
 """
 import os
@@ -22,7 +23,7 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC", "-Wall",
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter",
               f"-I{ROCM}/include", "-D__HIP_PLATFORM_AMD__"]
 
-HIP_SOURCES = ["kernels.hip", "abi.hip", "batch.hip", "unet.hip"]
+HIP_SOURCES = ["passes.hip", "kernels.hip", "abi.hip", "batch.hip", "unet.hip"]
 HOST_SOURCES = ["config.cpp", "cdi.cpp", "imgio.cpp"]
 
 
@@ -51,7 +52,7 @@ def build(verbose=True, jobs=None):
     with ThreadPoolExecutor(jobs) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-Wl,-z,defs", "-o", tmp] + objs)
     os.replace(tmp, LIB)
     # the command-line driver (cdi_run equivalent)
     cli_src = os.path.join(CSRC, "cdi_run.cpp")

@@ -67,6 +67,7 @@ struct cdx_ctx {
   int ring_cap;
   int rcolor;
   double scale, beta, relax;
+  int col_lines = 0, row_lines = 0;  // lines per workgroup of the passes (0: default)
 };
 
 // twiddle tables: twf[k1*Q+q] = W^(q*k1), twi[n1*P+k1] = W^(n1*k1), W = exp(-2 pi i/N)
@@ -156,7 +157,9 @@ extern "C" int cdx_create(cdx_ctx** out, int rows, int cols, int precision, void
     c->st = (hipStream_t)stream;
     c->own_stream = false;
   } else {
-    if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+    // a blocking stream: ordered with null-stream copies, like the reference's
+    // single legacy stream (default priority)
+    if (hipStreamCreate(&c->st) != hipSuccess) {
       delete c;
       return fail(CDX_ERR_HIP, "cdx_create: hipStreamCreate failed");
     }
@@ -521,12 +524,14 @@ struct Loop {
   int iterate(int algo, double* rf) {
     pa.rf = rf;
     if (f_pending)
-      HIPCHK(launch_col<T>(COL_PROJ_INV, c->rows, c->cols, F, c->T2, c->tw_col, (const T*)c->Ipk, pa, 0, c->st));
+      HIPCHK(launch_col<T>(COL_PROJ_INV, c->rows, c->cols, F, c->T2, c->tw_col, (const T*)c->Ipk, pa, 0, c->st,
+                           c->col_lines));
     else
-      HIPCHK(launch_col<T>(COL_FWD_PROJ_INV, c->rows, c->cols, c->T1, c->T2, c->tw_col, (const T*)c->Ipk, pa, 0, c->st));
+      HIPCHK(launch_col<T>(COL_FWD_PROJ_INV, c->rows, c->cols, c->T1, c->T2, c->tw_col, (const T*)c->Ipk, pa, 0,
+                           c->st, c->col_lines));
     UpdArgs<T> ua = upd_args<T>(c, algo, p->beta, p->relax);
     HIPCHK(launch_row<T>(ROW_INV_UPDATE_FWD, c->rows, c->cols, c->T2, c->T1, c->tw_row, g, spt, ua, (T)1, nullptr,
-                         ReduceSlots{}, c->st));
+                         ReduceSlots{}, c->st, c->row_lines));
     f_pending = false;
     return CDX_OK;
   }
@@ -536,12 +541,13 @@ struct Loop {
       // applyGaussMult(patternWave, ., Real(row>>2)/gaussianSigma, 1) (cdi.cc:327)
       T sf = (T)(c->rows >> 2) / (T)sigma;
       if (f_pending)
-        HIPCHK(launch_col<T>(COL_GAUSS_INV, c->rows, c->cols, F, c->T2, c->tw_col, nullptr, none, (double)sf, c->st));
+        HIPCHK(launch_col<T>(COL_GAUSS_INV, c->rows, c->cols, F, c->T2, c->tw_col, nullptr, none, (double)sf, c->st,
+                             c->col_lines));
       else
         HIPCHK(launch_col<T>(COL_FWD_GAUSS_INV, c->rows, c->cols, c->T1, c->T2, c->tw_col, nullptr, none, (double)sf,
-                             c->st));
+                             c->st, c->col_lines));
       HIPCHK(launch_row<T>(ROW_INV_MOD2, c->rows, c->cols, c->T2, nullptr, c->tw_row, nullptr, nullptr, UpdArgs<T>{},
-                           (T)1, spt, slots(c), c->st));
+                           (T)1, spt, slots(c), c->st, c->row_lines));
       HIPCHK(launch_threshold<T>(c->scal, slots(c), p->shrinkThreshold, c->rows, c->cols, 0, c->st));
     } else {
       HIPCHK(launch_blur_real<T>(c->rows, c->cols, g, 1, (T*)c->tmpr, spt, (T)sigma, slots(c), c->st));
@@ -679,11 +685,12 @@ static int time_t_(cdx_ctx* c, const cdx_params* p, int algo, void* F, void* g, 
     UpdArgs<T> ua = upd_args<T>(c, algo, p->beta, p->relax);
     for (int i = 0; i < n; i++) {
       HIPCHK(hipEventRecord(ev[4 * i], c->st));
-      HIPCHK(launch_col<T>(COL_FWD_PROJ_INV, c->rows, c->cols, c->T1, c->T2, c->tw_col, (const T*)c->Ipk, L.pa, 0, c->st));
+      HIPCHK(launch_col<T>(COL_FWD_PROJ_INV, c->rows, c->cols, c->T1, c->T2, c->tw_col, (const T*)c->Ipk, L.pa, 0,
+                           c->st, c->col_lines));
       HIPCHK(hipEventRecord(ev[4 * i + 1], c->st));
       HIPCHK(hipEventRecord(ev[4 * i + 2], c->st));
       HIPCHK(launch_row<T>(ROW_INV_UPDATE_FWD, c->rows, c->cols, c->T2, c->T1, c->tw_row, g, (const T*)spt, ua, (T)1,
-                           nullptr, ReduceSlots{}, c->st));
+                           nullptr, ReduceSlots{}, c->st, c->row_lines));
       HIPCHK(hipEventRecord(ev[4 * i + 3], c->st));
     }
     HIPCHK(hipStreamSynchronize(c->st));
@@ -817,4 +824,90 @@ extern "C" int cdx_create_wavefront(cdx_ctx* c, const void* I, const void* P, in
   else
     HIPCHK(launch_wavefront<float>(c->rows, c->cols, (const float*)I, (const float*)P, orow, ocol, out, c->st));
   return CDX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// tiling of the passes and its tuner
+// ---------------------------------------------------------------------------
+extern "C" int cdx_set_tiling(cdx_ctx* c, int col_lines, int row_lines) {
+  CTX_CHECK(c);
+  if (col_lines && !tiling_valid(c->rows, c->csz, col_lines))
+    return fail(CDX_ERR_ARG, "cdx_set_tiling: %d columns per workgroup is not valid for %d-point columns", col_lines,
+                c->rows);
+  if (row_lines && !tiling_valid(c->cols, c->csz, row_lines))
+    return fail(CDX_ERR_ARG, "cdx_set_tiling: %d rows per workgroup is not valid for %d-point rows", row_lines,
+                c->cols);
+  c->col_lines = col_lines;
+  c->row_lines = row_lines;
+  return CDX_OK;
+}
+extern "C" int cdx_get_tiling(cdx_ctx* c, int* col_lines, int* row_lines) {
+  CTX_CHECK(c);
+  if (col_lines) *col_lines = c->col_lines ? c->col_lines : tiling_default(c->rows, c->csz, 1);
+  if (row_lines) *row_lines = c->row_lines ? c->row_lines : tiling_default(c->cols, c->csz, 0);
+  return CDX_OK;
+}
+
+template <class T>
+static int autotune_t(cdx_ctx* c, int reps, double* report, int cap) {
+  void* g = nullptr;
+  void* spt = nullptr;
+  HIPCHK(hipMalloc(&g, c->n * c->csz));
+  HIPCHK(hipMalloc(&spt, c->n * c->rsz));
+  HIPCHK(hipMemsetAsync(g, 0, c->n * c->csz, c->st));
+  HIPCHK(hipMemsetAsync(spt, 0, c->n * c->rsz, c->st));
+  HIPCHK(hipMemsetAsync(c->T1, 0, c->n * c->csz, c->st));
+  HIPCHK(hipMemsetAsync(c->Ipk, 0, c->n * c->rsz, c->st));
+  ProjArgs<T> pa{};
+  pa.scale = 1;
+  pa.rcolor = 65535;
+  UpdArgs<T> ua = upd_args<T>(c, CDX_HIO, 1.0, 1.9);
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  int nrep = 0;
+  double best_col = 1e30, best_row = 1e30;
+  int bc = 0, br = 0;
+  for (int pass = 0; pass < 2; pass++) {
+    const int line = pass == 0 ? c->rows : c->cols;
+    for (int lines = 1; lines <= 256; lines *= 2) {
+      if (!tiling_valid(line, c->csz, lines)) continue;
+      float ms = 0;
+      for (int trial = 0; trial < 2; trial++) {  // the first trial warms caches and code
+        HIPCHK(hipEventRecord(e0, c->st));
+        for (int i = 0; i < reps; i++) {
+          if (pass == 0)
+            HIPCHK(launch_col<T>(COL_FWD_PROJ_INV, c->rows, c->cols, c->T1, c->T2, c->tw_col, (const T*)c->Ipk, pa,
+                                 0, c->st, lines));
+          else
+            HIPCHK(launch_row<T>(ROW_INV_UPDATE_FWD, c->rows, c->cols, c->T2, c->T1, c->tw_row, g, (const T*)spt, ua,
+                                 (T)1, nullptr, ReduceSlots{}, c->st, lines));
+        }
+        HIPCHK(hipEventRecord(e1, c->st));
+        HIPCHK(hipEventSynchronize(e1));
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+      }
+      const double us = 1000.0 * ms / reps;
+      if (report && nrep + 3 <= cap) {
+        report[nrep++] = pass;
+        report[nrep++] = lines;
+        report[nrep++] = us;
+      }
+      if (pass == 0 && us < best_col) best_col = us, bc = lines;
+      if (pass == 1 && us < best_row) best_row = us, br = lines;
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipFree(g);
+  hipFree(spt);
+  c->col_lines = bc;
+  c->row_lines = br;
+  return CDX_OK;
+}
+
+extern "C" int cdx_autotune(cdx_ctx* c, int reps, double* report, int cap) {
+  CTX_CHECK(c);
+  if (reps <= 0) reps = 50;
+  return DISPATCH(c, autotune_t, c, reps, report, cap);
 }

@@ -1,0 +1,218 @@
+// device_ops.hpp -- device helpers shared by the CDI kernels: line
+// configurations of the FFT, the reference's pointwise operations, workgroup
+// reductions, grid-stride helpers and the Philox generator.
+#pragma once
+#include "fft_line.hpp"
+#include "kernels.hpp"
+#include <math.h>
+
+namespace cdx {
+
+// ---------------------------------------------------------------------------
+// line configurations
+// ---------------------------------------------------------------------------
+template <int N>
+struct LineCfg;
+#define CDX_LINE(n, p, q, tl)                         \
+  template <>                                         \
+  struct LineCfg<n> {                                 \
+    static constexpr int P = p, Q = q, TL = tl;       \
+  };
+CDX_LINE(16, 4, 4, 4)
+CDX_LINE(32, 4, 8, 4)
+CDX_LINE(64, 8, 8, 8)
+CDX_LINE(128, 8, 16, 8)
+CDX_LINE(256, 16, 16, 16)
+CDX_LINE(384, 16, 24, 8)
+CDX_LINE(512, 16, 32, 16)
+CDX_LINE(768, 32, 24, 8)
+CDX_LINE(1024, 32, 32, 32)
+CDX_LINE(2048, 32, 64, 32)
+#undef CDX_LINE
+
+// LDS pitch per line: at least the transpose footprint, == 8 (mod 32)
+// complex elements so neighbouring lines land on different banks.
+template <class T, int N>
+constexpr int lds_pitch() {
+  using L = Line<T, LineCfg<N>::P, LineCfg<N>::Q, LineCfg<N>::TL>;
+  int e = L::LDS_ELEMS;
+  return ((e + 23) / 32) * 32 + 8;
+}
+
+// ---------------------------------------------------------------------------
+// pointwise device helpers (reference semantics)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float dsqrt(float x) { return sqrtf(x); }
+__device__ __forceinline__ double dsqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float dhypot(float a, float b) { return hypotf(a, b); }
+__device__ __forceinline__ double dhypot(double a, double b) { return hypot(a, b); }
+
+// applyModAbs (cudaConfig.cu:827-841) with state == NULL
+template <class T>
+__device__ __forceinline__ cplx<T> mod_abs(cplx<T> f, T I) {
+  T mod = dhypot(f.x, f.y);
+  T rat = I > (T)0 ? dsqrt(I) : (T)0;
+  if (mod == (T)0) return {rat, (T)0};
+  rat /= mod;
+  return {f.x * rat, f.y * rat};
+}
+
+// applyMod, 5-argument form (cdi.cu:438-485) with bsnorm = 1; beamstop pixels
+// arrive as NaN intensities and are left unchanged.
+template <class T>
+__device__ __forceinline__ cplx<T> mod_noise(cplx<T> f, T I, int noise, T scale, T rcolor) {
+  if (isnan(I)) return f;
+  T mod2 = I > (T)0 ? I : (T)0;
+  T src2 = f.x * f.x + f.y * f.y;
+  T maximum = scale * (T)0.95;
+  if (mod2 >= maximum) mod2 = mod2 > src2 ? mod2 : src2;
+  T tol = dsqrt((T)noise + mod2 * rcolor / scale) * scale / rcolor;
+  T diff = mod2 - src2;
+  T val = mod2;
+  if (diff > tol) val -= tol;
+  else if (diff < -tol) val += tol;
+  if (src2 == (T)0) return {val, (T)0};
+  T s = dsqrt((val > (T)0 ? val : (T)0) / src2);
+  return {s * f.x, s * f.y};
+}
+
+template <class T>
+__device__ __forceinline__ cplx<T> project(cplx<T> f, T I, const ProjArgs<T>& pa) {
+  if (pa.mode == PROJ_MOD_NOISE) return mod_noise(f, I, pa.noiseLevel, pa.scale, pa.rcolor);
+  if (pa.mode == PROJ_MOD_ABS_BS && isnan(I)) return f;
+  return mod_abs(f, I);
+}
+
+// Apply*Support (cdi.cu:518-616) behind the applySupport dispatcher
+// (cdi.cc:25-36).  g = current object estimate, p = projected estimate g'.
+template <class T>
+__device__ __forceinline__ cplx<T> support_update(cplx<T> g, cplx<T> p, bool in, int algo, T beta, T relax) {
+  switch (algo) {
+    case 0:  // RAAR
+      if (in) return p;
+      return {beta * g.x + ((T)1 - (T)2 * beta) * p.x, beta * g.y + ((T)1 - (T)2 * beta) * p.y};
+    case 1:  // ER
+      if (in) return {g.x + relax * (p.x - g.x), g.y + relax * (p.y - g.y)};
+      return {(T)0, (T)0};
+    case 2:  // POSER
+      if (in && p.x > (T)0) return p;
+      return {(T)0, (T)0};
+    case 3:  // HIO
+      if (in) return p;
+      return {g.x - beta * p.x, g.y - beta * p.y};
+    case 4: {  // POSHIO
+      cplx<T> r = g;
+      if (p.x > (T)0 && in) r.x = g.x + relax * (p.x - g.x);
+      else r.x = g.x - beta * p.x;
+      r.y = g.y - beta * p.y;
+      return r;
+    }
+    default:  // FHIO, KKT, XCORRELATION, ...
+      if (in) return {g.x + relax * (p.x - g.x), g.y + relax * (p.y - g.y)};
+      return {g.x - (T)1.2 * p.x, g.y - (T)1.2 * p.y};
+  }
+}
+
+// ---------------------------------------------------------------------------
+// workgroup reductions
+// ---------------------------------------------------------------------------
+template <class V>
+__device__ __forceinline__ V wave_sum(V v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+template <class V>
+__device__ __forceinline__ V wave_max(V v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    V w = __shfl_xor(v, o, 64);
+    v = w > v ? w : v;
+  }
+  return v;
+}
+// sum of a per-thread double over the workgroup, added atomically to *dst
+__device__ __forceinline__ void block_atomic_sum(double v, double* dst, double* sh) {
+  v = wave_sum(v);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int nw = (blockDim.x + 63) >> 6;
+  __syncthreads();
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0;
+    for (int i = 0; i < nw; i++) s += sh[i];
+    atomicAdd(dst, s);
+  }
+}
+// workgroup sum stored at rs.partials[blockIdx.x] (fixed-order reduction)
+__device__ __forceinline__ void block_partial_sum(double v, const ReduceSlots& rs, double* sh) {
+  v = wave_sum(v);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int nw = (blockDim.x + 63) >> 6;
+  __syncthreads();
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0;
+    for (int i = 0; i < nw; i++) s += sh[i];
+    rs.partials[blockIdx.x] = s;
+    if (blockIdx.x == 0) *rs.nparts = (int)gridDim.x;
+  }
+}
+__device__ __forceinline__ void atomic_max_pos(void* bits, float v) {
+  atomicMax((unsigned int*)bits, __float_as_uint(v));
+}
+__device__ __forceinline__ void atomic_max_pos(void* bits, double v) {
+  atomicMax((unsigned long long*)bits, (unsigned long long)__double_as_longlong(v));
+}
+template <class T>
+__device__ __forceinline__ void block_atomic_max(T v, void* dst, double* sh) {
+  v = wave_max(v);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int nw = (blockDim.x + 63) >> 6;
+  __syncthreads();
+  if (l == 0) sh[w] = (double)v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0;
+    for (int i = 0; i < nw; i++) m = sh[i] > m ? sh[i] : m;
+    atomic_max_pos(dst, (T)m);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// elementwise kernels (grid-stride, 256 threads)
+// ---------------------------------------------------------------------------
+inline dim3 ew_grid(int64_t n) {
+  int64_t b = (n + 255) / 256;
+  if (b > 65536) b = 65536;
+  if (b < 1) b = 1;
+  return dim3((unsigned)b);
+}
+#define EW_LOOP(n) for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (n); i += (int64_t)gridDim.x * blockDim.x)
+
+// ---------------------------------------------------------------------------
+// Philox4x64-10 (Salmon et al., SC'11) for random starts and the forward model
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void philox4x64(uint64_t c[4], uint64_t k0, uint64_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; r++) {
+    const uint64_t lo0 = 0xD2E7470EE14C6C93ULL * c[0], hi0 = __umul64hi(0xD2E7470EE14C6C93ULL, c[0]);
+    const uint64_t lo1 = 0xCA5A826395121157ULL * c[2], hi1 = __umul64hi(0xCA5A826395121157ULL, c[2]);
+    const uint64_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0;
+    c[1] = lo1;
+    c[2] = n2;
+    c[3] = lo0;
+    k0 += 0x9E3779B97F4A7C15ULL;
+    k1 += 0xBB67AE8584CAA73BULL;
+  }
+}
+__device__ __forceinline__ double philox_uniform(uint64_t seed, uint64_t stream, uint64_t idx) {
+  uint64_t c[4] = {idx, 0, 0, 0};
+  philox4x64(c, seed, stream);
+  return (double)(c[0] >> 11) * 0x1.0p-53;
+}
+
+}  // namespace cdx

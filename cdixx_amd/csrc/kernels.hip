@@ -1,509 +1,8 @@
-// kernels.hip -- CDNA4 kernels of the CDI phase-retrieval loop.
-//
-// One iteration of CDI::phaseRetrieve (CDIxx src/post_gpu/cdi.cc:349-354,381)
-// is projection -> IFFT -> 1/N -> support update -> FFT.  Here it is two
-// kernels that each stream the image once:
-//   column pass  (k_col, COL_FWD_PROJ_INV): forward column FFT of T1 = rowFFT(g),
-//                 modulus projection (+R-factor sums), inverse column FFT -> T2
-//   row pass     (k_row, ROW_INV_UPDATE_FWD): inverse row FFT of T2, 1/N,
-//                 HIO/RAAR/ER/... update against g and the support, store g,
-//                 forward row FFT -> T1
-// so F = colFFT(T1) is never stored between iterations.  Every other kernel
-// here serves setup, shrink-wrap, diagnostics or the standalone free
-// functions of the reference's cudaConfig.hpp.
-#include "fft_line.hpp"
-#include "kernels.hpp"
-#include <math.h>
+// kernels.hip -- setup, shrink-wrap, diagnostic and forward-model kernels of
+// the CDI path (the fused FFT passes are in passes.hip).
+#include "device_ops.hpp"
 
 namespace cdx {
-
-// ---------------------------------------------------------------------------
-// line configurations
-// ---------------------------------------------------------------------------
-template <int N>
-struct LineCfg;
-#define CDX_LINE(n, p, q, tl)                         \
-  template <>                                         \
-  struct LineCfg<n> {                                 \
-    static constexpr int P = p, Q = q, TL = tl;       \
-  };
-CDX_LINE(16, 4, 4, 4)
-CDX_LINE(32, 4, 8, 4)
-CDX_LINE(64, 8, 8, 8)
-CDX_LINE(128, 8, 16, 8)
-CDX_LINE(256, 16, 16, 16)
-CDX_LINE(384, 16, 24, 8)
-CDX_LINE(512, 16, 32, 16)
-CDX_LINE(768, 32, 24, 8)
-CDX_LINE(1024, 32, 32, 32)
-CDX_LINE(2048, 32, 64, 32)
-#undef CDX_LINE
-
-int line_supported(int n) {
-  switch (n) {
-    case 16: case 32: case 64: case 128: case 256: case 384: case 512: case 768: case 1024: case 2048:
-      return 1;
-    default:
-      return 0;
-  }
-}
-void line_factors(int n, int* P, int* Q) {
-  switch (n) {
-#define F(n_) case n_: *P = LineCfg<n_>::P; *Q = LineCfg<n_>::Q; return;
-    F(16) F(32) F(64) F(128) F(256) F(384) F(512) F(768) F(1024) F(2048)
-#undef F
-    default: *P = *Q = 0;
-  }
-}
-
-// LDS pitch per line: at least the transpose footprint, == 8 (mod 32)
-// complex elements so neighbouring lines land on different banks.
-template <class T, int N>
-constexpr int lds_pitch() {
-  using L = Line<T, LineCfg<N>::P, LineCfg<N>::Q, LineCfg<N>::TL>;
-  int e = L::LDS_ELEMS;
-  return ((e + 23) / 32) * 32 + 8;
-}
-// columns per workgroup (coalescing: C*sizeof(cplx) >= 64 B)
-// (at least one full wavefront per workgroup for the shuffle reductions)
-template <class T, int N>
-constexpr int col_lines() {
-  constexpr int base = sizeof(T) == 4 ? 8 : 4;
-  constexpr int wave = 64 / LineCfg<N>::TL;
-  return base > wave ? base : wave;
-}
-// rows per workgroup
-template <class T, int N>
-constexpr int row_lines() {
-  constexpr int r = 128 / LineCfg<N>::TL;
-  return r < 1 ? 1 : r;
-}
-
-// ---------------------------------------------------------------------------
-// pointwise device helpers (reference semantics)
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ float dsqrt(float x) { return sqrtf(x); }
-__device__ __forceinline__ double dsqrt(double x) { return sqrt(x); }
-__device__ __forceinline__ float dhypot(float a, float b) { return hypotf(a, b); }
-__device__ __forceinline__ double dhypot(double a, double b) { return hypot(a, b); }
-
-// applyModAbs (cudaConfig.cu:827-841) with state == NULL
-template <class T>
-__device__ __forceinline__ cplx<T> mod_abs(cplx<T> f, T I) {
-  T mod = dhypot(f.x, f.y);
-  T rat = I > (T)0 ? dsqrt(I) : (T)0;
-  if (mod == (T)0) return {rat, (T)0};
-  rat /= mod;
-  return {f.x * rat, f.y * rat};
-}
-
-// applyMod, 5-argument form (cdi.cu:438-485) with bsnorm = 1; beamstop pixels
-// arrive as NaN intensities and are left unchanged.
-template <class T>
-__device__ __forceinline__ cplx<T> mod_noise(cplx<T> f, T I, int noise, T scale, T rcolor) {
-  if (isnan(I)) return f;
-  T mod2 = I > (T)0 ? I : (T)0;
-  T src2 = f.x * f.x + f.y * f.y;
-  T maximum = scale * (T)0.95;
-  if (mod2 >= maximum) mod2 = mod2 > src2 ? mod2 : src2;
-  T tol = dsqrt((T)noise + mod2 * rcolor / scale) * scale / rcolor;
-  T diff = mod2 - src2;
-  T val = mod2;
-  if (diff > tol) val -= tol;
-  else if (diff < -tol) val += tol;
-  if (src2 == (T)0) return {val, (T)0};
-  T s = dsqrt((val > (T)0 ? val : (T)0) / src2);
-  return {s * f.x, s * f.y};
-}
-
-template <class T>
-__device__ __forceinline__ cplx<T> project(cplx<T> f, T I, const ProjArgs<T>& pa) {
-  if (pa.mode == PROJ_MOD_NOISE) return mod_noise(f, I, pa.noiseLevel, pa.scale, pa.rcolor);
-  if (pa.mode == PROJ_MOD_ABS_BS && isnan(I)) return f;
-  return mod_abs(f, I);
-}
-
-// Apply*Support (cdi.cu:518-616) behind the applySupport dispatcher
-// (cdi.cc:25-36).  g = current object estimate, p = projected estimate g'.
-template <class T>
-__device__ __forceinline__ cplx<T> support_update(cplx<T> g, cplx<T> p, bool in, int algo, T beta, T relax) {
-  switch (algo) {
-    case 0:  // RAAR
-      if (in) return p;
-      return {beta * g.x + ((T)1 - (T)2 * beta) * p.x, beta * g.y + ((T)1 - (T)2 * beta) * p.y};
-    case 1:  // ER
-      if (in) return {g.x + relax * (p.x - g.x), g.y + relax * (p.y - g.y)};
-      return {(T)0, (T)0};
-    case 2:  // POSER
-      if (in && p.x > (T)0) return p;
-      return {(T)0, (T)0};
-    case 3:  // HIO
-      if (in) return p;
-      return {g.x - beta * p.x, g.y - beta * p.y};
-    case 4: {  // POSHIO
-      cplx<T> r = g;
-      if (p.x > (T)0 && in) r.x = g.x + relax * (p.x - g.x);
-      else r.x = g.x - beta * p.x;
-      r.y = g.y - beta * p.y;
-      return r;
-    }
-    default:  // FHIO, KKT, XCORRELATION, ...
-      if (in) return {g.x + relax * (p.x - g.x), g.y + relax * (p.y - g.y)};
-      return {g.x - (T)1.2 * p.x, g.y - (T)1.2 * p.y};
-  }
-}
-
-// ---------------------------------------------------------------------------
-// workgroup reductions
-// ---------------------------------------------------------------------------
-template <class V>
-__device__ __forceinline__ V wave_sum(V v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-template <class V>
-__device__ __forceinline__ V wave_max(V v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    V w = __shfl_xor(v, o, 64);
-    v = w > v ? w : v;
-  }
-  return v;
-}
-// sum of a per-thread double over the workgroup, added atomically to *dst
-__device__ __forceinline__ void block_atomic_sum(double v, double* dst, double* sh) {
-  v = wave_sum(v);
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  const int nw = (blockDim.x + 63) >> 6;
-  __syncthreads();
-  if (l == 0) sh[w] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double s = 0;
-    for (int i = 0; i < nw; i++) s += sh[i];
-    atomicAdd(dst, s);
-  }
-}
-// workgroup sum stored at rs.partials[blockIdx.x] (fixed-order reduction)
-__device__ __forceinline__ void block_partial_sum(double v, const ReduceSlots& rs, double* sh) {
-  v = wave_sum(v);
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  const int nw = (blockDim.x + 63) >> 6;
-  __syncthreads();
-  if (l == 0) sh[w] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double s = 0;
-    for (int i = 0; i < nw; i++) s += sh[i];
-    rs.partials[blockIdx.x] = s;
-    if (blockIdx.x == 0) *rs.nparts = (int)gridDim.x;
-  }
-}
-__device__ __forceinline__ void atomic_max_pos(void* bits, float v) {
-  atomicMax((unsigned int*)bits, __float_as_uint(v));
-}
-__device__ __forceinline__ void atomic_max_pos(void* bits, double v) {
-  atomicMax((unsigned long long*)bits, (unsigned long long)__double_as_longlong(v));
-}
-template <class T>
-__device__ __forceinline__ void block_atomic_max(T v, void* dst, double* sh) {
-  v = wave_max(v);
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  const int nw = (blockDim.x + 63) >> 6;
-  __syncthreads();
-  if (l == 0) sh[w] = (double)v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double m = 0;
-    for (int i = 0; i < nw; i++) m = sh[i] > m ? sh[i] : m;
-    atomic_max_pos(dst, (T)m);
-  }
-}
-
-// ---------------------------------------------------------------------------
-// column pass
-// ---------------------------------------------------------------------------
-template <class T, int N, int OP>
-__global__ void __launch_bounds__((col_lines<T, N>() * LineCfg<N>::TL))
-k_col(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out, int cols, const cplx<T>* __restrict__ tw,
-      const T* __restrict__ I, ProjArgs<T> pa, T gsig) {
-  constexpr int P = LineCfg<N>::P, Q = LineCfg<N>::Q, TL = LineCfg<N>::TL;
-  constexpr int C = col_lines<T, N>();
-  constexpr int PITCH = lds_pitch<T, N>();
-  using L = Line<T, P, Q, TL>;
-  extern __shared__ __align__(16) unsigned char smem[];
-  cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem);
-  const int c = threadIdx.x % C;
-  const int t = threadIdx.x / C;
-  const int col = blockIdx.x * C + c;
-  const bool ok = col < cols;
-  cplx<T>* my = lds + c * PITCH;
-  const cplx<T>* twf = tw;
-  const cplx<T>* twi = tw + N;
-  cplx<T> v[L::E];
-
-  constexpr bool from_f = (OP == COL_INV || OP == COL_PROJ_INV || OP == COL_GAUSS_INV);
-  if constexpr (from_f) {
-#pragma unroll
-    for (int i = 0; i < L::E; i++) {
-      const int r = L::dout_pos(i, t);
-      v[i] = ok ? in[(size_t)r * cols + col] : cplx<T>{(T)0, (T)0};
-    }
-  } else {
-#pragma unroll
-    for (int i = 0; i < L::E; i++) {
-      const int r = L::din_pos(i, t);
-      v[i] = ok ? in[(size_t)r * cols + col] : cplx<T>{(T)0, (T)0};
-    }
-    L::forward(v, my, twf, t);
-  }
-
-  if constexpr (OP == COL_FWD) {
-    if (ok) {
-#pragma unroll
-      for (int i = 0; i < L::E; i++) out[(size_t)L::dout_pos(i, t) * cols + col] = v[i];
-    }
-    return;
-  }
-
-  if constexpr (OP == COL_FWD_PROJ_INV || OP == COL_PROJ_INV) {
-    double rnum = 0, rden = 0;
-#pragma unroll
-    for (int i = 0; i < L::E; i++) {
-      const int r = L::dout_pos(i, t);
-      const T Iv = ok ? I[(size_t)r * cols + col] : (T)NAN;
-      if (pa.rf && !isnan(Iv)) {
-        const T s = Iv > (T)0 ? dsqrt(Iv) : (T)0;
-        rnum += (double)fabs(dhypot(v[i].x, v[i].y) - s);
-        rden += (double)s;
-      }
-      if (ok) v[i] = project(v[i], Iv, pa);
-    }
-    if (pa.rf) {
-      __shared__ double sh[16];
-      block_atomic_sum(rnum, pa.rf, sh);
-      block_atomic_sum(rden, pa.rf + 1, sh);
-    }
-  }
-  if constexpr (OP == COL_FWD_GAUSS_INV || OP == COL_GAUSS_INV) {
-    // applyGaussMult(isFreq=1), cudaConfig.cu:380-395, wrapped coordinates
-    const int rows = N;
-    const T yrel = (T)(col >= cols / 2 ? col - cols : col);
-#pragma unroll
-    for (int i = 0; i < L::E; i++) {
-      const int r = L::dout_pos(i, t);
-      const T xrel = (T)(r >= rows / 2 ? r - rows : r);
-      const T f = (T)exp(-(xrel * xrel + yrel * yrel) / ((T)2 * gsig * gsig));
-      v[i].x *= f;
-      v[i].y *= f;
-    }
-  }
-
-  __syncthreads();  // LDS region reuse
-  L::inverse(v, my, twi, t);
-  if (ok) {
-#pragma unroll
-    for (int i = 0; i < L::E; i++) out[(size_t)L::din_pos(i, t) * cols + col] = v[i];
-  }
-}
-
-// ---------------------------------------------------------------------------
-// row pass
-// ---------------------------------------------------------------------------
-template <class T, int N, int OP>
-__global__ void __launch_bounds__((row_lines<T, N>() * LineCfg<N>::TL))
-k_row(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out, int rows, const cplx<T>* __restrict__ tw,
-      cplx<T>* __restrict__ g, const T* __restrict__ spt, UpdArgs<T> ua, T scale, T* __restrict__ sout,
-      ReduceSlots rs) {
-  constexpr int P = LineCfg<N>::P, Q = LineCfg<N>::Q, TL = LineCfg<N>::TL;
-  constexpr int R = row_lines<T, N>();
-  constexpr int PITCH = lds_pitch<T, N>();
-  using L = Line<T, P, Q, TL>;
-  extern __shared__ __align__(16) unsigned char smem[];
-  cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem);
-  const int t = threadIdx.x % TL;
-  const int rr = threadIdx.x / TL;
-  const int row = blockIdx.x * R + rr;
-  const bool ok = row < rows;
-  cplx<T>* my = lds + rr * PITCH;
-  const cplx<T>* twf = tw;
-  const cplx<T>* twi = tw + N;
-  const size_t base = (size_t)row * N;
-  cplx<T> v[L::E];
-
-  if constexpr (OP == ROW_FWD) {
-#pragma unroll
-    for (int i = 0; i < L::E; i++) v[i] = ok ? in[base + L::din_pos(i, t)] : cplx<T>{(T)0, (T)0};
-    L::forward(v, my, twf, t);
-    if (ok) {
-#pragma unroll
-      for (int i = 0; i < L::E; i++) out[base + L::dout_pos(i, t)] = v[i];
-    }
-    return;
-  } else {
-#pragma unroll
-    for (int i = 0; i < L::E; i++) v[i] = ok ? in[base + L::dout_pos(i, t)] : cplx<T>{(T)0, (T)0};
-    L::inverse(v, my, twi, t);
-  }
-
-  if constexpr (OP == ROW_INV) {
-    if (ok) {
-#pragma unroll
-      for (int i = 0; i < L::E; i++) out[base + L::din_pos(i, t)] = {v[i].x * scale, v[i].y * scale};
-    }
-    return;
-  }
-
-  if constexpr (OP == ROW_INV_MOD2) {
-    T m = 0;
-    double s = 0;
-#pragma unroll
-    for (int i = 0; i < L::E; i++) {
-      const T a = v[i].x * v[i].x + v[i].y * v[i].y;
-      if (ok) sout[base + L::din_pos(i, t)] = a;
-      if (ok) {
-        m = a > m ? a : m;
-        s += (double)a;
-      }
-    }
-    __shared__ double sh[16];
-    block_atomic_max(m, rs.maxbits, sh);
-    block_partial_sum(s, rs, sh);
-    return;
-  }
-
-  if constexpr (OP == ROW_INV_UPDATE_FWD) {
-    const T thr = (T)ua.scal[0];
-#pragma unroll
-    for (int i = 0; i < L::E; i++) {
-      const size_t a = base + L::din_pos(i, t);
-      const cplx<T> p = {v[i].x * ua.invN, v[i].y * ua.invN};
-      const cplx<T> go = ok ? g[a] : cplx<T>{(T)0, (T)0};
-      const T sv = ok ? spt[a] : (T)0;
-      v[i] = support_update(go, p, sv > thr, ua.algo, ua.beta, ua.relax);
-      if (ok) g[a] = v[i];
-    }
-    __syncthreads();  // LDS region reuse
-    L::forward(v, my, twf, t);
-    if (ok) {
-#pragma unroll
-      for (int i = 0; i < L::E; i++) out[base + L::dout_pos(i, t)] = v[i];
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// launchers
-// ---------------------------------------------------------------------------
-// raise the dynamic-LDS limit of a kernel above 64 KiB (once per kernel)
-template <class K>
-static void ensure_lds(K kern, size_t bytes) {
-  static size_t done = 0;
-  if (bytes > 64 * 1024 && bytes > done) {
-    (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    done = bytes;
-  }
-}
-
-template <class T, int N, int OP>
-static hipError_t col_launch_n(int cols, const void* in, void* out, const void* tw, const T* I,
-                               const ProjArgs<T>& pa, double gs, hipStream_t st) {
-  constexpr int C = col_lines<T, N>();
-  constexpr int TL = LineCfg<N>::TL;
-  const size_t lds = (size_t)C * lds_pitch<T, N>() * sizeof(cplx<T>);
-  auto kern = k_col<T, N, OP>;
-  ensure_lds(kern, lds);
-  dim3 grid((cols + C - 1) / C), block(C * TL);
-  hipLaunchKernelGGL(kern, grid, block, lds, st, (const cplx<T>*)in, (cplx<T>*)out, cols, (const cplx<T>*)tw, I,
-                     pa, (T)gs);
-  return hipGetLastError();
-}
-
-template <class T, int OP>
-static hipError_t col_launch_op(int rows, int cols, const void* in, void* out, const void* tw, const T* I,
-                                const ProjArgs<T>& pa, double gs, hipStream_t st) {
-  switch (rows) {
-#define CASE(n) case n: return col_launch_n<T, n, OP>(cols, in, out, tw, I, pa, gs, st);
-    CASE(16) CASE(32) CASE(64) CASE(128) CASE(256) CASE(384) CASE(512) CASE(768) CASE(1024) CASE(2048)
-#undef CASE
-    default: return hipErrorInvalidValue;
-  }
-}
-
-template <class T>
-hipError_t launch_col(int op, int rows, int cols, const void* in, void* out, const void* tw, const T* I,
-                      const ProjArgs<T>& pa, double gs, hipStream_t st) {
-  switch (op) {
-    case COL_FWD: return col_launch_op<T, COL_FWD>(rows, cols, in, out, tw, I, pa, gs, st);
-    case COL_INV: return col_launch_op<T, COL_INV>(rows, cols, in, out, tw, I, pa, gs, st);
-    case COL_FWD_PROJ_INV: return col_launch_op<T, COL_FWD_PROJ_INV>(rows, cols, in, out, tw, I, pa, gs, st);
-    case COL_PROJ_INV: return col_launch_op<T, COL_PROJ_INV>(rows, cols, in, out, tw, I, pa, gs, st);
-    case COL_FWD_GAUSS_INV: return col_launch_op<T, COL_FWD_GAUSS_INV>(rows, cols, in, out, tw, I, pa, gs, st);
-    case COL_GAUSS_INV: return col_launch_op<T, COL_GAUSS_INV>(rows, cols, in, out, tw, I, pa, gs, st);
-    default: return hipErrorInvalidValue;
-  }
-}
-
-template <class T, int N, int OP>
-static hipError_t row_launch_n(int rows, const void* in, void* out, const void* tw, void* g, const T* spt,
-                               const UpdArgs<T>& ua, T scale, T* sout, const ReduceSlots& rs, hipStream_t st) {
-  constexpr int R = row_lines<T, N>();
-  constexpr int TL = LineCfg<N>::TL;
-  const size_t lds = (size_t)R * lds_pitch<T, N>() * sizeof(cplx<T>);
-  auto kern = k_row<T, N, OP>;
-  ensure_lds(kern, lds);
-  dim3 grid((rows + R - 1) / R), block(R * TL);
-  hipLaunchKernelGGL(kern, grid, block, lds, st, (const cplx<T>*)in, (cplx<T>*)out, rows, (const cplx<T>*)tw,
-                     (cplx<T>*)g, spt, ua, scale, sout, rs);
-  return hipGetLastError();
-}
-
-template <class T, int OP>
-static hipError_t row_launch_op(int rows, int cols, const void* in, void* out, const void* tw, void* g,
-                                const T* spt, const UpdArgs<T>& ua, T scale, T* sout, const ReduceSlots& rs,
-                                hipStream_t st) {
-  switch (cols) {
-#define CASE(n) case n: return row_launch_n<T, n, OP>(rows, in, out, tw, g, spt, ua, scale, sout, rs, st);
-    CASE(16) CASE(32) CASE(64) CASE(128) CASE(256) CASE(384) CASE(512) CASE(768) CASE(1024) CASE(2048)
-#undef CASE
-    default: return hipErrorInvalidValue;
-  }
-}
-
-template <class T>
-hipError_t launch_row(int op, int rows, int cols, const void* in, void* out, const void* tw, void* g,
-                      const T* spt, const UpdArgs<T>& ua, T scale, T* sout, const ReduceSlots& rs,
-                      hipStream_t st) {
-  switch (op) {
-    case ROW_FWD: return row_launch_op<T, ROW_FWD>(rows, cols, in, out, tw, g, spt, ua, scale, sout, rs, st);
-    case ROW_INV: return row_launch_op<T, ROW_INV>(rows, cols, in, out, tw, g, spt, ua, scale, sout, rs, st);
-    case ROW_INV_UPDATE_FWD:
-      return row_launch_op<T, ROW_INV_UPDATE_FWD>(rows, cols, in, out, tw, g, spt, ua, scale, sout, rs, st);
-    case ROW_INV_MOD2: return row_launch_op<T, ROW_INV_MOD2>(rows, cols, in, out, tw, g, spt, ua, scale, sout, rs, st);
-    default: return hipErrorInvalidValue;
-  }
-}
-
-template hipError_t launch_col<float>(int, int, int, const void*, void*, const void*, const float*,
-                                      const ProjArgs<float>&, double, hipStream_t);
-template hipError_t launch_col<double>(int, int, int, const void*, void*, const void*, const double*,
-                                       const ProjArgs<double>&, double, hipStream_t);
-template hipError_t launch_row<float>(int, int, int, const void*, void*, const void*, void*, const float*,
-                                      const UpdArgs<float>&, float, float*, const ReduceSlots&, hipStream_t);
-template hipError_t launch_row<double>(int, int, int, const void*, void*, const void*, void*, const double*,
-                                       const UpdArgs<double>&, double, double*, const ReduceSlots&, hipStream_t);
-
-// ---------------------------------------------------------------------------
-// elementwise kernels (grid-stride, 256 threads)
-// ---------------------------------------------------------------------------
-static inline dim3 ew_grid(int64_t n) {
-  int64_t b = (n + 255) / 256;
-  if (b > 65536) b = 65536;
-  if (b < 1) b = 1;
-  return dim3((unsigned)b);
-}
-#define EW_LOOP(n) for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (n); i += (int64_t)gridDim.x * blockDim.x)
 
 // beamstop folded into the intensities as a NaN sentinel (bs > 0.5)
 template <class T>
@@ -815,33 +314,15 @@ hipError_t launch_blur_real(int rows, int cols, const void* g, int cpx_in, T* tm
   size_t l2 = (size_t)((rows + 2 * r) * CW + 2 * r + 1) * sizeof(T);
   if (l2 > 160 * 1024) return hipErrorInvalidValue;
   auto kern = k_blur_cols<T>;
-  ensure_lds(kern, l2);
+  static size_t lds_done = 0;  // this instantiation's dynamic-LDS limit
+  if (l2 > 64 * 1024 && l2 > lds_done) {
+    e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l2);
+    if (e != hipSuccess) return e;
+    lds_done = l2;
+  }
   hipLaunchKernelGGL(kern, dim3((cols + CW - 1) / CW), dim3(256), l2, st, rows, cols, (const T*)tmp, spt, sigma, r,
                      rs);
   return hipGetLastError();
-}
-
-// ---------------------------------------------------------------------------
-// Philox4x64-10 (Salmon et al., SC'11) for random starts and the forward model
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void philox4x64(uint64_t c[4], uint64_t k0, uint64_t k1) {
-#pragma unroll
-  for (int r = 0; r < 10; r++) {
-    const uint64_t lo0 = 0xD2E7470EE14C6C93ULL * c[0], hi0 = __umul64hi(0xD2E7470EE14C6C93ULL, c[0]);
-    const uint64_t lo1 = 0xCA5A826395121157ULL * c[2], hi1 = __umul64hi(0xCA5A826395121157ULL, c[2]);
-    const uint64_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
-    c[0] = n0;
-    c[1] = lo1;
-    c[2] = n2;
-    c[3] = lo0;
-    k0 += 0x9E3779B97F4A7C15ULL;
-    k1 += 0xBB67AE8584CAA73BULL;
-  }
-}
-__device__ __forceinline__ double philox_uniform(uint64_t seed, uint64_t stream, uint64_t idx) {
-  uint64_t c[4] = {idx, 0, 0, 0};
-  philox4x64(c, seed, stream);
-  return (double)(c[0] >> 11) * 0x1.0p-53;
 }
 
 // Start state (prepareIter, cdi.cc:222-223): createWaveFront(sqrt(I)) then

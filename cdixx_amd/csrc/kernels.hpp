@@ -57,17 +57,23 @@ struct ReduceSlots {
 };
 constexpr int kMaxPartials = 4096;
 
-// twiddle tables for a line of length n: twf (n entries) then twi (n entries)
+// line lengths of the FFT and their factorisation N = P*Q
 int line_supported(int n);
 void line_factors(int n, int* P, int* Q);
+// lines per workgroup of a pass: valid (power of two, 64..256 threads, LDS
+// within 160 KiB) and the default choice; csz = bytes per complex element
+int tiling_valid(int n, size_t csz, int lines);
+int tiling_default(int n, size_t csz, int column_pass);
 
+// The passes; tw = twiddle table of the transformed line length (2n
+// entries); lines = lines per workgroup (0: default).
 template <class T>
 hipError_t launch_col(int op, int rows, int cols, const void* in, void* out, const void* tw,
-                      const T* I, const ProjArgs<T>& pa, double gauss_sigma, hipStream_t st);
+                      const T* I, const ProjArgs<T>& pa, double gauss_sigma, hipStream_t st, int lines = 0);
 template <class T>
 hipError_t launch_row(int op, int rows, int cols, const void* in, void* out, const void* tw,
                       void* g, const T* spt, const UpdArgs<T>& ua, T scale, T* sout,
-                      const ReduceSlots& rs, hipStream_t st);
+                      const ReduceSlots& rs, hipStream_t st, int lines = 0);
 
 // elementwise / small kernels
 template <class T> hipError_t launch_pack_intensity(int64_t n, const T* I, const T* bs, T* out, hipStream_t st);

@@ -187,6 +187,16 @@ int cdx_time_iterations(cdx_ctx* ctx, const cdx_params* p, int algo, void* F, vo
                         const void* bs, const void* spt, int n_iter, int use_graph, double* ms_total,
                         double* col_ms, double* row_ms);
 
+/* ---- tiling of the FFT passes (no counterpart in the reference, whose
+ * cuFFT plan chooses internally; init_fft, src/gpu/cudaConfig.cu:137-160) ----
+ * lines = columns (column pass) / rows (row pass) per workgroup; 0 = default. */
+int cdx_set_tiling(cdx_ctx* ctx, int col_lines, int row_lines);
+int cdx_get_tiling(cdx_ctx* ctx, int* col_lines, int* row_lines);
+/* Times every valid tiling of both passes (reps launches each, HIP events)
+ * and keeps the fastest.  report (optional) receives triples
+ * (pass 0=column/1=row, lines, microseconds per launch), up to cap doubles. */
+int cdx_autotune(cdx_ctx* ctx, int reps, double* report, int cap);
+
 /* ---- synthetic forward model (Philox4x64-10 streams) ---- */
 /* n x n object: smoothed-noise amplitude in [0.2,1] and phase in [-pi,pi]
  * inside a centred disk of the given radius.  Context must be n x n. */

@@ -23,8 +23,21 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC", "-Wall",
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter",
               f"-I{ROCM}/include", "-D__HIP_PLATFORM_AMD__"]
 
-HIP_SOURCES = ["passes.hip", "kernels.hip", "abi.hip", "batch.hip", "unet.hip"]
-HOST_SOURCES = ["config.cpp", "cdi.cpp", "imgio.cpp"]
+# (source, extra defines, object name); passes.hip builds once per precision
+# and pass direction so its template instantiations compile in parallel
+UNITS = [
+    ("passes.hip", ["-DCDX_PASS_T=float", "-DCDX_PASS_ROW=0", "-DCDX_PASS_HOST"], "passes_col_f32.o"),
+    ("passes.hip", ["-DCDX_PASS_T=double", "-DCDX_PASS_ROW=0"], "passes_col_f64.o"),
+    ("passes.hip", ["-DCDX_PASS_T=float", "-DCDX_PASS_ROW=1"], "passes_row_f32.o"),
+    ("passes.hip", ["-DCDX_PASS_T=double", "-DCDX_PASS_ROW=1"], "passes_row_f64.o"),
+    ("kernels.hip", [], "kernels.o"),
+    ("abi.hip", [], "abi.o"),
+    ("batch.hip", [], "batch.o"),
+    ("unet.hip", [], "unet.o"),
+    ("config.cpp", [], "config.o"),
+    ("cdi.cpp", [], "cdi.o"),
+    ("imgio.cpp", [], "imgio.o"),
+]
 
 
 def _run(cmd):
@@ -34,23 +47,24 @@ def _run(cmd):
     return r
 
 
-def _compile(src):
+def _compile(unit):
+    src, defs, objname = unit
     path = os.path.join(CSRC, src)
-    obj = os.path.join(OUT, src + ".o")
+    obj = os.path.join(OUT, objname)
     if src.endswith(".hip"):
-        cmd = [HIPCC] + HIP_FLAGS + ["-c", path, "-o", obj]
+        cmd = [HIPCC] + HIP_FLAGS + defs + ["-c", path, "-o", obj]
     else:
-        cmd = ["g++"] + HOST_FLAGS + ["-c", path, "-o", obj]
+        cmd = ["g++"] + HOST_FLAGS + defs + ["-c", path, "-o", obj]
     _run(cmd)
     return obj
 
 
 def build(verbose=True, jobs=None):
     os.makedirs(OUT, exist_ok=True)
-    srcs = [s for s in HIP_SOURCES + HOST_SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    units = [u for u in UNITS if os.path.exists(os.path.join(CSRC, u[0]))]
     jobs = jobs or min(8, os.cpu_count() or 4)
     with ThreadPoolExecutor(jobs) as ex:
-        objs = list(ex.map(_compile, srcs))
+        objs = list(ex.map(_compile, units))
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-Wl,-z,defs", "-o", tmp] + objs)
     os.replace(tmp, LIB)

@@ -70,28 +70,27 @@ struct cdx_ctx {
   int col_lines = 0, row_lines = 0;  // lines per workgroup of the passes (0: default)
 };
 
-// twiddle tables: twf[k1*Q+q] = W^(q*k1), twi[n1*P+k1] = W^(n1*k1), W = exp(-2 pi i/N)
+// twiddle table of a line length n: tw[m] = W_n^m = exp(-2 pi i m/n), m < n,
+// evaluated in double with exact integer reduction of the angle
 static int make_twiddles(int n, int prec, void** dptr) {
-  int P, Q;
-  line_factors(n, &P, &Q);
-  std::vector<double> re(2 * (size_t)n), im(2 * (size_t)n);
-  for (int k1 = 0; k1 < P; k1++)
-    for (int q = 0; q < Q; q++) {
-      long e = ((long)q * k1) % n;
-      double a = -2.0 * M_PI * (double)e / n;
-      re[(size_t)k1 * Q + q] = cos(a);
-      im[(size_t)k1 * Q + q] = sin(a);
+  std::vector<double> re((size_t)n), im((size_t)n);
+  for (int m = 0; m < n; m++) {
+    // reduce to the first octant so cos/sin see |angle| <= pi/4
+    const long e8 = 8L * m;
+    const int quad = (int)((e8 / n + 1) / 2);
+    const double r = -2.0 * M_PI * (double)(4L * m - (long)quad * n) / (4.0 * n);
+    const double c = cos(r), s = sin(r);
+    // multiply (c + i s) by (-i)^quad
+    switch (quad & 3) {
+      case 0: re[m] = c; im[m] = s; break;
+      case 1: re[m] = s; im[m] = -c; break;
+      case 2: re[m] = -c; im[m] = -s; break;
+      default: re[m] = -s; im[m] = c; break;
     }
-  for (int n1 = 0; n1 < Q; n1++)
-    for (int k1 = 0; k1 < P; k1++) {
-      long e = ((long)n1 * k1) % n;
-      double a = -2.0 * M_PI * (double)e / n;
-      re[n + (size_t)n1 * P + k1] = cos(a);
-      im[n + (size_t)n1 * P + k1] = sin(a);
-    }
+  }
   size_t es = prec == CDX_F64 ? 16 : 8;
-  std::vector<unsigned char> host(es * 2 * n);
-  for (size_t i = 0; i < 2 * (size_t)n; i++) {
+  std::vector<unsigned char> host(es * (size_t)n);
+  for (size_t i = 0; i < (size_t)n; i++) {
     if (prec == CDX_F64) {
       double v[2] = {re[i], im[i]};
       memcpy(&host[i * es], v, 16);

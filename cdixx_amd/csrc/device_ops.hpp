@@ -11,31 +11,36 @@ namespace cdx {
 // ---------------------------------------------------------------------------
 // line configurations
 // ---------------------------------------------------------------------------
+// lanes per line (TL) and Stockham radices (forward order) per line length;
+// every radix divides the E = N/TL points a lane holds.
 template <int N>
 struct LineCfg;
-#define CDX_LINE(n, p, q, tl)                         \
-  template <>                                         \
-  struct LineCfg<n> {                                 \
-    static constexpr int P = p, Q = q, TL = tl;       \
+#define CDX_LINE(n, tl, ...)                 \
+  template <>                                \
+  struct LineCfg<n> {                        \
+    static constexpr int TL = tl;            \
+    using RL = Radices<__VA_ARGS__>;         \
   };
 CDX_LINE(16, 4, 4, 4)
 CDX_LINE(32, 4, 8, 4)
 CDX_LINE(64, 8, 8, 8)
-CDX_LINE(128, 8, 16, 8)
+CDX_LINE(128, 16, 8, 8, 2)
 CDX_LINE(256, 16, 16, 16)
-CDX_LINE(384, 16, 24, 8)
-CDX_LINE(512, 16, 32, 16)
-CDX_LINE(768, 32, 24, 8)
-CDX_LINE(1024, 32, 32, 32)
-CDX_LINE(2048, 32, 64, 32)
+CDX_LINE(384, 16, 8, 8, 6)
+CDX_LINE(512, 32, 16, 16, 2)
+CDX_LINE(768, 32, 8, 8, 12)
+CDX_LINE(1024, 64, 16, 16, 4)
+CDX_LINE(2048, 128, 16, 16, 8)
 #undef CDX_LINE
 
-// LDS pitch per line: at least the transpose footprint, == 8 (mod 32)
+template <class T, int N>
+using LineOf = Line<T, N, LineCfg<N>::TL, typename LineCfg<N>::RL>;
+
+// LDS pitch per line: the exchange footprint rounded to == 8 (mod 32)
 // complex elements so neighbouring lines land on different banks.
 template <class T, int N>
 constexpr int lds_pitch() {
-  using L = Line<T, LineCfg<N>::P, LineCfg<N>::Q, LineCfg<N>::TL>;
-  int e = L::LDS_ELEMS;
+  int e = LineOf<T, N>::LDS_ELEMS;
   return ((e + 23) / 32) * 32 + 8;
 }
 

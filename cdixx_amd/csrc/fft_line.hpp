@@ -1,17 +1,19 @@
 // fft_line.hpp -- device building blocks of the 2D FFT used by the CDI loop.
 //
-// A line (one image row or one image column) of length N = P*Q is
-// transformed by TL lanes with one LDS transpose:
-//   stage 1: every lane runs P-point DFTs in registers on the samples it
-//            loaded straight from global memory, multiplies by W_N^(q*k1);
-//   transpose through LDS (row pitch Q+1 / P+1 keeps it bank-conflict free);
-//   stage 2: every lane runs Q-point DFTs in registers.
-// Register layouts (t = lane within the line):
-//   Din : v[s*P + p]  = x[Q*p + t + TL*s]          s < Q/TL, p < P
-//   Dout: v[s*Q + k2] = X[t + TL*s + P*k2]         s < P/TL, k2 < Q
-// forward() maps Din -> Dout, inverse() maps Dout -> Din, so a
-// forward/pointwise/inverse chain never leaves registers except for the
-// transposes, and global loads/stores are coalesced over t in both layouts.
+// A line (one image row or one image column) of N points is transformed by
+// TL lanes, E = N/TL points per lane, with a mixed-radix Stockham schedule:
+// every pass runs radix-R DFTs in registers (E/R per lane) and the passes
+// exchange data through LDS.  For a pass with radix R after passes whose
+// radices multiply to Ns, butterfly j in [0, N/R) reads x[j + r*N/R],
+// multiplies by W_{Ns*R}^{(j mod Ns)*r}, and writes its outputs to
+// y[(j/Ns)*Ns*R + (j mod Ns) + r*Ns]; the last pass leaves the result in
+// natural order.  Lane t owns butterflies j = t + TL*s, so
+//   Din  (input of the forward transform):   v[s*R0 + r] = x[t + TL*s + r*N/R0]
+//   Dout (output of the forward transform):  v[s*RL + r] = X[t + TL*s + r*N/RL]
+// with R0 / RL the first / last radix.  The inverse runs the radices in the
+// opposite order, so it maps Dout back to Din: a forward -> pointwise ->
+// inverse chain never reorders data outside the passes, and global loads and
+// stores in both layouts are contiguous over t.
 // The transforms are unnormalised: forward exp(-2 pi i nk/N), inverse
 // exp(+2 pi i nk/N), the convention of the cuFFT plan behind myFFT/myIFFT in
 // CDIxx (src/gpu/cudaConfig.cu:137-187).
@@ -41,8 +43,8 @@ __device__ __forceinline__ cplx<T> cmulc(cplx<T> a, cplx<T> b) {
 
 // ---------------------------------------------------------------------------
 // Compile-time roots of unity.  cos/sin(2 pi e / R) are evaluated by a
-// constexpr Taylor series after exact octant reduction on the integer e, so
-// every twiddle inside an in-register DFT is a literal.
+// constexpr Taylor series after exact quadrant reduction on the integer e,
+// so every twiddle inside an in-register DFT is a literal.
 // ---------------------------------------------------------------------------
 constexpr double kPi = 3.14159265358979323846264338327950288;
 constexpr double taylor_sin(double x) {
@@ -61,22 +63,17 @@ constexpr double taylor_cos(double x) {
   }
   return sum;
 }
-// cos and sin of 2*pi*e/R, reduced to |angle| <= pi/4 with exact symmetries
-// (8*e is compared against R, so no rounding enters the reduction).
 struct CS {
   double c, s;
 };
 constexpr CS root_cs(long e, long R) {
   e %= R;
   if (e < 0) e += R;
-  // angle = 2 pi e / R.  Work in units of R/8 (an octant).
-  long e8 = 8 * e;  // angle in octants = e8 / R
-  int oct = (int)(e8 / R);
-  // angle = 2 pi e / R; choose the nearest quadrant multiple m*pi/2
-  int quad = (oct + 1) / 2;  // 0..4
-  // residual angle r = 2 pi e / R - quad * pi/2 = 2 pi (4 e - quad R) / (4R)
-  double r = 2.0 * kPi * (double)(4 * e - quad * R) / (4.0 * (double)R);
-  double c = taylor_cos(r), s = taylor_sin(r);
+  const long e8 = 8 * e;  // angle in octants = e8 / R
+  const int oct = (int)(e8 / R);
+  const int quad = (oct + 1) / 2;  // nearest multiple of pi/2
+  const double r = 2.0 * kPi * (double)(4 * e - quad * R) / (4.0 * (double)R);  // |r| <= pi/4
+  const double c = taylor_cos(r), s = taylor_sin(r);
   switch (quad & 3) {
     case 0: return {c, s};
     case 1: return {-s, c};
@@ -92,17 +89,17 @@ __device__ __forceinline__ void static_for(F&& f) {
   }(std::make_integer_sequence<int, N>{});
 }
 
-// v <- v * W_R^(DIR*E) with W_R = exp(2 pi i / R); DIR=-1 forward.
+// v * W_R^(DIR*E) with W_R = exp(2 pi i / R); DIR = -1 forward
 template <int E, int R, int DIR, class T>
 __device__ __forceinline__ cplx<T> twiddle_const(cplx<T> v) {
   constexpr long e = ((long)E % R + R) % R;
   if constexpr (e == 0) {
     return v;
-  } else if constexpr (4 * e == R) {  // angle pi/2: multiply by DIR*i
+  } else if constexpr (4 * e == R) {
     return DIR < 0 ? cplx<T>{v.y, -v.x} : cplx<T>{-v.y, v.x};
   } else if constexpr (2 * e == R) {
     return {-v.x, -v.y};
-  } else if constexpr (4 * e == 3 * R) {  // angle 3pi/2
+  } else if constexpr (4 * e == 3 * R) {
     return DIR < 0 ? cplx<T>{-v.y, v.x} : cplx<T>{v.y, -v.x};
   } else {
     constexpr CS w = root_cs(e, R);
@@ -113,8 +110,7 @@ __device__ __forceinline__ cplx<T> twiddle_const(cplx<T> v) {
 }
 
 // ---------------------------------------------------------------------------
-// In-register DFT of size R (R = 2^a or 3*2^a, R <= 64), natural order in
-// and out, strided access v[i*STRIDE].
+// In-register DFT of size R (R = 2^a * 3^b), natural order in and out.
 // ---------------------------------------------------------------------------
 template <int R, int DIR, int STRIDE = 1, class T>
 __device__ __forceinline__ void dft(cplx<T>* v) {
@@ -129,7 +125,6 @@ __device__ __forceinline__ void dft(cplx<T>* v) {
     cplx<T> a = v[0], b = v[STRIDE], c = v[2 * STRIDE];
     cplx<T> s = cadd(b, c), d = csub(b, c);
     cplx<T> m = {a.x - (T)0.5 * s.x, a.y - (T)0.5 * s.y};
-    // DIR*i*h*d
     cplx<T> r = DIR < 0 ? cplx<T>{h * d.y, -h * d.x} : cplx<T>{-h * d.y, h * d.x};
     v[0] = cadd(a, s);
     v[STRIDE] = cadd(m, r);
@@ -137,7 +132,6 @@ __device__ __forceinline__ void dft(cplx<T>* v) {
   } else if constexpr (R == 4) {
     cplx<T> a = v[0], b = v[STRIDE], c = v[2 * STRIDE], d = v[3 * STRIDE];
     cplx<T> s0 = cadd(a, c), d0 = csub(a, c), s1 = cadd(b, d), d1 = csub(b, d);
-    // DIR*i*d1
     cplx<T> r = DIR < 0 ? cplx<T>{d1.y, -d1.x} : cplx<T>{-d1.y, d1.x};
     v[0] = cadd(s0, s1);
     v[2 * STRIDE] = csub(s0, s1);
@@ -145,11 +139,9 @@ __device__ __forceinline__ void dft(cplx<T>* v) {
     v[3 * STRIDE] = csub(d0, r);
   } else {
     // decimation in time: R = F*M, n = F*m + f, k = k1 + M*k2
-    constexpr int F = (R % 4 == 0 && R != 12 && R != 24 && R != 48) ? 4 : (R % 3 == 0 ? 3 : 2);
+    constexpr int F = (R % 4 == 0 && R % 3 != 0) ? 4 : (R % 3 == 0 ? 3 : 2);
     constexpr int M = R / F;
-    // sub-DFTs over m for each residue f: elements v[(F*m+f)*STRIDE]
     static_for<F>([&](auto f) { dft<M, DIR, STRIDE * F>(v + f.value * STRIDE); });
-    // now v[(F*k1 + f)*STRIDE] holds Y_f[k1]; combine
     cplx<T> out[R];
     static_for<M>([&](auto k1) {
       cplx<T> z[F];
@@ -164,71 +156,113 @@ __device__ __forceinline__ void dft(cplx<T>* v) {
 }
 
 // ---------------------------------------------------------------------------
-// Line transform.  lds points at this line's LDS region, which must hold
-// max(P*(Q+1), Q*(P+1)) elements.  twf[k1*Q+q] = W_N^(q*k1) and
-// twi[n1*P+k1] = W_N^(n1*k1) (forward sign), both read through L1.
-// The caller synchronises the workgroup between a transform and any reuse
-// of the LDS region.
+// Stockham line transform.
 // ---------------------------------------------------------------------------
-template <class T, int P, int Q, int TL>
+template <int... Rs>
+struct Radices {
+  static constexpr int M = sizeof...(Rs);
+  static constexpr int R[M] = {Rs...};
+};
+
+template <class T, int N, int TL, class RL>
 struct Line {
-  static constexpr int N = P * Q;
+  static constexpr int M = RL::M;
   static constexpr int E = N / TL;
-  static constexpr int SQ = Q / TL;  // stage-1 sub-DFTs per lane (forward)
-  static constexpr int SP = P / TL;  // stage-2 sub-DFTs per lane (forward)
-  static constexpr int LDQ = Q + 1;
-  static constexpr int LDP = P + 1;
-  static constexpr int LDS_ELEMS = (P * LDQ > Q * LDP) ? P * LDQ : Q * LDP;
-  static_assert(Q % TL == 0 && P % TL == 0, "TL must divide P and Q");
+  static constexpr int R0 = RL::R[0];
+  static constexpr int RLAST = RL::R[M - 1];
+  // one pad element per 32 keeps strided pass outputs off a single bank
+  static constexpr int LDS_ELEMS = N + N / 32 + 1;
+  static_assert(TL * E == N, "TL must divide N");
 
-  // Din -> Dout
-  __device__ __forceinline__ static void forward(cplx<T>* v, cplx<T>* lds, const cplx<T>* __restrict__ twf,
-                                                 int t) {
-    static_for<SQ>([&](auto s) {
-      dft<P, -1>(v + s.value * P);
-      const int q = t + TL * s.value;
-      static_for<P - 1>([&](auto k) {
-        constexpr int k1 = k.value + 1;
-        v[s.value * P + k1] = cmul(v[s.value * P + k1], twf[k1 * Q + q]);
+  __device__ __forceinline__ static int swz(int p) { return p + (p >> 5); }
+
+  // radix of pass p in direction DIR (forward runs the list, inverse reverses it)
+  template <int DIR>
+  static constexpr int radix(int p) {
+    return DIR < 0 ? RL::R[p] : RL::R[M - 1 - p];
+  }
+  template <int DIR>
+  static constexpr int ns(int p) {
+    int s = 1;
+    for (int q = 0; q < p; q++) s *= radix<DIR>(q);
+    return s;
+  }
+
+  // v[r] *= W_N^(DIR*r*b) for r = 1..R-1, from table loads of W_N^(2^k b)
+  template <int R, int DIR>
+  __device__ __forceinline__ static void twiddle(cplx<T>* v, const cplx<T>* __restrict__ tw, int b) {
+    if constexpr ((R & (R - 1)) == 0) {
+      cplx<T> w[R];
+      static_for<R>([&](auto r) {
+        constexpr int rr = r.value;
+        if constexpr (rr > 0 && (rr & (rr - 1)) == 0) w[rr] = tw[rr * b];
       });
-      static_for<P>([&](auto k1) { lds[k1.value * LDQ + q] = v[s.value * P + k1.value]; });
-    });
-    __syncthreads();
-    static_for<SP>([&](auto s) {
-      const int k1 = t + TL * s.value;
-      static_for<Q>([&](auto q) { v[s.value * Q + q.value] = lds[k1 * LDQ + q.value]; });
-      dft<Q, -1>(v + s.value * Q);
-    });
-  }
-
-  // Dout -> Din
-  __device__ __forceinline__ static void inverse(cplx<T>* v, cplx<T>* lds, const cplx<T>* __restrict__ twi,
-                                                 int t) {
-    static_for<SP>([&](auto s) {
-      dft<Q, +1>(v + s.value * Q);
-      const int k1 = t + TL * s.value;
-      static_for<Q - 1>([&](auto n) {
-        constexpr int n1 = n.value + 1;
-        v[s.value * Q + n1] = cmulc(v[s.value * Q + n1], twi[n1 * P + k1]);
+      static_for<R>([&](auto r) {
+        constexpr int rr = r.value;
+        if constexpr (rr > 0 && (rr & (rr - 1)) != 0) {
+          constexpr int hi = 1 << (31 - __builtin_clz(rr));
+          w[rr] = cmul(w[hi], w[rr - hi]);
+        }
       });
-      static_for<Q>([&](auto n1) { lds[n1.value * LDP + k1] = v[s.value * Q + n1.value]; });
-    });
-    __syncthreads();
-    static_for<SQ>([&](auto s) {
-      const int n1 = t + TL * s.value;
-      static_for<P>([&](auto k1) { v[s.value * P + k1.value] = lds[n1 * LDP + k1.value]; });
-      dft<P, +1>(v + s.value * P);
-    });
+      static_for<R - 1>([&](auto r) {
+        constexpr int rr = r.value + 1;
+        v[rr] = DIR < 0 ? cmul(v[rr], w[rr]) : cmulc(v[rr], w[rr]);
+      });
+    } else {
+      static_for<R - 1>([&](auto r) {
+        constexpr int rr = r.value + 1;
+        const cplx<T> w = tw[rr * b];
+        v[rr] = DIR < 0 ? cmul(v[rr], w) : cmulc(v[rr], w);
+      });
+    }
   }
 
-  // global offsets (in elements along the line) of register i in each layout
-  __device__ __forceinline__ static int din_pos(int i, int t) {
-    int s = i / P, p = i % P;
-    return Q * p + t + TL * s;
+  // DIR = -1: Din -> Dout (forward); DIR = +1: Dout -> Din (inverse).
+  // lds: this line's region (LDS_ELEMS).  tw[m] = W_N^m (forward sign).
+  // Ends without a barrier; the caller synchronises before reusing lds.
+  template <int DIR>
+  __device__ __forceinline__ static void run(cplx<T>* v, cplx<T>* lds, const cplx<T>* __restrict__ tw, int t) {
+    static_for<M>([&](auto pi) {
+      constexpr int p = pi.value;
+      constexpr int R = radix<DIR>(p);
+      constexpr int Ns = ns<DIR>(p);
+      constexpr int S = E / R;
+      static_assert(E % R == 0, "every radix must divide the points per lane");
+      static_for<S>([&](auto s) {
+        const int j = t + TL * s.value;
+        if constexpr (Ns > 1) {
+          constexpr int stride = N / (Ns * R);
+          twiddle<R, DIR>(v + s.value * R, tw, (j % Ns) * stride);
+        }
+        dft<R, DIR>(v + s.value * R);
+      });
+      if constexpr (p < M - 1) {
+        static_for<S>([&](auto s) {
+          const int j = t + TL * s.value;
+          const int base = (j / Ns) * Ns * R + (j % Ns);
+          static_for<R>([&](auto r) { lds[swz(base + r.value * Ns)] = v[s.value * R + r.value]; });
+        });
+        __syncthreads();
+        constexpr int R2 = radix<DIR>(p + 1);
+        static_for<E / R2>([&](auto s) {
+          const int j = t + TL * s.value;
+          static_for<R2>([&](auto r) { v[s.value * R2 + r.value] = lds[swz(j + r.value * (N / R2))]; });
+        });
+        __syncthreads();
+      }
+    });
   }
+  __device__ __forceinline__ static void forward(cplx<T>* v, cplx<T>* lds, const cplx<T>* __restrict__ tw, int t) {
+    run<-1>(v, lds, tw, t);
+  }
+  __device__ __forceinline__ static void inverse(cplx<T>* v, cplx<T>* lds, const cplx<T>* __restrict__ tw, int t) {
+    run<+1>(v, lds, tw, t);
+  }
+
+  // element index along the line of register i in each layout
+  __device__ __forceinline__ static int din_pos(int i, int t) { return t + TL * (i / R0) + (i % R0) * (N / R0); }
   __device__ __forceinline__ static int dout_pos(int i, int t) {
-    int s = i / Q, k2 = i % Q;
-    return t + TL * s + P * k2;
+    return t + TL * (i / RLAST) + (i % RLAST) * (N / RLAST);
   }
 };
 

@@ -57,9 +57,8 @@ struct ReduceSlots {
 };
 constexpr int kMaxPartials = 4096;
 
-// line lengths of the FFT and their factorisation N = P*Q
+// line lengths of the FFT
 int line_supported(int n);
-void line_factors(int n, int* P, int* Q);
 // lines per workgroup of a pass: valid (power of two, 64..256 threads, LDS
 // within 160 KiB) and the default choice; csz = bytes per complex element
 int tiling_valid(int n, size_t csz, int lines);

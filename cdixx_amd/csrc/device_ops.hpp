@@ -195,6 +195,12 @@ inline dim3 ew_grid(int64_t n) {
   if (b < 1) b = 1;
   return dim3((unsigned)b);
 }
+// grids of kernels that end in one atomic per workgroup: one workgroup per CU
+inline dim3 red_grid(int64_t n) {
+  dim3 g = ew_grid(n);
+  if (g.x > 256) g.x = 256;
+  return g;
+}
 #define EW_LOOP(n) for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (n); i += (int64_t)gridDim.x * blockDim.x)
 
 // ---------------------------------------------------------------------------

@@ -379,7 +379,7 @@ __global__ void k_residual(int64_t n, const cplx<T>* F, const T* I, const T* bs,
 }
 template <class T>
 hipError_t launch_residual(int64_t n, const void* F, const T* I, const T* bs, double* acc, hipStream_t st) {
-  hipLaunchKernelGGL(k_residual<T>, ew_grid(n), dim3(256), 0, st, n, (const cplx<T>*)F, I, bs, acc);
+  hipLaunchKernelGGL(k_residual<T>, red_grid(n), dim3(256), 0, st, n, (const cplx<T>*)F, I, bs, acc);
   return hipGetLastError();
 }
 
@@ -400,7 +400,7 @@ __global__ void k_rfactor(int64_t n, const cplx<T>* F, const T* Ipk, double* acc
 }
 template <class T>
 hipError_t launch_rfactor(int64_t n, const void* F, const T* Ipk, double* acc, hipStream_t st) {
-  hipLaunchKernelGGL(k_rfactor<T>, ew_grid(n), dim3(256), 0, st, n, (const cplx<T>*)F, Ipk, acc);
+  hipLaunchKernelGGL(k_rfactor<T>, red_grid(n), dim3(256), 0, st, n, (const cplx<T>*)F, Ipk, acc);
   return hipGetLastError();
 }
 
@@ -478,7 +478,7 @@ __global__ void k_minmax(int64_t n, const double* x, double* mm) {
   }
 }
 hipError_t launch_minmax(int64_t n, const double* x, double* mm, hipStream_t st) {
-  hipLaunchKernelGGL(k_minmax, ew_grid(n), dim3(256), 0, st, n, x, mm);
+  hipLaunchKernelGGL(k_minmax, red_grid(n), dim3(256), 0, st, n, x, mm);
   return hipGetLastError();
 }
 __device__ __forceinline__ double unkey(unsigned long long k) {
@@ -521,7 +521,7 @@ __global__ void k_sum_mod2(int64_t n, const cplx<T>* F, double* acc) {
 }
 template <class T>
 hipError_t launch_sum_mod2(int64_t n, const void* F, double* acc, hipStream_t st) {
-  hipLaunchKernelGGL(k_sum_mod2<T>, ew_grid(n), dim3(256), 0, st, n, (const cplx<T>*)F, acc);
+  hipLaunchKernelGGL(k_sum_mod2<T>, red_grid(n), dim3(256), 0, st, n, (const cplx<T>*)F, acc);
   return hipGetLastError();
 }
 

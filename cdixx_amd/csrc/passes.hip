@@ -105,6 +105,15 @@ __device__ __forceinline__ cplx<T> project_m(cplx<T> f, T I, const ProjArgs<T>& 
   else return project(f, I, pa);  // runtime mode
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  Column tiles are narrower than a cache line (8 complex64 = 64 B), so
+// the tiles sharing a line must run on the same XCD: give each XCD a
+// contiguous run of tiles.
+__device__ __forceinline__ int xcd_tile(int bid, int nb) {
+  if (nb & 7) return bid;
+  return (bid & 7) * (nb >> 3) + (bid >> 3);
+}
+
 // ---------------------------------------------------------------------------
 // column pass.  MODE/RF = -1: projection mode and R-factor chosen at run time
 // ---------------------------------------------------------------------------
@@ -119,7 +128,7 @@ k_col(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out, int cols, const
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem);
   const int c = threadIdx.x & ((1 << lsh) - 1);
   const int t = threadIdx.x >> lsh;
-  const int col = (blockIdx.x << lsh) + c;
+  const int col = (xcd_tile(blockIdx.x, gridDim.x) << lsh) + c;
   const bool ok = col < cols;
   cplx<T>* my = lds + c * PITCH;
   constexpr bool from_f = (OP == COL_INV || OP == COL_PROJ_INV || OP == COL_GAUSS_INV);

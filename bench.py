@@ -82,6 +82,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--copy-calib-mb", type=int, default=0,
+                    help="run a plain copy kernel of this many MiB first (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -89,6 +91,14 @@ def main():
         sys.exit(2)
 
     import cdixx_amd as cdx
+    if args.copy_calib_mb:
+        nb = args.copy_calib_mb << 20
+        src, dst = cdx.DeviceBuffer(nb), cdx.DeviceBuffer(nb)
+        src.zero()
+        for _ in range(3):
+            cdx.check(cdx.lib.cdx_copy(None, dst.ptr, src.ptr, nb))
+        src.free()
+        dst.free()
     ctx = cdx.Context(N, precision=cdx.F32)
     ctx.set_threshold(0.5)
     d = make_inputs(cdx, ctx)

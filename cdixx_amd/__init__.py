@@ -92,6 +92,7 @@ _SIGS = {
     "cdx_sim_object": ([_vp, _d, _u64, _d, _vp], _i),
     "cdx_sim_intensity": ([_vp, _vp, _d, _u64, _i, _vp], _i),
     "cdx_read_config_json": ([ctypes.c_char_p, ctypes.c_char_p, _i], _i),
+    "cdx_copy": ([_vp, _vp, _vp, _sz], _i),
     "cdx_set_tiling": ([_vp, _i, _i], _i), "cdx_get_tiling": ([_vp, _ip, _ip], _i),
     "cdx_autotune": ([_vp, _i, _dp, _i], _i),
     "cdx_convert_fo_phase": ([_vp, _vp, _d], _i), "cdx_mask_multiply": ([_vp, _vp, _vp, _i], _i),

@@ -32,6 +32,7 @@ UNITS = [
     ("passes.hip", ["-DCDX_PASS_T=double", "-DCDX_PASS_ROW=1"], "passes_row_f64.o"),
     ("kernels.hip", [], "kernels.o"),
     ("abi.hip", [], "abi.o"),
+    ("abi_extra.hip", [], "abi_extra.o"),
     ("batch.hip", [], "batch.o"),
     ("unet.hip", [], "unet.o"),
     ("config.cpp", [], "config.o"),

@@ -355,6 +355,20 @@ hipError_t launch_start(int64_t n, const T* I, const T* bs, double thr, uint64_t
   return hipGetLastError();
 }
 
+// plain device copy in 16-byte words (tail bytes by the last thread)
+__global__ void k_copy(int64_t n16, const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t tail,
+                       const unsigned char* __restrict__ s8, unsigned char* __restrict__ d8) {
+  EW_LOOP(n16) dst[i] = src[i];
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    for (int64_t k = 0; k < tail; k++) d8[n16 * 16 + k] = s8[n16 * 16 + k];
+}
+hipError_t launch_copy(size_t bytes, const void* src, void* dst, hipStream_t st) {
+  const int64_t n16 = (int64_t)(bytes / 16);
+  hipLaunchKernelGGL(k_copy, ew_grid(n16 > 0 ? n16 : 1), dim3(256), 0, st, n16, (const uint4*)src, (uint4*)dst,
+                     (int64_t)(bytes % 16), (const unsigned char*)src, (unsigned char*)dst);
+  return hipGetLastError();
+}
+
 __global__ void k_fill_f64(int64_t n, double* p, double v) {
   EW_LOOP(n) p[i] = v;
 }

@@ -100,6 +100,7 @@ template <class T> hipError_t launch_blur_real(int rows, int cols, const void* g
 template <class T> hipError_t launch_start(int64_t n, const T* I, const T* bs, double thr, uint64_t seed,
                                            int mode, void* F, hipStream_t st);
 hipError_t launch_fill_f64(int64_t n, double* p, double v, hipStream_t st);
+hipError_t launch_copy(size_t bytes, const void* src, void* dst, hipStream_t st);
 template <class T> hipError_t launch_residual(int64_t n, const void* F, const T* I, const T* bs, double* acc,
                                               hipStream_t st);
 template <class T> hipError_t launch_rfactor(int64_t n, const void* F, const T* Ipk, double* acc,

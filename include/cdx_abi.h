@@ -58,6 +58,8 @@ int cdx_memcpy_d2h(void* dst, const void* src, size_t bytes);
 int cdx_memcpy_d2d(void* dst, const void* src, size_t bytes);
 int cdx_memset0(void* ptr, size_t bytes);
 int cdx_device_synchronize(void);
+/* myMemcpyD2D as a kernel on `stream` (NULL: legacy stream); blocking */
+int cdx_copy(void* stream, void* dst, const void* src, size_t bytes);
 
 /* ---- context: replaces resize_cuda_image + init_fft + init_cuda_image ----
  * include/cudaConfig.hpp:22-23,45; src/gpu/cudaConfig.cu:69-74,137-160;

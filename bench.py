@@ -74,6 +74,55 @@ def cpu_baseline(seconds=15.0):
                 sample=f"{n} RAAR iterations at 1024x1024 complex64 (FP32 oracle, OpenMP) in {el:.1f} s")
 
 
+BATCH_SIZES = [256] * 24 + [384] * 24 + [512] * 16
+
+
+def batched_leg(cdx, iters):
+    """64 frames (24x256^2, 24x384^2, 16x512^2), support radius 0.16 N,
+    photons log-uniform in [1e6, 1e8], per-frame seeds: `iters` HIO
+    iterations for the whole batch in one captured graph, against the same
+    frames run one after another through single-frame contexts."""
+    rs = np.random.default_rng(3)
+    photons = 10 ** rs.uniform(6, 8, len(BATCH_SIZES))
+    radii = [0.16 * n for n in BATCH_SIZES]
+    seeds = list(range(len(BATCH_SIZES)))
+    b = cdx.Batch(BATCH_SIZES)
+    b.simulate(radii, photons, seeds, support_scale=1.25)
+    p = cdx.Params.default(beta=0.9)
+    b.set_threshold(0.5)
+    b.time_iterations(p, cdx.HIO, 20)  # warm
+    ms_batch = b.time_iterations(p, cdx.HIO, iters)
+    # per-frame loop: the same frames through single-frame contexts
+    frames = [{k: b.get(i, k) for k in ("F", "g", "I", "spt")} for i in range(len(BATCH_SIZES))]
+    ctxs = {n: cdx.Context(n) for n in set(BATCH_SIZES)}
+    bufs = []
+    for i, n in enumerate(BATCH_SIZES):
+        c = ctxs[n]
+        bufs.append((c, c.cplx(frames[i]["F"]), c.cplx(frames[i]["g"]), c.real(frames[i]["I"]),
+                     c.real(frames[i]["spt"])))
+    for c, F, g, I, s in bufs[:3]:
+        c.time_iterations(p, cdx.HIO, F, g, I, s, 20, per_kernel=False)  # warm
+    ms_loop = 0.0
+    for c, F, g, I, s in bufs:
+        c.sync()
+        t0 = time.perf_counter()
+        c.time_iterations(p, cdx.HIO, F, g, I, s, iters, per_kernel=False)
+        c.sync()
+        ms_loop += (time.perf_counter() - t0) * 1e3
+    pix = sum(n * n for n in BATCH_SIZES)
+    fps_b = len(BATCH_SIZES) / (ms_batch * 1e-3) * 1.0
+    fps_l = len(BATCH_SIZES) / (ms_loop * 1e-3)
+    for c in ctxs.values():
+        c.close()
+    b.close()
+    return {"frames": len(BATCH_SIZES), "sizes": "24x256^2 + 24x384^2 + 16x512^2", "iterations": iters,
+            "batched_ms": round(ms_batch, 3), "per_frame_loop_ms": round(ms_loop, 3),
+            "frames_per_s_batched": round(fps_b, 2), "frames_per_s_loop": round(fps_l, 2),
+            "speedup": round(ms_loop / ms_batch, 2),
+            "batched_us_per_iteration": round(1e3 * ms_batch / iters, 2),
+            "effective_GBps": round(80 * pix * iters / (ms_batch * 1e-3) / 1e9, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,6 +131,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-iters", type=int, default=500)
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="run a plain copy kernel of this many MiB first (counter calibration)")
     args = ap.parse_args()
@@ -155,6 +206,8 @@ def main():
         "end_to_end": {"schedule": "50*(20*RAAR+1*shrinkWrap)", "iterations": nit, "seconds": round(e2e, 4),
                        "final_R": float(rt[-1]) if rt is not None and len(rt) else None},
     }
+    if not args.no_batch:
+        out["batched"] = batched_leg(cdx, args.batch_iters)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -93,6 +93,12 @@ _SIGS = {
     "cdx_sim_intensity": ([_vp, _vp, _d, _u64, _i, _vp], _i),
     "cdx_read_config_json": ([ctypes.c_char_p, ctypes.c_char_p, _i], _i),
     "cdx_copy": ([_vp, _vp, _vp, _sz], _i),
+    "cdx_batch_create": ([ctypes.POINTER(_vp), _i, _ip, _i, _vp], _i), "cdx_batch_destroy": ([_vp], _i),
+    "cdx_batch_frame": ([_vp, _i] + [ctypes.POINTER(_vp)] * 5 + [_ip], _i),
+    "cdx_batch_context": ([_vp, _i], _vp), "cdx_batch_set_threshold": ([_vp, _d], _i),
+    "cdx_batch_get_thresholds": ([_vp, _dp], _i), "cdx_batch_synchronize": ([_vp], _i),
+    "cdx_batch_phase_retrieve": ([_vp, ctypes.c_char_p, ctypes.POINTER(Params), _dp, _i, _ip], _i),
+    "cdx_batch_time_iterations": ([_vp, ctypes.POINTER(Params), _i, _i, _i, _dp], _i),
     "cdx_set_tiling": ([_vp, _i, _i], _i), "cdx_get_tiling": ([_vp, _ip, _ip], _i),
     "cdx_autotune": ([_vp, _i, _dp, _i], _i),
     "cdx_convert_fo_phase": ([_vp, _vp, _d], _i), "cdx_mask_multiply": ([_vp, _vp, _vp, _i], _i),
@@ -356,3 +362,126 @@ class Context:
 
     def __exit__(self, *a):
         self.close()
+
+
+class DeviceView:
+    """A slice of memory owned elsewhere (e.g. a frame of a batch)."""
+
+    def __init__(self, ptr, nbytes):
+        self.ptr, self.nbytes = ptr, nbytes
+
+    upload = DeviceBuffer.upload
+    download = DeviceBuffer.download
+
+    def zero(self):
+        check(lib.cdx_memset0(self.ptr, self.nbytes))
+
+
+class _Borrowed(Context):
+    """A context owned by a batch (not destroyed here)."""
+
+    def __init__(self, handle, n, precision):
+        self.rows = self.cols = n
+        self.precision = precision
+        self.rdtype = np.float64 if precision == F64 else np.float32
+        self.cdtype = np.complex128 if precision == F64 else np.complex64
+        self.h = handle
+
+    def close(self):
+        self.h = None
+
+
+class Batch:
+    """Many frames of sizes 128/256/384/512 advanced together (cdx_batch_*)."""
+
+    def __init__(self, sizes, precision=F32):
+        self.sizes = [int(s) for s in sizes]
+        self.precision = precision
+        arr = (ctypes.c_int * len(self.sizes))(*self.sizes)
+        h = _vp()
+        check(lib.cdx_batch_create(ctypes.byref(h), len(self.sizes), arr, precision, None))
+        self.h = h.value
+        self.rdtype = np.float64 if precision == F64 else np.float32
+        self.cdtype = np.complex128 if precision == F64 else np.complex64
+
+    def __len__(self):
+        return len(self.sizes)
+
+    def frame(self, i):
+        ptrs = [_vp() for _ in range(5)]
+        n = ctypes.c_int()
+        check(lib.cdx_batch_frame(self.h, i, *[ctypes.byref(p) for p in ptrs], ctypes.byref(n)))
+        n = n.value
+        cb, rb = n * n * np.dtype(self.cdtype).itemsize, n * n * np.dtype(self.rdtype).itemsize
+        F, g, I, bs, spt = (DeviceView(p.value, b) for p, b in zip(ptrs, (cb, cb, rb, rb, rb)))
+        return dict(F=F, g=g, I=I, bs=bs, spt=spt, n=n)
+
+    def context(self, i):
+        return _Borrowed(lib.cdx_batch_context(self.h, i), self.sizes[i], self.precision)
+
+    def set_threshold(self, t):
+        check(lib.cdx_batch_set_threshold(self.h, t))
+
+    def thresholds(self):
+        out = (ctypes.c_double * len(self.sizes))()
+        check(lib.cdx_batch_get_thresholds(self.h, out))
+        return np.array(out[:])
+
+    def sync(self):
+        check(lib.cdx_batch_synchronize(self.h))
+
+    def get(self, i, name):
+        self.sync()
+        f = self.frame(i)
+        dt = self.cdtype if name in ("F", "g") else self.rdtype
+        return f[name].download(dt, (f["n"], f["n"]))
+
+    def put(self, i, name, arr):
+        f = self.frame(i)
+        dt = self.cdtype if name in ("F", "g") else self.rdtype
+        f[name].upload(np.asarray(arr, dtype=dt))
+
+    def phase_retrieve(self, algorithm, params, rtrace_cap=0):
+        nf = len(self.sizes)
+        rt = (ctypes.c_double * max(nf * rtrace_cap, 1))()
+        n_iter = ctypes.c_int()
+        check(lib.cdx_batch_phase_retrieve(self.h, algorithm.encode(), ctypes.byref(params),
+                                           rt if rtrace_cap else None, rtrace_cap, ctypes.byref(n_iter)))
+        trace = np.array(rt[:nf * rtrace_cap]).reshape(nf, rtrace_cap) if rtrace_cap else None
+        return n_iter.value, trace
+
+    def time_iterations(self, params, algo, n, use_graph=True):
+        ms = ctypes.c_double()
+        check(lib.cdx_batch_time_iterations(self.h, ctypes.byref(params), algo, n, int(use_graph),
+                                            ctypes.byref(ms)))
+        return ms.value
+
+    def simulate(self, radii, photons, seeds, bs_radius=0, support_scale=1.0, start_mode=1, poisson=True):
+        """Fills every frame: seeded object of the given radius, Poisson
+        intensities, beamstop, disk support of radius support_scale*radius,
+        random start F and g = IFFT(F)/n (as prepareIter leaves objectWave)."""
+        for i in range(len(self.sizes)):
+            f = self.frame(i)
+            ctx = self.context(i)
+            ctx.sim_object(f["F"], radii[i], seed=int(seeds[i]))
+            ctx.sim_intensity(f["F"], f["I"], photons[i], seed=int(seeds[i]), poisson=poisson)
+            if bs_radius:
+                ctx.create_beamstop(f["bs"], bs_radius)
+            else:
+                f["bs"].zero()
+            ctx.create_circle(f["spt"], support_scale * radii[i])
+            ctx.random_start(f["I"], f["bs"] if bs_radius else None, f["F"], seed=int(seeds[i]), mode=start_mode)
+            ctx.fft(f["F"], f["g"], INVERSE)
+            ctx.apply_norm(f["g"], 1.0 / f["n"])
+        self.sync()
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.cdx_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

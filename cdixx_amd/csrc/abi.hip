@@ -72,7 +72,7 @@ struct cdx_ctx {
 
 // twiddle table of a line length n: tw[m] = W_n^m = exp(-2 pi i m/n), m < n,
 // evaluated in double with exact integer reduction of the angle
-static int make_twiddles(int n, int prec, void** dptr) {
+int make_twiddles(int n, int prec, void** dptr) {
   std::vector<double> re((size_t)n), im((size_t)n);
   for (int m = 0; m < n; m++) {
     // reduce to the first octant so cos/sin see |angle| <= pi/4

@@ -1,0 +1,613 @@
+// batch.hip -- many independent reconstructions of mixed sizes advanced
+// together: every pass of every frame in one launch.
+//
+// Frames are packed back to back in shared buffers (F, g, I, beamstop,
+// support and the T1/T2 intermediates).  A workgroup map lists, for every
+// workgroup of a pass, its (frame, tile); the kernels switch on the frame's
+// size and run the same pass bodies as the single-frame path, so a frame's
+// arithmetic is identical in both.  Each frame keeps its own support
+// threshold, shrink-wrap state and R-factor ring.  Workgroups have 256
+// threads for every size (16 columns/rows of 256 or 384 points, 8 of 512),
+// so column tiles are 128 bytes wide.
+#include "../../include/cdx_abi.h"
+#include "errors.hpp"
+#include "pass_bodies.hpp"
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+#include <algorithm>
+#include <string>
+#include <vector>
+
+namespace cdx {
+
+constexpr int kBatchThreads = 256;
+constexpr int kBatchSizes[] = {128, 256, 384, 512};
+constexpr int kNumBatchSizes = 4;
+constexpr int kScal = 16;  // scalars per frame (same layout as a context's)
+
+__host__ __device__ constexpr int bsize_index(int n) {
+  return n == 128 ? 0 : n == 256 ? 1 : n == 384 ? 2 : n == 512 ? 3 : -1;
+}
+template <int N>
+constexpr int blines() { return kBatchThreads / LineCfg<N>::TL; }
+template <int N>
+constexpr int blsh() {
+  int s = 0;
+  while ((1 << s) < blines<N>()) s++;
+  return s;
+}
+static int blines_rt(int n) {
+  switch (n) {
+    case 128: return blines<128>();
+    case 256: return blines<256>();
+    case 384: return blines<384>();
+    case 512: return blines<512>();
+    default: return 0;
+  }
+}
+
+struct BFrame {
+  int n;
+  int wg_row0, nwg_row;  // this frame's workgroups in the row pass
+  int pad;
+  int64_t off;           // element offset of the frame in the packed buffers
+  double* scal;          // kScal scalars: [0] threshold, [1] avg, [4] max bits
+  double* rf;            // R-factor ring (2 doubles per projection)
+};
+
+template <class T>
+struct BArgs {
+  const BFrame* frames;
+  const cplx<T>* src;
+  cplx<T>* dst;
+  cplx<T>* g;
+  const T* Ipk;
+  T* spt;
+  const cplx<T>* tw[kNumBatchSizes];
+  ProjArgs<T> pa;
+  T beta, relax;
+  double sigma;      // shrink-wrap blur sigma (the frame's sigma_f = (n>>2)/sigma)
+  double* partials;
+  int* nparts;
+  int rslot;         // R-factor ring slot of this projection, -1: none
+};
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+template <class T, int OP, int MODE, int RF>
+__global__ void __launch_bounds__(kBatchThreads) kb_col(BArgs<T> a, const int2* __restrict__ map) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int2 m = map[blockIdx.x];
+  if (m.x < 0) return;  // padding entry (whole workgroup)
+  const BFrame f = a.frames[m.x];
+  ProjArgs<T> pa = a.pa;
+  pa.rf = (RF && a.rslot >= 0) ? f.rf + 2 * a.rslot : nullptr;
+  const T gs = (T)(f.n >> 2) / (T)a.sigma;
+  const cplx<T>* in = a.src + f.off;
+  cplx<T>* out = a.dst + f.off;
+  const T* I = a.Ipk + f.off;
+  switch (f.n) {
+#define CASE(n_)                                                                                       \
+  case n_:                                                                                             \
+    col_body<T, n_, OP, MODE, RF>(in, out, n_, a.tw[bsize_index(n_)], I, pa, gs, m.y, blsh<n_>(), smem); \
+    break;
+    CASE(128) CASE(256) CASE(384) CASE(512)
+#undef CASE
+    default: break;
+  }
+}
+
+template <class T, int OP, int ALGO>
+__global__ void __launch_bounds__(kBatchThreads) kb_row(BArgs<T> a, const int2* __restrict__ map) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int2 m = map[blockIdx.x];
+  if (m.x < 0) return;
+  const BFrame f = a.frames[m.x];
+  UpdArgs<T> ua;
+  ua.algo = ALGO;
+  ua.beta = a.beta;
+  ua.relax = a.relax;
+  ua.invN = (T)(1.0 / ((double)f.n * (double)f.n));
+  ua.scal = f.scal;
+  ReduceSlots rs{(void*)(f.scal + 4), a.partials, a.nparts};
+  const cplx<T>* in = a.src + f.off;
+  cplx<T>* out = a.dst + f.off;
+  switch (f.n) {
+#define CASE(n_)                                                                                              \
+  case n_:                                                                                                    \
+    row_body<T, n_, OP, ALGO>(in, out, n_, a.tw[bsize_index(n_)], a.g + f.off, a.spt + f.off, ua, (T)1,        \
+                              a.spt + f.off, rs, m.y, blsh<n_>(), smem);                                      \
+    break;
+    CASE(128) CASE(256) CASE(384) CASE(512)
+#undef CASE
+    default: break;
+  }
+}
+
+// per-frame shrink-wrap threshold (frequency form, cdi.cc:330-333; real form
+// holo.cc:141): one wavefront per frame adds its partials in a fixed order
+template <class T>
+__global__ void __launch_bounds__(64) kb_threshold(const BFrame* frames, const double* partials,
+                                                   double shrinkThreshold, int mode) {
+  const BFrame f = frames[blockIdx.x];
+  double s = 0;
+  for (int i = threadIdx.x; i < f.nwg_row; i += 64) s += partials[f.wg_row0 + i];
+  s = wave_sum(s);
+  if (threadIdx.x != 0) return;
+  double* scal = f.scal;
+  T maxs;
+  if (sizeof(T) == 4) maxs = (T)__uint_as_float(*(unsigned int*)(scal + 4));
+  else maxs = (T)__longlong_as_double(*(long long*)(scal + 4));
+  if (mode == 0) {
+    T avg = (T)scal[1];
+    if (avg == (T)0) {
+      avg = (T)s * (T)4 / (T)f.n / (T)f.n;
+      scal[1] = (double)avg;
+    }
+    if (avg * (T)5 < maxs) maxs = avg * (T)5;
+  }
+  scal[0] = (double)(T)((double)maxs * shrinkThreshold);
+  scal[2] = (double)maxs;
+  scal[4] = 0;
+}
+
+// elementwise over all frames: g *= s_f with s_f = 1/n_f (dir 0) or n_f (dir 1)
+template <class T>
+__global__ void kb_scale_g(const BFrame* frames, int nf, cplx<T>* g, int dir) {
+  const BFrame f = frames[blockIdx.y];
+  const int64_t n = (int64_t)f.n * f.n;
+  const T s = dir == 0 ? (T)(1.0 / f.n) : (T)f.n;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    cplx<T> v = g[f.off + i];
+    g[f.off + i] = {v.x * s, v.y * s};
+  }
+}
+
+}  // namespace cdx
+
+using namespace cdx;
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct cdx_batch {
+  int nf = 0, prec = 0;
+  size_t rsz = 4, csz = 8;
+  std::vector<int> n;
+  std::vector<int64_t> off;
+  int64_t total = 0;
+  hipStream_t st = nullptr;
+  bool own_stream = false;
+  void *F = nullptr, *g = nullptr, *T1 = nullptr, *T2 = nullptr, *Ipk = nullptr, *I = nullptr, *bs = nullptr,
+       *spt = nullptr;
+  void* tw[kNumBatchSizes] = {nullptr, nullptr, nullptr, nullptr};
+  BFrame* dframes = nullptr;
+  int2* col_map = nullptr;
+  int2* row_map = nullptr;
+  int ncol = 0, nrow = 0;
+  double* scal = nullptr;
+  double* ring = nullptr;
+  int ring_cap = 0;
+  double* partials = nullptr;
+  int* nparts = nullptr;
+  size_t lds_col = 0, lds_row = 0;
+  std::vector<cdx_ctx*> ctx;  // a single-frame context per frame size (setup operations on slices)
+  std::vector<int> ctx_n;
+};
+
+static thread_local char g_berr[512];
+static int bfail(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_berr, sizeof g_berr, fmt, ap);
+  va_end(ap);
+  set_last_error(g_berr);
+  return code;
+}
+#define BCHK(expr)                                                                             \
+  do {                                                                                         \
+    hipError_t e_ = (expr);                                                                    \
+    if (e_ != hipSuccess) return bfail(CDX_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_));   \
+  } while (0)
+
+static size_t pitch_bytes(int n, size_t csz) {
+  switch (n) {
+#define F(n_) case n_: return (size_t)(csz == 8 ? lds_pitch<float, n_>() : lds_pitch<double, n_>()) * csz;
+    F(128) F(256) F(384) F(512)
+#undef F
+    default: return 0;
+  }
+}
+
+extern "C" int cdx_batch_destroy(cdx_batch* b) {
+  if (!b) return CDX_OK;
+  if (b->st) hipStreamSynchronize(b->st);
+  for (void* p : {b->F, b->g, b->T1, b->T2, b->Ipk, b->I, b->bs, b->spt}) hipFree(p);
+  for (void* p : b->tw) hipFree(p);
+  hipFree(b->dframes);
+  hipFree(b->col_map);
+  hipFree(b->row_map);
+  hipFree(b->scal);
+  hipFree(b->ring);
+  hipFree(b->partials);
+  hipFree(b->nparts);
+  for (auto* c : b->ctx) cdx_destroy(c);
+  if (b->own_stream && b->st) hipStreamDestroy(b->st);
+  delete b;
+  return CDX_OK;
+}
+
+extern "C" int cdx_batch_create(cdx_batch** out, int nframes, const int* sizes, int precision, void* stream) {
+  if (!out || nframes <= 0 || !sizes) return bfail(CDX_ERR_ARG, "cdx_batch_create: bad arguments");
+  if (precision != CDX_F32 && precision != CDX_F64) return bfail(CDX_ERR_ARG, "cdx_batch_create: bad precision");
+  *out = nullptr;
+  for (int i = 0; i < nframes; i++)
+    if (bsize_index(sizes[i]) < 0)
+      return bfail(CDX_ERR_ARG, "cdx_batch_create: frame %d has size %d (batched sizes: 128, 256, 384, 512)", i,
+                   sizes[i]);
+  cdx_batch* b = new cdx_batch();
+  b->nf = nframes;
+  b->prec = precision;
+  b->rsz = precision == CDX_F64 ? 8 : 4;
+  b->csz = 2 * b->rsz;
+  b->n.assign(sizes, sizes + nframes);
+  b->off.resize(nframes);
+  for (int i = 0; i < nframes; i++) {
+    b->off[i] = b->total;
+    b->total += (int64_t)sizes[i] * sizes[i];
+  }
+  int rc;
+  if (stream) {
+    b->st = (hipStream_t)stream;
+  } else {
+    if (hipStreamCreate(&b->st) != hipSuccess) {
+      delete b;
+      return bfail(CDX_ERR_HIP, "cdx_batch_create: hipStreamCreate failed");
+    }
+    b->own_stream = true;
+  }
+  auto bad = [&](int code) {
+    cdx_batch_destroy(b);
+    return code;
+  };
+  const size_t tc = (size_t)b->total * b->csz, tr = (size_t)b->total * b->rsz;
+  if (hipMalloc(&b->F, tc) || hipMalloc(&b->g, tc) || hipMalloc(&b->T1, tc) || hipMalloc(&b->T2, tc) ||
+      hipMalloc(&b->Ipk, tr) || hipMalloc(&b->I, tr) || hipMalloc(&b->bs, tr) || hipMalloc(&b->spt, tr))
+    return bad(bfail(CDX_ERR_HIP, "cdx_batch_create: out of device memory"));
+  hipMemset(b->bs, 0, tr);
+  // twiddles and single-frame contexts for the sizes present
+  for (int k = 0; k < kNumBatchSizes; k++) {
+    bool used = false;
+    for (int s : b->n) used |= s == kBatchSizes[k];
+    if (!used) continue;
+    if ((rc = make_twiddles(kBatchSizes[k], precision, &b->tw[k]))) return bad(rc);
+    cdx_ctx* c;
+    if ((rc = cdx_create(&c, kBatchSizes[k], kBatchSizes[k], precision, b->st))) return bad(rc);
+    b->ctx.push_back(c);
+    b->ctx_n.push_back(kBatchSizes[k]);
+    b->lds_col = std::max(b->lds_col, (size_t)blines_rt(kBatchSizes[k]) * pitch_bytes(kBatchSizes[k], b->csz));
+  }
+  b->lds_row = b->lds_col;
+  // workgroup maps
+  std::vector<int2> cols, rows;
+  std::vector<BFrame> fr(nframes);
+  b->ring_cap = 4096;
+  if (hipMalloc((void**)&b->scal, (size_t)nframes * kScal * sizeof(double)) ||
+      hipMalloc((void**)&b->ring, (size_t)nframes * b->ring_cap * 2 * sizeof(double)))
+    return bad(bfail(CDX_ERR_HIP, "cdx_batch_create: out of device memory"));
+  for (int i = 0; i < nframes; i++) {
+    const int n = b->n[i], lines = blines_rt(n);
+    for (int t = 0; t < n / lines; t++) cols.push_back(make_int2(i, t));
+    fr[i].n = n;
+    fr[i].wg_row0 = (int)rows.size();
+    fr[i].nwg_row = n / lines;
+    for (int t = 0; t < n / lines; t++) rows.push_back(make_int2(i, t));
+    fr[i].pad = 0;
+    fr[i].off = b->off[i];
+    fr[i].scal = b->scal + (size_t)i * kScal;
+    fr[i].rf = b->ring + (size_t)i * b->ring_cap * 2;
+  }
+  // column tiles sharing cache lines go to the same XCD: workgroup id b runs
+  // tile xcd(b) of the (padded) tile list
+  while (cols.size() % 8) cols.push_back(make_int2(-1, 0));
+  std::vector<int2> cmap(cols.size());
+  const int nb = (int)cols.size();
+  for (int bid = 0; bid < nb; bid++) cmap[bid] = cols[(bid & 7) * (nb >> 3) + (bid >> 3)];
+  b->ncol = nb;
+  b->nrow = (int)rows.size();
+  if (b->nrow > kMaxPartials) return bad(bfail(CDX_ERR_ARG, "cdx_batch_create: too many rows in the batch"));
+  if (hipMalloc((void**)&b->col_map, cmap.size() * sizeof(int2)) ||
+      hipMalloc((void**)&b->row_map, rows.size() * sizeof(int2)) ||
+      hipMalloc((void**)&b->dframes, fr.size() * sizeof(BFrame)) ||
+      hipMalloc((void**)&b->partials, kMaxPartials * sizeof(double)) || hipMalloc((void**)&b->nparts, sizeof(int)))
+    return bad(bfail(CDX_ERR_HIP, "cdx_batch_create: out of device memory"));
+  std::vector<double> sc((size_t)nframes * kScal, 0.0);
+  for (int i = 0; i < nframes; i++) sc[(size_t)i * kScal] = 0.5;  // init_cuda_image threshold
+  if (hipMemcpy(b->col_map, cmap.data(), cmap.size() * sizeof(int2), hipMemcpyHostToDevice) ||
+      hipMemcpy(b->row_map, rows.data(), rows.size() * sizeof(int2), hipMemcpyHostToDevice) ||
+      hipMemcpy(b->dframes, fr.data(), fr.size() * sizeof(BFrame), hipMemcpyHostToDevice) ||
+      hipMemcpy(b->scal, sc.data(), sc.size() * sizeof(double), hipMemcpyHostToDevice))
+    return bad(bfail(CDX_ERR_HIP, "cdx_batch_create: upload failed"));
+  *out = b;
+  return CDX_OK;
+}
+
+extern "C" int cdx_batch_frame(cdx_batch* b, int i, void** F, void** g, void** I, void** bs, void** spt, int* n) {
+  if (!b || i < 0 || i >= b->nf) return bfail(CDX_ERR_ARG, "cdx_batch_frame: bad frame index");
+  const int64_t o = b->off[i];
+  if (F) *F = (char*)b->F + o * b->csz;
+  if (g) *g = (char*)b->g + o * b->csz;
+  if (I) *I = (char*)b->I + o * b->rsz;
+  if (bs) *bs = (char*)b->bs + o * b->rsz;
+  if (spt) *spt = (char*)b->spt + o * b->rsz;
+  if (n) *n = b->n[i];
+  return CDX_OK;
+}
+
+extern "C" cdx_ctx* cdx_batch_context(cdx_batch* b, int i) {
+  if (!b || i < 0 || i >= b->nf) return nullptr;
+  for (size_t k = 0; k < b->ctx.size(); k++)
+    if (b->ctx_n[k] == b->n[i]) return b->ctx[k];
+  return nullptr;
+}
+
+extern "C" int cdx_batch_set_threshold(cdx_batch* b, double thr) {
+  if (!b) return bfail(CDX_ERR_ARG, "cdx_batch_set_threshold: null batch");
+  std::vector<double> h((size_t)b->nf * kScal);
+  BCHK(hipMemcpy(h.data(), b->scal, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int i = 0; i < b->nf; i++) {
+    h[(size_t)i * kScal] = b->prec == CDX_F64 ? thr : (double)(float)thr;
+    h[(size_t)i * kScal + 1] = 0;  // shrink-wrap avg: taken again on the next first call
+  }
+  BCHK(hipMemcpy(b->scal, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+  return CDX_OK;
+}
+
+extern "C" int cdx_batch_get_thresholds(cdx_batch* b, double* out) {
+  if (!b || !out) return bfail(CDX_ERR_ARG, "cdx_batch_get_thresholds: bad arguments");
+  std::vector<double> h((size_t)b->nf * kScal);
+  BCHK(hipStreamSynchronize(b->st));
+  BCHK(hipMemcpy(h.data(), b->scal, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int i = 0; i < b->nf; i++) out[i] = h[(size_t)i * kScal];
+  return CDX_OK;
+}
+
+namespace {
+template <class T>
+struct BLoop {
+  cdx_batch* b;
+  const cdx_params* p;
+  ProjArgs<T> pa;
+  bool f_pending = true;
+
+  BArgs<T> args(const void* src, void* dst) {
+    BArgs<T> a{};
+    a.frames = b->dframes;
+    a.src = (const cplx<T>*)src;
+    a.dst = (cplx<T>*)dst;
+    a.g = (cplx<T>*)b->g;
+    a.Ipk = (const T*)b->Ipk;
+    a.spt = (T*)b->spt;
+    for (int k = 0; k < kNumBatchSizes; k++) a.tw[k] = (const cplx<T>*)b->tw[k];
+    a.pa = pa;
+    a.beta = (T)p->beta;
+    a.relax = (T)p->relax;
+    a.sigma = 1;
+    a.partials = b->partials;
+    a.nparts = b->nparts;
+    a.rslot = -1;
+    return a;
+  }
+  template <class K>
+  hipError_t launch(K kern, int nwg, const int2* map, const BArgs<T>& a, size_t lds) {
+    if (lds > 64 * 1024) {
+      hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(kern, dim3(nwg), dim3(kBatchThreads), lds, b->st, a, map);
+    return hipGetLastError();
+  }
+  int setup() {
+    const bool pack_bs = p->useBS && (p->simCCDbit || p->freeBeamstop);
+    BCHK(launch_pack_intensity<T>(b->total, (const T*)b->I, pack_bs ? (const T*)b->bs : nullptr, (T*)b->Ipk, b->st));
+    pa = ProjArgs<T>{};
+    pa.mode = p->simCCDbit ? PROJ_MOD_NOISE : (pack_bs ? PROJ_MOD_ABS_BS : PROJ_MOD_ABS);
+    pa.noiseLevel = p->noiseLevel;
+    pa.scale = (T)(1.0 / p->exposure);
+    pa.rcolor = (T)p->rcolor;
+    return CDX_OK;
+  }
+  int iterate(int algo, int rslot) {
+    BArgs<T> a = args(f_pending ? b->F : b->T1, b->T2);
+    a.rslot = rslot;
+    const bool rf = rslot >= 0;
+    if (f_pending) {
+      ProjArgs<T> rpa = pa;
+      a.pa = rpa;
+#define M(mode)                                                                                                  \
+  BCHK(rf ? launch(kb_col<T, COL_PROJ_INV, mode, 1>, b->ncol, b->col_map, a, b->lds_col)                         \
+          : launch(kb_col<T, COL_PROJ_INV, mode, 0>, b->ncol, b->col_map, a, b->lds_col));
+      switch (pa.mode) {
+        case PROJ_MOD_ABS: M(PROJ_MOD_ABS) break;
+        case PROJ_MOD_NOISE: M(PROJ_MOD_NOISE) break;
+        default: M(PROJ_MOD_ABS_BS) break;
+      }
+#undef M
+    } else {
+#define M(mode)                                                                                                  \
+  BCHK(rf ? launch(kb_col<T, COL_FWD_PROJ_INV, mode, 1>, b->ncol, b->col_map, a, b->lds_col)                     \
+          : launch(kb_col<T, COL_FWD_PROJ_INV, mode, 0>, b->ncol, b->col_map, a, b->lds_col));
+      switch (pa.mode) {
+        case PROJ_MOD_ABS: M(PROJ_MOD_ABS) break;
+        case PROJ_MOD_NOISE: M(PROJ_MOD_NOISE) break;
+        default: M(PROJ_MOD_ABS_BS) break;
+      }
+#undef M
+    }
+    BArgs<T> r = args(b->T2, b->T1);
+    switch (algo) {
+#define A(k) \
+  case k: BCHK(launch(kb_row<T, ROW_INV_UPDATE_FWD, k>, b->nrow, b->row_map, r, b->lds_row)); break;
+      A(0) A(1) A(2) A(3) A(4)
+      default: BCHK(launch(kb_row<T, ROW_INV_UPDATE_FWD, 5>, b->nrow, b->row_map, r, b->lds_row)); break;
+#undef A
+    }
+    f_pending = false;
+    return CDX_OK;
+  }
+  int shrink_wrap(double sigma) {
+    if (p->shrinkMode != 0) return bfail(CDX_ERR_UNSUPPORTED, "batched shrink-wrap: frequency form only");
+    BArgs<T> a = args(f_pending ? b->F : b->T1, b->T2);
+    a.sigma = sigma;
+    if (f_pending) BCHK(launch(kb_col<T, COL_GAUSS_INV, 0, 0>, b->ncol, b->col_map, a, b->lds_col));
+    else BCHK(launch(kb_col<T, COL_FWD_GAUSS_INV, 0, 0>, b->ncol, b->col_map, a, b->lds_col));
+    BArgs<T> r = args(b->T2, nullptr);
+    BCHK(launch(kb_row<T, ROW_INV_MOD2, 0>, b->nrow, b->row_map, r, b->lds_row));
+    hipLaunchKernelGGL(kb_threshold<T>, dim3(b->nf), dim3(64), 0, b->st, (const BFrame*)b->dframes,
+                       (const double*)b->partials, p->shrinkThreshold, 0);
+    BCHK(hipGetLastError());
+    return CDX_OK;
+  }
+  int finish() {
+    if (!f_pending) {
+      BArgs<T> a = args(b->T1, b->F);
+      BCHK(launch(kb_col<T, COL_FWD, 0, 0>, b->ncol, b->col_map, a, b->lds_col));
+      f_pending = true;
+    }
+    return CDX_OK;
+  }
+  int scale_g(int dir) {
+    int maxn = 0;
+    for (int s : b->n) maxn = std::max(maxn, s);
+    dim3 grid((unsigned)std::min<int64_t>(((int64_t)maxn * maxn + 255) / 256, 1024), (unsigned)b->nf);
+    hipLaunchKernelGGL(kb_scale_g<T>, grid, dim3(256), 0, b->st, (const BFrame*)b->dframes, b->nf, (cplx<T>*)b->g,
+                       dir);
+    BCHK(hipGetLastError());
+    return CDX_OK;
+  }
+};
+
+template <class T>
+int batch_retrieve_t(cdx_batch* b, const char* algorithm, const cdx_params* p, double* rtrace, int rcap,
+                     int* n_iter) {
+  int nt = cdx_expand_schedule(algorithm, nullptr, 0);
+  if (nt < 0) return bfail(CDX_ERR_SCHEDULE, "cdx_batch_phase_retrieve: bad algorithm string \"%s\"", algorithm);
+  std::vector<int> toks(nt);
+  cdx_expand_schedule(algorithm, toks.data(), nt);
+  for (int t : toks)
+    if (t == CDX_TV) return bfail(CDX_ERR_UNSUPPORTED, "cdx_batch_phase_retrieve: TV steps are outside this path");
+  BLoop<T> L{b, p};
+  int rc = L.setup();
+  if (rc) return rc;
+  const int nrec = p->recordR ? std::min(rcap, b->ring_cap) : 0;
+  if (nrec > 0) BCHK(launch_fill_f64((int64_t)b->nf * b->ring_cap * 2, b->ring, 0.0, b->st));
+  if ((rc = L.scale_g(0))) return rc;  // cdi.cc:313
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  if (p->useGraph) BCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+  double sigma = p->sigma0;
+  int iter = 0, nproj = 0;
+  for (int tok : toks) {
+    if (tok == CDX_SHRINKWRAP) {
+      if ((rc = L.shrink_wrap(sigma))) break;
+      if (sigma > p->sigmaMin) sigma *= p->sigmaDecay;
+    } else {
+      if ((rc = L.iterate(tok, nproj < nrec ? nproj : -1))) break;
+      nproj++;
+    }
+    iter++;
+  }
+  if (!rc) rc = L.finish();
+  if (p->useGraph) {
+    hipError_t e = hipStreamEndCapture(b->st, &graph);
+    if (!rc && e != hipSuccess) rc = bfail(CDX_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    if (!rc && (e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0)) != hipSuccess)
+      rc = bfail(CDX_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    if (!rc && (e = hipGraphLaunch(exec, b->st)) != hipSuccess)
+      rc = bfail(CDX_ERR_HIP, "hipGraphLaunch: %s", hipGetErrorString(e));
+    if (exec) hipGraphExecDestroy(exec);
+    if (graph) hipGraphDestroy(graph);
+  }
+  if (rc) return rc;
+  if ((rc = L.scale_g(1))) return rc;  // cdi.cc:406
+  BCHK(hipStreamSynchronize(b->st));
+  if (n_iter) *n_iter = iter;
+  if (rtrace && nrec > 0) {
+    std::vector<double> h((size_t)b->nf * b->ring_cap * 2);
+    BCHK(hipMemcpy(h.data(), b->ring, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    const int m = std::min(nproj, nrec);
+    for (int f = 0; f < b->nf; f++)
+      for (int i = 0; i < m; i++) {
+        const double* s = &h[((size_t)f * b->ring_cap + i) * 2];
+        rtrace[(size_t)f * rcap + i] = s[1] > 0 ? s[0] / s[1] : 0;
+      }
+  }
+  return CDX_OK;
+}
+
+template <class T>
+int batch_time_t(cdx_batch* b, const cdx_params* p, int algo, int n, int use_graph, double* ms_out) {
+  BLoop<T> L{b, p};
+  int rc = L.setup();
+  if (rc) return rc;
+  if ((rc = L.iterate(algo, -1))) return rc;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  if (use_graph) {
+    BCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < n && !rc; i++) rc = L.iterate(algo, -1);
+    hipError_t e = hipStreamEndCapture(b->st, &graph);
+    if (rc) return rc;
+    BCHK(e);
+    BCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    BCHK(hipGraphLaunch(exec, b->st));
+  }
+  hipEvent_t e0, e1;
+  BCHK(hipEventCreate(&e0));
+  BCHK(hipEventCreate(&e1));
+  BCHK(hipStreamSynchronize(b->st));
+  BCHK(hipEventRecord(e0, b->st));
+  if (use_graph) {
+    BCHK(hipGraphLaunch(exec, b->st));
+  } else {
+    for (int i = 0; i < n && !rc; i++) rc = L.iterate(algo, -1);
+    if (rc) return rc;
+  }
+  BCHK(hipEventRecord(e1, b->st));
+  BCHK(hipEventSynchronize(e1));
+  float ms = 0;
+  BCHK(hipEventElapsedTime(&ms, e0, e1));
+  *ms_out = ms;
+  if (exec) hipGraphExecDestroy(exec);
+  if (graph) hipGraphDestroy(graph);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if ((rc = L.finish())) return rc;
+  BCHK(hipStreamSynchronize(b->st));
+  return CDX_OK;
+}
+}  // namespace
+
+extern "C" int cdx_batch_phase_retrieve(cdx_batch* b, const char* algorithm, const cdx_params* p, double* rtrace,
+                                        int rcap, int* n_iter) {
+  if (!b || !algorithm || !p) return bfail(CDX_ERR_ARG, "cdx_batch_phase_retrieve: null argument");
+  if (!(p->exposure != 0)) return bfail(CDX_ERR_ARG, "cdx_batch_phase_retrieve: exposure must be non-zero");
+  return b->prec == CDX_F64 ? batch_retrieve_t<double>(b, algorithm, p, rtrace, rcap, n_iter)
+                            : batch_retrieve_t<float>(b, algorithm, p, rtrace, rcap, n_iter);
+}
+
+extern "C" int cdx_batch_time_iterations(cdx_batch* b, const cdx_params* p, int algo, int n, int use_graph,
+                                         double* ms) {
+  if (!b || !p || n <= 0 || !ms) return bfail(CDX_ERR_ARG, "cdx_batch_time_iterations: bad arguments");
+  return b->prec == CDX_F64 ? batch_time_t<double>(b, p, algo, n, use_graph, ms)
+                            : batch_time_t<float>(b, p, algo, n, use_graph, ms);
+}
+
+extern "C" int cdx_batch_synchronize(cdx_batch* b) {
+  if (!b) return bfail(CDX_ERR_ARG, "cdx_batch_synchronize: null batch");
+  BCHK(hipStreamSynchronize(b->st));
+  return CDX_OK;
+}

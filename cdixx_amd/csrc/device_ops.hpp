@@ -29,7 +29,7 @@ CDX_LINE(256, 16, 16, 16)
 CDX_LINE(384, 16, 8, 8, 6)
 CDX_LINE(512, 32, 16, 16, 2)
 CDX_LINE(768, 32, 8, 8, 12)
-CDX_LINE(1024, 64, 16, 16, 4)
+CDX_LINE(1024, 128, 8, 8, 8, 2)
 CDX_LINE(2048, 128, 16, 16, 8)
 #undef CDX_LINE
 

@@ -59,6 +59,10 @@ constexpr int kMaxPartials = 4096;
 
 // line lengths of the FFT
 int line_supported(int n);
+}  // namespace cdx
+// twiddle table tw[m] = exp(-2 pi i m/n), m < n, uploaded to *dptr (C ABI status)
+int make_twiddles(int n, int prec, void** dptr);
+namespace cdx {
 // lines per workgroup of a pass: valid (power of two, 64..256 threads, LDS
 // within 160 KiB) and the default choice; csz = bytes per complex element
 int tiling_valid(int n, size_t csz, int lines);

@@ -1,0 +1,234 @@
+// pass_bodies.hpp -- device bodies of the fused FFT passes (shared by the
+// single-frame kernels in passes.hip and the batched kernels in batch.hip).
+//
+// One iteration of CDI::phaseRetrieve (CDIxx src/post_gpu/cdi.cc:349-354,381)
+// is projection -> IFFT -> 1/N -> support update -> FFT.  Here it is two
+// passes that each stream the image once:
+//   column pass  (COL_FWD_PROJ_INV): forward column FFT of T1 = rowFFT(g),
+//                 modulus projection (+R-factor sums), inverse column FFT -> T2
+//   row pass     (ROW_INV_UPDATE_FWD): inverse row FFT of T2, 1/N,
+//                 HIO/RAAR/ER/... update against g and the support, store g,
+//                 forward row FFT -> T1
+// so F = colFFT(T1) is never stored between iterations.  Other OP values
+// serve the standalone 2D FFT and shrink-wrap.
+//
+// The projection mode, the R-factor switch and the update rule are template
+// parameters (no per-pixel branching on them), and every operand a pass
+// reads besides the line itself (I; g and S) is loaded at the start so its
+// latency overlaps the first transform.
+#pragma once
+#include "device_ops.hpp"
+
+namespace cdx {
+
+// ---------------------------------------------------------------------------
+// pointwise operations with compile-time selection
+// ---------------------------------------------------------------------------
+// applyModAbs (cudaConfig.cu:827-841): the FP32 build scales by
+// sqrt(I)*rsqrt(|F|^2) (within 2 ulp of the reference's sqrt(I)/hypot(F))
+template <class T>
+__device__ __forceinline__ cplx<T> mod_abs_fast(cplx<T> f, T I) {
+  if constexpr (sizeof(T) == 4) {
+    const float m2 = f.x * f.x + f.y * f.y;
+    const float r = I > 0.f ? sqrtf(I) : 0.f;
+    if (m2 > 0.f) {
+      const float s = r * rsqrtf(m2);
+      return {f.x * s, f.y * s};
+    }
+    return {r, 0.f};
+  } else {
+    return mod_abs(f, I);
+  }
+}
+
+template <class T, int MODE>
+__device__ __forceinline__ cplx<T> project_m(cplx<T> f, T I, const ProjArgs<T>& pa) {
+  if constexpr (MODE == PROJ_MOD_NOISE) return mod_noise(f, I, pa.noiseLevel, pa.scale, pa.rcolor);
+  else if constexpr (MODE == PROJ_MOD_ABS_BS) return isnan(I) ? f : mod_abs_fast(f, I);
+  else if constexpr (MODE == PROJ_MOD_ABS) return mod_abs_fast(f, I);
+  else {  // runtime mode, same arithmetic as the compile-time variants
+    if (pa.mode == PROJ_MOD_NOISE) return mod_noise(f, I, pa.noiseLevel, pa.scale, pa.rcolor);
+    if (pa.mode == PROJ_MOD_ABS_BS && isnan(I)) return f;
+    return mod_abs_fast(f, I);
+  }
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  Column tiles are narrower than a cache line (8 complex64 = 64 B), so
+// the tiles sharing a line must run on the same XCD: give each XCD a
+// contiguous run of tiles.
+__device__ __forceinline__ int xcd_tile(int bid, int nb) {
+  if (nb & 7) return bid;
+  return (bid & 7) * (nb >> 3) + (bid >> 3);
+}
+
+// ---------------------------------------------------------------------------
+// column pass body: columns [tile << lsh, (tile+1) << lsh).  MODE/RF = -1:
+// projection mode and R-factor chosen at run time
+// ---------------------------------------------------------------------------
+template <class T, int N, int OP, int MODE, int RF>
+__device__ __forceinline__ void col_body(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out, int cols,
+                                         const cplx<T>* __restrict__ tw, const T* __restrict__ I,
+                                         const ProjArgs<T>& pa, T gsig, int tile, int lsh, unsigned char* smem) {
+  using L = LineOf<T, N>;
+  constexpr int PITCH = lds_pitch<T, N>();
+  constexpr int E = L::E;
+  cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem);
+  const int c = threadIdx.x & ((1 << lsh) - 1);
+  const int t = threadIdx.x >> lsh;
+  const int col = (tile << lsh) + c;
+  const bool ok = col < cols;
+  cplx<T>* my = lds + c * PITCH;
+  constexpr bool from_f = (OP == COL_INV || OP == COL_PROJ_INV || OP == COL_GAUSS_INV);
+  constexpr bool proj = (OP == COL_FWD_PROJ_INV || OP == COL_PROJ_INV);
+  cplx<T> v[E];
+  T iv[proj ? E : 1];
+
+  if constexpr (from_f) {
+#pragma unroll
+    for (int i = 0; i < E; i++) v[i] = ok ? in[(size_t)L::dout_pos(i, t) * cols + col] : cplx<T>{(T)0, (T)0};
+  } else {
+#pragma unroll
+    for (int i = 0; i < E; i++) v[i] = ok ? in[(size_t)L::din_pos(i, t) * cols + col] : cplx<T>{(T)0, (T)0};
+  }
+  if constexpr (proj) {
+#pragma unroll
+    for (int i = 0; i < E; i++) iv[i] = ok ? I[(size_t)L::dout_pos(i, t) * cols + col] : (T)NAN;
+  }
+  if constexpr (!from_f) L::forward(v, my, tw, t);
+
+  if constexpr (OP == COL_FWD) {
+    if (ok) {
+#pragma unroll
+      for (int i = 0; i < E; i++) out[(size_t)L::dout_pos(i, t) * cols + col] = v[i];
+    }
+    return;
+  }
+
+  if constexpr (proj) {
+    const bool rf = RF < 0 ? pa.rf != nullptr : RF > 0;
+    T rnum = 0, rden = 0;
+#pragma unroll
+    for (int i = 0; i < E; i++) {
+      if (rf && !isnan(iv[i])) {
+        const T s = iv[i] > (T)0 ? dsqrt(iv[i]) : (T)0;
+        rnum += fabs(dsqrt(v[i].x * v[i].x + v[i].y * v[i].y) - s);
+        rden += s;
+      }
+      if (ok) v[i] = project_m<T, MODE>(v[i], iv[i], pa);
+    }
+    if (rf) {
+      __shared__ double sh[16];
+      block_atomic_sum((double)rnum, pa.rf, sh);
+      block_atomic_sum((double)rden, pa.rf + 1, sh);
+    }
+  }
+  if constexpr (OP == COL_FWD_GAUSS_INV || OP == COL_GAUSS_INV) {
+    // applyGaussMult(isFreq=1), cudaConfig.cu:380-395, wrapped coordinates
+    const T yrel = (T)(col >= cols / 2 ? col - cols : col);
+#pragma unroll
+    for (int i = 0; i < E; i++) {
+      const int r = L::dout_pos(i, t);
+      const T xrel = (T)(r >= N / 2 ? r - N : r);
+      const T f = (T)exp(-(xrel * xrel + yrel * yrel) / ((T)2 * gsig * gsig));
+      v[i].x *= f;
+      v[i].y *= f;
+    }
+  }
+
+  L::inverse(v, my, tw, t);
+  if (ok) {
+#pragma unroll
+    for (int i = 0; i < E; i++) out[(size_t)L::din_pos(i, t) * cols + col] = v[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// row pass body: rows [tile << lsh, (tile+1) << lsh).  ALGO: support-update
+// rule of ROW_INV_UPDATE_FWD (enum Algorithm)
+// ---------------------------------------------------------------------------
+template <class T, int N, int OP, int ALGO>
+__device__ __forceinline__ void row_body(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out, int rows,
+                                         const cplx<T>* __restrict__ tw, cplx<T>* __restrict__ g,
+                                         const T* __restrict__ spt, const UpdArgs<T>& ua, T scale,
+                                         T* __restrict__ sout, const ReduceSlots& rs, int tile, int lsh,
+                                         unsigned char* smem) {
+  using L = LineOf<T, N>;
+  constexpr int TL = LineCfg<N>::TL;
+  constexpr int PITCH = lds_pitch<T, N>();
+  constexpr int E = L::E;
+  cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem);
+  const int t = threadIdx.x % TL;
+  const int rr = threadIdx.x / TL;
+  const int row = (tile << lsh) + rr;
+  const bool ok = row < rows;
+  cplx<T>* my = lds + rr * PITCH;
+  const size_t base = (size_t)row * N;
+  constexpr bool upd = OP == ROW_INV_UPDATE_FWD;
+  cplx<T> v[E];
+
+  if constexpr (OP == ROW_FWD) {
+#pragma unroll
+    for (int i = 0; i < E; i++) v[i] = ok ? in[base + L::din_pos(i, t)] : cplx<T>{(T)0, (T)0};
+    L::forward(v, my, tw, t);
+    if (ok) {
+#pragma unroll
+      for (int i = 0; i < E; i++) out[base + L::dout_pos(i, t)] = v[i];
+    }
+    return;
+  } else {
+    cplx<T> gv[upd ? E : 1];
+    T sv[upd ? E : 1];
+#pragma unroll
+    for (int i = 0; i < E; i++) v[i] = ok ? in[base + L::dout_pos(i, t)] : cplx<T>{(T)0, (T)0};
+    if constexpr (upd) {
+#pragma unroll
+      for (int i = 0; i < E; i++) {
+        gv[i] = ok ? g[base + L::din_pos(i, t)] : cplx<T>{(T)0, (T)0};
+        sv[i] = ok ? spt[base + L::din_pos(i, t)] : (T)0;
+      }
+    }
+    L::inverse(v, my, tw, t);
+
+    if constexpr (OP == ROW_INV) {
+      if (ok) {
+#pragma unroll
+        for (int i = 0; i < E; i++) out[base + L::din_pos(i, t)] = {v[i].x * scale, v[i].y * scale};
+      }
+      return;
+    }
+    if constexpr (OP == ROW_INV_MOD2) {
+      T m = 0;
+      double s = 0;
+#pragma unroll
+      for (int i = 0; i < E; i++) {
+        const T a = v[i].x * v[i].x + v[i].y * v[i].y;
+        if (ok) {
+          sout[base + L::din_pos(i, t)] = a;
+          m = a > m ? a : m;
+          s += (double)a;
+        }
+      }
+      __shared__ double sh[16];
+      block_atomic_max(m, rs.maxbits, sh);
+      block_partial_sum(s, rs, sh);
+      return;
+    }
+    if constexpr (upd) {
+      const T thr = (T)ua.scal[0];
+#pragma unroll
+      for (int i = 0; i < E; i++) {
+        const cplx<T> p = {v[i].x * ua.invN, v[i].y * ua.invN};
+        v[i] = support_update(gv[i], p, sv[i] > thr, ALGO, ua.beta, ua.relax);
+        if (ok) g[base + L::din_pos(i, t)] = v[i];
+      }
+      L::forward(v, my, tw, t);
+      if (ok) {
+#pragma unroll
+        for (int i = 0; i < E; i++) out[base + L::dout_pos(i, t)] = v[i];
+      }
+    }
+  }
+}
+
+}  // namespace cdx

@@ -170,6 +170,29 @@ void cdx_params_default(cdx_params* p);
 int cdx_phase_retrieve(cdx_ctx* ctx, const char* algorithm, const cdx_params* p, void* F, void* g,
                        const void* I, const void* bs, void* spt, double* rtrace, int rtrace_cap, int* n_iter);
 
+/* ---- batched reconstructions (no counterpart in the reference, which runs
+ * one frame per process through CDI::phaseRetrieve): frames of sizes 128,
+ * 256, 384 or 512 (square) advance together, one launch per pass for all
+ * frames.  The batch owns packed buffers; cdx_batch_frame returns the device
+ * slices of frame i (same layouts as the single-frame entry points), and
+ * cdx_batch_context a single-frame context of that frame's size on the
+ * batch's stream for setup operations on the slices. ---- */
+typedef struct cdx_batch cdx_batch;
+int cdx_batch_create(cdx_batch** out, int nframes, const int* sizes, int precision, void* stream);
+int cdx_batch_destroy(cdx_batch* b);
+int cdx_batch_frame(cdx_batch* b, int i, void** F, void** g, void** I, void** bs, void** spt, int* n);
+cdx_ctx* cdx_batch_context(cdx_batch* b, int i);
+int cdx_batch_set_threshold(cdx_batch* b, double threshold);
+int cdx_batch_get_thresholds(cdx_batch* b, double* thresholds);
+int cdx_batch_synchronize(cdx_batch* b);
+/* CDI::phaseRetrieve on every frame with one schedule (frequency-form
+ * shrink-wrap, per-frame thresholds).  rtrace (optional, host):
+ * nframes x rtrace_cap R-factors, frame-major. */
+int cdx_batch_phase_retrieve(cdx_batch* b, const char* algorithm, const cdx_params* p, double* rtrace,
+                             int rtrace_cap, int* n_iter);
+/* times n iterations of `algo` for the whole batch (graph or stream launches) */
+int cdx_batch_time_iterations(cdx_batch* b, const cdx_params* p, int algo, int n, int use_graph, double* ms);
+
 /* ---- the schedule grammar: AlgoParser, src/independant/readConfig.cc:82-163 ---- */
 /* expands the schedule into tokens; returns the token count or -CDX_ERR_SCHEDULE */
 int cdx_expand_schedule(const char* algorithm, int* out, int cap);

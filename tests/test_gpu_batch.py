@@ -1,0 +1,75 @@
+"""Batched reconstructions (frames of mixed sizes, one launch per pass) against
+the single-frame path on the same inputs.  Both run the same pass bodies, so
+without shrink-wrap the results are identical bit for bit; with shrink-wrap
+the per-frame threshold sums group the partials differently and agree to
+rounding."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [128, 256, 384, 512, 256, 384]
+
+
+@pytest.fixture(scope="module")
+def cdx():
+    import cdixx_amd
+    return cdixx_amd
+
+
+def _fill(cdx, b, seed0=100):
+    radii = [0.16 * n for n in b.sizes]
+    photons = [1e6 * 10 ** (i % 3) for i in range(len(b))]
+    b.simulate(radii, photons, [seed0 + i for i in range(len(b))], bs_radius=3, support_scale=1.3)
+    return {i: {k: b.get(i, k) for k in ("F", "g", "I", "bs", "spt")} for i in range(len(b))}
+
+
+def _single(cdx, n, prec, d, alg, p, rcap=0):
+    ctx = cdx.Context(n, precision=prec)
+    ctx.set_threshold(0.5)
+    dF, dg, dI, ds, dbs = ctx.cplx(d["F"]), ctx.cplx(d["g"]), ctx.real(d["I"]), ctx.real(d["spt"]), ctx.real(d["bs"])
+    _, rt = ctx.phase_retrieve(alg, p, dF, dg, dI, ds, bs=dbs, rtrace_cap=rcap)
+    out = dict(F=ctx.get_cplx(dF), g=ctx.get_cplx(dg), spt=ctx.get_real(ds), thr=ctx.threshold(), rt=rt)
+    ctx.close()
+    return out
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_batch_matches_single_frames_bitwise(cdx, prec):
+    print(f"batch bitwise prec={prec}", flush=True)
+    b = cdx.Batch(SIZES, precision=prec)
+    d = _fill(cdx, b)
+    p = cdx.Params.default(beta=0.9, useBS=1, freeBeamstop=1)
+    b.set_threshold(0.5)
+    alg = "10*HIO+3*RAAR+4*ER"
+    nit, _ = b.phase_retrieve(alg, p)
+    assert nit == 17
+    for i, n in enumerate(SIZES):
+        ref = _single(cdx, n, prec, d[i], alg, p)
+        assert np.array_equal(b.get(i, "g"), ref["g"]), i
+        assert np.array_equal(b.get(i, "F"), ref["F"]), i
+    b.close()
+
+
+def test_batch_shrinkwrap_and_rtrace(cdx):
+    b = cdx.Batch(SIZES, precision=1)
+    d = _fill(cdx, b, seed0=7)
+    p = cdx.Params.default(beta=0.9, shrinkThreshold=0.05, recordR=1)
+    b.set_threshold(0.5)
+    alg = "2*(8*HIO+1*shrinkWrap)+3*ER"
+    nit, rt = b.phase_retrieve(alg, p, rtrace_cap=19)
+    thr = b.thresholds()
+    for i, n in enumerate(SIZES):
+        ref = _single(cdx, n, 1, d[i], alg, p, rcap=19)
+        assert abs(thr[i] - ref["thr"]) <= 1e-12 * abs(ref["thr"]), i
+        assert rel_l2(b.get(i, "spt"), ref["spt"]) < 1e-10
+        assert rel_l2(b.get(i, "g"), ref["g"]) < 1e-9
+        assert np.allclose(rt[i], ref["rt"], rtol=1e-9, atol=1e-13)
+    b.close()
+
+
+def test_batch_rejects_unsupported_sizes(cdx):
+    with pytest.raises(cdx.CdxError):
+        cdx.Batch([256, 1024])

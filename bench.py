@@ -102,13 +102,11 @@ def batched_leg(cdx, iters):
                      c.real(frames[i]["spt"])))
     for c, F, g, I, s in bufs[:3]:
         c.time_iterations(p, cdx.HIO, F, g, I, s, 20, per_kernel=False)  # warm
+    # each frame's `iters` iterations as one captured graph, timed with
+    # events like the batched run (capture and instantiation excluded)
     ms_loop = 0.0
     for c, F, g, I, s in bufs:
-        c.sync()
-        t0 = time.perf_counter()
-        c.time_iterations(p, cdx.HIO, F, g, I, s, iters, per_kernel=False)
-        c.sync()
-        ms_loop += (time.perf_counter() - t0) * 1e3
+        ms_loop += c.time_iterations(p, cdx.HIO, F, g, I, s, iters, per_kernel=False)[0]
     pix = sum(n * n for n in BATCH_SIZES)
     fps_b = len(BATCH_SIZES) / (ms_batch * 1e-3) * 1.0
     fps_l = len(BATCH_SIZES) / (ms_loop * 1e-3)

@@ -469,10 +469,10 @@ extern "C" int cdx_rfactor(cdx_ctx* c, const void* F, const void* I, const void*
   if (!F || !I || !out) return fail(CDX_ERR_ARG, "cdx_rfactor: null buffer");
   HIPCHK(launch_fill_f64(2, c->scal + S_ACC, 0.0, c->st));
   if (c->prec == CDX_F64) {
-    HIPCHK(launch_pack_intensity<double>(c->n, (const double*)I, (const double*)bs, (double*)c->Ipk, c->st));
+    HIPCHK(launch_pack_intensity<double>(c->n, (const double*)I, (const double*)bs, (double*)c->Ipk, 0, c->st));
     HIPCHK(launch_rfactor<double>(c->n, F, (const double*)c->Ipk, c->scal + S_ACC, c->st));
   } else {
-    HIPCHK(launch_pack_intensity<float>(c->n, (const float*)I, (const float*)bs, (float*)c->Ipk, c->st));
+    HIPCHK(launch_pack_intensity<float>(c->n, (const float*)I, (const float*)bs, (float*)c->Ipk, 0, c->st));
     HIPCHK(launch_rfactor<float>(c->n, F, (const float*)c->Ipk, c->scal + S_ACC, c->st));
   }
   double a, b;
@@ -512,7 +512,8 @@ struct Loop {
 
   int setup(const void* I, const void* bs) {
     const bool pack_bs = p->useBS && bs && (p->simCCDbit || p->freeBeamstop);
-    HIPCHK(launch_pack_intensity<T>(c->n, (const T*)I, pack_bs ? (const T*)bs : nullptr, (T*)c->Ipk, c->st));
+    HIPCHK(launch_pack_intensity<T>(c->n, (const T*)I, pack_bs ? (const T*)bs : nullptr, (T*)c->Ipk, !p->simCCDbit,
+                                    c->st));
     pa.mode = p->simCCDbit ? PROJ_MOD_NOISE : (pack_bs ? PROJ_MOD_ABS_BS : PROJ_MOD_ABS);
     pa.noiseLevel = p->noiseLevel;
     pa.scale = (T)(1.0 / p->exposure);
@@ -677,20 +678,19 @@ static int time_t_(cdx_ctx* c, const cdx_params* p, int algo, void* F, void* g, 
   if (exec) hipGraphExecDestroy(exec);
   if (graph) hipGraphDestroy(graph);
   if (col_ms || row_ms) {
-    // per-launch brackets on a bounded sample of the same iterations
+    // kernel execution times (start/end timestamps of each launch, as a
+    // kernel-trace profiler reports them) on a bounded sample of the same
+    // iterations, launched back to back
     n = n < 200 ? n : 200;
     std::vector<hipEvent_t> ev(4 * (size_t)n);
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
     UpdArgs<T> ua = upd_args<T>(c, algo, p->beta, p->relax);
     for (int i = 0; i < n; i++) {
-      HIPCHK(hipEventRecord(ev[4 * i], c->st));
+      const KTimer kc{ev[4 * i], ev[4 * i + 1]}, kr{ev[4 * i + 2], ev[4 * i + 3]};
       HIPCHK(launch_col<T>(COL_FWD_PROJ_INV, c->rows, c->cols, c->T1, c->T2, c->tw_col, (const T*)c->Ipk, L.pa, 0,
-                           c->st, c->col_lines));
-      HIPCHK(hipEventRecord(ev[4 * i + 1], c->st));
-      HIPCHK(hipEventRecord(ev[4 * i + 2], c->st));
+                           c->st, c->col_lines, &kc));
       HIPCHK(launch_row<T>(ROW_INV_UPDATE_FWD, c->rows, c->cols, c->T2, c->T1, c->tw_row, g, (const T*)spt, ua, (T)1,
-                           nullptr, ReduceSlots{}, c->st, c->row_lines));
-      HIPCHK(hipEventRecord(ev[4 * i + 3], c->st));
+                           nullptr, ReduceSlots{}, c->st, c->row_lines, &kr));
     }
     HIPCHK(hipStreamSynchronize(c->st));
     double sc = 0, sr = 0;

@@ -412,7 +412,8 @@ struct BLoop {
   }
   int setup() {
     const bool pack_bs = p->useBS && (p->simCCDbit || p->freeBeamstop);
-    BCHK(launch_pack_intensity<T>(b->total, (const T*)b->I, pack_bs ? (const T*)b->bs : nullptr, (T*)b->Ipk, b->st));
+    BCHK(launch_pack_intensity<T>(b->total, (const T*)b->I, pack_bs ? (const T*)b->bs : nullptr, (T*)b->Ipk,
+                                  !p->simCCDbit, b->st));
     pa = ProjArgs<T>{};
     pa.mode = p->simCCDbit ? PROJ_MOD_NOISE : (pack_bs ? PROJ_MOD_ABS_BS : PROJ_MOD_ABS);
     pa.noiseLevel = p->noiseLevel;

@@ -6,12 +6,15 @@ namespace cdx {
 
 // beamstop folded into the intensities as a NaN sentinel (bs > 0.5)
 template <class T>
-__global__ void k_pack(int64_t n, const T* __restrict__ I, const T* __restrict__ bs, T* __restrict__ out) {
-  EW_LOOP(n) out[i] = (bs && bs[i] > (T)0.5) ? (T)NAN : I[i];
+__global__ void k_pack(int64_t n, const T* __restrict__ I, const T* __restrict__ bs, T* __restrict__ out, int amp) {
+  EW_LOOP(n) {
+    const T v = I[i];
+    out[i] = (bs && bs[i] > (T)0.5) ? (T)NAN : amp ? (v > (T)0 ? dsqrt(v) : (T)0) : v;
+  }
 }
 template <class T>
-hipError_t launch_pack_intensity(int64_t n, const T* I, const T* bs, T* out, hipStream_t st) {
-  hipLaunchKernelGGL(k_pack<T>, ew_grid(n), dim3(256), 0, st, n, I, bs, out);
+hipError_t launch_pack_intensity(int64_t n, const T* I, const T* bs, T* out, int amp, hipStream_t st) {
+  hipLaunchKernelGGL(k_pack<T>, ew_grid(n), dim3(256), 0, st, n, I, bs, out, amp);
   return hipGetLastError();
 }
 
@@ -691,7 +694,7 @@ hipError_t launch_wavefront(int rows, int cols, const T* I, const T* P, int orow
   template hipError_t launch_poisson_wo<T>(int64_t, T*, double, T, T, uint64_t, hipStream_t);                    \
   template hipError_t launch_extend<T>(int64_t, const T*, void*, hipStream_t);                                   \
   template hipError_t launch_wavefront<T>(int, int, const T*, const T*, int, int, void*, hipStream_t);           \
-  template hipError_t launch_pack_intensity<T>(int64_t, const T*, const T*, T*, hipStream_t);                    \
+  template hipError_t launch_pack_intensity<T>(int64_t, const T*, const T*, T*, int, hipStream_t);                    \
   template hipError_t launch_mod_abs<T>(int64_t, void*, const T*, hipStream_t);                                  \
   template hipError_t launch_mod_noise<T>(int64_t, void*, const T*, const T*, int, T, T, T, hipStream_t);        \
   template hipError_t launch_support<T>(int64_t, void*, const void*, const T*, const UpdArgs<T>&, hipStream_t);  \

@@ -68,18 +68,27 @@ namespace cdx {
 int tiling_valid(int n, size_t csz, int lines);
 int tiling_default(int n, size_t csz, int column_pass);
 
+// Events that receive the start and end timestamps of one kernel's
+// execution (hipExtLaunchKernel), the figure a kernel-trace profiler reports.
+struct KTimer {
+  hipEvent_t start, stop;
+};
+
 // The passes; tw = twiddle table of the transformed line length (2n
-// entries); lines = lines per workgroup (0: default).
+// entries); lines = lines per workgroup (0: default); kt = optional timer.
 template <class T>
 hipError_t launch_col(int op, int rows, int cols, const void* in, void* out, const void* tw,
-                      const T* I, const ProjArgs<T>& pa, double gauss_sigma, hipStream_t st, int lines = 0);
+                      const T* I, const ProjArgs<T>& pa, double gauss_sigma, hipStream_t st, int lines = 0,
+                      const KTimer* kt = nullptr);
 template <class T>
 hipError_t launch_row(int op, int rows, int cols, const void* in, void* out, const void* tw,
                       void* g, const T* spt, const UpdArgs<T>& ua, T scale, T* sout,
-                      const ReduceSlots& rs, hipStream_t st, int lines = 0);
+                      const ReduceSlots& rs, hipStream_t st, int lines = 0, const KTimer* kt = nullptr);
 
 // elementwise / small kernels
-template <class T> hipError_t launch_pack_intensity(int64_t n, const T* I, const T* bs, T* out, hipStream_t st);
+// packed measurement: NaN where bs > 0.5 (unmeasured), else I, or the
+// amplitude sqrt(max(I, 0)) when amp != 0 (what the modulus projection uses)
+template <class T> hipError_t launch_pack_intensity(int64_t n, const T* I, const T* bs, T* out, int amp, hipStream_t st);
 template <class T> hipError_t launch_mod_abs(int64_t n, void* F, const T* I, hipStream_t st);
 template <class T> hipError_t launch_mod_noise(int64_t n, void* F, const T* I, const T* bs, int noise, T scale,
                                                T rcolor, T bsnorm, hipStream_t st);

@@ -24,32 +24,43 @@ namespace cdx {
 // ---------------------------------------------------------------------------
 // pointwise operations with compile-time selection
 // ---------------------------------------------------------------------------
-// applyModAbs (cudaConfig.cu:827-841): the FP32 build scales by
-// sqrt(I)*rsqrt(|F|^2) (within 2 ulp of the reference's sqrt(I)/hypot(F))
+// applyModAbs (cudaConfig.cu:827-841) on the packed amplitude A = sqrt(max(I,0)):
+// F *= A/|F|, F = (A, 0) where |F| = 0.  The FP32 build scales by A*rsqrt(|F|^2)
+// (within 2 ulp of the reference's sqrt(I)/hypot(F)).
 template <class T>
-__device__ __forceinline__ cplx<T> mod_abs_fast(cplx<T> f, T I) {
+__device__ __forceinline__ cplx<T> mod_amp(cplx<T> f, T A) {
   if constexpr (sizeof(T) == 4) {
     const float m2 = f.x * f.x + f.y * f.y;
-    const float r = I > 0.f ? sqrtf(I) : 0.f;
     if (m2 > 0.f) {
-      const float s = r * rsqrtf(m2);
+      const float s = A * rsqrtf(m2);
       return {f.x * s, f.y * s};
     }
-    return {r, 0.f};
+    return {A, 0.f};
   } else {
-    return mod_abs(f, I);
+    const double mod = dhypot(f.x, f.y);
+    if (mod == 0.0) return {A, 0.0};
+    const double s = A / mod;
+    return {f.x * s, f.y * s};
   }
 }
 
+// The packed operand holds the amplitude for the modulus modes and the
+// intensity for the noise-tolerant mode (NaN marks unmeasured pixels).
+template <int MODE>
+__device__ __forceinline__ bool packed_amplitude(int runtime_mode) {
+  if constexpr (MODE < 0) return runtime_mode != PROJ_MOD_NOISE;
+  else return MODE != PROJ_MOD_NOISE;
+}
+
 template <class T, int MODE>
-__device__ __forceinline__ cplx<T> project_m(cplx<T> f, T I, const ProjArgs<T>& pa) {
-  if constexpr (MODE == PROJ_MOD_NOISE) return mod_noise(f, I, pa.noiseLevel, pa.scale, pa.rcolor);
-  else if constexpr (MODE == PROJ_MOD_ABS_BS) return isnan(I) ? f : mod_abs_fast(f, I);
-  else if constexpr (MODE == PROJ_MOD_ABS) return mod_abs_fast(f, I);
+__device__ __forceinline__ cplx<T> project_m(cplx<T> f, T m, const ProjArgs<T>& pa) {
+  if constexpr (MODE == PROJ_MOD_NOISE) return mod_noise(f, m, pa.noiseLevel, pa.scale, pa.rcolor);
+  else if constexpr (MODE == PROJ_MOD_ABS_BS) return isnan(m) ? f : mod_amp(f, m);
+  else if constexpr (MODE == PROJ_MOD_ABS) return mod_amp(f, m);
   else {  // runtime mode, same arithmetic as the compile-time variants
-    if (pa.mode == PROJ_MOD_NOISE) return mod_noise(f, I, pa.noiseLevel, pa.scale, pa.rcolor);
-    if (pa.mode == PROJ_MOD_ABS_BS && isnan(I)) return f;
-    return mod_abs_fast(f, I);
+    if (pa.mode == PROJ_MOD_NOISE) return mod_noise(f, m, pa.noiseLevel, pa.scale, pa.rcolor);
+    if (pa.mode == PROJ_MOD_ABS_BS && isnan(m)) return f;
+    return mod_amp(f, m);
   }
 }
 
@@ -105,22 +116,20 @@ __device__ __forceinline__ void col_body(const cplx<T>* __restrict__ in, cplx<T>
     return;
   }
 
+  // R-factor sums (|F| against the measured amplitude), reduced after the
+  // stores so the workgroup barrier does not stall the transforms
+  const bool rf = proj && (RF < 0 ? pa.rf != nullptr : RF > 0);
+  T rnum = 0, rden = 0;
   if constexpr (proj) {
-    const bool rf = RF < 0 ? pa.rf != nullptr : RF > 0;
-    T rnum = 0, rden = 0;
+    const bool amp = packed_amplitude<MODE>(pa.mode);
 #pragma unroll
     for (int i = 0; i < E; i++) {
       if (rf && !isnan(iv[i])) {
-        const T s = iv[i] > (T)0 ? dsqrt(iv[i]) : (T)0;
+        const T s = amp ? iv[i] : (iv[i] > (T)0 ? dsqrt(iv[i]) : (T)0);
         rnum += fabs(dsqrt(v[i].x * v[i].x + v[i].y * v[i].y) - s);
         rden += s;
       }
       if (ok) v[i] = project_m<T, MODE>(v[i], iv[i], pa);
-    }
-    if (rf) {
-      __shared__ double sh[16];
-      block_atomic_sum((double)rnum, pa.rf, sh);
-      block_atomic_sum((double)rden, pa.rf + 1, sh);
     }
   }
   if constexpr (OP == COL_FWD_GAUSS_INV || OP == COL_GAUSS_INV) {
@@ -140,6 +149,11 @@ __device__ __forceinline__ void col_body(const cplx<T>* __restrict__ in, cplx<T>
   if (ok) {
 #pragma unroll
     for (int i = 0; i < E; i++) out[(size_t)L::din_pos(i, t) * cols + col] = v[i];
+  }
+  if (rf) {
+    __shared__ double sh[16];
+    block_atomic_sum((double)rnum, pa.rf, sh);
+    block_atomic_sum((double)rden, pa.rf + 1, sh);
   }
 }
 

@@ -22,6 +22,8 @@
 // This file is compiled four times (CDX_PASS_T = float/double, CDX_PASS_ROW =
 // 0/1) so the instantiations build in parallel; CDX_PASS_HOST marks the
 // translation unit that also defines the host-side helpers.
+#include <hip/hip_ext.h>
+
 #include "pass_bodies.hpp"
 
 #ifndef CDX_PASS_T
@@ -119,7 +121,7 @@ static int log2i(int v) {
 #if CDX_PASS_ROW == 0
 template <class T, int N, int OP, int MODE, int RF>
 static hipError_t col_launch_n(int cols, const void* in, void* out, const void* tw, const T* I,
-                               const ProjArgs<T>& pa, double gs, hipStream_t st, int lines) {
+                               const ProjArgs<T>& pa, double gs, hipStream_t st, int lines, const KTimer* kt) {
   static size_t lds_done = 0;
   constexpr int TL = LineCfg<N>::TL;
   if (!tiling_valid(N, sizeof(cplx<T>), lines)) return hipErrorInvalidConfiguration;
@@ -128,22 +130,26 @@ static hipError_t col_launch_n(int cols, const void* in, void* out, const void* 
   hipError_t e = lds_limit(kern, lds, lds_done);
   if (e != hipSuccess) return e;
   dim3 grid((cols + lines - 1) / lines), block(lines * TL);
-  hipLaunchKernelGGL(kern, grid, block, lds, st, (const cplx<T>*)in, (cplx<T>*)out, cols, (const cplx<T>*)tw, I,
-                     pa, (T)gs, log2i(lines));
+  if (kt)
+    hipExtLaunchKernelGGL(kern, grid, block, (uint32_t)lds, st, kt->start, kt->stop, 0, (const cplx<T>*)in,
+                          (cplx<T>*)out, cols, (const cplx<T>*)tw, I, pa, (T)gs, log2i(lines));
+  else
+    hipLaunchKernelGGL(kern, grid, block, lds, st, (const cplx<T>*)in, (cplx<T>*)out, cols, (const cplx<T>*)tw, I,
+                       pa, (T)gs, log2i(lines));
   return hipGetLastError();
 }
 
 template <class T, int N>
 static hipError_t col_launch_sz(int op, int cols, const void* in, void* out, const void* tw, const T* I,
-                                const ProjArgs<T>& pa, double gs, hipStream_t st, int lines) {
+                                const ProjArgs<T>& pa, double gs, hipStream_t st, int lines, const KTimer* kt) {
   const bool rf = pa.rf != nullptr;
   switch (op) {
-    case COL_FWD: return col_launch_n<T, N, COL_FWD, 0, 0>(cols, in, out, tw, I, pa, gs, st, lines);
-    case COL_INV: return col_launch_n<T, N, COL_INV, 0, 0>(cols, in, out, tw, I, pa, gs, st, lines);
+    case COL_FWD: return col_launch_n<T, N, COL_FWD, 0, 0>(cols, in, out, tw, I, pa, gs, st, lines, kt);
+    case COL_INV: return col_launch_n<T, N, COL_INV, 0, 0>(cols, in, out, tw, I, pa, gs, st, lines, kt);
     case COL_FWD_PROJ_INV:
 #define M(mode)                                                                                     \
-  return rf ? col_launch_n<T, N, COL_FWD_PROJ_INV, mode, 1>(cols, in, out, tw, I, pa, gs, st, lines) \
-            : col_launch_n<T, N, COL_FWD_PROJ_INV, mode, 0>(cols, in, out, tw, I, pa, gs, st, lines);
+  return rf ? col_launch_n<T, N, COL_FWD_PROJ_INV, mode, 1>(cols, in, out, tw, I, pa, gs, st, lines, kt) \
+            : col_launch_n<T, N, COL_FWD_PROJ_INV, mode, 0>(cols, in, out, tw, I, pa, gs, st, lines, kt);
       switch (pa.mode) {
         case PROJ_MOD_ABS: M(PROJ_MOD_ABS)
         case PROJ_MOD_NOISE: M(PROJ_MOD_NOISE)
@@ -151,32 +157,32 @@ static hipError_t col_launch_sz(int op, int cols, const void* in, void* out, con
         default: return hipErrorInvalidValue;
       }
 #undef M
-    case COL_PROJ_INV: return col_launch_n<T, N, COL_PROJ_INV, -1, -1>(cols, in, out, tw, I, pa, gs, st, lines);
+    case COL_PROJ_INV: return col_launch_n<T, N, COL_PROJ_INV, -1, -1>(cols, in, out, tw, I, pa, gs, st, lines, kt);
     case COL_FWD_GAUSS_INV:
-      return col_launch_n<T, N, COL_FWD_GAUSS_INV, 0, 0>(cols, in, out, tw, I, pa, gs, st, lines);
-    case COL_GAUSS_INV: return col_launch_n<T, N, COL_GAUSS_INV, 0, 0>(cols, in, out, tw, I, pa, gs, st, lines);
+      return col_launch_n<T, N, COL_FWD_GAUSS_INV, 0, 0>(cols, in, out, tw, I, pa, gs, st, lines, kt);
+    case COL_GAUSS_INV: return col_launch_n<T, N, COL_GAUSS_INV, 0, 0>(cols, in, out, tw, I, pa, gs, st, lines, kt);
     default: return hipErrorInvalidValue;
   }
 }
 
 template <class T>
 hipError_t launch_col(int op, int rows, int cols, const void* in, void* out, const void* tw, const T* I,
-                      const ProjArgs<T>& pa, double gs, hipStream_t st, int lines) {
+                      const ProjArgs<T>& pa, double gs, hipStream_t st, int lines, const KTimer* kt) {
   if (lines == 0) lines = tiling_default(rows, sizeof(cplx<T>), 1);
   switch (rows) {
-#define F(n) case n: return col_launch_sz<T, n>(op, cols, in, out, tw, I, pa, gs, st, lines);
+#define F(n) case n: return col_launch_sz<T, n>(op, cols, in, out, tw, I, pa, gs, st, lines, kt);
     CDX_SIZES(F)
 #undef F
     default: return hipErrorInvalidValue;
   }
 }
 template hipError_t launch_col<CDX_PASS_T>(int, int, int, const void*, void*, const void*, const CDX_PASS_T*,
-                                           const ProjArgs<CDX_PASS_T>&, double, hipStream_t, int);
+                                           const ProjArgs<CDX_PASS_T>&, double, hipStream_t, int, const KTimer*);
 #else
 template <class T, int N, int OP, int ALGO>
 static hipError_t row_launch_n(int rows, const void* in, void* out, const void* tw, void* g, const T* spt,
                                const UpdArgs<T>& ua, T scale, T* sout, const ReduceSlots& rs, hipStream_t st,
-                               int lines) {
+                               int lines, const KTimer* kt) {
   static size_t lds_done = 0;
   constexpr int TL = LineCfg<N>::TL;
   if (!tiling_valid(N, sizeof(cplx<T>), lines)) return hipErrorInvalidConfiguration;
@@ -185,23 +191,28 @@ static hipError_t row_launch_n(int rows, const void* in, void* out, const void* 
   hipError_t e = lds_limit(kern, lds, lds_done);
   if (e != hipSuccess) return e;
   dim3 grid((rows + lines - 1) / lines), block(lines * TL);
-  hipLaunchKernelGGL(kern, grid, block, lds, st, (const cplx<T>*)in, (cplx<T>*)out, rows, (const cplx<T>*)tw,
-                     (cplx<T>*)g, spt, ua, scale, sout, rs, log2i(lines));
+  if (kt)
+    hipExtLaunchKernelGGL(kern, grid, block, (uint32_t)lds, st, kt->start, kt->stop, 0, (const cplx<T>*)in,
+                          (cplx<T>*)out, rows, (const cplx<T>*)tw, (cplx<T>*)g, spt, ua, scale, sout, rs,
+                          log2i(lines));
+  else
+    hipLaunchKernelGGL(kern, grid, block, lds, st, (const cplx<T>*)in, (cplx<T>*)out, rows, (const cplx<T>*)tw,
+                       (cplx<T>*)g, spt, ua, scale, sout, rs, log2i(lines));
   return hipGetLastError();
 }
 
 template <class T, int N>
 static hipError_t row_launch_sz(int op, int rows, const void* in, void* out, const void* tw, void* g,
                                 const T* spt, const UpdArgs<T>& ua, T scale, T* sout, const ReduceSlots& rs,
-                                hipStream_t st, int lines) {
+                                hipStream_t st, int lines, const KTimer* kt) {
   switch (op) {
-    case ROW_FWD: return row_launch_n<T, N, ROW_FWD, 0>(rows, in, out, tw, g, spt, ua, scale, sout, rs, st, lines);
-    case ROW_INV: return row_launch_n<T, N, ROW_INV, 0>(rows, in, out, tw, g, spt, ua, scale, sout, rs, st, lines);
+    case ROW_FWD: return row_launch_n<T, N, ROW_FWD, 0>(rows, in, out, tw, g, spt, ua, scale, sout, rs, st, lines, kt);
+    case ROW_INV: return row_launch_n<T, N, ROW_INV, 0>(rows, in, out, tw, g, spt, ua, scale, sout, rs, st, lines, kt);
     case ROW_INV_MOD2:
-      return row_launch_n<T, N, ROW_INV_MOD2, 0>(rows, in, out, tw, g, spt, ua, scale, sout, rs, st, lines);
+      return row_launch_n<T, N, ROW_INV_MOD2, 0>(rows, in, out, tw, g, spt, ua, scale, sout, rs, st, lines, kt);
     case ROW_INV_UPDATE_FWD:
 #define A(a) \
-  return row_launch_n<T, N, ROW_INV_UPDATE_FWD, a>(rows, in, out, tw, g, spt, ua, scale, sout, rs, st, lines);
+  return row_launch_n<T, N, ROW_INV_UPDATE_FWD, a>(rows, in, out, tw, g, spt, ua, scale, sout, rs, st, lines, kt);
       switch (ua.algo) {
         case 0: A(0)
         case 1: A(1)
@@ -218,10 +229,10 @@ static hipError_t row_launch_sz(int op, int rows, const void* in, void* out, con
 template <class T>
 hipError_t launch_row(int op, int rows, int cols, const void* in, void* out, const void* tw, void* g,
                       const T* spt, const UpdArgs<T>& ua, T scale, T* sout, const ReduceSlots& rs,
-                      hipStream_t st, int lines) {
+                      hipStream_t st, int lines, const KTimer* kt) {
   if (lines == 0) lines = tiling_default(cols, sizeof(cplx<T>), 0);
   switch (cols) {
-#define F(n) case n: return row_launch_sz<T, n>(op, rows, in, out, tw, g, spt, ua, scale, sout, rs, st, lines);
+#define F(n) case n: return row_launch_sz<T, n>(op, rows, in, out, tw, g, spt, ua, scale, sout, rs, st, lines, kt);
     CDX_SIZES(F)
 #undef F
     default: return hipErrorInvalidValue;
@@ -229,7 +240,7 @@ hipError_t launch_row(int op, int rows, int cols, const void* in, void* out, con
 }
 template hipError_t launch_row<CDX_PASS_T>(int, int, int, const void*, void*, const void*, void*,
                                            const CDX_PASS_T*, const UpdArgs<CDX_PASS_T>&, CDX_PASS_T, CDX_PASS_T*,
-                                           const ReduceSlots&, hipStream_t, int);
+                                           const ReduceSlots&, hipStream_t, int, const KTimer*);
 #endif
 
 }  // namespace cdx

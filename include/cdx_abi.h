@@ -206,8 +206,9 @@ int cdx_read_config_json(const char* path, char* buf, int cap);
 /* Runs n_iter iterations of `algo` from the current state (F pending on the
  * first call) without shrink-wrap.  ms_total: wall time of the n_iter
  * iterations between two events.  If col_ms/row_ms are given, a second run
- * brackets every launch with events and returns the mean duration of the
- * column pass and of the row pass. */
+ * of min(n_iter, 200) iterations records each launch's kernel start and end
+ * timestamps (hipExtLaunchKernel events) and returns the mean execution time
+ * of the column pass and of the row pass. */
 int cdx_time_iterations(cdx_ctx* ctx, const cdx_params* p, int algo, void* F, void* g, const void* I,
                         const void* bs, const void* spt, int n_iter, int use_graph, double* ms_total,
                         double* col_ms, double* row_ms);

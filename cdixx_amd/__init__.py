@@ -45,7 +45,8 @@ class Params(ctypes.Structure):
                 ("noiseLevel", ctypes.c_int), ("rcolor", ctypes.c_int), ("exposure", ctypes.c_double),
                 ("beta", ctypes.c_double), ("relax", ctypes.c_double), ("shrinkThreshold", ctypes.c_double),
                 ("shrinkMode", ctypes.c_int), ("sigma0", ctypes.c_double), ("sigmaMin", ctypes.c_double),
-                ("sigmaDecay", ctypes.c_double), ("recordR", ctypes.c_int), ("useGraph", ctypes.c_int)]
+                ("sigmaDecay", ctypes.c_double), ("recordR", ctypes.c_int), ("useGraph", ctypes.c_int),
+                ("prtfIter", ctypes.c_int)]
 
     @classmethod
     def default(cls, **kw):
@@ -93,6 +94,7 @@ _SIGS = {
     "cdx_sim_intensity": ([_vp, _vp, _d, _u64, _i, _vp], _i),
     "cdx_read_config_json": ([ctypes.c_char_p, ctypes.c_char_p, _i], _i),
     "cdx_copy": ([_vp, _vp, _vp, _sz], _i),
+    "cdx_prtf": ([_vp, _dp, _dp, _i, _ip], _i),
     "cdx_batch_create": ([ctypes.POINTER(_vp), _i, _ip, _i, _vp], _i), "cdx_batch_destroy": ([_vp], _i),
     "cdx_batch_frame": ([_vp, _i] + [ctypes.POINTER(_vp)] * 5 + [_ip], _i),
     "cdx_batch_context": ([_vp, _i], _vp), "cdx_batch_set_threshold": ([_vp, _d], _i),
@@ -331,6 +333,14 @@ class Context:
                                      bs.ptr if bs else None, spt.ptr, rt if rtrace_cap else None, rtrace_cap,
                                      ctypes.byref(n_iter)))
         return n_iter.value, np.array(rt[:rtrace_cap]) if rtrace_cap else None
+
+    def prtf(self):
+        """PRTF of the last phase_retrieve with params.prtfIter >= 0 (cdi.cc:423-440):
+        (prtf sums, weights, count); prtf.txt holds prtf/weights."""
+        m = self.rows // 2
+        pr, wt, cnt = (ctypes.c_double * m)(), (ctypes.c_double * m)(), ctypes.c_int()
+        check(lib.cdx_prtf(self.h, pr, wt, m, ctypes.byref(cnt)))
+        return np.array(pr[:]), np.array(wt[:]), cnt.value
 
     def time_iterations(self, params, algo, F, g, I, spt, n_iter, bs=None, use_graph=True, per_kernel=True):
         tot, cm, rm = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()

@@ -46,6 +46,8 @@ enum {
   S_MMA = 8,     // [8],[9] min/max keys
   S_MMP = 10,    // [10],[11]
   S_TOT = 12,
+  S_PHI0 = 13,   // PRTF phi0 (cdi.cc:360)
+  S_THETA = 14,  // PRTF phase of the current step
   S_COUNT = 16
 };
 
@@ -68,6 +70,12 @@ struct cdx_ctx {
   int rcolor;
   double scale, beta, relax;
   int col_lines = 0, row_lines = 0;  // lines per workgroup of the passes (0: default)
+  // PRTF of the last cdx_phase_retrieve (allocated on first use)
+  void* prtf_map = nullptr;   // summed unit phasors, complex N
+  void* gamma0 = nullptr;     // reference object P_S(g'), complex N
+  double* prtf_dev = nullptr; // [0, rows/2): prtf sums; [1024, 1024 + rows/2): weights
+  std::vector<double> prtf_h, weights_h;
+  int prtf_count = 0;
 };
 
 // twiddle table of a line length n: tw[m] = W_n^m = exp(-2 pi i m/n), m < n,
@@ -199,6 +207,9 @@ extern "C" int cdx_destroy(cdx_ctx* c) {
   hipFree(c->scal);
   hipFree(c->partials);
   hipFree(c->ring);
+  hipFree(c->prtf_map);
+  hipFree(c->gamma0);
+  hipFree(c->prtf_dev);
   if (c->own_stream && c->st) hipStreamDestroy(c->st);
   delete c;
   return CDX_OK;
@@ -494,6 +505,7 @@ extern "C" void cdx_params_default(cdx_params* p) {
   p->sigma0 = 3;
   p->sigmaMin = 2;
   p->sigmaDecay = 0.99;
+  p->prtfIter = -1;
 }
 
 // ---------------------------------------------------------------------------
@@ -532,6 +544,29 @@ struct Loop {
     UpdArgs<T> ua = upd_args<T>(c, algo, p->beta, p->relax);
     HIPCHK(launch_row<T>(ROW_INV_UPDATE_FWD, c->rows, c->cols, c->T2, c->T1, c->tw_row, g, spt, ua, (T)1, nullptr,
                          ReduceSlots{}, c->st, c->row_lines));
+    f_pending = false;
+    return CDX_OK;
+  }
+  // one iteration with the PRTF accumulation of cdi.cc:355-380: the column
+  // pass also stores the projected F (patternWave), the row pass forms the
+  // phase-alignment sums, then map += phasor(F) e^{i theta}
+  int iterate_prtf(int algo, double* rf, bool first) {
+    pa.rf = rf;
+    pa.fsave = F;
+    hipError_t e = f_pending ? launch_col<T>(COL_PROJ_INV, c->rows, c->cols, F, c->T2, c->tw_col, (const T*)c->Ipk,
+                                             pa, 0, c->st, c->col_lines)
+                             : launch_col<T>(COL_FWD_PROJ_INV, c->rows, c->cols, c->T1, c->T2, c->tw_col,
+                                             (const T*)c->Ipk, pa, 0, c->st, c->col_lines);
+    pa.fsave = nullptr;
+    HIPCHK(e);
+    UpdArgs<T> ua = upd_args<T>(c, algo, p->beta, p->relax);
+    ua.gamma0 = c->gamma0;
+    ua.prtf = first ? 1 : 2;
+    HIPCHK(launch_row<T>(ROW_INV_UPDATE_FWD_PRTF, c->rows, c->cols, c->T2, c->T1, c->tw_row, g, spt, ua, (T)1,
+                         nullptr, slots(c), c->st, c->row_lines));
+    HIPCHK(launch_prtf_phase(c->scal + S_PHI0, c->scal + S_THETA, slots(c), first, c->st));
+    HIPCHK(launch_prtf_add<T>(c->n, c->prtf_map, F, c->scal + S_THETA, c->st));
+    if (first) HIPCHK(launch_radial<T>(F, c->rows, c->cols, 0, (T)1, c->prtf_dev + 1024, c->st));
     f_pending = false;
     return CDX_OK;
   }
@@ -585,21 +620,43 @@ static int phase_retrieve_t(cdx_ctx* c, const char* algorithm, const cdx_params*
   const int nrec = p->recordR ? (rcap < c->ring_cap ? rcap : c->ring_cap) : 0;
   if (nrec > 0) HIPCHK(launch_fill_f64(2 * (int64_t)nrec, c->ring, 0.0, c->st));
 
+  // PRTF buffers (before any capture); the map starts at zero
+  const bool want_prtf = p->prtfIter >= 0;
+  if (want_prtf) {
+    if (!c->prtf_map) HIPCHK(hipMalloc(&c->prtf_map, c->n * c->csz));
+    if (!c->gamma0) HIPCHK(hipMalloc(&c->gamma0, c->n * c->csz));
+    if (!c->prtf_dev) HIPCHK(hipMalloc((void**)&c->prtf_dev, 2048 * sizeof(double)));
+    HIPCHK(launch_fill_f64(c->n * (int64_t)c->csz / 8, (double*)c->prtf_map, 0.0, c->st));
+    HIPCHK(launch_fill_f64(2048, c->prtf_dev, 0.0, c->st));
+  }
+
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   if (p->useGraph) HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
   double sigma = p->sigma0;
   int iter = 0, nproj = 0;
+  bool prtf_started = false;
   for (int tok : toks) {
     if (tok == CDX_SHRINKWRAP) {
       if ((rc = L.shrink_wrap(sigma))) break;
       if (sigma > p->sigmaMin) sigma *= p->sigmaDecay;
     } else {
       double* rf = nproj < nrec ? c->ring + 2 * nproj : nullptr;
-      if ((rc = L.iterate(tok, rf))) break;
+      if (want_prtf && iter >= p->prtfIter) {
+        if ((rc = L.iterate_prtf(tok, rf, !prtf_started))) break;
+        prtf_started = true;
+      } else if ((rc = L.iterate(tok, rf))) {
+        break;
+      }
       nproj++;
     }
     iter++;
+  }
+  // final PRTF: radial sums of |map| / (iter - prtfIter) (cdi.cc:423-433)
+  if (!rc && prtf_started) {
+    hipError_t e = launch_radial<T>(c->prtf_map, c->rows, c->cols, 1, (T)(1. / (iter - p->prtfIter)), c->prtf_dev,
+                                    c->st);
+    if (e != hipSuccess) rc = fail(CDX_ERR_HIP, "launch_radial: %s", hipGetErrorString(e));
   }
   if (!rc) rc = L.finish();
   if (p->useGraph) {
@@ -619,6 +676,16 @@ static int phase_retrieve_t(cdx_ctx* c, const char* algorithm, const cdx_params*
   if (rc) return rc;
   HIPCHK(launch_norm_c<T>(c->n, g, rootN, c->st));  // cdi.cc:406
   if (n_iter) *n_iter = iter;
+  c->prtf_count = prtf_started ? iter - p->prtfIter : 0;
+  c->prtf_h.assign((size_t)c->rows / 2, 0.0);
+  c->weights_h.assign((size_t)c->rows / 2, 0.0);
+  if (prtf_started) {
+    HIPCHK(hipMemcpyAsync(c->prtf_h.data(), c->prtf_dev, c->prtf_h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                          c->st));
+    HIPCHK(hipMemcpyAsync(c->weights_h.data(), c->prtf_dev + 1024, c->weights_h.size() * sizeof(double),
+                          hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+  }
   if (nrec > 0 && rtrace) {
     std::vector<double> h(2 * (size_t)nrec);
     HIPCHK(hipMemcpyAsync(h.data(), c->ring, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
@@ -635,6 +702,17 @@ extern "C" int cdx_phase_retrieve(cdx_ctx* c, const char* algorithm, const cdx_p
   if (!algorithm || !p || !F || !g || !I || !spt) return fail(CDX_ERR_ARG, "cdx_phase_retrieve: null argument");
   if (!(p->exposure != 0)) return fail(CDX_ERR_ARG, "cdx_phase_retrieve: exposure must be non-zero");
   return DISPATCH(c, phase_retrieve_t, c, algorithm, p, F, g, I, bs, spt, rtrace, rcap, n_iter);
+}
+
+extern "C" int cdx_prtf(cdx_ctx* c, double* prtf, double* weights, int cap, int* count) {
+  CTX_CHECK(c);
+  const int m = (int)c->prtf_h.size() < cap ? (int)c->prtf_h.size() : cap;
+  for (int i = 0; i < m; i++) {
+    if (prtf) prtf[i] = c->prtf_h[i];
+    if (weights) weights[i] = c->weights_h[i];
+  }
+  if (count) *count = c->prtf_count;
+  return CDX_OK;
 }
 
 // ---------------------------------------------------------------------------

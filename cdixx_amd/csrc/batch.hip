@@ -596,6 +596,8 @@ extern "C" int cdx_batch_phase_retrieve(cdx_batch* b, const char* algorithm, con
                                         int rcap, int* n_iter) {
   if (!b || !algorithm || !p) return bfail(CDX_ERR_ARG, "cdx_batch_phase_retrieve: null argument");
   if (!(p->exposure != 0)) return bfail(CDX_ERR_ARG, "cdx_batch_phase_retrieve: exposure must be non-zero");
+  if (p->prtfIter >= 0)
+    return bfail(CDX_ERR_UNSUPPORTED, "cdx_batch_phase_retrieve: PRTF is computed by single-frame contexts only");
   return b->prec == CDX_F64 ? batch_retrieve_t<double>(b, algorithm, p, rtrace, rcap, n_iter)
                             : batch_retrieve_t<float>(b, algorithm, p, rtrace, rcap, n_iter);
 }

@@ -290,6 +290,7 @@ complexFormat* CDI::phaseRetrieve() {
   p.sigmaMin = shrinkSigmaMin >= 0 ? shrinkSigmaMin : (shrinkWrapMode == 0 ? 2.0 : 1.5);
   p.sigmaDecay = shrinkSigmaDecay;
   p.useGraph = useGraph;
+  p.prtfIter = prtfIter;
   int n_iter = 0;
   chk(cdx_phase_retrieve(ctx, algorithm, &p, patternWave, objectWave, patternData, useBS ? beamstop : nullptr,
                          support, nullptr, 0, &n_iter),
@@ -298,6 +299,18 @@ complexFormat* CDI::phaseRetrieve() {
   chk(cdx_residual(ctx, patternWave, patternData, useBS ? beamstop : nullptr, &res), "residual");
   residual = (Real)res;
   if (verbose >= 1) printf("residual= %f\n", residual);
+  // prtf.txt: "i, prtf/weight" for i < row/2 (cdi.cc:435-439); written only
+  // when a step accumulated (the reference divides uninitialised weights
+  // otherwise)
+  std::vector<double> pr((size_t)row / 2), wt((size_t)row / 2);
+  int count = 0;
+  chk(cdx_prtf(ctx, pr.data(), wt.data(), row / 2, &count), "prtf");
+  if (count > 0) {
+    FILE* f = fopen("prtf.txt", "w");
+    if (!f) throw cdx::Error(CDX_ERR_IO, "phaseRetrieve: cannot write prtf.txt");
+    for (int i = 0; i < row / 2; i++) fprintf(f, "%d, %f\n", i, (float)(pr[i] / wt[i]));
+    fclose(f);
+  }
   return patternWave;
 }
 

@@ -150,8 +150,8 @@ __device__ __forceinline__ void block_atomic_sum(double v, double* dst, double* 
     atomicAdd(dst, s);
   }
 }
-// workgroup sum stored at rs.partials[blockIdx.x] (fixed-order reduction)
-__device__ __forceinline__ void block_partial_sum(double v, const ReduceSlots& rs, double* sh) {
+// workgroup sum of a per-thread double stored at *dst by thread 0
+__device__ __forceinline__ void block_store_sum(double v, double* dst, double* sh) {
   v = wave_sum(v);
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int nw = (blockDim.x + 63) >> 6;
@@ -161,9 +161,13 @@ __device__ __forceinline__ void block_partial_sum(double v, const ReduceSlots& r
   if (threadIdx.x == 0) {
     double s = 0;
     for (int i = 0; i < nw; i++) s += sh[i];
-    rs.partials[blockIdx.x] = s;
-    if (blockIdx.x == 0) *rs.nparts = (int)gridDim.x;
+    *dst = s;
   }
+}
+// workgroup sum stored at rs.partials[blockIdx.x] (fixed-order reduction)
+__device__ __forceinline__ void block_partial_sum(double v, const ReduceSlots& rs, double* sh) {
+  block_store_sum(v, rs.partials + blockIdx.x, sh);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *rs.nparts = (int)gridDim.x;
 }
 __device__ __forceinline__ void atomic_max_pos(void* bits, float v) {
   atomicMax((unsigned int*)bits, __float_as_uint(v));

@@ -687,6 +687,97 @@ hipError_t launch_wavefront(int rows, int cols, const T* I, const T* P, int orow
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// PRTF (cdi.cc:355-380 accumulation, 423-440 final)
+// ---------------------------------------------------------------------------
+// One wavefront: lane-strided partial sums in a fixed order, then a wave sum.
+__global__ void k_prtf_phase(double* phi0, double* theta, const double* partials, const int* nparts, int first) {
+  const int np = *nparts;
+  double re = 0, im = 0;
+  for (int i = threadIdx.x; i < np; i += 64) {
+    re += partials[2 * i];
+    im += partials[2 * i + 1];
+  }
+  re = wave_sum(re);
+  im = wave_sum(im);
+  if (threadIdx.x == 0) {
+    if (first) {
+      *phi0 = -atan2(im, re) / 2;
+      *theta = *phi0;
+    } else {
+      *theta = *phi0 + atan2(im, re);
+    }
+  }
+}
+hipError_t launch_prtf_phase(double* phi0, double* theta, const ReduceSlots& rs, int first, hipStream_t st) {
+  hipLaunchKernelGGL(k_prtf_phase, dim3(1), dim3(64), 0, st, phi0, theta, (const double*)rs.partials, rs.nparts,
+                     first);
+  return hipGetLastError();
+}
+
+// addPhase (cudaConfig.cu:876-887)
+template <class T>
+__global__ void k_prtf_add(int64_t n, cplx<T>* __restrict__ map, const cplx<T>* __restrict__ F,
+                           const double* theta) {
+  const T th = (T)*theta;
+  const T c = cos(th), s = sin(th);
+  EW_LOOP(n) {
+    const cplx<T> b = F[i];
+    const T m = dhypot(b.x, b.y);
+    if (m > (T)1e-6) {
+      const T r = b.x / m, q = b.y / m;
+      cplx<T> a = map[i];
+      a.x += r * c - q * s;
+      a.y += r * s + q * c;
+      map[i] = a;
+    }
+  }
+}
+template <class T>
+hipError_t launch_prtf_add(int64_t n, void* map, const void* F, const double* theta, hipStream_t st) {
+  hipLaunchKernelGGL(k_prtf_add<T>, ew_grid(n), dim3(256), 0, st, n, (cplx<T>*)map, (const cplx<T>*)F, theta);
+  return hipGetLastError();
+}
+
+// cart2polar_kernel + edgeReduce (cudaConfig.cu:1027-1063) over the fftshifted
+// image (cudaConvertFO, even sizes: a half-size shift on both axes), one
+// workgroup per radius.  Bilinear weights as bilinear(), cudaConfig.cu:1027-1047.
+template <class T>
+__device__ __forceinline__ T radial_sample(const cplx<T>* src, int rows, int cols, int x, int y, int kind, T norm) {
+  const int xs = x + (rows >> 1) < rows ? x + (rows >> 1) : x - (rows >> 1);
+  const int ys = y + (cols >> 1) < cols ? y + (cols >> 1) : y - (cols >> 1);
+  const cplx<T> v = src[(size_t)xs * cols + ys];
+  const T m = dhypot(v.x, v.y);
+  return kind == 0 ? (m > (T)1e-6 ? (T)1 : (T)0) : m * norm;
+}
+template <class T>
+__global__ void k_radial(const cplx<T>* __restrict__ src, int rows, int cols, int kind, T norm, double* out) {
+  const int r = blockIdx.x;
+  const int nang = 2 * cols;
+  double acc = 0;
+  for (int a = threadIdx.x; a < nang; a += blockDim.x) {
+    const T th = (T)(a * (2.0 * M_PI / nang));
+    const T X = (T)r * cos(th) + (T)(rows >> 1);
+    const T Y = (T)r * sin(th) + (T)(cols >> 1);
+    const int x0 = (int)floor(X), y0 = (int)floor(Y);
+    if (x0 < 0 || x0 + 1 >= rows || y0 < 0 || y0 + 1 >= cols) continue;
+    const T fx = X - (T)x0, fy = Y - (T)y0;
+    const T v00 = radial_sample(src, rows, cols, x0, y0, kind, norm);
+    const T v01 = radial_sample(src, rows, cols, x0, y0 + 1, kind, norm);
+    const T v10 = radial_sample(src, rows, cols, x0 + 1, y0, kind, norm);
+    const T v11 = radial_sample(src, rows, cols, x0 + 1, y0 + 1, kind, norm);
+    acc += (double)((1 - fx) * (1 - fy) * v00 + (1 - fx) * fy * v01 + fx * (1 - fy) * v10 + fx * fy * v11);
+  }
+  __shared__ double sh[16];
+  block_store_sum(acc, out + r, sh);
+}
+template <class T>
+hipError_t launch_radial(const void* src, int rows, int cols, int kind, T norm, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_radial<T>, dim3(rows / 2), dim3(256), 0, st, (const cplx<T>*)src, rows, cols, kind, norm,
+                     out);
+  return hipGetLastError();
+}
+
 // explicit instantiations
 #define INST(T)                                                                                                  \
   template hipError_t launch_fo_phase<T>(int, int, void*, T, hipStream_t);                                       \
@@ -716,7 +807,9 @@ hipError_t launch_wavefront(int rows, int cols, const T* I, const T* P, int orow
                                                float, void*, hipStream_t);                                       \
   template hipError_t launch_poisson_intensity<T>(int64_t, const void*, const double*, double, uint64_t, int, T*, \
                                                   hipStream_t);                                                  \
-  template hipError_t launch_sum_mod2<T>(int64_t, const void*, double*, hipStream_t);
+  template hipError_t launch_sum_mod2<T>(int64_t, const void*, double*, hipStream_t);                          \
+  template hipError_t launch_prtf_add<T>(int64_t, void*, const void*, const double*, hipStream_t);               \
+  template hipError_t launch_radial<T>(const void*, int, int, int, T, double*, hipStream_t);
 INST(float)
 INST(double)
 #undef INST

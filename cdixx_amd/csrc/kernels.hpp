@@ -28,6 +28,7 @@ enum RowOp : int {
   ROW_INV = 1,            // optional scale
   ROW_INV_UPDATE_FWD = 2, // inverse -> 1/N -> support update -> store g -> forward
   ROW_INV_MOD2 = 3,       // inverse -> |.|^2 -> S, max and sum (shrink-wrap)
+  ROW_INV_UPDATE_FWD_PRTF = 4,  // ROW_INV_UPDATE_FWD with the PRTF phase-alignment sums
 };
 
 template <class T>
@@ -37,6 +38,7 @@ struct ProjArgs {
   T scale;   // cudaVars.scale = 1/exposure
   T rcolor;  // cudaVars.rcolor
   double* rf;  // R-factor accumulators (num, den) or nullptr
+  void* fsave;  // if set (run-time-mode kernels only): projected F stored here
 };
 
 template <class T>
@@ -44,6 +46,8 @@ struct UpdArgs {
   int algo;
   T beta, relax, invN;
   const double* scal;  // scal[0] = support threshold
+  void* gamma0;        // PRTF reference object P_S(g') (ROW_INV_UPDATE_FWD_PRTF)
+  int prtf;            // 1: store gamma0, sum gamma0^2; 2: sum P_S(g') conj(gamma0)
 };
 
 // Deterministic reductions: every workgroup of the producing kernel stores
@@ -84,6 +88,18 @@ template <class T>
 hipError_t launch_row(int op, int rows, int cols, const void* in, void* out, const void* tw,
                       void* g, const T* spt, const UpdArgs<T>& ua, T scale, T* sout,
                       const ReduceSlots& rs, hipStream_t st, int lines = 0, const KTimer* kt = nullptr);
+
+// PRTF (cdi.cc:355-380,423-440).  prtf_phase sums the complex partials of
+// ROW_INV_UPDATE_FWD_PRTF: first step phi0 = -arg(sum)/2, theta = phi0;
+// later steps theta = phi0 + arg(sum).  prtf_add: map += (F/|F|) e^{i theta}
+// where |F| > 1e-6 (addPhase, cudaConfig.cu:876-887).  radial: out[r], r <
+// rows/2 = sum over 2 cols angles of the bilinear sample of the fftshifted
+// image at radius r (cart2polar_kernel + edgeReduce, cudaConfig.cu:1027-1063);
+// kind 0 samples the map |src| > 1e-6, kind 1 samples |src| * norm.
+hipError_t launch_prtf_phase(double* phi0, double* theta, const ReduceSlots& rs, int first, hipStream_t st);
+template <class T> hipError_t launch_prtf_add(int64_t n, void* map, const void* F, const double* theta, hipStream_t st);
+template <class T> hipError_t launch_radial(const void* src, int rows, int cols, int kind, T norm, double* out,
+                                            hipStream_t st);
 
 // elementwise / small kernels
 // packed measurement: NaN where bs > 0.5 (unmeasured), else I, or the

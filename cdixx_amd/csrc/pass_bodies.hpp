@@ -130,6 +130,10 @@ __device__ __forceinline__ void col_body(const cplx<T>* __restrict__ in, cplx<T>
         rden += s;
       }
       if (ok) v[i] = project_m<T, MODE>(v[i], iv[i], pa);
+      if constexpr (MODE < 0) {
+        // patternWave after the projection, for the PRTF (cdi.cc:362,377)
+        if (pa.fsave && ok) ((cplx<T>*)pa.fsave)[(size_t)L::dout_pos(i, t) * cols + col] = v[i];
+      }
     }
   }
   if constexpr (OP == COL_FWD_GAUSS_INV || OP == COL_GAUSS_INV) {
@@ -178,7 +182,8 @@ __device__ __forceinline__ void row_body(const cplx<T>* __restrict__ in, cplx<T>
   const bool ok = row < rows;
   cplx<T>* my = lds + rr * PITCH;
   const size_t base = (size_t)row * N;
-  constexpr bool upd = OP == ROW_INV_UPDATE_FWD;
+  constexpr bool prtf = OP == ROW_INV_UPDATE_FWD_PRTF;
+  constexpr bool upd = OP == ROW_INV_UPDATE_FWD || prtf;
   cplx<T> v[E];
 
   if constexpr (OP == ROW_FWD) {
@@ -193,6 +198,8 @@ __device__ __forceinline__ void row_body(const cplx<T>* __restrict__ in, cplx<T>
   } else {
     cplx<T> gv[upd ? E : 1];
     T sv[upd ? E : 1];
+    cplx<T> g0[prtf ? E : 1];
+    cplx<T>* gamma0 = (cplx<T>*)ua.gamma0;
 #pragma unroll
     for (int i = 0; i < E; i++) v[i] = ok ? in[base + L::dout_pos(i, t)] : cplx<T>{(T)0, (T)0};
     if constexpr (upd) {
@@ -200,6 +207,12 @@ __device__ __forceinline__ void row_body(const cplx<T>* __restrict__ in, cplx<T>
       for (int i = 0; i < E; i++) {
         gv[i] = ok ? g[base + L::din_pos(i, t)] : cplx<T>{(T)0, (T)0};
         sv[i] = ok ? spt[base + L::din_pos(i, t)] : (T)0;
+      }
+    }
+    if constexpr (prtf) {
+      if (ua.prtf == 2) {
+#pragma unroll
+        for (int i = 0; i < E; i++) g0[i] = ok ? gamma0[base + L::din_pos(i, t)] : cplx<T>{(T)0, (T)0};
       }
     }
     L::inverse(v, my, tw, t);
@@ -230,16 +243,35 @@ __device__ __forceinline__ void row_body(const cplx<T>* __restrict__ in, cplx<T>
     }
     if constexpr (upd) {
       const T thr = (T)ua.scal[0];
+      double are = 0, aim = 0;  // PRTF alignment sums (cdi.cc:358-360,373-375)
 #pragma unroll
       for (int i = 0; i < E; i++) {
         const cplx<T> p = {v[i].x * ua.invN, v[i].y * ua.invN};
-        v[i] = support_update(gv[i], p, sv[i] > thr, ALGO, ua.beta, ua.relax);
+        const bool in = sv[i] > thr;
+        if constexpr (prtf) {
+          if (ua.prtf == 1) {  // gamma0 = P_S(g'), sum gamma0^2
+            const cplx<T> q = in ? p : cplx<T>{(T)0, (T)0};
+            if (ok) gamma0[base + L::din_pos(i, t)] = q;
+            are += (double)q.x * q.x - (double)q.y * q.y;
+            aim += 2.0 * q.x * q.y;
+          } else if (in) {  // sum P_S(g') conj(gamma0)
+            are += (double)p.x * g0[i].x + (double)p.y * g0[i].y;
+            aim += (double)p.y * g0[i].x - (double)p.x * g0[i].y;
+          }
+        }
+        v[i] = support_update(gv[i], p, in, ALGO >= 0 ? ALGO : ua.algo, ua.beta, ua.relax);
         if (ok) g[base + L::din_pos(i, t)] = v[i];
       }
       L::forward(v, my, tw, t);
       if (ok) {
 #pragma unroll
         for (int i = 0; i < E; i++) out[base + L::dout_pos(i, t)] = v[i];
+      }
+      if constexpr (prtf) {
+        __shared__ double sh[16];
+        block_store_sum(are, rs.partials + 2 * blockIdx.x, sh);
+        block_store_sum(aim, rs.partials + 2 * blockIdx.x + 1, sh);
+        if (blockIdx.x == 0 && threadIdx.x == 0) *rs.nparts = (int)gridDim.x;
       }
     }
   }

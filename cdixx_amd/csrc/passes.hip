@@ -147,6 +147,7 @@ static hipError_t col_launch_sz(int op, int cols, const void* in, void* out, con
     case COL_FWD: return col_launch_n<T, N, COL_FWD, 0, 0>(cols, in, out, tw, I, pa, gs, st, lines, kt);
     case COL_INV: return col_launch_n<T, N, COL_INV, 0, 0>(cols, in, out, tw, I, pa, gs, st, lines, kt);
     case COL_FWD_PROJ_INV:
+      if (pa.fsave) return col_launch_n<T, N, COL_FWD_PROJ_INV, -1, -1>(cols, in, out, tw, I, pa, gs, st, lines, kt);
 #define M(mode)                                                                                     \
   return rf ? col_launch_n<T, N, COL_FWD_PROJ_INV, mode, 1>(cols, in, out, tw, I, pa, gs, st, lines, kt) \
             : col_launch_n<T, N, COL_FWD_PROJ_INV, mode, 0>(cols, in, out, tw, I, pa, gs, st, lines, kt);
@@ -222,6 +223,10 @@ static hipError_t row_launch_sz(int op, int rows, const void* in, void* out, con
         default: A(5)  // FHIO and every other token (the dispatcher's default)
       }
 #undef A
+    case ROW_INV_UPDATE_FWD_PRTF:
+      if (2 * (size_t)((rows + lines - 1) / lines) > (size_t)kMaxPartials) return hipErrorInvalidConfiguration;
+      return row_launch_n<T, N, ROW_INV_UPDATE_FWD_PRTF, -1>(rows, in, out, tw, g, spt, ua, scale, sout, rs, st,
+                                                             lines, kt);
     default: return hipErrorInvalidValue;
   }
 }

@@ -157,6 +157,7 @@ typedef struct cdx_params {
   double sigmaDecay;    /* 0.99 */
   int recordR;          /* record the R-factor before every projection */
   int useGraph;         /* capture the schedule in a hipGraph and launch it once */
+  int prtfIter;         /* PRTF accumulation from this iteration counter on (cdi.cc:355-380); -1 off */
 } cdx_params;
 void cdx_params_default(cdx_params* p);
 
@@ -169,6 +170,12 @@ void cdx_params_default(cdx_params* p);
  * n_iter (optional): value of the reference's iteration counter at exit. */
 int cdx_phase_retrieve(cdx_ctx* ctx, const char* algorithm, const cdx_params* p, void* F, void* g,
                        const void* I, const void* bs, void* spt, double* rtrace, int rtrace_cap, int* n_iter);
+
+/* PRTF of the last cdx_phase_retrieve with p->prtfIter >= 0 (cdi.cc:423-440):
+ * prtf[r] and weights[r], r < rows/2, are the radial sums whose ratio the
+ * reference writes to prtf.txt; *count = iter - prtfIter (0: no step
+ * accumulated, both arrays zero).  Host arrays, up to cap values each. */
+int cdx_prtf(cdx_ctx* ctx, double* prtf, double* weights, int cap, int* count);
 
 /* ---- batched reconstructions (no counterpart in the reference, which runs
  * one frame per process through CDI::phaseRetrieve): frames of sizes 128,

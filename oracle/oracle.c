@@ -555,19 +555,79 @@ void orc_iteration(int rows, int cols, ocpx* F, ocpx* g, const REAL* I, const RE
   orc_fft2d(rows, cols, g, F, -1);
 }
 
-/* CDI::phaseRetrieve, src/post_gpu/cdi.cc:297-421 (without PRTF).  g enters
- * as prepareIter leaves it (IFFT(F)/sqrt(N)) and leaves scaled by sqrt(N)
- * (cdi.cc:313,406).  rtrace (optional) receives the R-factor of F before
- * each projection.  Returns 0, or -1 for a bad schedule / unsupported token. */
-int orc_phase_retrieve(int rows, int cols, const char* algorithm, ocpx* F, ocpx* g,
-                       const REAL* I, const REAL* bs, REAL* spt, const orc_params* p,
-                       REAL* threshold_inout, double* rtrace, int rtrace_cap, int* n_iter_out) {
+/* ---- PRTF (cdi.cc:355-380 accumulation, 423-440 final) ---- */
+
+/* addPhase, cudaConfig.cu:876-887: where |b| > 1e-6, a += (b/|b|) e^{i phi} */
+void orc_add_phase(int64_t n, ocpx* a, const ocpx* b, REAL phi) {
+  const REAL c = (REAL)cos(phi), s = (REAL)sin(phi);
+  for (int64_t i = 0; i < n; i++) {
+    const REAL m = (REAL)hypot(b[i].re, b[i].im);
+    if (m > (REAL)1e-6) {
+      const REAL r = b[i].re / m, q = b[i].im / m;
+      a[i].re += r * c - q * s;
+      a[i].im += r * s + q * c;
+    }
+  }
+}
+
+/* bilinear sample at centred coordinates (X, Y) of a w x h row-major image,
+ * 0 where the 2x2 stencil leaves the image (cudaConfig.cu:1027-1047) */
+static REAL bilinear_centred(const REAL* img, int w, int h, REAL X, REAL Y) {
+  X += (REAL)(w >> 1);
+  Y += (REAL)(h >> 1);
+  const int x0 = (int)floor(X), y0 = (int)floor(Y);
+  if (x0 < 0 || x0 + 1 >= w || y0 < 0 || y0 + 1 >= h) return 0;
+  const REAL fx = X - (REAL)x0, fy = Y - (REAL)y0;
+  const size_t i0 = (size_t)x0 * h + y0;
+  return (1 - fx) * (1 - fy) * img[i0] + (1 - fx) * fy * img[i0 + 1] + fx * (1 - fy) * img[i0 + h] +
+         fx * fy * img[i0 + h + 1];
+}
+
+/* cart2polar_kernel (cudaConfig.cu:1050-1055) on a (2 cols) x (rows/2) polar
+ * grid, then edgeReduce (:1057-1063): out[r] = sum over the 2 cols angles
+ * theta_a = a * 2pi / (2 cols) of the sample at (r cos theta_a, r sin theta_a). */
+void orc_radial_sum(int rows, int cols, const REAL* img, REAL* out) {
+  const int nang = 2 * cols, nrad = rows / 2;
+  for (int r = 0; r < nrad; r++) out[r] = 0;
+  for (int a = 0; a < nang; a++) {
+    const REAL th = (REAL)(a * (2.0 * M_PI / nang));
+    const REAL c = (REAL)cos(th), s = (REAL)sin(th);
+    for (int r = 0; r < nrad; r++) out[r] += bilinear_centred(img, rows, cols, (REAL)r * c, (REAL)r * s);
+  }
+}
+
+/* CDI::phaseRetrieve, src/post_gpu/cdi.cc:297-440.  g enters as prepareIter
+ * leaves it (IFFT(F)/sqrt(N)) and leaves scaled by sqrt(N) (cdi.cc:313,406).
+ * rtrace (optional) receives the R-factor of F before each projection.
+ *
+ * PRTF (prtfIter >= 0): from the first projection step whose counter is
+ * >= prtfIter, the projected F's unit phasors are summed into a map with the
+ * global-phase alignment of cdi.cc:358-378: gamma0 = P_S(g') at the first such
+ * step, phi0 = -arg(sum gamma0^2)/2; later steps add phi0 + arg(sum P_S(g')
+ * conj(gamma0)).  P_S keeps g' inside the support and zeroes it outside (the
+ * reference applies its relaxed ER kernel to a scratch buffer it never
+ * initialises).  weights = radial sums of the fftshifted map |F| > 1e-6 at
+ * the first step; prtf = radial sums of the fftshifted |map| / (iter -
+ * prtfIter).  prtf_out/weights_out hold rows/2 values; *prtf_count receives
+ * iter - prtfIter, or 0 when no step accumulated.
+ * Returns 0, or -1 for a bad schedule / unsupported token. */
+int orc_phase_retrieve_prtf(int rows, int cols, const char* algorithm, ocpx* F, ocpx* g, const REAL* I,
+                            const REAL* bs, REAL* spt, const orc_params* p, REAL* threshold_inout,
+                            double* rtrace, int rtrace_cap, int* n_iter_out, int prtfIter, REAL* prtf_out,
+                            REAL* weights_out, int* prtf_count) {
   int64_t n = (int64_t)rows * cols;
   int err;
   orc_parser* ap = orc_parser_new(algorithm, &err);
   if (!ap) return -1;
   ocpx* gp = (ocpx*)malloc(sizeof(ocpx) * n);
   REAL* rs = (REAL*)malloc(sizeof(REAL) * n);
+  ocpx *map = NULL, *gamma0 = NULL;
+  REAL phi0 = 0;
+  int started = 0;
+  if (prtfIter >= 0) {
+    map = (ocpx*)calloc(n, sizeof(ocpx));
+    gamma0 = (ocpx*)malloc(sizeof(ocpx) * n);
+  }
   orc_apply_norm(n, g, (REAL)(1.0 / sqrt((double)n)));
   orc_sw_state st = {p->sigmaStart, 0, *threshold_inout, p->sigmaDecay};
   int iter = 0, nr = 0, rc = 0;
@@ -588,14 +648,63 @@ int orc_phase_retrieve(int rows, int cols, const char* algorithm, ocpx* F, ocpx*
       orc_rfactor(n, F, I, p->useBS ? bs : NULL, &num, &den);
       rtrace[nr++] = den > 0 ? num / den : 0;
     }
-    orc_iteration(rows, cols, F, g, I, bs, spt, a, p, st.threshold, gp);
+    if (prtfIter < 0 || iter < prtfIter) {
+      orc_iteration(rows, cols, F, g, I, bs, spt, a, p, st.threshold, gp);
+      continue;
+    }
+    /* orc_iteration split around the PRTF accumulation */
+    if (p->simCCDbit) orc_apply_mod(n, F, I, p->useBS ? bs : NULL, p->noiseLevel, p->scale, p->rcolor, 1);
+    else orc_apply_mod_abs(n, F, I);
+    orc_fft2d(rows, cols, F, gp, +1);
+    orc_apply_norm(n, gp, (REAL)(1.0 / (double)n));
+    orc_apply_support(n, g, gp, a, spt, st.threshold, p->beta, p->relax);
+    double sre = 0, sim = 0;
+    if (!started) {
+      for (int64_t i = 0; i < n; i++) {
+        const int in = spt[i] > st.threshold;
+        gamma0[i].re = in ? gp[i].re : 0;
+        gamma0[i].im = in ? gp[i].im : 0;
+        sre += (double)gamma0[i].re * gamma0[i].re - (double)gamma0[i].im * gamma0[i].im;
+        sim += 2.0 * gamma0[i].re * gamma0[i].im;
+      }
+      phi0 = (REAL)(-atan2(sim, sre) / 2);
+      orc_add_phase(n, map, F, phi0);
+      for (int64_t i = 0; i < n; i++) rs[i] = (REAL)hypot(F[i].re, F[i].im) > (REAL)1e-6 ? 1 : 0;
+      orc_convert_fo_real(rows, cols, rs);
+      orc_radial_sum(rows, cols, rs, weights_out);
+      started = 1;
+    } else {
+      for (int64_t i = 0; i < n; i++) {
+        if (!(spt[i] > st.threshold)) continue;
+        sre += (double)gp[i].re * gamma0[i].re + (double)gp[i].im * gamma0[i].im;
+        sim += (double)gp[i].im * gamma0[i].re - (double)gp[i].re * gamma0[i].im;
+      }
+      orc_add_phase(n, map, F, (REAL)(phi0 + atan2(sim, sre)));
+    }
+    orc_fft2d(rows, cols, g, F, -1);
   }
   orc_apply_norm(n, g, (REAL)sqrt((double)n));
   *threshold_inout = st.threshold;
   if (n_iter_out) *n_iter_out = iter;
+  if (prtf_count) *prtf_count = started ? iter - prtfIter : 0;
+  if (started) {
+    const REAL f = (REAL)(1. / (iter - prtfIter));
+    for (int64_t i = 0; i < n; i++) rs[i] = (REAL)hypot(map[i].re, map[i].im) * f;
+    orc_convert_fo_real(rows, cols, rs);
+    orc_radial_sum(rows, cols, rs, prtf_out);
+  } else if (prtfIter >= 0) {
+    for (int r = 0; r < rows / 2; r++) prtf_out[r] = weights_out[r] = 0;
+  }
   orc_parser_free(ap);
-  free(gp); free(rs);
+  free(gp); free(rs); free(map); free(gamma0);
   return rc;
+}
+
+int orc_phase_retrieve(int rows, int cols, const char* algorithm, ocpx* F, ocpx* g,
+                       const REAL* I, const REAL* bs, REAL* spt, const orc_params* p,
+                       REAL* threshold_inout, double* rtrace, int rtrace_cap, int* n_iter_out) {
+  return orc_phase_retrieve_prtf(rows, cols, algorithm, F, g, I, bs, spt, p, threshold_inout, rtrace, rtrace_cap,
+                                 n_iter_out, -1, NULL, NULL, NULL);
 }
 
 /* ------------------------------------------------------------------------ */

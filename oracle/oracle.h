@@ -85,6 +85,14 @@ int  orc_phase_retrieve(int rows, int cols, const char* algorithm, ocpx* F, ocpx
                         const REAL* I, const REAL* bs, REAL* spt, const orc_params* p,
                         REAL* threshold_inout, double* rtrace, int rtrace_cap, int* n_iter_out);
 
+/* ---- PRTF (a11) ---- */
+void orc_add_phase(int64_t n, ocpx* a, const ocpx* b, REAL phi);
+void orc_radial_sum(int rows, int cols, const REAL* img, REAL* out);
+int  orc_phase_retrieve_prtf(int rows, int cols, const char* algorithm, ocpx* F, ocpx* g, const REAL* I,
+                             const REAL* bs, REAL* spt, const orc_params* p, REAL* threshold_inout,
+                             double* rtrace, int rtrace_cap, int* n_iter_out, int prtfIter, REAL* prtf_out,
+                             REAL* weights_out, int* prtf_count);
+
 /* ---- synthetic forward model (Philox4x64-10 streams) ---- */
 void orc_philox4x64(const uint64_t ctr[4], const uint64_t key[2], uint64_t out[4]);
 double orc_uniform(uint64_t seed, uint64_t stream, uint64_t idx);

@@ -60,6 +60,8 @@ class Oracle:
             "orc_poisson": [ctypes.c_double, u64, u64, u64],
             "orc_iteration": [i, i, vp, vp, vp, vp, vp, i, vp, r, vp],
             "orc_phase_retrieve": [i, i, ctypes.c_char_p, vp, vp, vp, vp, vp, vp, vp, vp, i, vp],
+            "orc_phase_retrieve_prtf": [i, i, ctypes.c_char_p, vp, vp, vp, vp, vp, vp, vp, vp, i, vp, i, vp, vp, vp],
+            "orc_add_phase": [i64, vp, vp, r], "orc_radial_sum": [i, i, vp, vp],
         }
         for name, args in sig.items():
             getattr(L, name).argtypes = args
@@ -70,6 +72,7 @@ class Oracle:
         L.orc_poisson.restype = ctypes.c_int64
         L.orc_parser_expand.restype = i
         L.orc_phase_retrieve.restype = i
+        L.orc_phase_retrieve_prtf.restype = i
 
     # helpers
     def _c(self, a):
@@ -212,18 +215,35 @@ class Oracle:
                                spt.ctypes.data, algo, ctypes.byref(p), threshold, gp.ctypes.data)
         return F, g
 
-    def phase_retrieve(self, algorithm, F, g, I, spt, threshold=0.5, bs=None, rtrace_cap=0, **kw):
+    def phase_retrieve(self, algorithm, F, g, I, spt, threshold=0.5, bs=None, rtrace_cap=0, prtf_iter=-1, **kw):
         F, g = self._c(F).copy(), self._c(g).copy()
         I, spt = self._r(I), self._r(spt).copy()
         bs = None if bs is None else self._r(bs)
         p = self.params(**kw)
         thr = self.creal(threshold)
         rt = np.zeros(max(rtrace_cap, 1), np.float64)
-        nit = ctypes.c_int()
-        rc = self.lib.orc_phase_retrieve(F.shape[0], F.shape[1], algorithm.encode(), F.ctypes.data, g.ctypes.data,
-                                         I.ctypes.data, self._p(bs), spt.ctypes.data, ctypes.byref(p),
-                                         ctypes.byref(thr), rt.ctypes.data if rtrace_cap else None, rtrace_cap,
-                                         ctypes.byref(nit))
+        nit, cnt = ctypes.c_int(), ctypes.c_int()
+        nr = F.shape[0] // 2
+        prtf, wts = np.zeros(nr, self.real), np.zeros(nr, self.real)
+        rc = self.lib.orc_phase_retrieve_prtf(F.shape[0], F.shape[1], algorithm.encode(), F.ctypes.data,
+                                              g.ctypes.data, I.ctypes.data, self._p(bs), spt.ctypes.data,
+                                              ctypes.byref(p), ctypes.byref(thr),
+                                              rt.ctypes.data if rtrace_cap else None, rtrace_cap, ctypes.byref(nit),
+                                              prtf_iter, prtf.ctypes.data, wts.ctypes.data, ctypes.byref(cnt))
         if rc != 0:
             raise RuntimeError("oracle phase_retrieve failed")
-        return dict(F=F, g=g, spt=spt, threshold=thr.value, n_iter=nit.value, rtrace=rt[:rtrace_cap])
+        out = dict(F=F, g=g, spt=spt, threshold=thr.value, n_iter=nit.value, rtrace=rt[:rtrace_cap])
+        if prtf_iter >= 0:
+            out.update(prtf=prtf, weights=wts, prtf_count=cnt.value)
+        return out
+
+    def add_phase(self, a, b, phi):
+        a, b = self._c(a).copy(), self._c(b)
+        self.lib.orc_add_phase(a.size, a.ctypes.data, b.ctypes.data, phi)
+        return a
+
+    def radial_sum(self, img):
+        img = self._r(img)
+        out = np.zeros(img.shape[0] // 2, self.real)
+        self.lib.orc_radial_sum(img.shape[0], img.shape[1], img.ctypes.data, out.ctypes.data)
+        return out

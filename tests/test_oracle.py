@@ -314,3 +314,70 @@ def test_phase_retrieve_converges(o64):
     r = out["rtrace"]
     assert r[0] < 1e-12            # the start has the measured modulus
     assert r[-1] < 0.05 and r[-1] < 0.1 * r[1]
+
+
+# ---------------------------------------------------------------------------
+# PRTF pieces (cdi.cc:355-380,423-440; cudaConfig.cu:876-887,1027-1063)
+# ---------------------------------------------------------------------------
+def test_radial_sum_counts_angles_and_is_centred(o64):
+    # a constant image: every in-range sample is 1, so the sum counts the
+    # 2*cols angles until the 2x2 stencil leaves the image
+    out = o64.radial_sum(np.ones((64, 64)))
+    assert np.all(out[:31] == 128.0)
+    assert out[31] < 128.0
+    rect = o64.radial_sum(np.ones((64, 32)))
+    assert np.all(rect[:15] == 64.0) and np.all(rect[17:] < 64.0)
+    # a disc of radius 12 about the centre (rows/2, cols/2): full count inside,
+    # nothing outside -- an off-centre sampling grid breaks both
+    x, y = np.mgrid[0:64, 0:64]
+    disc = (((x - 32) ** 2 + (y - 32) ** 2) < 12.5 ** 2).astype(float)
+    d = o64.radial_sum(disc)
+    assert np.all(d[:11] == 128.0) and np.all(d[14:] == 0.0)
+    # an affine image is reproduced exactly by bilinear sampling: the sum of
+    # r*cos(theta) over a full circle of angles vanishes, the offset counts
+    aff = o64.radial_sum(7.0 + (x - 32) + 0.5 * (y - 32))
+    assert np.allclose(aff[:31], 7.0 * 128, rtol=0, atol=1e-9)
+
+
+def test_add_phase_closed_form(o64):
+    a = np.array([0, 1 + 1j, 5, 0], np.complex128)
+    b = np.array([2j, 0, 1e-7, -3], np.complex128)
+    out = o64.add_phase(a, b, np.pi / 2)
+    # unit phasor of b rotated by pi/2, added where |b| > 1e-6
+    assert np.allclose(out, [-1, 1 + 1j, 5, -1j], atol=1e-15)
+
+
+def _fixed_point(o64, n=64, r=10, seed=5):
+    """An exact fixed point of the loop: F = FFT(obj), I = |F|^2, g =
+    IFFT(F)/sqrt(N) with obj inside the support."""
+    rs = np.random.default_rng(seed)
+    x, y = np.mgrid[0:n, 0:n]
+    spt = (((x - n // 2) ** 2 + (y - n // 2) ** 2) < r * r).astype(float)
+    obj = spt * (0.5 + rs.uniform(size=(n, n))) * np.exp(1j * rs.uniform(-1, 1, (n, n)))
+    F = np.fft.fft2(obj)
+    return F, np.fft.ifft2(F) * n, np.abs(F) ** 2, spt
+
+
+def test_prtf_is_one_at_a_fixed_point(o64):
+    F, g, I, spt = _fixed_point(o64)
+    out = o64.phase_retrieve("12*ER", F, g, I, spt, 0.5, prtf_iter=4, relax=1.0)
+    assert out["prtf_count"] == 12 - 4
+    w, p = out["weights"], out["prtf"]
+    assert w[0] > 0 and np.all(w[:30] > 100)
+    ok = w > 0
+    assert np.allclose(p[ok] / w[ok], 1.0, rtol=1e-12)
+
+
+def test_prtf_counts_and_accumulates_partially(o64):
+    F, g, I, spt = _fixed_point(o64, seed=6)
+    # shrinkWrap tokens advance the counter but accumulate nothing; the
+    # divisor is still iter - prtfIter (cdi.cc:425)
+    out = o64.phase_retrieve("6*ER+2*shrinkWrap+4*ER", F, g, I, spt, 0.5, prtf_iter=4, relax=1.0,
+                             shrinkThreshold=0.01)
+    assert out["n_iter"] == 12 and out["prtf_count"] == 8
+    ok = out["weights"] > 0
+    # 6 of the 8 counted steps accumulate unit phasors of a fixed F
+    assert np.allclose(out["prtf"][ok] / out["weights"][ok], 6 / 8, rtol=1e-9)
+    # prtfIter beyond the schedule: nothing accumulated
+    none = o64.phase_retrieve("5*ER", F, g, I, spt, 0.5, prtf_iter=10, relax=1.0)
+    assert none["prtf_count"] == 0 and not none["prtf"].any() and not none["weights"].any()

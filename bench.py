@@ -121,6 +121,95 @@ def batched_leg(cdx, iters):
             "effective_GBps": round(80 * pix * iters / (ms_batch * 1e-3) / 1e9, 1)}
 
 
+# RAAR with frequency-form shrink-wrap (the reference's cdi.cc:326-341 rule);
+# a threshold of 0.02 keeps the support from collapsing onto the brightest
+# blob of the iterate, which 0.1 and above do on these objects
+E2E_PARAMS = dict(beta=0.85, shrinkThreshold=0.02, useBS=1, freeBeamstop=1)
+
+# FP32 pipeline against an FP64 GPU run from the same inputs, per workload
+REPORT_CONFIGS = [
+    ("256x256 HIO+ER, real-space shrink-wrap (CPU reference path)",
+     dict(n=256, radius=40, photons=1e6, bs=0, support="rect",
+          schedule="10*(20*HIO+1*shrinkWrap)+2*(20*ER+1*shrinkWrap)+10*ER",
+          params=dict(beta=0.9, shrinkMode=1, shrinkThreshold=0.2, sigma0=3.0, sigmaMin=1.5,
+                      sigmaDecay=0.5 ** (1 / 11)))),
+    ("256x256 HIO+ER, real-space shrink-wrap, threshold 0.02",
+     dict(n=256, radius=40, photons=1e6, bs=0, support="rect",
+          schedule="10*(20*HIO+1*shrinkWrap)+2*(20*ER+1*shrinkWrap)+10*ER",
+          params=dict(beta=0.9, shrinkMode=1, shrinkThreshold=0.02, sigma0=3.0, sigmaMin=1.5,
+                      sigmaDecay=0.5 ** (1 / 11)))),
+    ("1024x1024 RAAR beta=0.85, 1000 iterations, shrink-wrap every 20",
+     dict(n=1024, radius=160, photons=1e8, bs=8, support="circle",
+          schedule="50*(20*RAAR+1*shrinkWrap)", params=E2E_PARAMS)),
+    ("2048x2048 RAAR beta=0.85, 2000 iterations, shrink-wrap every 20",
+     dict(n=2048, radius=320, photons=1e9, bs=16, support="circle",
+          schedule="100*(20*RAAR+1*shrinkWrap)", params=E2E_PARAMS)),
+]
+
+
+def _rfactor_host(F, I, bs):
+    m = np.ones(I.shape, bool) if bs is None else bs < 0.5
+    a = np.sqrt(np.maximum(I, 0))[m]
+    return float(np.abs(np.abs(F[m]) - a).sum() / a.sum())
+
+
+def fp64_report(cdx, seed=0):
+    """Each workload's seeded inputs (object, Poisson intensities, beamstop,
+    support, random-phase start) are made once in FP64 and run through the
+    FP32 and the FP64 pipelines.  Reported: R-factor of the support-projected
+    final object in each precision, and the relative L2 distance between the
+    two final objects (inside the FP64 support, global phase aligned)."""
+    out = {}
+    for name, c in REPORT_CONFIGS:
+        n = c["n"]
+        ctx = cdx.Context(n, precision=cdx.F64)
+        obj, I, bs, spt, F = ctx.cplx(), ctx.real(), ctx.real(), ctx.real(), ctx.cplx()
+        ctx.sim_object(obj, c["radius"], seed=seed)
+        ctx.sim_intensity(obj, I, c["photons"], seed=seed, poisson=True)
+        if c["bs"]:
+            ctx.create_beamstop(bs, c["bs"])
+        else:
+            bs.zero()
+        if c["support"] == "rect":
+            ctx.create_support(spt, 2.0)
+        else:
+            ctx.create_circle(spt, 0.2 * n)
+        ctx.random_start(I, bs if c["bs"] else None, F, seed=seed, mode=1)
+        host = dict(I=ctx.get_real(I), bs=ctx.get_real(bs) if c["bs"] else None, spt=ctx.get_real(spt),
+                    F=ctx.get_cplx(F))
+        ctx.close()
+        host["g"] = np.fft.ifft2(host["F"]) * n  # IFFT(F)/sqrt(N), as prepareIter leaves objectWave
+        res = {}
+        for prec in (cdx.F32, cdx.F64):
+            ctx = cdx.Context(n, precision=prec)
+            ctx.set_threshold(0.5)
+            p = cdx.Params.default(**c["params"])
+            dF, dg, dI, ds = ctx.cplx(host["F"]), ctx.cplx(host["g"]), ctx.real(host["I"]), ctx.real(host["spt"])
+            dbs = ctx.real(host["bs"]) if host["bs"] is not None else None
+            ctx.sync()
+            t0 = time.perf_counter()
+            nit, _ = ctx.phase_retrieve(c["schedule"], p, dF, dg, dI, ds, bs=dbs)
+            ctx.sync()
+            el = time.perf_counter() - t0
+            g = ctx.get_cplx(dg).astype(np.complex128) / n
+            mask = ctx.get_real(ds) > ctx.threshold()
+            res[prec] = dict(g=g, mask=mask, seconds=el, n_iter=nit)
+            ctx.close()
+        m64 = res[cdx.F64]["mask"]
+        for prec in res:
+            gs = res[prec]["g"] * m64
+            res[prec]["R"] = _rfactor_host(np.fft.fft2(gs), host["I"], host["bs"])
+            res[prec]["gs"] = gs
+        a, b = res[cdx.F32]["gs"], res[cdx.F64]["gs"]
+        ph = np.vdot(a.ravel(), b.ravel())
+        err = float(np.linalg.norm(a * (ph / abs(ph)) - b) / np.linalg.norm(b)) if abs(ph) > 0 else None
+        out[name] = {"iterations": res[cdx.F64]["n_iter"], "R_fp32": round(res[cdx.F32]["R"], 5),
+                     "R_fp64": round(res[cdx.F64]["R"], 5), "fp32_vs_fp64_rel_l2": err,
+                     "seconds_fp32": round(res[cdx.F32]["seconds"], 4),
+                     "seconds_fp64": round(res[cdx.F64]["seconds"], 4)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -130,6 +219,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-report", action="store_true", help="skip the FP32-vs-FP64 report per workload")
     ap.add_argument("--batch-iters", type=int, default=500)
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="run a plain copy kernel of this many MiB first (counter calibration)")
@@ -170,13 +260,18 @@ def main():
     # end-to-end: 1000 RAAR iterations with shrink-wrap every 20 (reference form)
     d2 = make_inputs(cdx, ctx, seed=1)
     ctx.set_threshold(0.5)
-    pe = cdx.Params.default(beta=0.85, useBS=1, freeBeamstop=1, shrinkThreshold=0.1, recordR=1)
+    pe = cdx.Params.default(recordR=1, **E2E_PARAMS)
     ctx.sync()
     t1 = time.perf_counter()
     nit, rt = ctx.phase_retrieve("50*(20*RAAR+1*shrinkWrap)", pe, d2["F"], d2["g"], d2["I"], d2["spt"],
                                  bs=d2["bs"], rtrace_cap=1000)
     ctx.sync()
     e2e = time.perf_counter() - t1
+    # R-factor of the support-projected final object (the RAAR iterate itself
+    # carries the reflected part outside the support)
+    gfin = ctx.get_cplx(d2["g"]).astype(np.complex128) / N
+    sfin = ctx.get_real(d2["spt"]) > ctx.threshold()
+    r_final = _rfactor_host(np.fft.fft2(gfin * sfin), ctx.get_real(d2["I"]), ctx.get_real(d2["bs"]))
 
     dom = "row" if row_ms >= col_ms else "col"
     t_dom = max(row_ms, col_ms)
@@ -202,10 +297,13 @@ def main():
                      "col_pass_ms": round(col_ms, 5), "row_pass_ms": round(row_ms, 5)},
         "wall_s_timed": round(wall, 4),
         "end_to_end": {"schedule": "50*(20*RAAR+1*shrinkWrap)", "iterations": nit, "seconds": round(e2e, 4),
-                       "final_R": float(rt[-1]) if rt is not None and len(rt) else None},
+                       "shrink_threshold": E2E_PARAMS["shrinkThreshold"], "R_iterate_last": float(rt[-1]),
+                       "R_support_projected": round(r_final, 5)},
     }
     if not args.no_batch:
         out["batched"] = batched_leg(cdx, args.batch_iters)
+    if not args.no_report:
+        out["fp64_report"] = fp64_report(cdx)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(out), flush=True)

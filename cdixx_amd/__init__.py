@@ -109,6 +109,21 @@ _SIGS = {
     "cdx_bin_read_header": ([ctypes.c_char_p, _ip, _ip, _ip], _i),
     "cdx_bin_read": ([ctypes.c_char_p, _vp, _sz], _i),
     "cdx_bin_write": ([ctypes.c_char_p, _i, _i, _i, _vp], _i),
+    "cdx_sim_object_shape": ([_vp, _i, _dp, _i, _u64, _d, _vp], _i),
+    "cdx_context_info": ([_vp, _ip, _ip, _ip], _i),
+    "cdx_unet_create": ([ctypes.POINTER(_vp), _i, _i, _i, _i, _u64], _i),
+    "cdx_unet_destroy": ([_vp], _i), "cdx_unet_num_params": ([_vp, ctypes.POINTER(_i64)], _i),
+    "cdx_unet_layers": ([_vp, ctypes.POINTER(_i64), _i], _i),
+    "cdx_unet_set_params": ([_vp, _vp, _i64], _i), "cdx_unet_get_params": ([_vp, _vp, _i64], _i),
+    "cdx_unet_get_grads": ([_vp, _vp, _i64], _i), "cdx_unet_zero_grads": ([_vp], _i),
+    "cdx_unet_set_batch": ([_vp, _i, _vp, _vp], _i), "cdx_unet_get_batch": ([_vp, _i, _vp, _vp], _i),
+    "cdx_unet_forward": ([_vp, _i], _i), "cdx_unet_output": ([_vp, _i, _vp], _i),
+    "cdx_unet_backward": ([_vp, _i, _dp], _i), "cdx_unet_adam": ([_vp, _d], _i),
+    "cdx_unet_synth_batch": ([_vp, _i, _u64, _d, _d], _i),
+    "cdx_unet_synth_frame": ([_vp, _u64, _d, _d, _vp, _vp], _i),
+    "cdx_unet_train": ([_vp, _i, _i, _u64, _d, _d, _d, _dp], _i),
+    "cdx_unet_warm_start": ([_vp, _vp, _vp, _d, _vp, _vp, _vp], _i),
+    "cdx_unet_save": ([_vp, ctypes.c_char_p], _i), "cdx_unet_load": ([_vp, ctypes.c_char_p], _i),
 }
 for _name, (_args, _res) in _SIGS.items():
     if hasattr(lib, _name):
@@ -353,6 +368,11 @@ class Context:
     def sim_object(self, obj, radius, seed=0, smooth_sigma=3.0):
         check(lib.cdx_sim_object(self.h, radius, seed, smooth_sigma, obj.ptr))
 
+    def sim_object_shape(self, obj, kind, params, seed=0, smooth_sigma=3.0):
+        """kind 0 disk [r], 1 ellipse [a, b, rot], 2 polygon [angles..., radii...]"""
+        prm = (ctypes.c_double * len(params))(*params)
+        check(lib.cdx_sim_object_shape(self.h, kind, prm, len(params), seed, smooth_sigma, obj.ptr))
+
     def sim_intensity(self, obj, I, photons, seed=0, poisson=True):
         check(lib.cdx_sim_intensity(self.h, obj.ptr, photons, seed, int(poisson), I.ptr))
 
@@ -399,6 +419,98 @@ class _Borrowed(Context):
 
     def close(self):
         self.h = None
+
+
+class UNet:
+    """Learned warm start (cdx_unet_*): U-Net on the centred log-intensity
+    predicting amplitude, phase and a support logit."""
+
+    def __init__(self, size=256, levels=4, base=32, max_batch=16, seed=1):
+        self.size, self.levels, self.base, self.max_batch = size, levels, base, max_batch
+        h = _vp()
+        check(lib.cdx_unet_create(ctypes.byref(h), size, levels, base, max_batch, seed))
+        self.h = h
+        n = _i64()
+        check(lib.cdx_unet_num_params(self.h, ctypes.byref(n)))
+        self.nparam = n.value
+
+    def layers(self):
+        buf = (_i64 * (6 * 64))()
+        nl = lib.cdx_unet_layers(self.h, buf, 6 * 64)
+        return [tuple(buf[6 * i:6 * i + 6]) for i in range(nl)]
+
+    def params(self):
+        out = np.zeros(self.nparam, np.float32)
+        check(lib.cdx_unet_get_params(self.h, out.ctypes.data, self.nparam))
+        return out
+
+    def set_params(self, flat):
+        flat = np.ascontiguousarray(flat, np.float32)
+        check(lib.cdx_unet_set_params(self.h, flat.ctypes.data, self.nparam))
+
+    def grads(self):
+        out = np.zeros(self.nparam, np.float32)
+        check(lib.cdx_unet_get_grads(self.h, out.ctypes.data, self.nparam))
+        return out
+
+    def zero_grads(self):
+        check(lib.cdx_unet_zero_grads(self.h))
+
+    def set_batch(self, x, targets=None):
+        x = np.ascontiguousarray(x, np.float32)
+        t = None if targets is None else np.ascontiguousarray(targets, np.float32)
+        check(lib.cdx_unet_set_batch(self.h, x.shape[0], x.ctypes.data, None if t is None else t.ctypes.data))
+
+    def get_batch(self, B):
+        x = np.zeros((B, self.size, self.size), np.float32)
+        t = np.zeros((B, self.size, self.size, 3), np.float32)
+        check(lib.cdx_unet_get_batch(self.h, B, x.ctypes.data, t.ctypes.data))
+        return x, t
+
+    def forward(self, B):
+        check(lib.cdx_unet_forward(self.h, B))
+        out = np.zeros((B, self.size, self.size, 3), np.float32)
+        check(lib.cdx_unet_output(self.h, B, out.ctypes.data))
+        return out
+
+    def backward(self, B):
+        loss = ctypes.c_double()
+        check(lib.cdx_unet_backward(self.h, B, ctypes.byref(loss)))
+        return loss.value
+
+    def adam(self, lr):
+        check(lib.cdx_unet_adam(self.h, lr))
+
+    def synth_batch(self, B, seed0, photons=(1e5, 1e8)):
+        check(lib.cdx_unet_synth_batch(self.h, B, seed0, photons[0], photons[1]))
+
+    def synth_frame(self, seed, obj, I, photons=(1e5, 1e8)):
+        check(lib.cdx_unet_synth_frame(self.h, seed, photons[0], photons[1], obj.ptr, I.ptr))
+
+    def train(self, steps, B, seed0, lr, photons=(1e5, 1e8)):
+        losses = (ctypes.c_double * max(steps, 1))()
+        check(lib.cdx_unet_train(self.h, steps, B, seed0, lr, photons[0], photons[1], losses))
+        return np.array(losses[:steps])
+
+    def warm_start(self, ctx, I, F, g, spt, threshold=0.5):
+        check(lib.cdx_unet_warm_start(self.h, ctx.h, I.ptr, threshold, F.ptr, g.ptr, spt.ptr))
+
+    def save(self, path):
+        check(lib.cdx_unet_save(self.h, path.encode()))
+
+    def load(self, path):
+        check(lib.cdx_unet_load(self.h, path.encode()))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.cdx_unet_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Batch:

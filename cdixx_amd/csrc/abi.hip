@@ -197,20 +197,11 @@ extern "C" int cdx_create(cdx_ctx** out, int rows, int cols, int precision, void
 
 extern "C" int cdx_destroy(cdx_ctx* c) {
   if (!c) return CDX_OK;
-  if (c->st) hipStreamSynchronize(c->st);
-  hipFree(c->tw_row);
-  hipFree(c->tw_col);
-  hipFree(c->T1);
-  hipFree(c->T2);
-  hipFree(c->Ipk);
-  hipFree(c->tmpr);
-  hipFree(c->scal);
-  hipFree(c->partials);
-  hipFree(c->ring);
-  hipFree(c->prtf_map);
-  hipFree(c->gamma0);
-  hipFree(c->prtf_dev);
-  if (c->own_stream && c->st) hipStreamDestroy(c->st);
+  if (c->st) (void)hipStreamSynchronize(c->st);
+  for (void* p : {c->tw_row, c->tw_col, c->T1, c->T2, c->Ipk, c->tmpr, (void*)c->scal, (void*)c->partials,
+                  (void*)c->ring, c->prtf_map, c->gamma0, (void*)c->prtf_dev})
+    (void)hipFree(p);
+  if (c->own_stream && c->st) (void)hipStreamDestroy(c->st);
   delete c;
   return CDX_OK;
 }
@@ -219,6 +210,14 @@ extern "C" int cdx_destroy(cdx_ctx* c) {
   do {                                                        \
     if (!(c)) return fail(CDX_ERR_ARG, "%s: null context", __func__); \
   } while (0)
+
+extern "C" int cdx_context_info(cdx_ctx* c, int* rows, int* cols, int* precision) {
+  CTX_CHECK(c);
+  if (rows) *rows = c->rows;
+  if (cols) *cols = c->cols;
+  if (precision) *precision = c->prec;
+  return CDX_OK;
+}
 
 extern "C" int cdx_synchronize(cdx_ctx* c) {
   CTX_CHECK(c);
@@ -809,8 +808,7 @@ __global__ void k_seed_minmax(unsigned long long* mm) {
   mm[3] = 0ull;
 }
 
-extern "C" int cdx_sim_object(cdx_ctx* c, double radius, uint64_t seed, double sigma, void* obj) {
-  CTX_CHECK(c);
+static int sim_object_sp(cdx_ctx* c, const ShapeParams& sp, uint64_t seed, double sigma, void* obj) {
   if (!obj) return fail(CDX_ERR_ARG, "cdx_sim_object: null buffer");
   if (c->rows != c->cols) return fail(CDX_ERR_ARG, "cdx_sim_object: square contexts only");
   const int n = c->rows;
@@ -830,14 +828,49 @@ extern "C" int cdx_sim_object(cdx_ctx* c, double radius, uint64_t seed, double s
   HIPCHK(launch_minmax(c->n, a, c->scal + S_MMA, c->st));
   HIPCHK(launch_minmax(c->n, ph, c->scal + S_MMP, c->st));
   if (c->prec == CDX_F64)
-    HIPCHK(launch_compose_object<double>(n, a, ph, c->scal + S_MMA, c->scal + S_MMP, (float)radius, obj, c->st));
+    HIPCHK(launch_compose_object<double>(n, a, ph, c->scal + S_MMA, c->scal + S_MMP, sp, obj, c->st));
   else
-    HIPCHK(launch_compose_object<float>(n, a, ph, c->scal + S_MMA, c->scal + S_MMP, (float)radius, obj, c->st));
+    HIPCHK(launch_compose_object<float>(n, a, ph, c->scal + S_MMA, c->scal + S_MMP, sp, obj, c->st));
   HIPCHK(hipFreeAsync(a, c->st));
   HIPCHK(hipFreeAsync(ph, c->st));
   HIPCHK(hipFreeAsync(t, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
   return CDX_OK;
+}
+
+extern "C" int cdx_sim_object(cdx_ctx* c, double radius, uint64_t seed, double sigma, void* obj) {
+  CTX_CHECK(c);
+  ShapeParams sp{};
+  sp.kind = 0;
+  sp.a = (float)radius;
+  return sim_object_sp(c, sp, seed, sigma, obj);
+}
+
+extern "C" int cdx_sim_object_shape(cdx_ctx* c, int kind, const double* prm, int np, uint64_t seed, double sigma,
+                                    void* obj) {
+  CTX_CHECK(c);
+  ShapeParams sp{};
+  sp.kind = kind;
+  if (!prm) return fail(CDX_ERR_ARG, "cdx_sim_object_shape: null parameters");
+  if (kind == 0 && np >= 1) {
+    sp.a = (float)prm[0];
+  } else if (kind == 1 && np >= 3) {
+    sp.a = (float)prm[0];
+    sp.b = (float)prm[1];
+    sp.rot = (float)prm[2];
+  } else if (kind == 2 && np >= 6 && np <= 16 && np % 2 == 0) {
+    sp.nv = np / 2;
+    for (int i = 0; i < sp.nv; i++) {
+      sp.ang[i] = (float)prm[i];
+      sp.rad[i] = (float)prm[sp.nv + i];
+      if (i && !(sp.ang[i] > sp.ang[i - 1])) return fail(CDX_ERR_ARG, "cdx_sim_object_shape: angles must increase");
+    }
+    if (!(sp.ang[0] >= 0) || !(sp.ang[sp.nv - 1] < 2 * M_PI))
+      return fail(CDX_ERR_ARG, "cdx_sim_object_shape: angles outside [0, 2 pi)");
+  } else {
+    return fail(CDX_ERR_ARG, "cdx_sim_object_shape: bad shape kind %d / parameter count %d", kind, np);
+  }
+  return sim_object_sp(c, sp, seed, sigma, obj);
 }
 
 extern "C" int cdx_sim_intensity(cdx_ctx* c, const void* obj, double photons, uint64_t seed, int poisson, void* I) {

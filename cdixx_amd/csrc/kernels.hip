@@ -502,17 +502,36 @@ __device__ __forceinline__ double unkey(unsigned long long k) {
   const unsigned long long b = (k & 0x8000000000000000ULL) ? (k & 0x7FFFFFFFFFFFFFFFULL) : ~k;
   return __longlong_as_double((long long)b);
 }
+// inside test of a support shape centred at (c, c): disk |d| < a (strict, as
+// createMask<C_circle>), ellipse with semi-axes a, b rotated by rot, or a
+// polygon star-shaped about the centre (vertices at increasing angles)
+__device__ __forceinline__ bool shape_inside(const ShapeParams& sp, float dx, float dy) {
+  if (sp.kind == 0) return sqrtf(dx * dx + dy * dy) < sp.a;
+  if (sp.kind == 1) {
+    const float c = cosf(sp.rot), s = sinf(sp.rot);
+    const float u = (dx * c + dy * s) / sp.a, v = (-dx * s + dy * c) / sp.b;
+    return u * u + v * v < 1.f;
+  }
+  float phi = atan2f(dy, dx);
+  if (phi < 0.f) phi += 2.f * (float)kPi;
+  int i = sp.nv - 1;  // sector [ang[i], ang[i+1]), wrapping after the last vertex
+  for (int k = 0; k + 1 < sp.nv; k++)
+    if (phi >= sp.ang[k] && phi < sp.ang[k + 1]) i = k;
+  const int j = i + 1 < sp.nv ? i + 1 : 0;
+  const float x0 = sp.rad[i] * cosf(sp.ang[i]), y0 = sp.rad[i] * sinf(sp.ang[i]);
+  const float x1 = sp.rad[j] * cosf(sp.ang[j]), y1 = sp.rad[j] * sinf(sp.ang[j]);
+  return (x1 - x0) * (dy - y0) - (y1 - y0) * (dx - x0) >= 0.f;
+}
 template <class T>
 __global__ void k_compose(int n, const double* a, const double* ph, const double* mma, const double* mmp,
-                          float radius, cplx<T>* obj) {
+                          ShapeParams sp, cplx<T>* obj) {
   const int64_t N = (int64_t)n * n;
   const double amin = unkey(((const unsigned long long*)mma)[0]), amax = unkey(((const unsigned long long*)mma)[1]);
   const double pmin = unkey(((const unsigned long long*)mmp)[0]), pmax = unkey(((const unsigned long long*)mmp)[1]);
   const int c = n >> 1;
   EW_LOOP(N) {
     const int x = (int)(i / n), y = (int)(i % n);
-    const float dx = (float)(x - c), dy = (float)(y - c);
-    if (!(sqrtf(dx * dx + dy * dy) < radius)) {
+    if (!shape_inside(sp, (float)(x - c), (float)(y - c))) {
       obj[i] = {(T)0, (T)0};
       continue;
     }
@@ -523,8 +542,8 @@ __global__ void k_compose(int n, const double* a, const double* ph, const double
 }
 template <class T>
 hipError_t launch_compose_object(int n, const double* a, const double* ph, const double* mma, const double* mmp,
-                                 float radius, void* obj, hipStream_t st) {
-  hipLaunchKernelGGL(k_compose<T>, ew_grid((int64_t)n * n), dim3(256), 0, st, n, a, ph, mma, mmp, radius,
+                                 const ShapeParams& sp, void* obj, hipStream_t st) {
+  hipLaunchKernelGGL(k_compose<T>, ew_grid((int64_t)n * n), dim3(256), 0, st, n, a, ph, mma, mmp, sp,
                      (cplx<T>*)obj);
   return hipGetLastError();
 }
@@ -804,7 +823,7 @@ hipError_t launch_radial(const void* src, int rows, int cols, int kind, T norm, 
   template hipError_t launch_rfactor<T>(int64_t, const void*, const T*, double*, hipStream_t);                   \
   template hipError_t launch_cast_c<T>(int64_t, const void*, int, void*, hipStream_t);                           \
   template hipError_t launch_compose_object<T>(int, const double*, const double*, const double*, const double*, \
-                                               float, void*, hipStream_t);                                       \
+                                               const ShapeParams&, void*, hipStream_t);                          \
   template hipError_t launch_poisson_intensity<T>(int64_t, const void*, const double*, double, uint64_t, int, T*, \
                                                   hipStream_t);                                                  \
   template hipError_t launch_sum_mod2<T>(int64_t, const void*, double*, hipStream_t);                          \

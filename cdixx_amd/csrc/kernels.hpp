@@ -150,8 +150,17 @@ hipError_t launch_uniform_field(int n, uint64_t seed, uint64_t stream, double* o
 hipError_t launch_smooth_periodic(int n, const double* in, double* out, double sigma, int axis,
                                   hipStream_t st);
 hipError_t launch_minmax(int64_t n, const double* x, double* mm, hipStream_t st);
+// support shape of a synthetic object: kind 0 disk (radius a), 1 ellipse
+// (semi-axes a, b, rotation rot), 2 polygon (nv <= 8 vertices at increasing
+// angles ang[] in [0, 2 pi) and radii rad[], star-shaped about the centre)
+struct ShapeParams {
+  int kind, nv;
+  float a, b, rot;
+  float ang[8], rad[8];
+};
 template <class T> hipError_t launch_compose_object(int n, const double* a, const double* ph, const double* mma,
-                                                    const double* mmp, float radius, void* obj, hipStream_t st);
+                                                    const double* mmp, const ShapeParams& sp, void* obj,
+                                                    hipStream_t st);
 template <class T> hipError_t launch_poisson_intensity(int64_t n, const void* F, const double* tot, double photons,
                                                        uint64_t seed, int poisson, T* I, hipStream_t st);
 template <class T> hipError_t launch_sum_mod2(int64_t n, const void* F, double* acc, hipStream_t st);

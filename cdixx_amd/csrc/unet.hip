@@ -1,0 +1,1076 @@
+// unet.hip -- learned warm start for the phase-retrieval loop (no counterpart
+// in the reference, whose loop always starts from random signs,
+// src/post_gpu/cdi.cc:222-223).
+//
+// A 2D U-Net on the centred log-intensity predicts the object amplitude, its
+// phase and a support logit.  Everything runs on hand-written kernels:
+//   * 3x3 / 1x1 convolutions as implicit GEMMs on BF16 MFMA
+//     (v_mfma_f32_32x32x16_bf16, FP32 accumulation), NHWC activations;
+//   * input gradients as the same convolution with flipped, transposed
+//     weights; weight gradients as a split-K GEMM over pixels with FP32
+//     atomics; bias gradients, ReLU/pool/upsample/concat backward;
+//   * the loss, Adam (FP32 master weights, BF16 copies for the MFMA);
+//   * training pairs from the project's own GPU forward model (random
+//     ellipse / polygon supports, smoothed-noise amplitude and phase, Poisson
+//     intensities).
+// Layout of a conv layer's weights: w[co][k], k = tap * Cin + ci (tap = 3 dy
+// + dx for 3x3), zero-padded to Kp = roundup(taps * Cin, 32) per row.
+#include "../../include/cdx_abi.h"
+#include "device_ops.hpp"
+#include "errors.hpp"
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+#include <string>
+#include <vector>
+
+using namespace cdx;
+
+namespace {
+
+int ufail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  cdx::set_last_error(buf);
+  return code;
+}
+#define UCHK(expr)                                                                           \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess) return ufail(CDX_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+#define RCHK(expr)          \
+  do {                      \
+    int r_ = (expr);        \
+    if (r_) return r_;      \
+  } while (0)
+
+using u16 = unsigned short;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__host__ __device__ inline float bf2f(u16 h) {
+  const unsigned u = (unsigned)h << 16;
+  float f;
+  __builtin_memcpy(&f, &u, 4);
+  return f;
+}
+// round to nearest even
+__host__ __device__ inline u16 f2bf(float f) {
+  unsigned u;
+  __builtin_memcpy(&u, &f, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (u16)(u >> 16);
+}
+
+constexpr int kTile = 64;  // output tile (rows x cols) of the GEMM kernels
+constexpr int kBK = 32;    // K step
+constexpr int kLdsRow = kBK + 8;  // LDS row pitch in bf16 (80 B, 16-byte aligned rows)
+
+// ---------------------------------------------------------------------------
+// convolution: out[m][co] = act(bias[co] + sum_k A[m][k] W[co][k]),
+// A[m][k] = x[b, h + dy, w + dx, ci] (zero outside the image)
+// ---------------------------------------------------------------------------
+struct ConvArgs {
+  const u16* x;       // [B][H][W][Cin]
+  const u16* w;       // [Cout][Kp]
+  const float* bias;  // [Cout] or null
+  u16* y;             // [B][H][W][Cout] or null
+  float* yf;          // FP32 copy of the output or null
+  int B, H, W, Cin, Cout, Kp, taps, relu;
+};
+
+__device__ __forceinline__ void pixel_of(int m, int HW, int W, int& b, int& h, int& w) {
+  b = m / HW;
+  const int r = m - b * HW;
+  h = r / W;
+  w = r - h * W;
+}
+
+// 8 consecutive k (one tap, 8 consecutive input channels) of pixel (b,h,w)
+__device__ __forceinline__ uint4 load_patch8(const u16* x, int H, int W, int Cin, int taps, int b, int h, int w,
+                                             int k) {
+  const int tap = k / Cin, ci = k - tap * Cin;
+  int dy = 0, dx = 0;
+  if (taps == 9) {
+    dy = tap / 3 - 1;
+    dx = tap - (tap / 3) * 3 - 1;
+  }
+  const int hh = h + dy, ww = w + dx;
+  if (hh < 0 || hh >= H || ww < 0 || ww >= W) return make_uint4(0, 0, 0, 0);
+  return *(const uint4*)(x + ((size_t)(b * H + hh) * W + ww) * Cin + ci);
+}
+
+// Four waves, each a 32x32 quarter of the 64x64 tile: D += A B with A rows
+// from As[row][k], B columns from Bs[col][k] (both K-contiguous).  The MFMA
+// pairs A and B values by lane and element, so any fixed assignment of the
+// 16 k values of a step to (lane half, element) gives the same dot product.
+__device__ __forceinline__ void mfma_tile(const u16 (*As)[kLdsRow], const u16 (*Bs)[kLdsRow], int wm, int wn,
+                                          int lane, f32x16& acc) {
+#pragma unroll
+  for (int s = 0; s < kBK / 16; s++) {
+    const int kk = s * 16 + (lane >> 5) * 8;
+    const bf16x8 a = *(const bf16x8*)&As[wm * 32 + (lane & 31)][kk];
+    const bf16x8 b = *(const bf16x8*)&Bs[wn * 32 + (lane & 31)][kk];
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+  }
+}
+// row of accumulator element r in the 32x32 block (column = lane & 31)
+__device__ __forceinline__ int acc_row(int r, int lane) { return (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3); }
+
+__global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
+  __shared__ __align__(16) u16 As[kTile][kLdsRow];
+  __shared__ __align__(16) u16 Bs[kTile][kLdsRow];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int HW = a.H * a.W, M = a.B * HW, K = a.taps * a.Cin;
+  const int m0 = blockIdx.x * kTile, n0 = blockIdx.y * kTile;
+  const int lr = tid >> 2, lk = (tid & 3) * 8;
+  const int m = m0 + lr;
+  const bool mok = m < M;
+  int b = 0, h = 0, w = 0;
+  if (mok) pixel_of(m, HW, a.W, b, h, w);
+  const int co = n0 + lr;
+  const bool cok = co < a.Cout;
+  f32x16 acc = {};
+  for (int k0 = 0; k0 < a.Kp; k0 += kBK) {
+    const int k = k0 + lk;
+    const uint4 av = (mok && k < K) ? load_patch8(a.x, a.H, a.W, a.Cin, a.taps, b, h, w, k) : make_uint4(0, 0, 0, 0);
+    const uint4 bv = cok ? *(const uint4*)(a.w + (size_t)co * a.Kp + k) : make_uint4(0, 0, 0, 0);
+    __syncthreads();
+    *(uint4*)&As[lr][lk] = av;
+    *(uint4*)&Bs[lr][lk] = bv;
+    __syncthreads();
+    mfma_tile(As, Bs, wm, wn, lane, acc);
+  }
+  const int coo = n0 + wn * 32 + (lane & 31);
+  if (coo >= a.Cout) return;
+  const float bias = a.bias ? a.bias[coo] : 0.f;
+#pragma unroll
+  for (int r = 0; r < 16; r++) {
+    const int mm = m0 + wm * 32 + acc_row(r, lane);
+    if (mm < M) {
+      float v = acc[r] + bias;
+      if (a.relu) v = fmaxf(v, 0.f);
+      if (a.y) a.y[(size_t)mm * a.Cout + coo] = f2bf(v);
+      if (a.yf) a.yf[(size_t)mm * a.Cout + coo] = v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// weight gradient: dw[co][n] += sum_m dz[m][co] A[m][n] over a pixel chunk
+// (grid.z splits the pixels; FP32 atomics combine the chunks)
+// ---------------------------------------------------------------------------
+struct WgradArgs {
+  const u16* dz;  // [M][Cout]
+  const u16* x;   // [B][H][W][Cin]
+  float* dw;      // [Cout][Kp]
+  int B, H, W, Cin, Cout, Kp, taps, chunk;
+};
+
+__global__ void __launch_bounds__(256) k_wgrad(WgradArgs a) {
+  __shared__ __align__(16) u16 As[kTile][kLdsRow];  // [co][pixel]
+  __shared__ __align__(16) u16 Bs[kTile][kLdsRow];  // [n][pixel]
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int HW = a.H * a.W, M = a.B * HW, K = a.taps * a.Cin;
+  const int co0 = blockIdx.x * kTile, n0 = blockIdx.y * kTile;
+  const int p_lo = blockIdx.z * a.chunk;
+  const int p_hi = min(M, p_lo + a.chunk);
+  const int lp = tid >> 3, lc = (tid & 7) * 8;  // pixel 0..31 of the step, 8 columns
+  f32x16 acc = {};
+  for (int p0 = p_lo; p0 < p_hi; p0 += kBK) {
+    const int m = p0 + lp;
+    uint4 av = make_uint4(0, 0, 0, 0), bv = make_uint4(0, 0, 0, 0);
+    if (m < p_hi) {
+      if (co0 + lc < a.Cout) av = *(const uint4*)(a.dz + (size_t)m * a.Cout + co0 + lc);
+      if (n0 + lc < K) {
+        int b, h, w;
+        pixel_of(m, HW, a.W, b, h, w);
+        bv = load_patch8(a.x, a.H, a.W, a.Cin, a.taps, b, h, w, n0 + lc);
+      }
+    }
+    __syncthreads();
+    const u16* ae = (const u16*)&av;
+    const u16* be = (const u16*)&bv;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      As[lc + e][lp] = ae[e];
+      Bs[lc + e][lp] = be[e];
+    }
+    __syncthreads();
+    mfma_tile(As, Bs, wm, wn, lane, acc);
+  }
+  const int n = n0 + wn * 32 + (lane & 31);
+  if (n >= K) return;
+#pragma unroll
+  for (int r = 0; r < 16; r++) {
+    const int co = co0 + wm * 32 + acc_row(r, lane);
+    if (co < a.Cout) atomicAdd(a.dw + (size_t)co * a.Kp + n, acc[r]);
+  }
+}
+
+// db[c] += sum over pixels of dz[m][c]
+__global__ void k_bias_grad(const u16* __restrict__ dz, int M, int C, float* __restrict__ db) {
+  for (int c = threadIdx.x; c < C; c += blockDim.x) {
+    float s = 0.f;
+    for (int m = blockIdx.x; m < M; m += gridDim.x) s += bf2f(dz[(size_t)m * C + c]);
+    atomicAdd(db + c, s);
+  }
+}
+
+// dy *= (y > 0)
+__global__ void k_relu_bwd(int64_t n, u16* __restrict__ dy, const u16* __restrict__ y) {
+  EW_LOOP(n) {
+    if (!(bf2f(y[i]) > 0.f)) dy[i] = 0;
+  }
+}
+
+// 2x2 average pool, NHWC
+__global__ void k_pool(int B, int H, int W, int C, const u16* __restrict__ x, u16* __restrict__ y) {
+  const int Ho = H / 2, Wo = W / 2;
+  const int64_t n = (int64_t)B * Ho * Wo * C;
+  EW_LOOP(n) {
+    const int c = (int)(i % C);
+    const int64_t p = i / C;
+    const int wo = (int)(p % Wo), ho = (int)((p / Wo) % Ho), b = (int)(p / ((int64_t)Wo * Ho));
+    const size_t base = ((size_t)(b * H + 2 * ho) * W + 2 * wo) * C + c;
+    const float s = bf2f(x[base]) + bf2f(x[base + C]) + bf2f(x[base + (size_t)W * C]) +
+                    bf2f(x[base + (size_t)W * C + C]);
+    y[i] = f2bf(0.25f * s);
+  }
+}
+// dx[b,h,w,c] += dy[b,h/2,w/2,c] / 4
+__global__ void k_pool_bwd_add(int B, int H, int W, int C, const u16* __restrict__ dy, u16* __restrict__ dx) {
+  const int64_t n = (int64_t)B * H * W * C;
+  EW_LOOP(n) {
+    const int c = (int)(i % C);
+    const int64_t p = i / C;
+    const int w = (int)(p % W), h = (int)((p / W) % H), b = (int)(p / ((int64_t)W * H));
+    const float g = bf2f(dy[((size_t)(b * (H / 2) + h / 2) * (W / 2) + w / 2) * C + c]);
+    dx[i] = f2bf(bf2f(dx[i]) + 0.25f * g);
+  }
+}
+// cat[b,h,w,:] = (low[b,h/2,w/2,0:Cu], skip[b,h,w,0:Cs])
+__global__ void k_upcat(int B, int H, int W, int Cu, int Cs, const u16* __restrict__ low,
+                        const u16* __restrict__ skip, u16* __restrict__ cat) {
+  const int C = Cu + Cs;
+  const int64_t n = (int64_t)B * H * W * C;
+  EW_LOOP(n) {
+    const int c = (int)(i % C);
+    const int64_t p = i / C;
+    const int w = (int)(p % W), h = (int)((p / W) % H), b = (int)(p / ((int64_t)W * H));
+    cat[i] = c < Cu ? low[((size_t)(b * (H / 2) + h / 2) * (W / 2) + w / 2) * Cu + c] : skip[(size_t)p * Cs + c - Cu];
+  }
+}
+// dskip = dcat[..., Cu:]
+__global__ void k_upcat_bwd_skip(int64_t P, int Cu, int Cs, const u16* __restrict__ dcat, u16* __restrict__ dskip) {
+  const int64_t n = P * Cs;
+  EW_LOOP(n) dskip[i] = dcat[(i / Cs) * (Cu + Cs) + Cu + i % Cs];
+}
+// dlow[b,h,w,c] = sum over the 2x2 block of dcat[..., c]
+__global__ void k_upcat_bwd_low(int B, int Hl, int Wl, int Cu, int Cs, const u16* __restrict__ dcat,
+                                u16* __restrict__ dlow) {
+  const int C = Cu + Cs, H = 2 * Hl, W = 2 * Wl;
+  const int64_t n = (int64_t)B * Hl * Wl * Cu;
+  EW_LOOP(n) {
+    const int c = (int)(i % Cu);
+    const int64_t p = i / Cu;
+    const int w = (int)(p % Wl), h = (int)((p / Wl) % Hl), b = (int)(p / ((int64_t)Wl * Hl));
+    const size_t base = ((size_t)(b * H + 2 * h) * W + 2 * w) * C + c;
+    const float s = bf2f(dcat[base]) + bf2f(dcat[base + C]) + bf2f(dcat[base + (size_t)W * C]) +
+                    bf2f(dcat[base + (size_t)W * C + C]);
+    dlow[i] = f2bf(s);
+  }
+}
+
+// wf[ci][t' * Cout + co] = w[co][t * Cin + ci], t' = taps - 1 - t
+__global__ void k_flip(int Cout, int Cin, int taps, int Kp, const u16* __restrict__ w, int Kp2,
+                       u16* __restrict__ wf) {
+  const int64_t n = (int64_t)Cin * taps * Cout;
+  EW_LOOP(n) {
+    const int co = (int)(i % Cout);
+    const int t2 = (int)((i / Cout) % taps);
+    const int ci = (int)(i / ((int64_t)Cout * taps));
+    wf[(size_t)ci * Kp2 + t2 * Cout + co] = w[(size_t)co * Kp + (taps - 1 - t2) * Cin + ci];
+  }
+}
+
+// Loss per pixel over the head output (amplitude, phase, support logit) and
+// the target (amplitude, phase, mask):
+//   (a - a*)^2 + lp * m* (1 - cos(p - p*)) + ls * BCE(s, m*), averaged over pixels;
+// dz = dLoss/dout as BF16 (channels 3..7 zero).
+__global__ void k_loss(int M, const float* __restrict__ yf, const float* __restrict__ tg, float lp, float ls,
+                       u16* __restrict__ dz, double* loss) {
+  double acc = 0;
+  const float inv = 1.f / (float)M;
+  EW_LOOP(M) {
+    const float a = yf[i * 8], p = yf[i * 8 + 1], s = yf[i * 8 + 2];
+    const float at = tg[i * 4], pt = tg[i * 4 + 1], mt = tg[i * 4 + 2];
+    const float sg = 1.f / (1.f + expf(-s));
+    const float d = p - pt;
+    acc += (double)((a - at) * (a - at) + lp * mt * (1.f - cosf(d)) +
+                    ls * (fmaxf(s, 0.f) - s * mt + log1pf(expf(-fabsf(s)))));
+    u16* o = dz + (size_t)i * 8;
+    o[0] = f2bf(2.f * (a - at) * inv);
+    o[1] = f2bf(lp * mt * sinf(d) * inv);
+    o[2] = f2bf(ls * (sg - mt) * inv);
+#pragma unroll
+    for (int c = 3; c < 8; c++) o[c] = 0;
+  }
+  __shared__ double sh[16];
+  block_store_sum(acc, loss + blockIdx.x, sh);
+}
+
+// Adam on the flat FP32 parameters; refreshes the BF16 copy, clears the gradient
+__global__ void k_adam(int64_t n, float* __restrict__ w, float* __restrict__ g, float* __restrict__ m1,
+                       float* __restrict__ m2, u16* __restrict__ wb, float lr, float b1, float b2, float eps,
+                       float c1, float c2) {
+  EW_LOOP(n) {
+    const float gi = g[i];
+    const float a = b1 * m1[i] + (1.f - b1) * gi;
+    const float v = b2 * m2[i] + (1.f - b2) * gi * gi;
+    m1[i] = a;
+    m2[i] = v;
+    const float wi = w[i] - lr * (a / c1) / (sqrtf(v / c2) + eps);
+    w[i] = wi;
+    wb[i] = f2bf(wi);
+    g[i] = 0.f;
+  }
+}
+__global__ void k_to_bf16(int64_t n, const float* __restrict__ w, u16* __restrict__ wb) {
+  EW_LOOP(n) wb[i] = f2bf(w[i]);
+}
+__global__ void k_zero_f32(int64_t n, float* p) {
+  EW_LOOP(n) p[i] = 0.f;
+}
+__global__ void k_zero_u16(int64_t n, u16* p) {
+  EW_LOOP(n) p[i] = 0;
+}
+
+// network input of sample b: channel 0 = log(1 + I) / 16 of the centred
+// (fftshifted) pattern, channels 1..7 zero
+__global__ void k_pack_input(int n, const float* __restrict__ I, u16* __restrict__ x) {
+  const int64_t N = (int64_t)n * n;
+  EW_LOOP(N) {
+    const int h = (int)(i / n), w = (int)(i % n);
+    const int hs = h + n / 2 < n ? h + n / 2 : h - n / 2, ws = w + n / 2 < n ? w + n / 2 : w - n / 2;
+    const float v = I[(size_t)hs * n + ws];
+    u16* o = x + (size_t)i * 8;
+    o[0] = f2bf(log1pf(fmaxf(v, 0.f)) * (1.f / 16.f));
+#pragma unroll
+    for (int c = 1; c < 8; c++) o[c] = 0;
+  }
+}
+// one workgroup: sum of a complex float image (fixed order)
+__global__ void k_csum(int64_t n, const cplx<float>* __restrict__ z, double* out) {
+  double re = 0, im = 0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    re += z[i].x;
+    im += z[i].y;
+  }
+  __shared__ double sh[16];
+  block_store_sum(re, out, sh);
+  block_store_sum(im, out + 1, sh);
+}
+// targets: amplitude, phase relative to the global phase arg(sum obj), mask
+__global__ void k_pack_target(int64_t n, const cplx<float>* __restrict__ obj, const double* csum,
+                              float* __restrict__ tg) {
+  const float th = (float)atan2(csum[1], csum[0]);
+  const float c = cosf(th), s = sinf(th);
+  EW_LOOP(n) {
+    const cplx<float> z = obj[i];
+    const float re = z.x * c + z.y * s, im = z.y * c - z.x * s;  // z e^{-i th}
+    const float a = sqrtf(z.x * z.x + z.y * z.y);
+    float* o = tg + (size_t)i * 4;
+    o[0] = a;
+    o[1] = a > 0.f ? atan2f(im, re) : 0.f;
+    o[2] = a > 0.f ? 1.f : 0.f;
+    o[3] = 0.f;
+  }
+}
+// warm start from the head output of sample 0: support = sigmoid(logit) > thr,
+// g0 = amplitude e^{i phase} inside the support
+__global__ void k_ws_compose(int64_t n, const float* __restrict__ yf, float thr, cplx<float>* __restrict__ g0,
+                             float* __restrict__ spt) {
+  EW_LOOP(n) {
+    const float a = fmaxf(yf[i * 8], 0.f), p = yf[i * 8 + 1], s = yf[i * 8 + 2];
+    const bool in = 1.f / (1.f + expf(-s)) > thr;
+    g0[i] = in ? cplx<float>{a * cosf(p), a * sinf(p)} : cplx<float>{0.f, 0.f};
+    spt[i] = in ? 1.f : 0.f;
+  }
+}
+__global__ void k_sum_sq(int64_t n, const cplx<float>* __restrict__ z, const float* __restrict__ I, double* out) {
+  double a = 0, b = 0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    a += (double)z[i].x * z[i].x + (double)z[i].y * z[i].y;
+    b += (double)fmaxf(I[i], 0.f);
+  }
+  __shared__ double sh[16];
+  block_store_sum(a, out, sh);
+  block_store_sum(b, out + 1, sh);
+}
+
+// host Philox4x64-10 (Salmon et al., SC'11), same streams as the device one
+void philox_host(uint64_t c[4], uint64_t k0, uint64_t k1) {
+  for (int r = 0; r < 10; r++) {
+    const __uint128_t p0 = (__uint128_t)0xD2E7470EE14C6C93ULL * c[0];
+    const __uint128_t p1 = (__uint128_t)0xCA5A826395121157ULL * c[2];
+    const uint64_t n0 = (uint64_t)(p1 >> 64) ^ c[1] ^ k0, n2 = (uint64_t)(p0 >> 64) ^ c[3] ^ k1;
+    c[0] = n0;
+    c[1] = (uint64_t)p1;
+    c[2] = n2;
+    c[3] = (uint64_t)p0;
+    k0 += 0x9E3779B97F4A7C15ULL;
+    k1 += 0xBB67AE8584CAA73BULL;
+  }
+}
+double uniform_host(uint64_t seed, uint64_t stream, uint64_t idx) {
+  uint64_t c[4] = {idx, 0, 0, 0};
+  philox_host(c, seed, stream);
+  return (double)(c[0] >> 11) * 0x1.0p-53;
+}
+
+int64_t ew_blocks(int64_t n) {
+  int64_t b = (n + 255) / 256;
+  return b < 1 ? 1 : (b > 65536 ? 65536 : b);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// the network
+// ---------------------------------------------------------------------------
+struct UConv {
+  int cin, cout, taps, kp, kp2, relu;
+  int64_t w_off, b_off;  // offsets into the flat parameter arrays
+  u16* wflip = nullptr;  // [cin][kp2] for the input gradient
+};
+struct UTensor {
+  u16* y = nullptr;
+  u16* dy = nullptr;
+  int h = 0, c = 0;
+};
+
+struct cdx_unet {
+  int size, levels, base, max_batch;
+  hipStream_t st = nullptr;
+  cdx_ctx* ctx = nullptr;  // size x size FP32 context for the forward model
+  std::vector<UConv> conv;  // enc (a,b) per level, dec (a,b) per level < L-1, head
+  int64_t nparam = 0;
+  float *w = nullptr, *g = nullptr, *m1 = nullptr, *m2 = nullptr;
+  u16* wb = nullptr;
+  // activations
+  UTensor in0;
+  std::vector<UTensor> ea, e, p, cat, da, d;
+  float* yf = nullptr;  // head output [B][H][W][8]
+  u16* dz_head = nullptr;
+  float* tg = nullptr;  // targets [B][H][W][4]
+  double* dscal = nullptr;  // reductions (loss partials, sums)
+  void *obj = nullptr, *I = nullptr;  // forward-model scratch (one frame)
+  int64_t adam_t = 0;
+};
+
+namespace {
+
+int enc_a(int l) { return 2 * l; }
+int enc_b(int l) { return 2 * l + 1; }
+int dec_a(const cdx_unet* u, int l) { return 2 * u->levels + 2 * l; }
+int dec_b(const cdx_unet* u, int l) { return 2 * u->levels + 2 * l + 1; }
+int head(const cdx_unet* u) { return 2 * u->levels + 2 * (u->levels - 1); }
+
+int alloc_tensor(UTensor& t, int B, int h, int c) {
+  t.h = h;
+  t.c = c;
+  const size_t bytes = (size_t)B * h * h * c * sizeof(u16);
+  UCHK(hipMalloc(&t.y, bytes));
+  UCHK(hipMalloc(&t.dy, bytes));
+  return CDX_OK;
+}
+void free_tensor(UTensor& t) {
+  hipFree(t.y);
+  hipFree(t.dy);
+  t.y = t.dy = nullptr;
+}
+
+int conv_fwd(cdx_unet* u, const UConv& L, int B, int h, const u16* x, u16* y, float* yf, bool relu) {
+  ConvArgs a{x, u->wb + L.w_off, u->w + L.b_off, y, yf, B, h, h, L.cin, L.cout, L.kp, L.taps, relu ? 1 : 0};
+  const int M = B * h * h;
+  hipLaunchKernelGGL(k_conv, dim3((M + kTile - 1) / kTile, (L.cout + kTile - 1) / kTile), dim3(256), 0, u->st, a);
+  UCHK(hipGetLastError());
+  return CDX_OK;
+}
+// input gradient: dx = conv(dz, flipped weights), no bias
+int conv_dgrad(cdx_unet* u, const UConv& L, int B, int h, const u16* dz, u16* dx) {
+  ConvArgs a{dz, L.wflip, nullptr, dx, nullptr, B, h, h, L.cout, L.cin, L.kp2, L.taps, 0};
+  const int M = B * h * h;
+  hipLaunchKernelGGL(k_conv, dim3((M + kTile - 1) / kTile, (L.cin + kTile - 1) / kTile), dim3(256), 0, u->st, a);
+  UCHK(hipGetLastError());
+  return CDX_OK;
+}
+int conv_wgrad(cdx_unet* u, const UConv& L, int B, int h, const u16* dz, const u16* x) {
+  const int M = B * h * h, K = L.taps * L.cin;
+  const int tm = (L.cout + kTile - 1) / kTile, tn = (K + kTile - 1) / kTile;
+  int nsplit = 2048 / (tm * tn);
+  if (nsplit < 1) nsplit = 1;
+  if (nsplit > M / 256) nsplit = M / 256 > 0 ? M / 256 : 1;
+  int chunk = (M + nsplit - 1) / nsplit;
+  chunk = (chunk + kBK - 1) / kBK * kBK;
+  nsplit = (M + chunk - 1) / chunk;
+  WgradArgs a{dz, x, u->g + L.w_off, B, h, h, L.cin, L.cout, L.kp, L.taps, chunk};
+  hipLaunchKernelGGL(k_wgrad, dim3(tm, tn, nsplit), dim3(256), 0, u->st, a);
+  UCHK(hipGetLastError());
+  int gb = M / 64;
+  gb = gb < 1 ? 1 : (gb > 1024 ? 1024 : gb);
+  hipLaunchKernelGGL(k_bias_grad, dim3(gb), dim3(256), 0, u->st, dz, M, L.cout, u->g + L.b_off);
+  UCHK(hipGetLastError());
+  return CDX_OK;
+}
+int relu_bwd(cdx_unet* u, int64_t n, u16* dy, const u16* y) {
+  hipLaunchKernelGGL(k_relu_bwd, dim3(ew_blocks(n)), dim3(256), 0, u->st, n, dy, y);
+  UCHK(hipGetLastError());
+  return CDX_OK;
+}
+int refresh_weights(cdx_unet* u) {
+  hipLaunchKernelGGL(k_to_bf16, dim3(ew_blocks(u->nparam)), dim3(256), 0, u->st, u->nparam, u->w, u->wb);
+  UCHK(hipGetLastError());
+  for (auto& L : u->conv) {
+    const int64_t n = (int64_t)L.cin * L.taps * L.cout;
+    hipLaunchKernelGGL(k_flip, dim3(ew_blocks(n)), dim3(256), 0, u->st, L.cout, L.cin, L.taps, L.kp,
+                       (const u16*)(u->wb + L.w_off), L.kp2, L.wflip);
+    UCHK(hipGetLastError());
+  }
+  return CDX_OK;
+}
+
+int forward(cdx_unet* u, int B) {
+  const int Lv = u->levels;
+  for (int l = 0; l < Lv; l++) {
+    const int h = u->size >> l;
+    const u16* x = l == 0 ? u->in0.y : u->p[l].y;
+    RCHK(conv_fwd(u, u->conv[enc_a(l)], B, h, x, u->ea[l].y, nullptr, true));
+    RCHK(conv_fwd(u, u->conv[enc_b(l)], B, h, u->ea[l].y, u->e[l].y, nullptr, true));
+    if (l + 1 < Lv) {
+      const int64_t n = (int64_t)B * (h / 2) * (h / 2) * u->e[l].c;
+      hipLaunchKernelGGL(k_pool, dim3(ew_blocks(n)), dim3(256), 0, u->st, B, h, h, u->e[l].c, u->e[l].y,
+                         u->p[l + 1].y);
+      UCHK(hipGetLastError());
+    }
+  }
+  for (int l = Lv - 2; l >= 0; l--) {
+    const int h = u->size >> l;
+    const UTensor& low = l == Lv - 2 ? u->e[Lv - 1] : u->d[l + 1];
+    const int64_t n = (int64_t)B * h * h * u->cat[l].c;
+    hipLaunchKernelGGL(k_upcat, dim3(ew_blocks(n)), dim3(256), 0, u->st, B, h, h, low.c, u->e[l].c, low.y,
+                       u->e[l].y, u->cat[l].y);
+    UCHK(hipGetLastError());
+    RCHK(conv_fwd(u, u->conv[dec_a(u, l)], B, h, u->cat[l].y, u->da[l].y, nullptr, true));
+    RCHK(conv_fwd(u, u->conv[dec_b(u, l)], B, h, u->da[l].y, u->d[l].y, nullptr, true));
+  }
+  const UTensor& top = Lv >= 2 ? u->d[0] : u->e[0];
+  RCHK(conv_fwd(u, u->conv[head(u)], B, u->size, top.y, nullptr, u->yf, false));
+  return CDX_OK;
+}
+
+int backward(cdx_unet* u, int B) {
+  const int Lv = u->levels;
+  const int H0 = u->size;
+  UTensor& top = Lv >= 2 ? u->d[0] : u->e[0];
+  const UConv& hd = u->conv[head(u)];
+  RCHK(conv_wgrad(u, hd, B, H0, u->dz_head, top.y));
+  RCHK(conv_dgrad(u, hd, B, H0, u->dz_head, top.dy));
+  for (int l = 0; l + 1 < Lv; l++) {
+    const int h = H0 >> l;
+    const int64_t np = (int64_t)B * h * h;
+    RCHK(relu_bwd(u, np * u->d[l].c, u->d[l].dy, u->d[l].y));
+    RCHK(conv_wgrad(u, u->conv[dec_b(u, l)], B, h, u->d[l].dy, u->da[l].y));
+    RCHK(conv_dgrad(u, u->conv[dec_b(u, l)], B, h, u->d[l].dy, u->da[l].dy));
+    RCHK(relu_bwd(u, np * u->da[l].c, u->da[l].dy, u->da[l].y));
+    RCHK(conv_wgrad(u, u->conv[dec_a(u, l)], B, h, u->da[l].dy, u->cat[l].y));
+    RCHK(conv_dgrad(u, u->conv[dec_a(u, l)], B, h, u->da[l].dy, u->cat[l].dy));
+    UTensor& low = l == Lv - 2 ? u->e[Lv - 1] : u->d[l + 1];
+    const int Cs = u->e[l].c;
+    hipLaunchKernelGGL(k_upcat_bwd_skip, dim3(ew_blocks(np * Cs)), dim3(256), 0, u->st, np, low.c, Cs,
+                       u->cat[l].dy, u->e[l].dy);
+    UCHK(hipGetLastError());
+    const int64_t nl = (int64_t)B * (h / 2) * (h / 2) * low.c;
+    hipLaunchKernelGGL(k_upcat_bwd_low, dim3(ew_blocks(nl)), dim3(256), 0, u->st, B, h / 2, h / 2, low.c, Cs,
+                       u->cat[l].dy, low.dy);
+    UCHK(hipGetLastError());
+  }
+  for (int l = Lv - 1; l >= 0; l--) {
+    const int h = H0 >> l;
+    const int64_t np = (int64_t)B * h * h;
+    RCHK(relu_bwd(u, np * u->e[l].c, u->e[l].dy, u->e[l].y));
+    RCHK(conv_wgrad(u, u->conv[enc_b(l)], B, h, u->e[l].dy, u->ea[l].y));
+    RCHK(conv_dgrad(u, u->conv[enc_b(l)], B, h, u->e[l].dy, u->ea[l].dy));
+    RCHK(relu_bwd(u, np * u->ea[l].c, u->ea[l].dy, u->ea[l].y));
+    const u16* x = l == 0 ? u->in0.y : u->p[l].y;
+    RCHK(conv_wgrad(u, u->conv[enc_a(l)], B, h, u->ea[l].dy, x));
+    if (l > 0) {
+      RCHK(conv_dgrad(u, u->conv[enc_a(l)], B, h, u->ea[l].dy, u->p[l].dy));
+      const int hp = h * 2;
+      const int64_t n = (int64_t)B * hp * hp * u->e[l - 1].c;
+      hipLaunchKernelGGL(k_pool_bwd_add, dim3(ew_blocks(n)), dim3(256), 0, u->st, B, hp, hp, u->e[l - 1].c,
+                         u->p[l].dy, u->e[l - 1].dy);
+      UCHK(hipGetLastError());
+    }
+  }
+  return CDX_OK;
+}
+
+int loss_grad(cdx_unet* u, int B, double* loss) {
+  const int M = B * u->size * u->size;
+  const int nb = 256;  // fixed grid: partial sums in dscal[0..nb)
+  hipLaunchKernelGGL(k_loss, dim3(nb), dim3(256), 0, u->st, M, u->yf, u->tg, 0.5f, 1.0f, u->dz_head, u->dscal);
+  UCHK(hipGetLastError());
+  if (loss) {
+    std::vector<double> h(nb);
+    UCHK(hipMemcpyAsync(h.data(), u->dscal, nb * sizeof(double), hipMemcpyDeviceToHost, u->st));
+    UCHK(hipStreamSynchronize(u->st));
+    double s = 0;
+    for (double v : h) s += v;
+    *loss = s / M;
+  }
+  return CDX_OK;
+}
+
+int adam(cdx_unet* u, double lr) {
+  u->adam_t++;
+  const double b1 = 0.9, b2 = 0.999;
+  const float c1 = (float)(1 - pow(b1, (double)u->adam_t)), c2 = (float)(1 - pow(b2, (double)u->adam_t));
+  hipLaunchKernelGGL(k_adam, dim3(ew_blocks(u->nparam)), dim3(256), 0, u->st, u->nparam, u->w, u->g, u->m1, u->m2,
+                     u->wb, (float)lr, (float)b1, (float)b2, 1e-8f, c1, c2);
+  UCHK(hipGetLastError());
+  for (auto& L : u->conv) {
+    const int64_t n = (int64_t)L.cin * L.taps * L.cout;
+    hipLaunchKernelGGL(k_flip, dim3(ew_blocks(n)), dim3(256), 0, u->st, L.cout, L.cin, L.taps, L.kp,
+                       (const u16*)(u->wb + L.w_off), L.kp2, L.wflip);
+    UCHK(hipGetLastError());
+  }
+  return CDX_OK;
+}
+
+// forward-model sample: shape, photons and seeds drawn from `seed`
+struct SampleSpec {
+  int kind, np;
+  double prm[16];
+  double photons;
+};
+SampleSpec draw_sample(uint64_t seed, double lo, double hi) {
+  SampleSpec s{};
+  auto U = [&](int k) { return uniform_host(seed, 10, (uint64_t)k); };
+  s.kind = U(0) < 0.5 ? 1 : 2;
+  if (s.kind == 1) {
+    s.np = 3;
+    s.prm[0] = 20 + 40 * U(1);
+    s.prm[1] = 20 + 40 * U(2);
+    s.prm[2] = M_PI * U(3);
+  } else {
+    const int nv = 3 + (int)(6 * U(1));
+    s.np = 2 * nv;
+    for (int i = 0; i < nv; i++) {
+      s.prm[i] = 2 * M_PI * (i + 0.4 + 0.2 * U(2 + i)) / nv;
+      s.prm[nv + i] = 20 + 40 * U(10 + i);
+    }
+  }
+  s.photons = pow(10.0, log10(lo) + (log10(hi) - log10(lo)) * U(20));
+  return s;
+}
+
+int synth_frame(cdx_unet* u, uint64_t seed, double lo, double hi, void* obj, void* I) {
+  const SampleSpec s = draw_sample(seed, lo, hi);
+  RCHK(cdx_sim_object_shape(u->ctx, s.kind, s.prm, s.np, seed, 3.0, obj));
+  RCHK(cdx_sim_intensity(u->ctx, obj, s.photons, seed, 1, I));
+  return CDX_OK;
+}
+
+int synth_batch(cdx_unet* u, int B, uint64_t seed0, double lo, double hi) {
+  const int n = u->size;
+  const int64_t N = (int64_t)n * n;
+  for (int b = 0; b < B; b++) {
+    RCHK(synth_frame(u, seed0 + (uint64_t)b, lo, hi, u->obj, u->I));
+    hipLaunchKernelGGL(k_pack_input, dim3(ew_blocks(N)), dim3(256), 0, u->st, n, (const float*)u->I,
+                       u->in0.y + (size_t)b * N * 8);
+    UCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_csum, dim3(1), dim3(1024), 0, u->st, N, (const cplx<float>*)u->obj, u->dscal);
+    UCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_pack_target, dim3(ew_blocks(N)), dim3(256), 0, u->st, N, (const cplx<float>*)u->obj,
+                       (const double*)u->dscal, u->tg + (size_t)b * N * 4);
+    UCHK(hipGetLastError());
+  }
+  return CDX_OK;
+}
+
+void destroy(cdx_unet* u) {
+  if (!u) return;
+  if (u->st) hipStreamSynchronize(u->st);
+  for (auto& L : u->conv) hipFree(L.wflip);
+  for (void* p : {(void*)u->w, (void*)u->g, (void*)u->m1, (void*)u->m2, (void*)u->wb, (void*)u->yf,
+                  (void*)u->dz_head, (void*)u->tg, (void*)u->dscal, u->obj, u->I})
+    hipFree(p);
+  free_tensor(u->in0);
+  for (auto* v : {&u->ea, &u->e, &u->p, &u->cat, &u->da, &u->d})
+    for (auto& t : *v) free_tensor(t);
+  if (u->ctx) cdx_destroy(u->ctx);
+  delete u;
+}
+
+}  // namespace
+
+#define UNET_CHECK(u) \
+  if (!(u)) return ufail(CDX_ERR_ARG, "%s: null network", __func__)
+
+extern "C" int cdx_unet_create(cdx_unet** out, int size, int levels, int base, int max_batch, uint64_t seed) {
+  if (!out) return ufail(CDX_ERR_ARG, "cdx_unet_create: null out pointer");
+  *out = nullptr;
+  if (levels < 1 || levels > 6 || base < 8 || base % 8 || max_batch < 1 || size < 16 || (size >> (levels - 1)) < 2 ||
+      (size >> (levels - 1)) << (levels - 1) != size || !cdx_size_supported(size))
+    return ufail(CDX_ERR_ARG, "cdx_unet_create: unsupported geometry size %d levels %d base %d batch %d", size,
+                 levels, base, max_batch);
+  cdx_unet* u = new cdx_unet();
+  u->size = size;
+  u->levels = levels;
+  u->base = base;
+  u->max_batch = max_batch;
+  int rc = cdx_create(&u->ctx, size, size, CDX_F32, nullptr);
+  if (rc) {
+    delete u;
+    return rc;
+  }
+  u->st = (hipStream_t)cdx_stream(u->ctx);
+  // layer table
+  auto add = [&](int cin, int cout, int taps, int relu) {
+    UConv L{};
+    L.cin = cin;
+    L.cout = cout;
+    L.taps = taps;
+    L.relu = relu;
+    L.kp = (taps * cin + kBK - 1) / kBK * kBK;
+    L.kp2 = (taps * cout + kBK - 1) / kBK * kBK;
+    L.w_off = u->nparam;
+    u->nparam += (int64_t)cout * L.kp;
+    L.b_off = u->nparam;
+    u->nparam += (cout + 7) / 8 * 8;
+    u->conv.push_back(L);
+  };
+  for (int l = 0; l < levels; l++) {
+    const int c = base << l;
+    add(l == 0 ? 8 : (base << (l - 1)), c, 9, 1);
+    add(c, c, 9, 1);
+  }
+  for (int l = 0; l + 1 < levels; l++) {
+    const int c = base << l;
+    add((base << (l + 1)) + c, c, 9, 1);
+    add(c, c, 9, 1);
+  }
+  add(base, 8, 1, 0);
+  const size_t pb = (size_t)u->nparam * sizeof(float);
+  const int B = max_batch;
+  const int64_t M = (int64_t)B * size * size;
+  bool ok = hipMalloc(&u->w, pb) == hipSuccess && hipMalloc(&u->g, pb) == hipSuccess &&
+            hipMalloc(&u->m1, pb) == hipSuccess && hipMalloc(&u->m2, pb) == hipSuccess &&
+            hipMalloc(&u->wb, (size_t)u->nparam * sizeof(u16)) == hipSuccess &&
+            hipMalloc(&u->yf, (size_t)M * 8 * sizeof(float)) == hipSuccess &&
+            hipMalloc(&u->dz_head, (size_t)M * 8 * sizeof(u16)) == hipSuccess &&
+            hipMalloc(&u->tg, (size_t)M * 4 * sizeof(float)) == hipSuccess &&
+            hipMalloc(&u->dscal, 4096 * sizeof(double)) == hipSuccess &&
+            hipMalloc(&u->obj, (size_t)size * size * 8) == hipSuccess &&
+            hipMalloc(&u->I, (size_t)size * size * 4) == hipSuccess;
+  for (auto& L : u->conv) {
+    ok = ok && hipMalloc(&L.wflip, (size_t)L.cin * L.kp2 * sizeof(u16)) == hipSuccess;
+    if (ok) {  // the K padding of the flipped weights stays zero
+      const int64_t nf = (int64_t)L.cin * L.kp2;
+      hipLaunchKernelGGL(k_zero_u16, dim3(ew_blocks(nf)), dim3(256), 0, u->st, nf, L.wflip);
+    }
+  }
+  u->ea.resize(levels);
+  u->e.resize(levels);
+  u->p.resize(levels);
+  u->cat.resize(levels);
+  u->da.resize(levels);
+  u->d.resize(levels);
+  ok = ok && alloc_tensor(u->in0, B, size, 8) == CDX_OK;
+  for (int l = 0; l < levels && ok; l++) {
+    const int h = size >> l, c = base << l;
+    ok = alloc_tensor(u->ea[l], B, h, c) == CDX_OK && alloc_tensor(u->e[l], B, h, c) == CDX_OK;
+    if (ok && l > 0) ok = alloc_tensor(u->p[l], B, h, base << (l - 1)) == CDX_OK;
+    if (ok && l + 1 < levels)
+      ok = alloc_tensor(u->cat[l], B, h, (base << (l + 1)) + c) == CDX_OK &&
+           alloc_tensor(u->da[l], B, h, c) == CDX_OK && alloc_tensor(u->d[l], B, h, c) == CDX_OK;
+  }
+  if (!ok) {
+    destroy(u);
+    return ufail(CDX_ERR_HIP, "cdx_unet_create: out of device memory");
+  }
+  // He-normal weights (fan-in of the real input channels), zero biases and
+  // padding; the head starts small
+  std::vector<float> h((size_t)u->nparam, 0.f);
+  uint64_t ctr = 0;
+  auto normal = [&]() {
+    const double u1 = uniform_host(seed, 11, ctr++), u2 = uniform_host(seed, 11, ctr++);
+    return sqrt(-2 * log(u1 + 1e-300)) * cos(2 * M_PI * u2);
+  };
+  for (size_t i = 0; i < u->conv.size(); i++) {
+    const UConv& L = u->conv[i];
+    const int cin_real = i == 0 ? 1 : L.cin;
+    const bool is_head = i + 1 == u->conv.size();
+    const double sd = is_head ? 0.1 / sqrt((double)L.cin) : sqrt(2.0 / (L.taps * cin_real));
+    for (int co = 0; co < L.cout; co++)
+      for (int t = 0; t < L.taps; t++)
+        for (int ci = 0; ci < L.cin; ci++)
+          h[L.w_off + (size_t)co * L.kp + t * L.cin + ci] = (i == 0 && ci > 0) ? 0.f : (float)(sd * normal());
+  }
+  rc = cdx_unet_set_params(u, h.data(), u->nparam);
+  if (!rc) {
+    hipLaunchKernelGGL(k_zero_f32, dim3(ew_blocks(u->nparam)), dim3(256), 0, u->st, u->nparam, u->g);
+    hipLaunchKernelGGL(k_zero_f32, dim3(ew_blocks(u->nparam)), dim3(256), 0, u->st, u->nparam, u->m1);
+    hipLaunchKernelGGL(k_zero_f32, dim3(ew_blocks(u->nparam)), dim3(256), 0, u->st, u->nparam, u->m2);
+    hipLaunchKernelGGL(k_zero_u16, dim3(ew_blocks(M * 8)), dim3(256), 0, u->st, M * 8, u->in0.y);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(u->st) != hipSuccess)
+      rc = ufail(CDX_ERR_HIP, "cdx_unet_create: initialisation failed");
+  }
+  if (rc) {
+    destroy(u);
+    return rc;
+  }
+  *out = u;
+  return CDX_OK;
+}
+
+extern "C" int cdx_unet_destroy(cdx_unet* u) {
+  destroy(u);
+  return CDX_OK;
+}
+
+extern "C" int cdx_unet_num_params(cdx_unet* u, int64_t* n) {
+  UNET_CHECK(u);
+  if (n) *n = u->nparam;
+  return CDX_OK;
+}
+
+extern "C" int cdx_unet_layers(cdx_unet* u, int64_t* info, int cap) {
+  UNET_CHECK(u);
+  const int nl = (int)u->conv.size();
+  for (int i = 0; i < nl && 6 * (i + 1) <= cap; i++) {
+    const UConv& L = u->conv[i];
+    int64_t* o = info + 6 * i;
+    o[0] = L.cin;
+    o[1] = L.cout;
+    o[2] = L.taps;
+    o[3] = L.kp;
+    o[4] = L.w_off;
+    o[5] = L.b_off;
+  }
+  return nl;
+}
+
+extern "C" int cdx_unet_set_params(cdx_unet* u, const float* host, int64_t n) {
+  UNET_CHECK(u);
+  if (!host || n != u->nparam) return ufail(CDX_ERR_ARG, "cdx_unet_set_params: expected %lld values", (long long)u->nparam);
+  UCHK(hipMemcpyAsync(u->w, host, (size_t)n * sizeof(float), hipMemcpyHostToDevice, u->st));
+  RCHK(refresh_weights(u));
+  UCHK(hipStreamSynchronize(u->st));
+  return CDX_OK;
+}
+extern "C" int cdx_unet_get_params(cdx_unet* u, float* host, int64_t n) {
+  UNET_CHECK(u);
+  if (!host || n != u->nparam) return ufail(CDX_ERR_ARG, "cdx_unet_get_params: expected %lld values", (long long)u->nparam);
+  UCHK(hipMemcpyAsync(host, u->w, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, u->st));
+  UCHK(hipStreamSynchronize(u->st));
+  return CDX_OK;
+}
+extern "C" int cdx_unet_get_grads(cdx_unet* u, float* host, int64_t n) {
+  UNET_CHECK(u);
+  if (!host || n != u->nparam) return ufail(CDX_ERR_ARG, "cdx_unet_get_grads: expected %lld values", (long long)u->nparam);
+  UCHK(hipMemcpyAsync(host, u->g, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, u->st));
+  UCHK(hipStreamSynchronize(u->st));
+  return CDX_OK;
+}
+extern "C" int cdx_unet_zero_grads(cdx_unet* u) {
+  UNET_CHECK(u);
+  hipLaunchKernelGGL(k_zero_f32, dim3(ew_blocks(u->nparam)), dim3(256), 0, u->st, u->nparam, u->g);
+  UCHK(hipGetLastError());
+  return CDX_OK;
+}
+
+// host batch: x [B][size][size] (network input channel 0, already scaled),
+// targets [B][size][size][3] (amplitude, phase, mask) or NULL
+extern "C" int cdx_unet_set_batch(cdx_unet* u, int B, const float* x, const float* targets) {
+  UNET_CHECK(u);
+  if (B < 1 || B > u->max_batch || !x) return ufail(CDX_ERR_ARG, "cdx_unet_set_batch: bad batch");
+  const size_t P = (size_t)B * u->size * u->size;
+  std::vector<u16> xi(P * 8, 0);
+  for (size_t i = 0; i < P; i++) xi[i * 8] = f2bf(x[i]);
+  UCHK(hipMemcpyAsync(u->in0.y, xi.data(), xi.size() * sizeof(u16), hipMemcpyHostToDevice, u->st));
+  if (targets) {
+    std::vector<float> t(P * 4, 0.f);
+    for (size_t i = 0; i < P; i++)
+      for (int c = 0; c < 3; c++) t[i * 4 + c] = targets[i * 3 + c];
+    UCHK(hipMemcpyAsync(u->tg, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice, u->st));
+  }
+  UCHK(hipStreamSynchronize(u->st));
+  return CDX_OK;
+}
+// current input / targets back to the host (same layouts as cdx_unet_set_batch)
+extern "C" int cdx_unet_get_batch(cdx_unet* u, int B, float* x, float* targets) {
+  UNET_CHECK(u);
+  if (B < 1 || B > u->max_batch) return ufail(CDX_ERR_ARG, "cdx_unet_get_batch: bad batch");
+  const size_t P = (size_t)B * u->size * u->size;
+  if (x) {
+    std::vector<u16> xi(P * 8);
+    UCHK(hipMemcpyAsync(xi.data(), u->in0.y, xi.size() * sizeof(u16), hipMemcpyDeviceToHost, u->st));
+    UCHK(hipStreamSynchronize(u->st));
+    for (size_t i = 0; i < P; i++) x[i] = bf2f(xi[i * 8]);
+  }
+  if (targets) {
+    std::vector<float> t(P * 4);
+    UCHK(hipMemcpyAsync(t.data(), u->tg, t.size() * sizeof(float), hipMemcpyDeviceToHost, u->st));
+    UCHK(hipStreamSynchronize(u->st));
+    for (size_t i = 0; i < P; i++)
+      for (int c = 0; c < 3; c++) targets[i * 3 + c] = t[i * 4 + c];
+  }
+  return CDX_OK;
+}
+
+extern "C" int cdx_unet_forward(cdx_unet* u, int B) {
+  UNET_CHECK(u);
+  if (B < 1 || B > u->max_batch) return ufail(CDX_ERR_ARG, "cdx_unet_forward: bad batch");
+  RCHK(forward(u, B));
+  UCHK(hipStreamSynchronize(u->st));
+  return CDX_OK;
+}
+// head output [B][size][size][3]: amplitude, phase, support logit
+extern "C" int cdx_unet_output(cdx_unet* u, int B, float* host) {
+  UNET_CHECK(u);
+  if (B < 1 || B > u->max_batch || !host) return ufail(CDX_ERR_ARG, "cdx_unet_output: bad batch");
+  const size_t P = (size_t)B * u->size * u->size;
+  std::vector<float> y(P * 8);
+  UCHK(hipMemcpyAsync(y.data(), u->yf, y.size() * sizeof(float), hipMemcpyDeviceToHost, u->st));
+  UCHK(hipStreamSynchronize(u->st));
+  for (size_t i = 0; i < P; i++)
+    for (int c = 0; c < 3; c++) host[i * 3 + c] = y[i * 8 + c];
+  return CDX_OK;
+}
+// loss of the last forward against the targets, gradients added to the
+// parameter gradients (cleared by cdx_unet_zero_grads and by each Adam step)
+extern "C" int cdx_unet_backward(cdx_unet* u, int B, double* loss) {
+  UNET_CHECK(u);
+  if (B < 1 || B > u->max_batch) return ufail(CDX_ERR_ARG, "cdx_unet_backward: bad batch");
+  RCHK(loss_grad(u, B, loss));
+  RCHK(backward(u, B));
+  UCHK(hipStreamSynchronize(u->st));
+  return CDX_OK;
+}
+extern "C" int cdx_unet_adam(cdx_unet* u, double lr) {
+  UNET_CHECK(u);
+  RCHK(adam(u, lr));
+  UCHK(hipStreamSynchronize(u->st));
+  return CDX_OK;
+}
+
+extern "C" int cdx_unet_synth_batch(cdx_unet* u, int B, uint64_t seed0, double photons_lo, double photons_hi) {
+  UNET_CHECK(u);
+  if (B < 1 || B > u->max_batch || !(photons_lo > 0) || !(photons_hi >= photons_lo))
+    return ufail(CDX_ERR_ARG, "cdx_unet_synth_batch: bad arguments");
+  RCHK(synth_batch(u, B, seed0, photons_lo, photons_hi));
+  UCHK(hipStreamSynchronize(u->st));
+  return CDX_OK;
+}
+extern "C" int cdx_unet_synth_frame(cdx_unet* u, uint64_t seed, double photons_lo, double photons_hi, void* obj,
+                                    void* I) {
+  UNET_CHECK(u);
+  if (!obj || !I) return ufail(CDX_ERR_ARG, "cdx_unet_synth_frame: null buffer");
+  return synth_frame(u, seed, photons_lo, photons_hi, obj, I);
+}
+
+extern "C" int cdx_unet_train(cdx_unet* u, int steps, int B, uint64_t seed0, double lr, double photons_lo,
+                              double photons_hi, double* losses) {
+  UNET_CHECK(u);
+  if (steps < 0 || B < 1 || B > u->max_batch) return ufail(CDX_ERR_ARG, "cdx_unet_train: bad arguments");
+  for (int s = 0; s < steps; s++) {
+    RCHK(synth_batch(u, B, seed0 + (uint64_t)s * B, photons_lo, photons_hi));
+    RCHK(forward(u, B));
+    double l = 0;
+    RCHK(loss_grad(u, B, losses ? &l : nullptr));
+    RCHK(backward(u, B));
+    RCHK(adam(u, lr));
+    if (losses) losses[s] = l;
+  }
+  UCHK(hipStreamSynchronize(u->st));
+  return CDX_OK;
+}
+
+// warm start of a reconstruction on `ctx` (size x size, FP32) from the
+// measured pattern I (FFT order): F = FFT(c g0), g = IFFT(F)/sqrt(N) (the
+// state CDI::prepareIter leaves), spt = predicted support (0/1, centred),
+// with c scaling g0 to the measured total intensity (Parseval)
+extern "C" int cdx_unet_warm_start(cdx_unet* u, cdx_ctx* ctx, const void* I, double support_threshold, void* F,
+                                   void* g, void* spt) {
+  UNET_CHECK(u);
+  if (!ctx || !I || !F || !g || !spt) return ufail(CDX_ERR_ARG, "cdx_unet_warm_start: null argument");
+  int rows = 0, cols = 0, prec = 0;
+  RCHK(cdx_context_info(ctx, &rows, &cols, &prec));
+  if (rows != u->size || cols != u->size || prec != CDX_F32)
+    return ufail(CDX_ERR_ARG, "cdx_unet_warm_start: context must be %dx%d FP32", u->size, u->size);
+  const int n = u->size;
+  const int64_t N = (int64_t)n * n;
+  RCHK(cdx_synchronize(ctx));
+  hipLaunchKernelGGL(k_pack_input, dim3(ew_blocks(N)), dim3(256), 0, u->st, n, (const float*)I, u->in0.y);
+  UCHK(hipGetLastError());
+  RCHK(forward(u, 1));
+  hipLaunchKernelGGL(k_ws_compose, dim3(ew_blocks(N)), dim3(256), 0, u->st, N, (const float*)u->yf,
+                     (float)support_threshold, (cplx<float>*)g, (float*)spt);
+  UCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_sum_sq, dim3(1), dim3(1024), 0, u->st, N, (const cplx<float>*)g, (const float*)I, u->dscal);
+  UCHK(hipGetLastError());
+  double sums[2];
+  UCHK(hipMemcpyAsync(sums, u->dscal, sizeof sums, hipMemcpyDeviceToHost, u->st));
+  UCHK(hipStreamSynchronize(u->st));
+  const double c = sums[0] > 0 ? sqrt(sums[1] / ((double)N * sums[0])) : 0.0;
+  RCHK(cdx_apply_norm(ctx, g, c));
+  RCHK(cdx_fft(ctx, g, F, CDX_FORWARD));
+  RCHK(cdx_apply_norm(ctx, g, sqrt((double)N)));
+  RCHK(cdx_synchronize(ctx));
+  return CDX_OK;
+}
+
+// parameters as a small binary file: "CDXUNET1", size, levels, base, count, FP32 values
+extern "C" int cdx_unet_save(cdx_unet* u, const char* path) {
+  UNET_CHECK(u);
+  std::vector<float> h((size_t)u->nparam);
+  RCHK(cdx_unet_get_params(u, h.data(), u->nparam));
+  FILE* f = path ? fopen(path, "wb") : nullptr;
+  if (!f) return ufail(CDX_ERR_IO, "cdx_unet_save: cannot open %s", path ? path : "(null)");
+  const int32_t hdr[3] = {u->size, u->levels, u->base};
+  const int64_t n = u->nparam;
+  bool ok = fwrite("CDXUNET1", 1, 8, f) == 8 && fwrite(hdr, sizeof hdr, 1, f) == 1 && fwrite(&n, 8, 1, f) == 1 &&
+            fwrite(h.data(), sizeof(float), h.size(), f) == h.size();
+  ok = (fclose(f) == 0) && ok;
+  return ok ? CDX_OK : ufail(CDX_ERR_IO, "cdx_unet_save: write failed on %s", path);
+}
+extern "C" int cdx_unet_load(cdx_unet* u, const char* path) {
+  UNET_CHECK(u);
+  FILE* f = path ? fopen(path, "rb") : nullptr;
+  if (!f) return ufail(CDX_ERR_IO, "cdx_unet_load: cannot open %s", path ? path : "(null)");
+  char magic[8];
+  int32_t hdr[3];
+  int64_t n = 0;
+  bool ok = fread(magic, 1, 8, f) == 8 && memcmp(magic, "CDXUNET1", 8) == 0 && fread(hdr, sizeof hdr, 1, f) == 1 &&
+            fread(&n, 8, 1, f) == 1;
+  if (!ok || hdr[0] != u->size || hdr[1] != u->levels || hdr[2] != u->base || n != u->nparam) {
+    fclose(f);
+    return ufail(CDX_ERR_IO, "cdx_unet_load: %s does not hold a %d/%d/%d network", path, u->size, u->levels, u->base);
+  }
+  std::vector<float> h((size_t)n);
+  ok = fread(h.data(), sizeof(float), h.size(), f) == h.size();
+  fclose(f);
+  if (!ok) return ufail(CDX_ERR_IO, "cdx_unet_load: short read on %s", path);
+  return cdx_unet_set_params(u, h.data(), n);
+}

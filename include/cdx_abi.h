@@ -68,6 +68,8 @@ int cdx_copy(void* stream, void* dst, const void* src, size_t bytes);
 int cdx_create(cdx_ctx** out, int rows, int cols, int precision, void* stream);
 int cdx_destroy(cdx_ctx* ctx);
 int cdx_synchronize(cdx_ctx* ctx);
+/* size and precision of a context */
+int cdx_context_info(cdx_ctx* ctx, int* rows, int* cols, int* precision);
 void* cdx_stream(cdx_ctx* ctx);
 /* init_cuda_image(rcolor, scale), include/cudaConfig.hpp:23 */
 int cdx_init_vars(cdx_ctx* ctx, int rcolor, double scale);
@@ -234,8 +236,66 @@ int cdx_autotune(cdx_ctx* ctx, int reps, double* report, int cap);
 /* n x n object: smoothed-noise amplitude in [0.2,1] and phase in [-pi,pi]
  * inside a centred disk of the given radius.  Context must be n x n. */
 int cdx_sim_object(cdx_ctx* ctx, double radius, uint64_t seed, double smooth_sigma, void* obj);
+/* the same object inside another support shape centred at (n/2, n/2):
+ * kind 0 disk {radius}; 1 ellipse {a, b, rotation}, |rotated (dx/a, dy/b)| < 1;
+ * 2 polygon {ang_0..ang_{k-1}, rad_0..rad_{k-1}}, 3 <= k <= 8 vertices at
+ * increasing angles in [0, 2 pi), star-shaped about the centre (np = 2k) */
+int cdx_sim_object_shape(cdx_ctx* ctx, int kind, const double* params, int np, uint64_t seed, double smooth_sigma,
+                         void* obj);
 /* I = |FFT(obj)|^2 scaled to `photons` in total, Poisson sampled if `poisson` */
 int cdx_sim_intensity(cdx_ctx* ctx, const void* obj, double photons, uint64_t seed, int poisson, void* I);
+
+/* ---- learned warm start (no counterpart in the reference, whose loop
+ * starts from random signs, src/post_gpu/cdi.cc:222-223): a U-Net on the
+ * centred log-intensity log(1 + I)/16 predicting the object amplitude, its
+ * phase (relative to the global phase arg(sum obj)) and a support logit.
+ * BF16 MFMA convolutions with FP32 accumulation; Adam on FP32 master weights.
+ * Geometry: size x size input (size a supported FFT length divisible by
+ * 2^(levels-1)), `levels` resolution levels of base*2^l channels, two 3x3
+ * convolutions + ReLU per level, 2x2 average pooling, nearest upsampling and
+ * skip concatenation, 1x1 head.  One network per host thread. ---- */
+typedef struct cdx_unet cdx_unet;
+int cdx_unet_create(cdx_unet** out, int size, int levels, int base, int max_batch, uint64_t seed);
+int cdx_unet_destroy(cdx_unet* net);
+int cdx_unet_num_params(cdx_unet* net, int64_t* n);
+/* per conv layer (encoder a,b per level; decoder a,b per level below the
+ * bottom; head): cin, cout, taps, Kp, weight offset, bias offset into the
+ * flat parameter array (weights [cout][Kp], k = tap*cin + ci); returns the
+ * layer count, fills up to cap values */
+int cdx_unet_layers(cdx_unet* net, int64_t* info, int cap);
+int cdx_unet_set_params(cdx_unet* net, const float* host, int64_t n);
+int cdx_unet_get_params(cdx_unet* net, float* host, int64_t n);
+int cdx_unet_get_grads(cdx_unet* net, float* host, int64_t n);
+int cdx_unet_zero_grads(cdx_unet* net);
+/* host batch in: x [B][size][size] network input, targets [B][size][size][3]
+ * (amplitude, phase, mask) or NULL; and back out */
+int cdx_unet_set_batch(cdx_unet* net, int batch, const float* x, const float* targets);
+int cdx_unet_get_batch(cdx_unet* net, int batch, float* x, float* targets);
+int cdx_unet_forward(cdx_unet* net, int batch);
+/* head output of the last forward: [B][size][size][3] amplitude, phase, logit */
+int cdx_unet_output(cdx_unet* net, int batch, float* host);
+/* loss of the last forward (mean over pixels of (a-a*)^2 + 0.5 m*(1-cos(p-p*))
+ * + BCE(logit, m*)) and its gradients added to the parameter gradients */
+int cdx_unet_backward(cdx_unet* net, int batch, double* loss);
+int cdx_unet_adam(cdx_unet* net, double lr);
+/* forward-model training pairs on the GPU: sample b uses seed seed0 + b
+ * (ellipse or polygon support of radius 20-60 px, photons log-uniform in
+ * [lo, hi], Poisson noise) */
+int cdx_unet_synth_batch(cdx_unet* net, int batch, uint64_t seed0, double photons_lo, double photons_hi);
+/* one frame of the same generator into caller buffers (complex obj, real I,
+ * FFT order; FP32, size x size) */
+int cdx_unet_synth_frame(cdx_unet* net, uint64_t seed, double photons_lo, double photons_hi, void* obj, void* I);
+/* steps x (synthesise batch, forward, backward, Adam); losses[steps] optional */
+int cdx_unet_train(cdx_unet* net, int steps, int batch, uint64_t seed0, double lr, double photons_lo,
+                   double photons_hi, double* losses);
+/* warm start on ctx (size x size, FP32) from the measured I (FFT order):
+ * spt = sigmoid(logit) > support_threshold (0/1, centred), g0 = amplitude
+ * e^{i phase} on the support scaled to the measured total intensity,
+ * F = FFT(g0), g = IFFT(F)/sqrt(N) (the state CDI::prepareIter leaves) */
+int cdx_unet_warm_start(cdx_unet* net, cdx_ctx* ctx, const void* I, double support_threshold, void* F, void* g,
+                        void* spt);
+int cdx_unet_save(cdx_unet* net, const char* path);
+int cdx_unet_load(cdx_unet* net, const char* path);
 
 #ifdef __cplusplus
 }

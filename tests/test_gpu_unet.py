@@ -1,0 +1,127 @@
+"""The learned warm start on the GPU: forward pass and gradients of the BF16
+MFMA U-Net against the CPU oracle (oracle/unet_ref.py, same BF16 rounding
+points, float64 arithmetic), training, the forward-model data path and the
+warm start itself."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def cdx():
+    import cdixx_amd
+    return cdixx_amd
+
+
+def _rand_net(cdx, size, levels, base, B, seed=0):
+    from oracle.unet_ref import UNetRef
+    net = cdx.UNet(size, levels, base, max_batch=B, seed=seed)
+    rs = np.random.default_rng(seed)
+    flat = net.params().astype(np.float64)
+    # keep the He init but give the biases and the head some spread
+    for cin, cout, taps, kp, wo, bo in net.layers():
+        flat[bo:bo + cout] = rs.uniform(-0.05, 0.2, cout)
+        flat[wo:wo + cout * kp] *= 1.2
+    flat = flat.astype(np.float32)
+    net.set_params(flat)
+    ref = UNetRef(net.layers(), size, levels, base, bf16=True)
+    return net, ref, flat.astype(np.float64)
+
+
+@pytest.mark.parametrize("size,levels,base,B", [(32, 3, 8, 2), (64, 4, 16, 1)])
+def test_forward_and_gradients_match_oracle(cdx, size, levels, base, B):
+    print(f"unet parity size={size} levels={levels} base={base} B={B}", flush=True)
+    net, ref, flat = _rand_net(cdx, size, levels, base, B)
+    rs = np.random.default_rng(5)
+    x = rs.uniform(0, 1, (B, size, size)).astype(np.float32)
+    tgt = np.stack([rs.uniform(0, 1, (B, size, size)), rs.uniform(-3, 3, (B, size, size)),
+                    (rs.uniform(size=(B, size, size)) > 0.5).astype(float)], axis=-1).astype(np.float32)
+    net.set_batch(x, tgt)
+    y = net.forward(B)
+    yr, cache = ref.forward(x, ref.unpack(flat))
+    assert rel_l2(y, yr[..., :3]) < 1e-2, rel_l2(y, yr[..., :3])
+    net.zero_grads()
+    loss = net.backward(B)
+    lr_, dy = ref.loss(yr, tgt)
+    assert abs(loss - lr_) <= 1e-3 * abs(lr_)
+    g = net.grads()
+    gr = ref.pack_grads(ref.backward(dy, cache), net.nparam)
+    for i, (cin, cout, taps, kp, wo, bo) in enumerate(net.layers()):
+        ew = rel_l2(g[wo:wo + cout * kp], gr[wo:wo + cout * kp])
+        eb = rel_l2(g[bo:bo + cout], gr[bo:bo + cout])
+        print(f"  layer {i} ({cin}->{cout}, {taps} taps): dW {ew:.2e} db {eb:.2e}")
+        assert ew < 3e-2 and eb < 3e-2, (i, ew, eb)
+    net.close()
+
+
+def test_training_reduces_the_loss_and_save_load(cdx, tmp_path):
+    net = cdx.UNet(64, 3, 16, max_batch=4, seed=3)
+    net.synth_batch(4, seed0=1000)
+    x, t = net.get_batch(4)
+    # forward-model pairs: masks of the drawn shapes, amplitudes in [0.2, 1]
+    m = t[..., 2] > 0
+    assert 0.02 < m.mean() < 0.9
+    assert t[..., 0][m].min() >= 0.2 - 1e-6 and t[..., 0][m].max() <= 1 + 1e-6
+    assert np.all(np.abs(t[..., 1]) <= np.pi + 1e-5) and np.all(t[..., 0][~m] == 0)
+    assert x.max() > 0.1
+    losses = []
+    for _ in range(60):
+        net.forward(4)
+        losses.append(net.backward(4))
+        net.adam(2e-3)
+    print(f"loss {losses[0]:.4f} -> {losses[-1]:.4f}", flush=True)
+    assert losses[-1] < 0.85 * losses[0]
+    p = str(tmp_path / "unet.bin")
+    net.save(p)
+    y0 = net.forward(4)
+    net2 = cdx.UNet(64, 3, 16, max_batch=4, seed=99)
+    net2.load(p)
+    net2.set_batch(x, t)
+    assert np.array_equal(net2.forward(4), y0)
+    assert np.array_equal(net2.params(), net.params())
+    net.close()
+    net2.close()
+
+
+def test_native_training_loop(cdx):
+    net = cdx.UNet(64, 2, 8, max_batch=2, seed=4)
+    losses = net.train(30, 2, seed0=5000, lr=2e-3)
+    assert np.all(np.isfinite(losses)) and losses[-5:].mean() < losses[:5].mean()
+    net.close()
+
+
+def test_shapes_and_warm_start(cdx):
+    n = 128
+    ctx = cdx.Context(n)
+    obj = ctx.cplx()
+    ctx.sim_object_shape(obj, 1, [30, 15, 0.5], seed=1)
+    m = np.abs(ctx.get_cplx(obj)) > 0
+    assert abs(m.sum() - np.pi * 30 * 15) < 0.05 * np.pi * 30 * 15
+    ang = [0.3, 2.2, 4.0]
+    rad = [30, 25, 35]
+    ctx.sim_object_shape(obj, 2, ang + rad, seed=1)
+    m = np.abs(ctx.get_cplx(obj)) > 0
+    xs, ys = [r * np.cos(a) for a, r in zip(ang, rad)], [r * np.sin(a) for a, r in zip(ang, rad)]
+    area = 0.5 * abs(sum(xs[i] * ys[(i + 1) % 3] - xs[(i + 1) % 3] * ys[i] for i in range(3)))
+    assert abs(m.sum() - area) < 0.06 * area and m[n // 2, n // 2]
+    with pytest.raises(cdx.CdxError):
+        ctx.sim_object_shape(obj, 2, [1.0, 0.5, 2.0, 10, 10, 10])  # angles not increasing
+    # warm start: F = FFT(g0) with g0 scaled to the measured intensity (Parseval)
+    net = cdx.UNet(n, 3, 8, max_batch=1, seed=2)
+    I = ctx.real()
+    net.synth_frame(77, obj, I)
+    F, g, spt = ctx.cplx(), ctx.cplx(), ctx.real()
+    net.warm_start(ctx, I, F, g, spt, threshold=0.3)
+    Fh, gh, sh, Ih = ctx.get_cplx(F), ctx.get_cplx(g), ctx.get_real(spt), ctx.get_real(I)
+    assert set(np.unique(sh)) <= {0.0, 1.0}
+    if sh.any():
+        assert np.allclose((np.abs(Fh) ** 2).sum(), Ih.sum(), rtol=1e-3)
+        assert rel_l2(gh, np.fft.ifft2(Fh) * n) < 1e-5
+        assert np.all(gh[sh == 0] == 0)
+    net.close()
+    ctx.close()

@@ -210,6 +210,61 @@ def fp64_report(cdx, seed=0):
     return out
 
 
+def unet_flops_per_sample(layers, size):
+    """multiply-adds x 2 of one training sample: forward, weight gradient and
+    input gradient (none for the first layer) of every convolution"""
+    fl = 0.0
+    h = {}
+    levels = (len(layers) - 1 + 2) // 4  # 2L + 2(L-1) + 1 layers
+    for i, (cin, cout, taps, kp, wo, bo) in enumerate(layers):
+        if i < 2 * levels:
+            lvl = i // 2
+        elif i < 2 * levels + 2 * (levels - 1):
+            lvl = (i - 2 * levels) // 2
+        else:
+            lvl = 0
+        px = (size >> lvl) ** 2
+        f = 2.0 * px * taps * cin * cout
+        fl += f * (2 if i == 0 else 3)
+    return fl
+
+
+def learned_leg(cdx):
+    """Learned warm start (U-Net, trained weights cdixx_amd/models/unet256.bin)
+    against random starts on 64 held-out frames per photon range, same
+    batched HIO/ER schedule; and the network's training throughput."""
+    from cdixx_amd.warmstart import DEFAULT_WEIGHTS, LEVELS, BASE, SIZE, evaluate
+    net = cdx.UNet(SIZE, LEVELS, BASE, max_batch=16, seed=1)
+    net.load(DEFAULT_WEIGHTS)
+    out = {}
+    keep = ("frames", "iterations", "target_R", "photons", "median_R_true_object", "median_support_iou",
+            "random_reached_target", "learned_reached_target", "random_median_iters_to_target",
+            "learned_median_iters_to_target", "random_median_final_R", "learned_median_final_R",
+            "learned_median_iters_to_random_final_R")
+    for ph in ((1e5, 1e8), (1e9, 1e10)):
+        _, summ = evaluate(cdx, net, 64, photons=ph, blocks=100, arms=("random", "learned"))
+        out[f"photons_{ph[0]:.0e}_{ph[1]:.0e}"] = {k: summ.get(k) for k in keep}
+    # training throughput: synthesis alone, then full steps (synthesis +
+    # forward + backward + Adam), batch 16 at 256^2
+    B, steps = 16, 20
+    net.train(2, B, seed0=3_000_000, lr=1e-6)
+    t0 = time.perf_counter()
+    for k in range(steps):
+        net.synth_batch(B, seed0=3_100_000 + k * B)
+    t_syn = (time.perf_counter() - t0) / steps
+    t0 = time.perf_counter()
+    net.train(steps, B, seed0=3_200_000, lr=1e-6)
+    t_step = (time.perf_counter() - t0) / steps
+    fl = unet_flops_per_sample(net.layers(), SIZE) * B
+    tf = fl / max(t_step - t_syn, 1e-9) / 1e12
+    out["training"] = {"batch": B, "step_ms": round(1e3 * t_step, 2), "synthesis_ms": round(1e3 * t_syn, 2),
+                       "samples_per_s": round(B / t_step, 1), "gflop_per_step": round(fl / 1e9, 1),
+                       "achieved_tflops_excl_synthesis": round(tf, 1), "peak_tflops_bf16_dense": 2500.0,
+                       "frac": round(tf / 2500.0, 4)}
+    net.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -220,6 +275,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-report", action="store_true", help="skip the FP32-vs-FP64 report per workload")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start leg")
     ap.add_argument("--batch-iters", type=int, default=500)
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="run a plain copy kernel of this many MiB first (counter calibration)")
@@ -304,6 +360,8 @@ def main():
         out["batched"] = batched_leg(cdx, args.batch_iters)
     if not args.no_report:
         out["fp64_report"] = fp64_report(cdx)
+    if not args.no_learned:
+        out["learned_warm_start"] = learned_leg(cdx)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(out), flush=True)

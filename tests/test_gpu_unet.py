@@ -125,3 +125,23 @@ def test_shapes_and_warm_start(cdx):
         assert np.all(gh[sh == 0] == 0)
     net.close()
     ctx.close()
+
+
+def test_trained_warm_start_beats_random_start(cdx):
+    """What the trained network gains: on held-out frames (seeds from
+    2,000,000) the same batched HIO/ER schedule from the learned start ends at
+    a lower R-factor than from random starts with the reference's rectangular
+    support, and more frames reach R <= 0.05 where the noise floor allows it."""
+    from cdixx_amd.warmstart import DEFAULT_WEIGHTS, LEVELS, BASE, SIZE, evaluate
+    assert os.path.exists(DEFAULT_WEIGHTS)
+    net = cdx.UNet(SIZE, LEVELS, BASE, max_batch=1, seed=1)
+    net.load(DEFAULT_WEIGHTS)
+    for ph in ((1e5, 1e8), (1e9, 1e10)):
+        _, s = evaluate(cdx, net, 32, photons=ph, blocks=100, arms=("random", "learned"))
+        print(f"photons {ph}: final R random {s['random_median_final_R']:.4f} learned "
+              f"{s['learned_median_final_R']:.4f}; reached 0.05: {s['random_reached_target']} vs "
+              f"{s['learned_reached_target']}; support IoU {s['median_support_iou']:.3f}", flush=True)
+        assert s["median_support_iou"] > 0.6
+        assert s["learned_median_final_R"] < s["random_median_final_R"]
+        assert s["learned_reached_target"] >= s["random_reached_target"]
+    net.close()

@@ -30,20 +30,23 @@ constexpr int kScal = 16;  // scalars per frame (same layout as a context's)
 __host__ __device__ constexpr int bsize_index(int n) {
   return n == 128 ? 0 : n == 256 ? 1 : n == 384 ? 2 : n == 512 ? 3 : -1;
 }
-template <int N>
-constexpr int blines() { return kBatchThreads / LineCfg<N>::TL; }
-template <int N>
+// threads per workgroup: 256 (FP32) / 128 (FP64, whose two exchange
+// buffers per line would otherwise exceed the LDS at 384)
+template <class T>
+constexpr int bthreads() { return sizeof(T) == 8 ? kBatchThreads / 2 : kBatchThreads; }
+template <class T, int N>
+constexpr int blines() { return bthreads<T>() / LineCfg<N>::TL; }
+template <class T, int N>
 constexpr int blsh() {
   int s = 0;
-  while ((1 << s) < blines<N>()) s++;
+  while ((1 << s) < blines<T, N>()) s++;
   return s;
 }
-static int blines_rt(int n) {
+static int blines_rt(int n, size_t csz) {
   switch (n) {
-    case 128: return blines<128>();
-    case 256: return blines<256>();
-    case 384: return blines<384>();
-    case 512: return blines<512>();
+#define F(n_) case n_: return csz == 8 ? blines<float, n_>() : blines<double, n_>();
+    F(128) F(256) F(384) F(512)
+#undef F
     default: return 0;
   }
 }
@@ -92,7 +95,7 @@ __global__ void __launch_bounds__(kBatchThreads) kb_col(BArgs<T> a, const int2* 
   switch (f.n) {
 #define CASE(n_)                                                                                       \
   case n_:                                                                                             \
-    col_body<T, n_, OP, MODE, RF>(in, out, n_, a.tw[bsize_index(n_)], I, pa, gs, m.y, blsh<n_>(), smem); \
+    col_body<T, n_, OP, MODE, RF>(in, out, n_, a.tw[bsize_index(n_)], I, pa, gs, m.y, blsh<T, n_>(), smem); \
     break;
     CASE(128) CASE(256) CASE(384) CASE(512)
 #undef CASE
@@ -119,7 +122,7 @@ __global__ void __launch_bounds__(kBatchThreads) kb_row(BArgs<T> a, const int2* 
 #define CASE(n_)                                                                                              \
   case n_:                                                                                                    \
     row_body<T, n_, OP, ALGO>(in, out, n_, a.tw[bsize_index(n_)], a.g + f.off, a.spt + f.off, ua, (T)1,        \
-                              a.spt + f.off, rs, m.y, blsh<n_>(), smem);                                      \
+                              a.spt + f.off, rs, m.y, blsh<T, n_>(), smem);                                      \
     break;
     CASE(128) CASE(256) CASE(384) CASE(512)
 #undef CASE
@@ -213,9 +216,10 @@ static int bfail(int code, const char* fmt, ...) {
     if (e_ != hipSuccess) return bfail(CDX_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_));   \
   } while (0)
 
-static size_t pitch_bytes(int n, size_t csz) {
+// LDS bytes of a batched workgroup of frame size n (both exchange buffers)
+static size_t wg_lds_bytes(int n, size_t csz) {
   switch (n) {
-#define F(n_) case n_: return (size_t)(csz == 8 ? lds_pitch<float, n_>() : lds_pitch<double, n_>()) * csz;
+#define F(n_) case n_: return csz == 8 ? lds_bytes<float, n_>(blsh<float, n_>()) : lds_bytes<double, n_>(blsh<double, n_>());
     F(128) F(256) F(384) F(512)
 #undef F
     default: return 0;
@@ -288,7 +292,7 @@ extern "C" int cdx_batch_create(cdx_batch** out, int nframes, const int* sizes, 
     if ((rc = cdx_create(&c, kBatchSizes[k], kBatchSizes[k], precision, b->st))) return bad(rc);
     b->ctx.push_back(c);
     b->ctx_n.push_back(kBatchSizes[k]);
-    b->lds_col = std::max(b->lds_col, (size_t)blines_rt(kBatchSizes[k]) * pitch_bytes(kBatchSizes[k], b->csz));
+    b->lds_col = std::max(b->lds_col, wg_lds_bytes(kBatchSizes[k], b->csz));
   }
   b->lds_row = b->lds_col;
   // workgroup maps
@@ -299,7 +303,7 @@ extern "C" int cdx_batch_create(cdx_batch** out, int nframes, const int* sizes, 
       hipMalloc((void**)&b->ring, (size_t)nframes * b->ring_cap * 2 * sizeof(double)))
     return bad(bfail(CDX_ERR_HIP, "cdx_batch_create: out of device memory"));
   for (int i = 0; i < nframes; i++) {
-    const int n = b->n[i], lines = blines_rt(n);
+    const int n = b->n[i], lines = blines_rt(n, b->csz);
     for (int t = 0; t < n / lines; t++) cols.push_back(make_int2(i, t));
     fr[i].n = n;
     fr[i].wg_row0 = (int)rows.size();
@@ -407,7 +411,7 @@ struct BLoop {
       hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
       if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(kern, dim3(nwg), dim3(kBatchThreads), lds, b->st, a, map);
+    hipLaunchKernelGGL(kern, dim3(nwg), dim3(bthreads<T>()), lds, b->st, a, map);
     return hipGetLastError();
   }
   int setup() {

@@ -36,12 +36,22 @@ CDX_LINE(2048, 128, 16, 16, 8)
 template <class T, int N>
 using LineOf = Line<T, N, LineCfg<N>::TL, typename LineCfg<N>::RL>;
 
-// LDS pitch per line: the exchange footprint rounded to == 8 (mod 32)
-// complex elements so neighbouring lines land on different banks.
+// LDS layout of a workgroup of 2^lsh lines: two exchange buffers (the
+// transforms alternate between them, one barrier per exchange); buffer b of
+// line c starts at (b * 2^lsh + c) * pitch.  The pitch is the exchange
+// footprint rounded to a 128-byte bank row (EPB complex elements) plus
+// EPB / 2^lsh, so the lines of a column tile (lanes (t, c) with c fastest)
+// start on different banks.
 template <class T, int N>
-constexpr int lds_pitch() {
-  int e = LineOf<T, N>::LDS_ELEMS;
-  return ((e + 23) / 32) * 32 + 8;
+__host__ __device__ constexpr int lds_pitch(int lsh) {
+  constexpr int EPB = 128 / (int)sizeof(cplx<T>);
+  constexpr int e = LineOf<T, N>::LDS_ELEMS;
+  constexpr int base = (e + EPB - 1) / EPB * EPB;
+  return base + (lsh < 31 ? (EPB >> lsh) & (EPB - 1) : 0);
+}
+template <class T, int N>
+__host__ __device__ constexpr size_t lds_bytes(int lsh) {
+  return (size_t)2 * ((size_t)1 << lsh) * lds_pitch<T, N>(lsh) * sizeof(cplx<T>);
 }
 
 // ---------------------------------------------------------------------------

@@ -218,10 +218,14 @@ struct Line {
   }
 
   // DIR = -1: Din -> Dout (forward); DIR = +1: Dout -> Din (inverse).
-  // lds: this line's region (LDS_ELEMS).  tw[m] = W_N^m (forward sign).
-  // Ends without a barrier; the caller synchronises before reusing lds.
+  // lds: this line's first exchange buffer (LDS_ELEMS), lds + alt its second;
+  // ph selects the buffer of the next exchange and flips after each one.  One
+  // barrier per exchange suffices: a buffer is written again two exchanges
+  // later, after every thread has passed the barrier of the exchange in
+  // between, which follows its reads.  tw[m] = W_N^m (forward sign).
   template <int DIR>
-  __device__ __forceinline__ static void run(cplx<T>* v, cplx<T>* lds, const cplx<T>* __restrict__ tw, int t) {
+  __device__ __forceinline__ static void run(cplx<T>* v, cplx<T>* lds0, int alt, int& ph,
+                                             const cplx<T>* __restrict__ tw, int t) {
     static_for<M>([&](auto pi) {
       constexpr int p = pi.value;
       constexpr int R = radix<DIR>(p);
@@ -237,6 +241,8 @@ struct Line {
         dft<R, DIR>(v + s.value * R);
       });
       if constexpr (p < M - 1) {
+        cplx<T>* lds = lds0 + ph * alt;
+        ph ^= 1;
         static_for<S>([&](auto s) {
           const int j = t + TL * s.value;
           const int base = (j / Ns) * Ns * R + (j % Ns);
@@ -248,15 +254,16 @@ struct Line {
           const int j = t + TL * s.value;
           static_for<R2>([&](auto r) { v[s.value * R2 + r.value] = lds[swz(j + r.value * (N / R2))]; });
         });
-        __syncthreads();
       }
     });
   }
-  __device__ __forceinline__ static void forward(cplx<T>* v, cplx<T>* lds, const cplx<T>* __restrict__ tw, int t) {
-    run<-1>(v, lds, tw, t);
+  __device__ __forceinline__ static void forward(cplx<T>* v, cplx<T>* lds, int alt, int& ph,
+                                                 const cplx<T>* __restrict__ tw, int t) {
+    run<-1>(v, lds, alt, ph, tw, t);
   }
-  __device__ __forceinline__ static void inverse(cplx<T>* v, cplx<T>* lds, const cplx<T>* __restrict__ tw, int t) {
-    run<+1>(v, lds, tw, t);
+  __device__ __forceinline__ static void inverse(cplx<T>* v, cplx<T>* lds, int alt, int& ph,
+                                                 const cplx<T>* __restrict__ tw, int t) {
+    run<+1>(v, lds, alt, ph, tw, t);
   }
 
   // element index along the line of register i in each layout

@@ -82,14 +82,16 @@ __device__ __forceinline__ void col_body(const cplx<T>* __restrict__ in, cplx<T>
                                          const cplx<T>* __restrict__ tw, const T* __restrict__ I,
                                          const ProjArgs<T>& pa, T gsig, int tile, int lsh, unsigned char* smem) {
   using L = LineOf<T, N>;
-  constexpr int PITCH = lds_pitch<T, N>();
+  const int pitch = lds_pitch<T, N>(lsh);
+  const int alt = pitch << lsh;
+  int ph = 0;
   constexpr int E = L::E;
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem);
   const int c = threadIdx.x & ((1 << lsh) - 1);
   const int t = threadIdx.x >> lsh;
   const int col = (tile << lsh) + c;
   const bool ok = col < cols;
-  cplx<T>* my = lds + c * PITCH;
+  cplx<T>* my = lds + c * pitch;
   constexpr bool from_f = (OP == COL_INV || OP == COL_PROJ_INV || OP == COL_GAUSS_INV);
   constexpr bool proj = (OP == COL_FWD_PROJ_INV || OP == COL_PROJ_INV);
   cplx<T> v[E];
@@ -106,7 +108,7 @@ __device__ __forceinline__ void col_body(const cplx<T>* __restrict__ in, cplx<T>
 #pragma unroll
     for (int i = 0; i < E; i++) iv[i] = ok ? I[(size_t)L::dout_pos(i, t) * cols + col] : (T)NAN;
   }
-  if constexpr (!from_f) L::forward(v, my, tw, t);
+  if constexpr (!from_f) L::forward(v, my, alt, ph, tw, t);
 
   if constexpr (OP == COL_FWD) {
     if (ok) {
@@ -149,7 +151,7 @@ __device__ __forceinline__ void col_body(const cplx<T>* __restrict__ in, cplx<T>
     }
   }
 
-  L::inverse(v, my, tw, t);
+  L::inverse(v, my, alt, ph, tw, t);
   if (ok) {
 #pragma unroll
     for (int i = 0; i < E; i++) out[(size_t)L::din_pos(i, t) * cols + col] = v[i];
@@ -173,14 +175,16 @@ __device__ __forceinline__ void row_body(const cplx<T>* __restrict__ in, cplx<T>
                                          unsigned char* smem) {
   using L = LineOf<T, N>;
   constexpr int TL = LineCfg<N>::TL;
-  constexpr int PITCH = lds_pitch<T, N>();
+  const int pitch = lds_pitch<T, N>(lsh);
+  const int alt = pitch << lsh;
+  int ph = 0;
   constexpr int E = L::E;
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem);
   const int t = threadIdx.x % TL;
   const int rr = threadIdx.x / TL;
   const int row = (tile << lsh) + rr;
   const bool ok = row < rows;
-  cplx<T>* my = lds + rr * PITCH;
+  cplx<T>* my = lds + rr * pitch;
   const size_t base = (size_t)row * N;
   constexpr bool prtf = OP == ROW_INV_UPDATE_FWD_PRTF;
   constexpr bool upd = OP == ROW_INV_UPDATE_FWD || prtf;
@@ -189,7 +193,7 @@ __device__ __forceinline__ void row_body(const cplx<T>* __restrict__ in, cplx<T>
   if constexpr (OP == ROW_FWD) {
 #pragma unroll
     for (int i = 0; i < E; i++) v[i] = ok ? in[base + L::din_pos(i, t)] : cplx<T>{(T)0, (T)0};
-    L::forward(v, my, tw, t);
+    L::forward(v, my, alt, ph, tw, t);
     if (ok) {
 #pragma unroll
       for (int i = 0; i < E; i++) out[base + L::dout_pos(i, t)] = v[i];
@@ -215,7 +219,7 @@ __device__ __forceinline__ void row_body(const cplx<T>* __restrict__ in, cplx<T>
         for (int i = 0; i < E; i++) g0[i] = ok ? gamma0[base + L::din_pos(i, t)] : cplx<T>{(T)0, (T)0};
       }
     }
-    L::inverse(v, my, tw, t);
+    L::inverse(v, my, alt, ph, tw, t);
 
     if constexpr (OP == ROW_INV) {
       if (ok) {
@@ -262,7 +266,7 @@ __device__ __forceinline__ void row_body(const cplx<T>* __restrict__ in, cplx<T>
         v[i] = support_update(gv[i], p, in, ALGO >= 0 ? ALGO : ua.algo, ua.beta, ua.relax);
         if (ok) g[base + L::din_pos(i, t)] = v[i];
       }
-      L::forward(v, my, tw, t);
+      L::forward(v, my, alt, ph, tw, t);
       if (ok) {
 #pragma unroll
         for (int i = 0; i < E; i++) out[base + L::dout_pos(i, t)] = v[i];

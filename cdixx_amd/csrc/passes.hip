@@ -48,9 +48,16 @@ static int line_lanes(int n) {
     default: return 0;
   }
 }
-static size_t line_pitch_bytes(int n, size_t csz) {
+static int log2i(int v) {
+  int s = 0;
+  while ((1 << s) < v) s++;
+  return s;
+}
+// LDS bytes of a workgroup of `lines` lines (both exchange buffers)
+static size_t wg_lds_bytes(int n, size_t csz, int lines) {
+  const int lsh = log2i(lines);
   switch (n) {
-#define F(n_) case n_: return (size_t)(csz == 8 ? lds_pitch<float, n_>() : lds_pitch<double, n_>()) * csz;
+#define F(n_) case n_: return csz == 8 ? lds_bytes<float, n_>(lsh) : lds_bytes<double, n_>(lsh);
     CDX_SIZES(F)
 #undef F
     default: return 0;
@@ -65,7 +72,7 @@ int tiling_valid(int n, size_t csz, int lines) {
   if (tl == 0 || lines <= 0 || (lines & (lines - 1))) return 0;
   const int threads = lines * tl;
   if (threads < 64 || threads > kMaxThreads) return 0;
-  return (size_t)lines * line_pitch_bytes(n, csz) <= kMaxLds;
+  return wg_lds_bytes(n, csz, lines) <= kMaxLds;
 }
 int tiling_default(int n, size_t csz, int column_pass) {
   const int tl = line_lanes(n);
@@ -112,12 +119,6 @@ static hipError_t lds_limit(K kern, size_t bytes, size_t& done) {
   }
   return hipSuccess;
 }
-static int log2i(int v) {
-  int s = 0;
-  while ((1 << s) < v) s++;
-  return s;
-}
-
 #if CDX_PASS_ROW == 0
 template <class T, int N, int OP, int MODE, int RF>
 static hipError_t col_launch_n(int cols, const void* in, void* out, const void* tw, const T* I,
@@ -125,7 +126,7 @@ static hipError_t col_launch_n(int cols, const void* in, void* out, const void* 
   static size_t lds_done = 0;
   constexpr int TL = LineCfg<N>::TL;
   if (!tiling_valid(N, sizeof(cplx<T>), lines)) return hipErrorInvalidConfiguration;
-  const size_t lds = (size_t)lines * lds_pitch<T, N>() * sizeof(cplx<T>);
+  const size_t lds = lds_bytes<T, N>(log2i(lines));
   auto kern = k_col<T, N, OP, MODE, RF>;
   hipError_t e = lds_limit(kern, lds, lds_done);
   if (e != hipSuccess) return e;
@@ -187,7 +188,7 @@ static hipError_t row_launch_n(int rows, const void* in, void* out, const void* 
   static size_t lds_done = 0;
   constexpr int TL = LineCfg<N>::TL;
   if (!tiling_valid(N, sizeof(cplx<T>), lines)) return hipErrorInvalidConfiguration;
-  const size_t lds = (size_t)lines * lds_pitch<T, N>() * sizeof(cplx<T>);
+  const size_t lds = lds_bytes<T, N>(log2i(lines));
   auto kern = k_row<T, N, OP, ALGO>;
   hipError_t e = lds_limit(kern, lds, lds_done);
   if (e != hipSuccess) return e;

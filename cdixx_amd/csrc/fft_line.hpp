@@ -188,33 +188,74 @@ struct Line {
     return s;
   }
 
-  // v[r] *= W_N^(DIR*r*b) for r = 1..R-1, from table loads of W_N^(2^k b)
+  // Table twiddles of a transform.  Pass p (radix R, Ns > 1) multiplies
+  // butterfly j = t + TL*s by W_N^(r * b), b = (j mod Ns) * N/(Ns*R): for a
+  // power-of-two R the lane loads W^(2^k b) and forms the other powers by
+  // products, otherwise it loads W^(r b), r = 1..R-1.  All of a transform's
+  // loads are issued together (tw_load) so their latency overlaps the data
+  // loads instead of stalling every exchange.
+  static constexpr bool pow2(int r) { return (r & (r - 1)) == 0; }
+  static constexpr int log2c(int r) {
+    int k = 0;
+    while ((1 << k) < r) k++;
+    return k;
+  }
+  template <int DIR>
+  static constexpr int tw_per(int p) {
+    return ns<DIR>(p) == 1 ? 0 : (pow2(radix<DIR>(p)) ? log2c(radix<DIR>(p)) : radix<DIR>(p) - 1);
+  }
+  template <int DIR>
+  static constexpr int tw_off(int p) {  // first slot of pass p
+    int o = 0;
+    for (int q = 0; q < p; q++) o += (E / radix<DIR>(q)) * tw_per<DIR>(q);
+    return o;
+  }
+  template <int DIR>
+  struct Tw {
+    static constexpr int K = tw_off<DIR>(M);
+    cplx<T> w[K > 0 ? K : 1];
+  };
+  template <int DIR>
+  __device__ __forceinline__ static Tw<DIR> tw_load(const cplx<T>* __restrict__ tw, int t) {
+    Tw<DIR> out;
+    static_for<M>([&](auto pi) {
+      constexpr int p = pi.value;
+      constexpr int R = radix<DIR>(p);
+      constexpr int Ns = ns<DIR>(p);
+      if constexpr (Ns > 1) {
+        constexpr int stride = N / (Ns * R);
+        constexpr int per = tw_per<DIR>(p);
+        static_for<E / R>([&](auto s) {
+          const int b = ((t + TL * s.value) % Ns) * stride;
+          static_for<per>([&](auto k) {
+            constexpr int rr = pow2(R) ? (1 << k.value) : k.value + 1;
+            out.w[tw_off<DIR>(p) + s.value * per + k.value] = tw[rr * b];
+          });
+        });
+      }
+    });
+    return out;
+  }
+  // v[r] *= W^(DIR r b) for r = 1..R-1 from the lane's preloaded values w
   template <int R, int DIR>
-  __device__ __forceinline__ static void twiddle(cplx<T>* v, const cplx<T>* __restrict__ tw, int b) {
-    if constexpr ((R & (R - 1)) == 0) {
-      cplx<T> w[R];
+  __device__ __forceinline__ static void twiddle(cplx<T>* v, const cplx<T>* wl) {
+    cplx<T> w[R];
+    if constexpr (pow2(R)) {
+      static_for<log2c(R)>([&](auto k) { w[1 << k.value] = wl[k.value]; });
       static_for<R>([&](auto r) {
         constexpr int rr = r.value;
-        if constexpr (rr > 0 && (rr & (rr - 1)) == 0) w[rr] = tw[rr * b];
-      });
-      static_for<R>([&](auto r) {
-        constexpr int rr = r.value;
-        if constexpr (rr > 0 && (rr & (rr - 1)) != 0) {
+        if constexpr (rr > 0 && !pow2(rr)) {
           constexpr int hi = 1 << (31 - __builtin_clz(rr));
           w[rr] = cmul(w[hi], w[rr - hi]);
         }
       });
-      static_for<R - 1>([&](auto r) {
-        constexpr int rr = r.value + 1;
-        v[rr] = DIR < 0 ? cmul(v[rr], w[rr]) : cmulc(v[rr], w[rr]);
-      });
     } else {
-      static_for<R - 1>([&](auto r) {
-        constexpr int rr = r.value + 1;
-        const cplx<T> w = tw[rr * b];
-        v[rr] = DIR < 0 ? cmul(v[rr], w) : cmulc(v[rr], w);
-      });
+      static_for<R - 1>([&](auto r) { w[r.value + 1] = wl[r.value]; });
     }
+    static_for<R - 1>([&](auto r) {
+      constexpr int rr = r.value + 1;
+      v[rr] = DIR < 0 ? cmul(v[rr], w[rr]) : cmulc(v[rr], w[rr]);
+    });
   }
 
   // DIR = -1: Din -> Dout (forward); DIR = +1: Dout -> Din (inverse).
@@ -222,10 +263,11 @@ struct Line {
   // ph selects the buffer of the next exchange and flips after each one.  One
   // barrier per exchange suffices: a buffer is written again two exchanges
   // later, after every thread has passed the barrier of the exchange in
-  // between, which follows its reads.  tw[m] = W_N^m (forward sign).
+  // between, which follows its reads.  tw: the lane's table twiddles
+  // (tw_load of the table tw[m] = W_N^m, forward sign).
   template <int DIR>
-  __device__ __forceinline__ static void run(cplx<T>* v, cplx<T>* lds0, int alt, int& ph,
-                                             const cplx<T>* __restrict__ tw, int t) {
+  __device__ __forceinline__ static void run(cplx<T>* v, cplx<T>* lds0, int alt, int& ph, const Tw<DIR>& tw,
+                                             int t) {
     static_for<M>([&](auto pi) {
       constexpr int p = pi.value;
       constexpr int R = radix<DIR>(p);
@@ -233,11 +275,7 @@ struct Line {
       constexpr int S = E / R;
       static_assert(E % R == 0, "every radix must divide the points per lane");
       static_for<S>([&](auto s) {
-        const int j = t + TL * s.value;
-        if constexpr (Ns > 1) {
-          constexpr int stride = N / (Ns * R);
-          twiddle<R, DIR>(v + s.value * R, tw, (j % Ns) * stride);
-        }
+        if constexpr (Ns > 1) twiddle<R, DIR>(v + s.value * R, tw.w + tw_off<DIR>(p) + s.value * tw_per<DIR>(p));
         dft<R, DIR>(v + s.value * R);
       });
       if constexpr (p < M - 1) {
@@ -257,12 +295,12 @@ struct Line {
       }
     });
   }
-  __device__ __forceinline__ static void forward(cplx<T>* v, cplx<T>* lds, int alt, int& ph,
-                                                 const cplx<T>* __restrict__ tw, int t) {
+  using TwF = Tw<-1>;
+  using TwI = Tw<+1>;
+  __device__ __forceinline__ static void forward(cplx<T>* v, cplx<T>* lds, int alt, int& ph, const TwF& tw, int t) {
     run<-1>(v, lds, alt, ph, tw, t);
   }
-  __device__ __forceinline__ static void inverse(cplx<T>* v, cplx<T>* lds, int alt, int& ph,
-                                                 const cplx<T>* __restrict__ tw, int t) {
+  __device__ __forceinline__ static void inverse(cplx<T>* v, cplx<T>* lds, int alt, int& ph, const TwI& tw, int t) {
     run<+1>(v, lds, alt, ph, tw, t);
   }
 

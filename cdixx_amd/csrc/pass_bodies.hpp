@@ -64,6 +64,18 @@ __device__ __forceinline__ cplx<T> project_m(cplx<T> f, T m, const ProjArgs<T>& 
   }
 }
 
+// Streaming store of one complex value that bypasses the L2 (the row pass
+// writes whole rows, so the lines are complete and nothing is left dirty
+// for the end-of-kernel write-back)
+template <class T>
+__device__ __forceinline__ void store_stream(cplx<T>* p, cplx<T> v) {
+  typedef T vec2 __attribute__((ext_vector_type(2)));
+  vec2 w;
+  w.x = v.x;
+  w.y = v.y;
+  __builtin_nontemporal_store(w, reinterpret_cast<vec2*>(p));
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
 // L2.  Column tiles are narrower than a cache line (8 complex64 = 64 B), so
 // the tiles sharing a line must run on the same XCD: give each XCD a
@@ -108,7 +120,12 @@ __device__ __forceinline__ void col_body(const cplx<T>* __restrict__ in, cplx<T>
 #pragma unroll
     for (int i = 0; i < E; i++) iv[i] = ok ? I[(size_t)L::dout_pos(i, t) * cols + col] : (T)NAN;
   }
-  if constexpr (!from_f) L::forward(v, my, alt, ph, tw, t);
+  // table twiddles of both transforms, loaded behind the data
+  typename L::TwF twf;
+  typename L::TwI twi;
+  if constexpr (!from_f) twf = L::template tw_load<-1>(tw, t);
+  if constexpr (OP != COL_FWD) twi = L::template tw_load<+1>(tw, t);
+  if constexpr (!from_f) L::forward(v, my, alt, ph, twf, t);
 
   if constexpr (OP == COL_FWD) {
     if (ok) {
@@ -151,7 +168,7 @@ __device__ __forceinline__ void col_body(const cplx<T>* __restrict__ in, cplx<T>
     }
   }
 
-  L::inverse(v, my, alt, ph, tw, t);
+  L::inverse(v, my, alt, ph, twi, t);
   if (ok) {
 #pragma unroll
     for (int i = 0; i < E; i++) out[(size_t)L::din_pos(i, t) * cols + col] = v[i];
@@ -193,7 +210,8 @@ __device__ __forceinline__ void row_body(const cplx<T>* __restrict__ in, cplx<T>
   if constexpr (OP == ROW_FWD) {
 #pragma unroll
     for (int i = 0; i < E; i++) v[i] = ok ? in[base + L::din_pos(i, t)] : cplx<T>{(T)0, (T)0};
-    L::forward(v, my, alt, ph, tw, t);
+    const typename L::TwF twf = L::template tw_load<-1>(tw, t);
+    L::forward(v, my, alt, ph, twf, t);
     if (ok) {
 #pragma unroll
       for (int i = 0; i < E; i++) out[base + L::dout_pos(i, t)] = v[i];
@@ -219,7 +237,8 @@ __device__ __forceinline__ void row_body(const cplx<T>* __restrict__ in, cplx<T>
         for (int i = 0; i < E; i++) g0[i] = ok ? gamma0[base + L::din_pos(i, t)] : cplx<T>{(T)0, (T)0};
       }
     }
-    L::inverse(v, my, alt, ph, tw, t);
+    const typename L::TwI twi = L::template tw_load<+1>(tw, t);
+    L::inverse(v, my, alt, ph, twi, t);
 
     if constexpr (OP == ROW_INV) {
       if (ok) {
@@ -246,6 +265,7 @@ __device__ __forceinline__ void row_body(const cplx<T>* __restrict__ in, cplx<T>
       return;
     }
     if constexpr (upd) {
+      const typename L::TwF twf = L::template tw_load<-1>(tw, t);
       const T thr = (T)ua.scal[0];
       double are = 0, aim = 0;  // PRTF alignment sums (cdi.cc:358-360,373-375)
 #pragma unroll
@@ -264,12 +284,12 @@ __device__ __forceinline__ void row_body(const cplx<T>* __restrict__ in, cplx<T>
           }
         }
         v[i] = support_update(gv[i], p, in, ALGO >= 0 ? ALGO : ua.algo, ua.beta, ua.relax);
-        if (ok) g[base + L::din_pos(i, t)] = v[i];
+        if (ok) store_stream(&g[base + L::din_pos(i, t)], v[i]);
       }
-      L::forward(v, my, alt, ph, tw, t);
+      L::forward(v, my, alt, ph, twf, t);
       if (ok) {
 #pragma unroll
-        for (int i = 0; i < E; i++) out[base + L::dout_pos(i, t)] = v[i];
+        for (int i = 0; i < E; i++) store_stream(&out[base + L::dout_pos(i, t)], v[i]);
       }
       if constexpr (prtf) {
         __shared__ double sh[16];

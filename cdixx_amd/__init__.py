@@ -110,6 +110,7 @@ _SIGS = {
     "cdx_bin_read": ([ctypes.c_char_p, _vp, _sz], _i),
     "cdx_bin_write": ([ctypes.c_char_p, _i, _i, _i, _vp], _i),
     "cdx_sim_object_shape": ([_vp, _i, _dp, _i, _u64, _d, _vp], _i),
+    "cdx_preprocess_pattern": ([_vp, _i, _i, _vp, _i, _i, _i, _i, _d, _d, _i, _vp, _i, _ip, _dp], _i),
     "cdx_context_info": ([_vp, _ip, _ip, _ip], _i),
     "cdx_unet_create": ([ctypes.POINTER(_vp), _i, _i, _i, _i, _u64], _i),
     "cdx_unet_destroy": ([_vp], _i), "cdx_unet_num_params": ([_vp, ctypes.POINTER(_i64)], _i),
@@ -150,6 +151,25 @@ def expand_schedule(algorithm):
     out = (ctypes.c_int * max(n, 1))()
     lib.cdx_expand_schedule(algorithm.encode(), out, n)
     return list(out[:n])
+
+
+def preprocess_pattern(sig, bkg=None, nmerge=1, centre=False, shift_row=0.0, shift_col=0.0, shift_mode=0):
+    """util/processimg.cpp: background subtraction, optional centring with a
+    sub-pixel FFT shift (mode 0) or spectrum-modulus re-centring (mode 1),
+    cropping to a multiple of 4*nmerge and nmerge x nmerge binning.
+    Returns (image, (midx, midy))."""
+    sig = np.ascontiguousarray(sig, np.float32)
+    b = None if bkg is None else np.ascontiguousarray(bkg, np.float32)
+    cap = sig.size
+    out = np.zeros(cap, np.float32)
+    n = ctypes.c_int()
+    mid = (ctypes.c_double * 2)()
+    check(lib.cdx_preprocess_pattern(sig.ctypes.data, sig.shape[0], sig.shape[1],
+                                     None if b is None else b.ctypes.data, 0 if b is None else b.shape[0],
+                                     0 if b is None else b.shape[1], nmerge, int(centre), shift_row, shift_col,
+                                     shift_mode, out.ctypes.data, cap, ctypes.byref(n), mid))
+    k = n.value
+    return out[:k * k].reshape(k, k).copy(), (mid[0], mid[1])
 
 
 def read_config(path):

@@ -35,6 +35,7 @@ UNITS = [
     ("abi_extra.hip", [], "abi_extra.o"),
     ("batch.hip", [], "batch.o"),
     ("unet.hip", [], "unet.o"),
+    ("preprocess.hip", [], "preprocess.o"),
     ("config.cpp", [], "config.o"),
     ("cdi.cpp", [], "cdi.o"),
     ("imgio.cpp", [], "imgio.o"),

@@ -245,6 +245,22 @@ int cdx_sim_object_shape(cdx_ctx* ctx, int kind, const double* params, int np, u
 /* I = |FFT(obj)|^2 scaled to `photons` in total, Poisson sampled if `poisson` */
 int cdx_sim_intensity(cdx_ctx* ctx, const void* obj, double photons, uint64_t seed, int poisson, void* I);
 
+/* ---- pattern preprocessing: util/processimg.cpp:11-109 (host arrays) ----
+ * sig (rows x cols) minus bkg (optional; the larger image is first cropped
+ * centrally to the smaller, comparing row counts as the reference does);
+ * the centroid of the result thresholded at half its maximum (findMiddle) is
+ * used when `centre` is set (then midx += shift_row, midy -= shift_col:
+ * argv[7], argv[6] of the reference), else the centre is (0, 0); the image is
+ * cropped around it to the largest multiple of 4*nmerge that fits, shifted by
+ * the sub-pixel remainder through the FFT (shift_mode 0, shiftWave) or
+ * re-centred by the modulus of its spectrum (shift_mode 1, shiftMiddle), and
+ * binned nmerge x nmerge.  out receives out_size^2 values (cap), mid the
+ * centre (midx, midy) in pixels.  The cropped size must be an FFT length of
+ * this build (cdx_size_supported). */
+int cdx_preprocess_pattern(const float* sig, int rows, int cols, const float* bkg, int brows, int bcols,
+                           int nmerge, int centre, double shift_row, double shift_col, int shift_mode, float* out,
+                           int cap, int* out_size, double* mid);
+
 /* ---- learned warm start (no counterpart in the reference, whose loop
  * starts from random signs, src/post_gpu/cdi.cc:222-223): a U-Net on the
  * centred log-intensity log(1 + I)/16 predicting the object amplitude, its

@@ -381,3 +381,33 @@ def test_prtf_counts_and_accumulates_partially(o64):
     # prtfIter beyond the schedule: nothing accumulated
     none = o64.phase_retrieve("5*ER", F, g, I, spt, 0.5, prtf_iter=10, relax=1.0)
     assert none["prtf_count"] == 0 and not none["prtf"].any() and not none["weights"].any()
+
+
+# ---------------------------------------------------------------------------
+# pattern preprocessing (util/processimg.cpp)
+# ---------------------------------------------------------------------------
+def test_preprocess_pieces_closed_forms():
+    from oracle import preprocess_ref as pp
+    # a band-limited periodic image shifted by a fraction of a pixel is the
+    # same sum of plane waves evaluated at shifted coordinates
+    n = 64
+    x, y = np.mgrid[0:n, 0:n]
+    f = lambda xx, yy: np.cos(2 * np.pi * (3 * xx + 5 * yy) / n) + 0.5 * np.sin(2 * np.pi * (7 * xx - 2 * yy) / n)
+    out = pp.shift_wave(f(x, y).astype(complex), 0.3, -0.45)
+    assert np.allclose(out.real, f(x - 0.3, y + 0.45), atol=1e-12) and np.allclose(out.imag, 0, atol=1e-12)
+    # integer shifts are circular shifts
+    z = np.random.default_rng(0).standard_normal((n, n))
+    assert np.allclose(pp.shift_wave(z, 2, -3).real, np.roll(z, (2, -3), axis=(0, 1)), atol=1e-12)
+    # binning of a ramp: the block means
+    a = np.arange(36.0).reshape(6, 6)
+    assert np.allclose(pp.merge_pixels(a, 2), [[3.5, 5.5, 7.5], [15.5, 17.5, 19.5], [27.5, 29.5, 31.5]])
+    # the centroid of a symmetric blob at (x0, y0) sits at x0 + 0.5 - R/2
+    R, C = 80, 96
+    X, Y = np.mgrid[0:R, 0:C]
+    blob = np.exp(-((X - 30) ** 2 + (Y - 61) ** 2) / 20.0)
+    mx, my = pp.centroid(blob)
+    assert abs(mx - (30 + 0.5 - R / 2)) < 1e-4 and abs(my - (61 + 0.5 - C / 2)) < 1e-4
+    # shiftMiddle re-centres: a displaced symmetric blob comes back centred
+    img = np.exp(-((x - 40) ** 2 + (y - 25) ** 2) / 8.0)
+    c = pp.shift_middle(img.astype(complex)).real
+    assert np.unravel_index(np.argmax(c), c.shape) == (n // 2, n // 2)

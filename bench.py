@@ -312,6 +312,10 @@ def main():
     wall = time.perf_counter() - t0
     ms_per_step = ev_ms / args.steps
     iters_s = 1000.0 / ms_per_step
+    # the same loop recording the R-factor every iteration
+    prf = cdx.Params.default(beta=0.85, useBS=1, freeBeamstop=1, recordR=1)
+    rf_ms, rf_col, _ = ctx.time_iterations(prf, cdx.RAAR, d["F"], d["g"], d["I"], d["spt"], args.steps,
+                                           bs=d["bs"], use_graph=not args.no_graph, per_kernel=True)
 
     # end-to-end: 1000 RAAR iterations with shrink-wrap every 20 (reference form)
     d2 = make_inputs(cdx, ctx, seed=1)
@@ -352,6 +356,7 @@ def main():
                      "kernel": f"{dom} pass", "avg_ms": round(t_dom, 5),
                      "col_pass_ms": round(col_ms, 5), "row_pass_ms": round(row_ms, 5)},
         "wall_s_timed": round(wall, 4),
+        "with_rfactor": {"iters_per_s": round(1000.0 * args.steps / rf_ms, 1), "col_pass_ms": round(rf_col, 5)},
         "end_to_end": {"schedule": "50*(20*RAAR+1*shrinkWrap)", "iterations": nit, "seconds": round(e2e, 4),
                        "shrink_threshold": E2E_PARAMS["shrinkThreshold"], "R_iterate_last": float(rt[-1]),
                        "R_support_projected": round(r_final, 5)},

@@ -181,7 +181,7 @@ extern "C" int cdx_create(cdx_ctx** out, int rows, int cols, int precision, void
       hipMalloc(&c->Ipk, c->n * c->rsz) != hipSuccess || hipMalloc(&c->tmpr, c->n * c->rsz) != hipSuccess ||
       hipMalloc((void**)&c->scal, S_COUNT * sizeof(double)) != hipSuccess ||
       hipMalloc((void**)&c->partials, kMaxPartials * sizeof(double)) != hipSuccess ||
-      hipMalloc((void**)&c->ring, 2 * (size_t)c->ring_cap * sizeof(double)) != hipSuccess) {
+      hipMalloc((void**)&c->ring, 2 * kRfSlots * (size_t)c->ring_cap * sizeof(double)) != hipSuccess) {
     cdx_destroy(c);
     return fail(CDX_ERR_HIP, "cdx_create: out of device memory for %dx%d", rows, cols);
   }
@@ -530,6 +530,7 @@ struct Loop {
     pa.scale = (T)(1.0 / p->exposure);
     pa.rcolor = (T)p->rcolor;
     pa.rf = nullptr;
+    pa.rfs = 2 * c->ring_cap;
     return CDX_OK;
   }
   int iterate(int algo, double* rf) {
@@ -617,7 +618,7 @@ static int phase_retrieve_t(cdx_ctx* c, const char* algorithm, const cdx_params*
   const T rootN = (T)sqrt((double)c->n);
   HIPCHK(launch_norm_c<T>(c->n, g, (T)1 / rootN, c->st));  // cdi.cc:313
   const int nrec = p->recordR ? (rcap < c->ring_cap ? rcap : c->ring_cap) : 0;
-  if (nrec > 0) HIPCHK(launch_fill_f64(2 * (int64_t)nrec, c->ring, 0.0, c->st));
+  if (nrec > 0) HIPCHK(launch_fill_f64(2 * kRfSlots * (int64_t)c->ring_cap, c->ring, 0.0, c->st));
 
   // PRTF buffers (before any capture); the map starts at zero
   const bool want_prtf = p->prtfIter >= 0;
@@ -640,7 +641,7 @@ static int phase_retrieve_t(cdx_ctx* c, const char* algorithm, const cdx_params*
       if ((rc = L.shrink_wrap(sigma))) break;
       if (sigma > p->sigmaMin) sigma *= p->sigmaDecay;
     } else {
-      double* rf = nproj < nrec ? c->ring + 2 * nproj : nullptr;
+      double* rf = nproj < nrec ? c->ring + 2 * (size_t)nproj : nullptr;
       if (want_prtf && iter >= p->prtfIter) {
         if ((rc = L.iterate_prtf(tok, rf, !prtf_started))) break;
         prtf_started = true;
@@ -686,11 +687,20 @@ static int phase_retrieve_t(cdx_ctx* c, const char* algorithm, const cdx_params*
     HIPCHK(hipStreamSynchronize(c->st));
   }
   if (nrec > 0 && rtrace) {
-    std::vector<double> h(2 * (size_t)nrec);
-    HIPCHK(hipMemcpyAsync(h.data(), c->ring, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    // the first nrec entries of every slot row
+    std::vector<double> h(2 * kRfSlots * (size_t)nrec);
+    HIPCHK(hipMemcpy2DAsync(h.data(), 2 * nrec * sizeof(double), c->ring, 2 * c->ring_cap * sizeof(double),
+                            2 * nrec * sizeof(double), kRfSlots, hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     int m = nproj < nrec ? nproj : nrec;
-    for (int i = 0; i < m; i++) rtrace[i] = h[2 * i + 1] > 0 ? h[2 * i] / h[2 * i + 1] : 0;
+    for (int i = 0; i < m; i++) {
+      double num = 0, den = 0;
+      for (int k = 0; k < kRfSlots; k++) {
+        num += h[2 * ((size_t)k * nrec + i)];
+        den += h[2 * ((size_t)k * nrec + i) + 1];
+      }
+      rtrace[i] = den > 0 ? num / den : 0;
+    }
   }
   return CDX_OK;
 }
@@ -723,8 +733,10 @@ static int time_t_(cdx_ctx* c, const cdx_params* p, int algo, void* F, void* g, 
   Loop<T> L{c, p, {}, true, F, g, (T*)spt};
   int rc = L.setup(I, bs);
   if (rc) return rc;
+  // R-factor sums when p->recordR (accumulated into the first ring entry)
+  double* rf = p->recordR ? c->ring : nullptr;
   // first iteration leaves the F-pending state; it is not timed
-  if ((rc = L.iterate(algo, nullptr))) return rc;
+  if ((rc = L.iterate(algo, rf))) return rc;
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
@@ -732,7 +744,7 @@ static int time_t_(cdx_ctx* c, const cdx_params* p, int algo, void* F, void* g, 
   hipGraphExec_t exec = nullptr;
   if (use_graph) {
     HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-    for (int i = 0; i < n && !rc; i++) rc = L.iterate(algo, nullptr);
+    for (int i = 0; i < n && !rc; i++) rc = L.iterate(algo, rf);
     hipError_t e = hipStreamEndCapture(c->st, &graph);
     if (rc) return rc;
     HIPCHK(e);
@@ -744,7 +756,7 @@ static int time_t_(cdx_ctx* c, const cdx_params* p, int algo, void* F, void* g, 
   if (use_graph) {
     HIPCHK(hipGraphLaunch(exec, c->st));
   } else {
-    for (int i = 0; i < n && !rc; i++) rc = L.iterate(algo, nullptr);
+    for (int i = 0; i < n && !rc; i++) rc = L.iterate(algo, rf);
     if (rc) return rc;
   }
   HIPCHK(hipEventRecord(e1, c->st));
@@ -762,6 +774,7 @@ static int time_t_(cdx_ctx* c, const cdx_params* p, int algo, void* F, void* g, 
     std::vector<hipEvent_t> ev(4 * (size_t)n);
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
     UpdArgs<T> ua = upd_args<T>(c, algo, p->beta, p->relax);
+    L.pa.rf = rf;
     for (int i = 0; i < n; i++) {
       const KTimer kc{ev[4 * i], ev[4 * i + 1]}, kr{ev[4 * i + 2], ev[4 * i + 3]};
       HIPCHK(launch_col<T>(COL_FWD_PROJ_INV, c->rows, c->cols, c->T1, c->T2, c->tw_col, (const T*)c->Ipk, L.pa, 0,

@@ -87,7 +87,7 @@ __global__ void __launch_bounds__(kBatchThreads) kb_col(BArgs<T> a, const int2* 
   if (m.x < 0) return;  // padding entry (whole workgroup)
   const BFrame f = a.frames[m.x];
   ProjArgs<T> pa = a.pa;
-  pa.rf = (RF && a.rslot >= 0) ? f.rf + 2 * a.rslot : nullptr;
+  pa.rf = (RF && a.rslot >= 0) ? f.rf + 2 * (size_t)a.rslot : nullptr;
   const T gs = (T)(f.n >> 2) / (T)a.sigma;
   const cplx<T>* in = a.src + f.off;
   cplx<T>* out = a.dst + f.off;
@@ -300,7 +300,7 @@ extern "C" int cdx_batch_create(cdx_batch** out, int nframes, const int* sizes, 
   std::vector<BFrame> fr(nframes);
   b->ring_cap = 4096;
   if (hipMalloc((void**)&b->scal, (size_t)nframes * kScal * sizeof(double)) ||
-      hipMalloc((void**)&b->ring, (size_t)nframes * b->ring_cap * 2 * sizeof(double)))
+      hipMalloc((void**)&b->ring, (size_t)nframes * b->ring_cap * 2 * kRfSlots * sizeof(double)))
     return bad(bfail(CDX_ERR_HIP, "cdx_batch_create: out of device memory"));
   for (int i = 0; i < nframes; i++) {
     const int n = b->n[i], lines = blines_rt(n, b->csz);
@@ -312,7 +312,7 @@ extern "C" int cdx_batch_create(cdx_batch** out, int nframes, const int* sizes, 
     fr[i].pad = 0;
     fr[i].off = b->off[i];
     fr[i].scal = b->scal + (size_t)i * kScal;
-    fr[i].rf = b->ring + (size_t)i * b->ring_cap * 2;
+    fr[i].rf = b->ring + (size_t)i * b->ring_cap * 2 * kRfSlots;
   }
   // column tiles sharing cache lines go to the same XCD: workgroup id b runs
   // tile xcd(b) of the (padded) tile list
@@ -423,6 +423,7 @@ struct BLoop {
     pa.noiseLevel = p->noiseLevel;
     pa.scale = (T)(1.0 / p->exposure);
     pa.rcolor = (T)p->rcolor;
+    pa.rfs = 2 * b->ring_cap;
     return CDX_OK;
   }
   int iterate(int algo, int rslot) {
@@ -508,7 +509,7 @@ int batch_retrieve_t(cdx_batch* b, const char* algorithm, const cdx_params* p, d
   int rc = L.setup();
   if (rc) return rc;
   const int nrec = p->recordR ? std::min(rcap, b->ring_cap) : 0;
-  if (nrec > 0) BCHK(launch_fill_f64((int64_t)b->nf * b->ring_cap * 2, b->ring, 0.0, b->st));
+  if (nrec > 0) BCHK(launch_fill_f64((int64_t)b->nf * b->ring_cap * 2 * kRfSlots, b->ring, 0.0, b->st));
   if ((rc = L.scale_g(0))) return rc;  // cdi.cc:313
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -541,13 +542,19 @@ int batch_retrieve_t(cdx_batch* b, const char* algorithm, const cdx_params* p, d
   BCHK(hipStreamSynchronize(b->st));
   if (n_iter) *n_iter = iter;
   if (rtrace && nrec > 0) {
-    std::vector<double> h((size_t)b->nf * b->ring_cap * 2);
+    std::vector<double> h((size_t)b->nf * b->ring_cap * 2 * kRfSlots);
     BCHK(hipMemcpy(h.data(), b->ring, h.size() * sizeof(double), hipMemcpyDeviceToHost));
     const int m = std::min(nproj, nrec);
     for (int f = 0; f < b->nf; f++)
       for (int i = 0; i < m; i++) {
-        const double* s = &h[((size_t)f * b->ring_cap + i) * 2];
-        rtrace[(size_t)f * rcap + i] = s[1] > 0 ? s[0] / s[1] : 0;
+        // frame f's ring: kRfSlots rows of ring_cap (num, den) pairs
+        const double* s = &h[(size_t)f * b->ring_cap * 2 * kRfSlots + 2 * (size_t)i];
+        double num = 0, den = 0;
+        for (int k = 0; k < kRfSlots; k++) {
+          num += s[2 * (size_t)k * b->ring_cap];
+          den += s[2 * (size_t)k * b->ring_cap + 1];
+        }
+        rtrace[(size_t)f * rcap + i] = den > 0 ? num / den : 0;
       }
   }
   return CDX_OK;

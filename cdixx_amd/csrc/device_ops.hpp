@@ -160,6 +160,38 @@ __device__ __forceinline__ void block_atomic_sum(double v, double* dst, double* 
     atomicAdd(dst, s);
   }
 }
+// Workgroup barrier ordering LDS only: unlike __syncthreads it does not wait
+// for the wave's outstanding global stores, so a reduction after a pass's
+// stores does not stall on their write latency.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// workgroup sums of two per-thread values (reduced together in the working
+// precision, one shuffle chain), added atomically in double to dst[0], dst[1]
+template <class V>
+__device__ __forceinline__ void block_atomic_sum2(V a, V b, double* dst, double* sh) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_xor(a, o, 64);
+    b += __shfl_xor(b, o, 64);
+  }
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int nw = (blockDim.x + 63) >> 6;
+  lds_barrier();
+  if (l == 0) {
+    sh[2 * w] = (double)a;
+    sh[2 * w + 1] = (double)b;
+  }
+  lds_barrier();
+  if (threadIdx.x == 0) {
+    double sa = 0, sb = 0;
+    for (int i = 0; i < nw; i++) {
+      sa += sh[2 * i];
+      sb += sh[2 * i + 1];
+    }
+    atomicAdd(dst, sa);
+    atomicAdd(dst + 1, sb);
+  }
+}
 // workgroup sum of a per-thread double stored at *dst by thread 0
 __device__ __forceinline__ void block_store_sum(double v, double* dst, double* sh) {
   v = wave_sum(v);

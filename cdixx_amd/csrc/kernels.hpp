@@ -31,13 +31,22 @@ enum RowOp : int {
   ROW_INV_UPDATE_FWD_PRTF = 4,  // ROW_INV_UPDATE_FWD with the PRTF phase-alignment sums
 };
 
+// R-factor sums of one projection: kRfSlots (num, den) pairs; workgroup b
+// adds into pair b mod kRfSlots, readers add the pairs.  Rings of these sums
+// are slot-major (pair k of entry i at ring[k * rfs + 2 i], rfs = 2 x entries),
+// so the pairs of one projection lie on distinct cache lines: double atomics
+// to one line serialize, and with all pairs of a projection on one or two
+// lines they cost about as much as the rest of the pass's tail.
+constexpr int kRfSlots = 16;
+
 template <class T>
 struct ProjArgs {
   int mode;
   int noiseLevel;
   T scale;   // cudaVars.scale = 1/exposure
   T rcolor;  // cudaVars.rcolor
-  double* rf;  // R-factor accumulators (num, den) or nullptr
+  double* rf;  // R-factor accumulators (kRfSlots (num, den) pairs, rfs doubles apart) or nullptr
+  int rfs;
   void* fsave;  // if set (run-time-mode kernels only): projected F stored here
 };
 

@@ -174,9 +174,8 @@ __device__ __forceinline__ void col_body(const cplx<T>* __restrict__ in, cplx<T>
     for (int i = 0; i < E; i++) out[(size_t)L::din_pos(i, t) * cols + col] = v[i];
   }
   if (rf) {
-    __shared__ double sh[16];
-    block_atomic_sum((double)rnum, pa.rf, sh);
-    block_atomic_sum((double)rden, pa.rf + 1, sh);
+    __shared__ double sh[32];
+    block_atomic_sum2(rnum, rden, pa.rf + (size_t)(blockIdx.x & (kRfSlots - 1)) * pa.rfs, sh);
   }
 }
 

@@ -217,7 +217,8 @@ int cdx_read_config_json(const char* path, char* buf, int cap);
  * iterations between two events.  If col_ms/row_ms are given, a second run
  * of min(n_iter, 200) iterations records each launch's kernel start and end
  * timestamps (hipExtLaunchKernel events) and returns the mean execution time
- * of the column pass and of the row pass. */
+ * of the column pass and of the row pass.  With p->recordR the column pass
+ * also forms the R-factor sums (the cost of recording R every iteration). */
 int cdx_time_iterations(cdx_ctx* ctx, const cdx_params* p, int algo, void* F, void* g, const void* I,
                         const void* bs, const void* spt, int n_iter, int use_graph, double* ms_total,
                         double* col_ms, double* row_ms);

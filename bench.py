@@ -265,6 +265,45 @@ def learned_leg(cdx):
     return out
 
 
+PTYCHO_N, PTYCHO_OBJ, PTYCHO_STEP = 256, 1280, 64
+PTYCHO_BYTES_PER_PX = 76  # 24 (gather O, P; store row spectra) + 20 (column pass) + 32 (row pass, O update)
+
+
+def ptycho_leg(cdx, sweeps=20):
+    """Ptychographic scan loop (util/ptycho.cpp:514-561): 256x256 probe on a
+    1280x1280 object (the reference's default object size, ptycho.cpp:42-43),
+    raster step 64 px (75% overlap, 17x17 = 289 positions), exact synthetic
+    patterns, rPIE object updates from a perturbed start, random order per
+    sweep.  Reports positions/s, launch groups per sweep and the effective
+    bandwidth at 76 B per probe pixel per position."""
+    import numpy as np
+    n, M, st = PTYCHO_N, PTYCHO_OBJ, PTYCHO_STEP
+    xs = np.arange(0, M - n + 1, st)
+    sx, sy = np.repeat(xs, len(xs)), np.tile(xs, len(xs))
+    rs = np.random.default_rng(0)
+    yy, xx = np.mgrid[0:M, 0:M] / M
+    obj = ((0.6 + 0.3 * np.sin(9 * xx) * np.cos(7 * yy)) * np.exp(1j * 2 * np.sin(5 * xx + 3 * yy))).astype(np.complex64)
+    x, y = np.mgrid[0:n, 0:n] - n / 2
+    probe = (np.exp(-(x ** 2 + y ** 2) / (2 * 40.0 ** 2)) * ((x ** 2 + y ** 2) < 70 ** 2)).astype(np.complex64)
+    pt = cdx.Ptycho(n, (M, M), sx, sy)
+    dO, dP = cdx.DeviceBuffer.from_array(obj), cdx.DeviceBuffer.from_array(probe)
+    dpat = cdx.DeviceBuffer(len(sx) * n * n * 4)
+    pt.simulate(dO, dP, dpat)
+    pt.set_patterns(dpat)
+    start = (obj * np.exp(0.5j * rs.standard_normal(obj.shape))).astype(np.complex64)
+    dO.upload(start)
+    order = rs.permutation(len(sx))
+    params = cdx.PtychoParams.default(scale=1e6)
+    ms, groups = pt.time_sweeps(params, dO, dP, sweeps, order=order)
+    per_pos_us = 1000.0 * ms / (sweeps * len(sx))
+    gbs = PTYCHO_BYTES_PER_PX * n * n * len(sx) * sweeps / (ms * 1e-3) / 1e9
+    pt.close()
+    return {"probe": [n, n], "object": [M, M], "positions": int(len(sx)), "raster_step": st, "sweeps": sweeps,
+            "launch_groups_per_sweep": groups, "positions_per_s": round(1e6 / per_pos_us, 1),
+            "us_per_position": round(per_pos_us, 3), "effective_GBps": round(gbs, 1),
+            "bytes_per_probe_px": PTYCHO_BYTES_PER_PX}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -276,6 +315,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-report", action="store_true", help="skip the FP32-vs-FP64 report per workload")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start leg")
+    ap.add_argument("--no-ptycho", action="store_true", help="skip the ptychography leg")
     ap.add_argument("--batch-iters", type=int, default=500)
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="run a plain copy kernel of this many MiB first (counter calibration)")
@@ -367,6 +407,8 @@ def main():
         out["fp64_report"] = fp64_report(cdx)
     if not args.no_learned:
         out["learned_warm_start"] = learned_leg(cdx)
+    if not args.no_ptycho:
+        out["ptycho"] = ptycho_leg(cdx)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -125,6 +125,13 @@ _SIGS = {
     "cdx_unet_train": ([_vp, _i, _i, _u64, _d, _d, _d, _dp], _i),
     "cdx_unet_warm_start": ([_vp, _vp, _vp, _d, _vp, _vp, _vp], _i),
     "cdx_unet_save": ([_vp, ctypes.c_char_p], _i), "cdx_unet_load": ([_vp, ctypes.c_char_p], _i),
+    "cdx_ptycho_create": ([ctypes.POINTER(_vp), _i, _i, _i, _i, _ip, _ip, _vp], _i),
+    "cdx_ptycho_destroy": ([_vp], _i), "cdx_ptycho_set_positions": ([_vp, _ip, _ip], _i),
+    "cdx_ptycho_set_patterns": ([_vp, _vp, _vp], _i), "cdx_ptycho_params_default": ([_vp], _i),
+    "cdx_ptycho_sweep": ([_vp, _ip, _i, _vp, _vp, _vp, _vp, _ip], _i),
+    "cdx_ptycho_apply_probe_step": ([_vp, _vp, _vp], _i),
+    "cdx_ptycho_simulate": ([_vp, _vp, _vp, _vp], _i), "cdx_ptycho_synchronize": ([_vp], _i),
+    "cdx_ptycho_time_sweeps": ([_vp, _ip, _i, _vp, _vp, _vp, _vp, _i, _dp, _ip], _i),
 }
 for _name, (_args, _res) in _SIGS.items():
     if hasattr(lib, _name):
@@ -620,6 +627,93 @@ class Batch:
     def close(self):
         if getattr(self, "h", None):
             lib.cdx_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+PT_ACCURATE, PT_NOISE = 0, 1
+
+
+class PtychoParams(ctypes.Structure):
+    """cdx_ptycho_params (include/cdx_abi.h)."""
+    _fields_ = [("mode", ctypes.c_int), ("noiseLevel", ctypes.c_int), ("scale", ctypes.c_double),
+                ("rcolor", ctypes.c_double), ("probeMax", ctypes.c_double), ("objMax", ctypes.c_double),
+                ("objStep", ctypes.c_double), ("probeStepSize", ctypes.c_double), ("updateProbe", ctypes.c_int),
+                ("maxConcurrent", ctypes.c_int)]
+
+    @classmethod
+    def default(cls, **kw):
+        p = cls()
+        check(lib.cdx_ptycho_params_default(ctypes.byref(p)))
+        for k, v in kw.items():
+            if not hasattr(p, k):
+                raise AttributeError(k)
+            setattr(p, k, v)
+        return p
+
+
+class Ptycho:
+    """The scan-position loop of CDIxx's ptychography (util/ptycho.cpp:514-561):
+    n x n probe, obj_shape object, integer window offsets sx (rows), sy
+    (columns) per scan position.  Object, probe, probe step and patterns are
+    device buffers (DeviceBuffer) owned by the caller."""
+
+    def __init__(self, n, obj_shape, sx, sy):
+        self.n = int(n)
+        self.obj_shape = (int(obj_shape[0]), int(obj_shape[1]))
+        self.nscan = len(sx)
+        h = _vp()
+        check(lib.cdx_ptycho_create(ctypes.byref(h), self.n, self.obj_shape[0], self.obj_shape[1], self.nscan,
+                                    self._ints(sx), self._ints(sy), None))
+        self.h = h.value
+
+    @staticmethod
+    def _ints(v):
+        v = [int(x) for x in v]
+        return (ctypes.c_int * max(len(v), 1))(*v)
+
+    def set_positions(self, sx, sy):
+        check(lib.cdx_ptycho_set_positions(self.h, self._ints(sx), self._ints(sy)))
+
+    def set_patterns(self, patterns, bs=None):
+        """patterns: device f32 [nscan][n][n] (FFT order, divided by the exposure)"""
+        check(lib.cdx_ptycho_set_patterns(self.h, patterns.ptr, bs.ptr if bs is not None else None))
+
+    def sweep(self, params, obj, probe, probe_step=None, order=None):
+        """one pass over `order` (default 0..nscan-1); returns the launch groups used"""
+        ng = ctypes.c_int()
+        o = self._ints(order) if order is not None else None
+        check(lib.cdx_ptycho_sweep(self.h, o, len(order) if order is not None else 0, ctypes.byref(params),
+                                   obj.ptr, probe.ptr, probe_step.ptr if probe_step is not None else None,
+                                   ctypes.byref(ng)))
+        return ng.value
+
+    def apply_probe_step(self, probe, probe_step):
+        check(lib.cdx_ptycho_apply_probe_step(self.h, probe.ptr, probe_step.ptr))
+
+    def simulate(self, obj, probe, patterns):
+        check(lib.cdx_ptycho_simulate(self.h, obj.ptr, probe.ptr, patterns.ptr))
+
+    def sync(self):
+        check(lib.cdx_ptycho_synchronize(self.h))
+
+    def time_sweeps(self, params, obj, probe, n, probe_step=None, order=None):
+        """(ms for n sweeps after one untimed sweep, launch groups per sweep)"""
+        ms, ng = ctypes.c_double(), ctypes.c_int()
+        o = self._ints(order) if order is not None else None
+        check(lib.cdx_ptycho_time_sweeps(self.h, o, len(order) if order is not None else 0, ctypes.byref(params),
+                                         obj.ptr, probe.ptr, probe_step.ptr if probe_step is not None else None,
+                                         int(n), ctypes.byref(ms), ctypes.byref(ng)))
+        return ms.value, ng.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.cdx_ptycho_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -36,6 +36,7 @@ UNITS = [
     ("batch.hip", [], "batch.o"),
     ("unet.hip", [], "unet.o"),
     ("preprocess.hip", [], "preprocess.o"),
+    ("ptycho.hip", [], "ptycho.o"),
     ("config.cpp", [], "config.o"),
     ("cdi.cpp", [], "cdi.o"),
     ("imgio.cpp", [], "imgio.o"),

@@ -91,6 +91,20 @@ __device__ __forceinline__ cplx<T> mod_noise(cplx<T> f, T I, int noise, T scale,
   return {s * f.x, s * f.y};
 }
 
+// applyModAccurate (cdi.cu:351-388) on r = bsnorm * F.  I: the intensity,
+// NaN on beamstop pixels (where the reference writes r back).  Where
+// |r|^2 == 0 the reference leaves F unwritten; r is zero there as well.
+template <class T>
+__device__ __forceinline__ cplx<T> mod_accurate(cplx<T> r, T I, T scale) {
+  if (isnan(I)) return r;
+  const T mod2 = I > (T)0 ? I : (T)0;
+  const T src2 = r.x * r.x + r.y * r.y;
+  if (src2 == (T)0) return r;
+  if (mod2 >= scale * (T)0.95 && src2 >= mod2) return r;
+  const T s = dsqrt(mod2 / src2);
+  return {s * r.x, s * r.y};
+}
+
 template <class T>
 __device__ __forceinline__ cplx<T> project(cplx<T> f, T I, const ProjArgs<T>& pa) {
   if (pa.mode == PROJ_MOD_NOISE) return mod_noise(f, I, pa.noiseLevel, pa.scale, pa.rcolor);

@@ -10,6 +10,9 @@ enum ProjMode : int {
   PROJ_MOD_ABS = 0,     // applyModAbs, src/gpu/cudaConfig.cu:827-841
   PROJ_MOD_NOISE = 1,   // applyMod, src/gpu/cdi.cu:438-485 (beamstop as NaN sentinel)
   PROJ_MOD_ABS_BS = 2,  // applyModAbs, beamstop pixels left unconstrained
+  // ptychography (util/ptycho.cpp:539-548): F *= bsnorm first, intensity packed, NaN on the beamstop
+  PROJ_PT_ACCURATE = 3,  // applyModAccurate, src/gpu/cdi.cu:351-388
+  PROJ_PT_NOISE = 4,     // applyMod, src/gpu/cdi.cu:438-485
 };
 
 // Column-pass operations (one workgroup owns C whole columns).
@@ -48,6 +51,7 @@ struct ProjArgs {
   double* rf;  // R-factor accumulators (kRfSlots (num, den) pairs, rfs doubles apart) or nullptr
   int rfs;
   void* fsave;  // if set (run-time-mode kernels only): projected F stored here
+  T bsnorm;     // PROJ_PT_* modes: F *= bsnorm before the projection
 };
 
 template <class T>

@@ -45,11 +45,11 @@ __device__ __forceinline__ cplx<T> mod_amp(cplx<T> f, T A) {
 }
 
 // The packed operand holds the amplitude for the modulus modes and the
-// intensity for the noise-tolerant mode (NaN marks unmeasured pixels).
+// intensity for the others (NaN marks unmeasured pixels).
 template <int MODE>
 __device__ __forceinline__ bool packed_amplitude(int runtime_mode) {
-  if constexpr (MODE < 0) return runtime_mode != PROJ_MOD_NOISE;
-  else return MODE != PROJ_MOD_NOISE;
+  if constexpr (MODE < 0) return runtime_mode == PROJ_MOD_ABS || runtime_mode == PROJ_MOD_ABS_BS;
+  else return MODE == PROJ_MOD_ABS || MODE == PROJ_MOD_ABS_BS;
 }
 
 template <class T, int MODE>
@@ -57,6 +57,9 @@ __device__ __forceinline__ cplx<T> project_m(cplx<T> f, T m, const ProjArgs<T>& 
   if constexpr (MODE == PROJ_MOD_NOISE) return mod_noise(f, m, pa.noiseLevel, pa.scale, pa.rcolor);
   else if constexpr (MODE == PROJ_MOD_ABS_BS) return isnan(m) ? f : mod_amp(f, m);
   else if constexpr (MODE == PROJ_MOD_ABS) return mod_amp(f, m);
+  else if constexpr (MODE == PROJ_PT_ACCURATE) return mod_accurate(cplx<T>{f.x * pa.bsnorm, f.y * pa.bsnorm}, m, pa.scale);
+  else if constexpr (MODE == PROJ_PT_NOISE)
+    return mod_noise(cplx<T>{f.x * pa.bsnorm, f.y * pa.bsnorm}, m, pa.noiseLevel, pa.scale, pa.rcolor);
   else {  // runtime mode, same arithmetic as the compile-time variants
     if (pa.mode == PROJ_MOD_NOISE) return mod_noise(f, m, pa.noiseLevel, pa.scale, pa.rcolor);
     if (pa.mode == PROJ_MOD_ABS_BS && isnan(m)) return f;

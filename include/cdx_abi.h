@@ -314,6 +314,57 @@ int cdx_unet_warm_start(cdx_unet* net, cdx_ctx* ctx, const void* I, double suppo
 int cdx_unet_save(cdx_unet* net, const char* path);
 int cdx_unet_load(cdx_unet* net, const char* path);
 
+/* ---- ptychography: the scan-position loop of util/ptycho.cpp:514-561 ----
+ * The reference keeps the object, probe, patterns and scan positions in
+ * `class ptycho` (util/ptycho.cpp:40-69) and runs the loop over the free
+ * functions of include/ptycho.hpp (getWindow, updateWindow, updateObject,
+ * updateObjectAndProbeStep), myFFT/myIFFT and applyModAccurate / applyMod
+ * (include/cudaConfig.hpp:107-110).  A cdx_ptycho holds the positions, the
+ * packed patterns and the work buffers; object, probe and probe step are the
+ * caller's device arrays (complex64, row-major).  FP32 only, as the
+ * reference (Real = float). */
+typedef struct cdx_ptycho cdx_ptycho;
+enum { CDX_PT_ACCURATE = 0, CDX_PT_NOISE = 1 };
+typedef struct cdx_ptycho_params {
+  int mode;              /* CDX_PT_ACCURATE: applyModAccurate (ptycho.cpp:540,545, iter < 2000);
+                            CDX_PT_NOISE: applyMod with noiseLevel (:542,547) */
+  int noiseLevel;
+  double scale;          /* cudaVars.scale = 1/exposure (init_cuda_image, ptycho.cpp:249) */
+  double rcolor;         /* cudaVars.rcolor (65535) */
+  double probeMax;       /* max |P|^2 (ptycho.cpp:479-480); < 0: computed on the device */
+  double objMax;         /* max |O|^2 used by the probe step (ptycho.cpp:481-488) */
+  double objStep;        /* object step: 1 (updateObject, :555), 0.5 (updateObjectAndProbeStep, :557) */
+  double probeStepSize;  /* probe-step weight (1/nscan, ptycho.cpp:453) */
+  int updateProbe;       /* 0: updateObject; 1: updateObjectAndProbeStep (iter >= update_probe_iter) */
+  int maxConcurrent;     /* positions per launch (0: up to 64); 1 runs the positions one by one */
+} cdx_ptycho_params;
+int cdx_ptycho_params_default(cdx_ptycho_params* p);
+/* probe n x n (64, 128, 256, 384, 512); object obj_rows x obj_cols; nscan
+ * window offsets sx (rows), sy (columns): round(scanpos + shift) of
+ * ptycho.cpp:517-523 (the sub-pixel shift there truncates to zero) */
+int cdx_ptycho_create(cdx_ptycho** out, int n, int obj_rows, int obj_cols, int nscan, const int* sx, const int* sy,
+                      void* stream);
+int cdx_ptycho_destroy(cdx_ptycho* pt);
+int cdx_ptycho_set_positions(cdx_ptycho* pt, const int* sx, const int* sy);
+/* patterns: device f32 [nscan][n][n], FFT order, divided by the exposure
+ * (ptycho.cpp:734-736); bs: device f32 [n][n] (> 0.5 = beamstop) or NULL */
+int cdx_ptycho_set_patterns(cdx_ptycho* pt, const void* patterns, const void* bs);
+/* one pass over the positions order[0..norder) (order NULL: 0..nscan-1):
+ * the object (and probe_step when updateProbe) as the reference's loop leaves
+ * them.  n_groups: launch groups used (optional). */
+int cdx_ptycho_sweep(cdx_ptycho* pt, const int* order, int norder, const cdx_ptycho_params* p, void* object,
+                     const void* probe, void* probe_step, int* n_groups);
+/* probe += probe_step; probe_step = 0 (ptycho.cpp:563-566) */
+int cdx_ptycho_apply_probe_step(cdx_ptycho* pt, void* probe, void* probe_step);
+/* forward model without noise: patterns[i] = |FFT(P * window_i(O))|^2 / (n n)
+ * (createPattern, ptycho.cpp:328-347) */
+int cdx_ptycho_simulate(cdx_ptycho* pt, const void* object, const void* probe, void* patterns);
+int cdx_ptycho_synchronize(cdx_ptycho* pt);
+/* n_sweeps sweeps with the same order after one untimed sweep; ms_total
+ * between two events on the context's stream */
+int cdx_ptycho_time_sweeps(cdx_ptycho* pt, const int* order, int norder, const cdx_ptycho_params* p, void* object,
+                           const void* probe, void* probe_step, int n_sweeps, double* ms_total, int* n_groups);
+
 #ifdef __cplusplus
 }
 #endif

@@ -316,6 +316,7 @@ def main():
     ap.add_argument("--no-report", action="store_true", help="skip the FP32-vs-FP64 report per workload")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start leg")
     ap.add_argument("--no-ptycho", action="store_true", help="skip the ptychography leg")
+    ap.add_argument("--ptycho-only", action="store_true", help="run only the ptychography leg (profiling)")
     ap.add_argument("--batch-iters", type=int, default=500)
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="run a plain copy kernel of this many MiB first (counter calibration)")
@@ -326,6 +327,9 @@ def main():
         sys.exit(2)
 
     import cdixx_amd as cdx
+    if args.ptycho_only:
+        print(json.dumps({"ptycho": ptycho_leg(cdx)}), flush=True)
+        return
     if args.copy_calib_mb:
         nb = args.copy_calib_mb << 20
         src, dst = cdx.DeviceBuffer(nb), cdx.DeviceBuffer(nb)

@@ -36,15 +36,15 @@
 
 namespace cdx {
 
-constexpr int kPtThreads = 256;
 constexpr int kPtSlots = 64;  // positions in flight per launch (slot buffers)
 constexpr float kAlpha = 0.1f;  // ALPHA, ptycho.cu:5 (object step)
 constexpr float kBeta = 0.5f;   // BETA, ptycho.cu:6 (probe step)
 
-template <int N>
+// lines per workgroup (log2) for NT threads per workgroup
+template <int N, int NT>
 constexpr int plsh() {
   int s = 0;
-  while ((1 << s) < kPtThreads / LineCfg<N>::TL) s++;
+  while ((1 << s) < NT / LineCfg<N>::TL) s++;
   return s;
 }
 
@@ -86,11 +86,11 @@ __device__ __forceinline__ cplx<float> rpie_step(cplx<float> s, cplx<float> d, f
 }
 
 // gather the window, multiply by the probe, forward row FFT -> slot A
-template <int N>
-__global__ void __launch_bounds__(kPtThreads) kp_fwd(PtArgs a) {
+template <int N, int NT>
+__global__ void __launch_bounds__(NT) kp_fwd(PtArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   using L = LineOf<float, N>;
-  constexpr int TL = LineCfg<N>::TL, E = L::E, LSH = plsh<N>();
+  constexpr int TL = LineCfg<N>::TL, E = L::E, LSH = plsh<N, NT>();
   const int pitch = lds_pitch<float, N>(LSH), alt = pitch << LSH;
   int ph = 0;
   const int t = threadIdx.x % TL, rr = threadIdx.x / TL;
@@ -114,30 +114,30 @@ __global__ void __launch_bounds__(kPtThreads) kp_fwd(PtArgs a) {
 }
 
 // column FFT, projection, inverse column FFT: slot A -> slot B
-template <int N, int MODE>
-__global__ void __launch_bounds__(kPtThreads) kp_col(PtArgs a) {
+template <int N, int NT, int MODE>
+__global__ void __launch_bounds__(NT) kp_col(PtArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   const PtPos p = a.pos[blockIdx.y];
   const size_t so = (size_t)blockIdx.y * N * N;
   col_body<float, N, COL_FWD_PROJ_INV, MODE, 0>(a.slotA + so, a.slotB + so, N, a.tw, a.Ipk + (size_t)p.scan * N * N,
-                                                a.pa, 0.f, blockIdx.x, plsh<N>(), smem);
+                                                a.pa, 0.f, blockIdx.x, plsh<N, NT>(), smem);
 }
 
 // forward column FFT only (forward model): slot A -> slot B
-template <int N>
-__global__ void __launch_bounds__(kPtThreads) kp_colfwd(PtArgs a) {
+template <int N, int NT>
+__global__ void __launch_bounds__(NT) kp_colfwd(PtArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   const size_t so = (size_t)blockIdx.y * N * N;
-  col_body<float, N, COL_FWD, 0, 0>(a.slotA + so, a.slotB + so, N, a.tw, nullptr, a.pa, 0.f, blockIdx.x, plsh<N>(),
-                                    smem);
+  col_body<float, N, COL_FWD, 0, 0>(a.slotA + so, a.slotB + so, N, a.tw, nullptr, a.pa, 0.f, blockIdx.x,
+                                    plsh<N, NT>(), smem);
 }
 
 // inverse row FFT of slot B, exit-wave difference, rPIE update, scatter
-template <int N>
-__global__ void __launch_bounds__(kPtThreads) kp_upd(PtArgs a) {
+template <int N, int NT>
+__global__ void __launch_bounds__(NT) kp_upd(PtArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   using L = LineOf<float, N>;
-  constexpr int TL = LineCfg<N>::TL, E = L::E, LSH = plsh<N>();
+  constexpr int TL = LineCfg<N>::TL, E = L::E, LSH = plsh<N, NT>();
   const int pitch = lds_pitch<float, N>(LSH), alt = pitch << LSH;
   int ph = 0;
   const int t = threadIdx.x % TL, rr = threadIdx.x / TL;
@@ -266,44 +266,40 @@ static int pfail(int code, const char* fmt, ...) {
 
 static bool pt_size_ok(int n) { return n == 64 || n == 128 || n == 256 || n == 384 || n == 512; }
 
-static size_t pt_lds(int n) {
-  switch (n) {
-#define F(n_) case n_: return lds_bytes<float, n_>(plsh<n_>());
-    F(64) F(128) F(256) F(384) F(512)
-#undef F
-    default: return 0;
-  }
-}
-
 template <class K>
-static hipError_t pt_launch(K kern, dim3 grid, size_t lds, hipStream_t st, const PtArgs& a) {
+static hipError_t pt_launch(K kern, dim3 grid, int nt, size_t lds, hipStream_t st, const PtArgs& a) {
   if (lds > 64 * 1024) {
     hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(kern, grid, dim3(kPtThreads), lds, st, a);
+  hipLaunchKernelGGL(kern, grid, dim3(nt), lds, st, a);
   return hipGetLastError();
 }
 
-// the three (or four) launches of one group of independent positions
+// The three launches of one group of independent positions.  A group holds
+// a few positions, so small workgroups fill the chip: the row kernels run one
+// wave per workgroup (rows are contiguous at any width), the column kernel two
+// waves (8 complex64 columns = 64-byte row segments for N = 128..384).
 template <int N>
 static hipError_t pt_group(const PtArgs& a, int k, int mode, hipStream_t st) {
-  const size_t lds = pt_lds(N);
-  const dim3 grid(N >> plsh<N>(), k);
-  hipError_t e = pt_launch(kp_fwd<N>, grid, lds, st, a);
+  constexpr int RT = 64, CT = 128;
+  const size_t lr = lds_bytes<float, N>(plsh<N, RT>()), lc = lds_bytes<float, N>(plsh<N, CT>());
+  const dim3 gr(N >> plsh<N, RT>(), k), gc(N >> plsh<N, CT>(), k);
+  hipError_t e = pt_launch(kp_fwd<N, RT>, gr, RT, lr, st, a);
   if (e == hipSuccess)
-    e = mode == CDX_PT_NOISE ? pt_launch(kp_col<N, PROJ_PT_NOISE>, grid, lds, st, a)
-                             : pt_launch(kp_col<N, PROJ_PT_ACCURATE>, grid, lds, st, a);
-  if (e == hipSuccess) e = pt_launch(kp_upd<N>, grid, lds, st, a);
+    e = mode == CDX_PT_NOISE ? pt_launch(kp_col<N, CT, PROJ_PT_NOISE>, gc, CT, lc, st, a)
+                             : pt_launch(kp_col<N, CT, PROJ_PT_ACCURATE>, gc, CT, lc, st, a);
+  if (e == hipSuccess) e = pt_launch(kp_upd<N, RT>, gr, RT, lr, st, a);
   return e;
 }
 
 template <int N>
 static hipError_t pt_sim_group(const PtArgs& a, int k, float* pat, hipStream_t st) {
-  const size_t lds = pt_lds(N);
-  const dim3 grid(N >> plsh<N>(), k);
-  hipError_t e = pt_launch(kp_fwd<N>, grid, lds, st, a);
-  if (e == hipSuccess) e = pt_launch(kp_colfwd<N>, grid, lds, st, a);
+  constexpr int NT = 256;
+  const size_t lds = lds_bytes<float, N>(plsh<N, NT>());
+  const dim3 grid(N >> plsh<N, NT>(), k);
+  hipError_t e = pt_launch(kp_fwd<N, NT>, grid, NT, lds, st, a);
+  if (e == hipSuccess) e = pt_launch(kp_colfwd<N, NT>, grid, NT, lds, st, a);
   if (e == hipSuccess) {
     hipLaunchKernelGGL(kp_mod2<N>, dim3(std::max(1, N * N / 4096), k), dim3(256), 0, st, a.pos,
                        (const cplx<float>*)a.slotB, pat);

@@ -265,6 +265,30 @@ def learned_leg(cdx):
     return out
 
 
+def slab_leg(cdx, iters=200):
+    """One GPU's share of a 2048x2048 frame split over 8 GPUs in row slabs
+    (the sharding described for frames larger than one GPU): the row pass on
+    a 256x2048 slab (row FFTs local) and, after the all-to-all transpose, the
+    column pass on 2048-point columns x 256.  Each is the kernel of this build
+    at that shape (autotuned, mean kernel time over `iters` launches); the
+    transposes themselves (RCCL over xGMI) are not run here."""
+    out = {}
+    for key, (r, c), pick in (("row_pass_256x2048", (256, 2048), 2), ("col_pass_2048x256", (2048, 256), 1)):
+        ctx = cdx.Context(r, c)
+        ctx.autotune(reps=20)
+        F, g, I, spt = ctx.cplx(), ctx.cplx(), ctx.real(), ctx.real()
+        for b in (F, g, I, spt):
+            b.zero()
+        p = cdx.Params.default(beta=0.85)
+        _, col_ms, row_ms = ctx.time_iterations(p, cdx.RAAR, F, g, I, spt, iters, per_kernel=True)
+        ms = row_ms if pick == 2 else col_ms
+        nbytes = (ROW_BYTES // (N * N) if pick == 2 else COL_BYTES // (N * N)) * r * c
+        out[key] = {"us": round(1000 * ms, 3), "GBps": round(nbytes / (ms * 1e-3) / 1e9, 1)}
+        ctx.close()
+    out["local_us_per_iteration"] = round(out["row_pass_256x2048"]["us"] + out["col_pass_2048x256"]["us"], 3)
+    return out
+
+
 PTYCHO_N, PTYCHO_OBJ, PTYCHO_STEP = 256, 1280, 64
 PTYCHO_BYTES_PER_PX = 76  # 24 (gather O, P; store row spectra) + 20 (column pass) + 32 (row pass, O update)
 
@@ -317,6 +341,7 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start leg")
     ap.add_argument("--no-ptycho", action="store_true", help="skip the ptychography leg")
     ap.add_argument("--ptycho-only", action="store_true", help="run only the ptychography leg (profiling)")
+    ap.add_argument("--slab-only", action="store_true", help="run only the one-GPU slab-share leg")
     ap.add_argument("--batch-iters", type=int, default=500)
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="run a plain copy kernel of this many MiB first (counter calibration)")
@@ -329,6 +354,9 @@ def main():
     import cdixx_amd as cdx
     if args.ptycho_only:
         print(json.dumps({"ptycho": ptycho_leg(cdx)}), flush=True)
+        return
+    if args.slab_only:
+        print(json.dumps({"slab_share_2048_over_8": slab_leg(cdx)}), flush=True)
         return
     if args.copy_calib_mb:
         nb = args.copy_calib_mb << 20
@@ -413,6 +441,7 @@ def main():
         out["learned_warm_start"] = learned_leg(cdx)
     if not args.no_ptycho:
         out["ptycho"] = ptycho_leg(cdx)
+    out["slab_share_2048_over_8"] = slab_leg(cdx)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(out), flush=True)

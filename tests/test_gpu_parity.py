@@ -29,7 +29,8 @@ TOL = {0: 2e-6, 1: 1e-13}  # FFT rel L2 by precision
 
 @pytest.mark.parametrize("prec", [0, 1])
 @pytest.mark.parametrize("shape", [(16, 16), (32, 64), (64, 64), (128, 256), (256, 256), (384, 384), (512, 512),
-                                   (768, 768), (1024, 1024), (2048, 2048), (256, 384), (1024, 512), (384, 16)])
+                                   (768, 768), (1024, 1024), (2048, 2048), (256, 384), (1024, 512), (384, 16),
+                                   (256, 2048), (2048, 256)])
 def test_fft_matches_numpy(cdx, shape, prec):
     print(f"fft {shape} prec={prec}", flush=True)
     ctx = cdx.Context(*shape, precision=prec)
@@ -152,6 +153,33 @@ def test_one_iteration_from_injected_state(cdx, o64, n, algo, prec):
         eF = rel_l2(ctx.get_cplx(dF), ref["F"])
         assert eg < tol and eF < tol, (proj, eg, eF)
         ctx.close()
+
+
+@pytest.mark.parametrize("shape", [(256, 2048), (2048, 256)])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_one_iteration_on_slab_shapes(cdx, o64, shape, prec):
+    """The passes at the shapes of one GPU's share of a 2048x2048 frame split
+    in row slabs (256x2048 rows, 2048-point columns x 256): one RAAR step
+    with beamstop from an injected state against the FP64 oracle."""
+    rows, cols = shape
+    print(f"one-iteration slab {shape} prec={prec}", flush=True)
+    rs = np.random.default_rng(3)
+    I = rs.uniform(0, 4, shape) ** 2
+    F = np.sqrt(I) * np.exp(2j * np.pi * rs.uniform(size=shape))
+    g = np.fft.ifft2(F) * np.sqrt(rows * cols) + 0.01 * (rs.standard_normal(shape) + 1j * rs.standard_normal(shape))
+    spt = rs.uniform(0, 1, shape)
+    bs = o64.beamstop(rows, cols, 6)
+    p = cdx.Params.default(beta=0.85, useBS=1, noiseLevel=24, simCCDbit=1, exposure=1e-6)
+    ctx = cdx.Context(rows, cols, precision=prec)
+    ctx.set_threshold(0.5)
+    dF, dg, dI, ds, dbs = ctx.cplx(F), ctx.cplx(g), ctx.real(I), ctx.real(spt), ctx.real(bs)
+    ctx.phase_retrieve("2*RAAR", p, dF, dg, dI, ds, bs=dbs)
+    ref = o64.phase_retrieve("2*RAAR", F, g, I, spt, 0.5, bs=bs, beta=0.85, simCCDbit=1, useBS=1, noiseLevel=24,
+                             scale=1e6)
+    tol = 1e-5 if prec == 0 else 1e-12
+    eg, eF = rel_l2(ctx.get_cplx(dg), ref["g"]), rel_l2(ctx.get_cplx(dF), ref["F"])
+    assert eg < tol and eF < tol, (eg, eF)
+    ctx.close()
 
 
 @pytest.mark.parametrize("mode", [0, 1])

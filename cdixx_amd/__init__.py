@@ -132,6 +132,7 @@ _SIGS = {
     "cdx_ptycho_apply_probe_step": ([_vp, _vp, _vp], _i),
     "cdx_ptycho_simulate": ([_vp, _vp, _vp, _vp], _i), "cdx_ptycho_synchronize": ([_vp], _i),
     "cdx_ptycho_time_sweeps": ([_vp, _ip, _i, _vp, _vp, _vp, _vp, _i, _dp, _ip], _i),
+    "cdx_ptycho_levels": ([_i, _i, _i, _i, _ip, _ip, _ip, _i, _i, _ip, _ip, _ip], _i),
 }
 for _name, (_args, _res) in _SIGS.items():
     if hasattr(lib, _name):
@@ -655,6 +656,22 @@ class PtychoParams(ctypes.Structure):
                 raise AttributeError(k)
             setattr(p, k, v)
         return p
+
+
+def ptycho_levels(n, obj_shape, sx, sy, order=None, max_concurrent=0):
+    """Launch schedule of a sweep (host only): list of groups, each the scan
+    indices launched together, in launch order (cdx_ptycho_levels)."""
+    m = len(order) if order is not None else len(sx)
+    ints = lambda v: (ctypes.c_int * max(len(v), 1))(*[int(x) for x in v])
+    groups, lst, ng = (ctypes.c_int * max(m, 1))(), (ctypes.c_int * max(m, 1))(), ctypes.c_int()
+    check(lib.cdx_ptycho_levels(int(n), int(obj_shape[0]), int(obj_shape[1]), len(sx), ints(sx), ints(sy),
+                                ints(order) if order is not None else None, m, int(max_concurrent), groups, lst,
+                                ctypes.byref(ng)))
+    out, k = [], 0
+    for g in groups[:ng.value]:
+        out.append(list(lst[k:k + g]))
+        k += g
+    return out
 
 
 class Ptycho:

@@ -423,23 +423,35 @@ extern "C" int cdx_ptycho_params_default(cdx_ptycho_params* q) {
 // Level lists of a sweep: positions grouped by level (stable within a level),
 // each level split into groups of at most cap positions.  Returns the groups'
 // sizes; fills list[] with the positions in launch order.
-static std::vector<int> pt_levels(const cdx_ptycho* p, const int* order, int m, int cap, PtPos* list) {
+static std::vector<int> pt_levels(int n, int orow, int ocol, const int* sx, const int* sy, const int* order, int m,
+                                  int cap, PtPos* list) {
   struct Rect {
     int x0, x1, y0, y1;
   };
   std::vector<Rect> r(m);
   std::vector<int> lvl(m, 0);
   int nl = 0;
+  // Clipped windows are at most n wide and start inside the object, so two
+  // that overlap start less than n apart in both directions: bucket the
+  // earlier positions by start / n and look only at the 3 x 3 neighbouring
+  // cells.
+  const int gx = orow / n + 1, gy = ocol / n + 1;
+  std::vector<std::vector<int>> cell((size_t)gx * gy);
   for (int j = 0; j < m; j++) {
     const int s = order[j];
-    r[j] = {std::max(p->sx[s], 0), std::min(p->sx[s] + p->n, p->orow), std::max(p->sy[s], 0),
-            std::min(p->sy[s] + p->n, p->ocol)};
+    r[j] = {std::max(sx[s], 0), std::min(sx[s] + n, orow), std::max(sy[s], 0),
+            std::min(sy[s] + n, ocol)};
     int l = 0;
     if (r[j].x0 < r[j].x1 && r[j].y0 < r[j].y1) {
-      for (int i = 0; i < j; i++) {
-        if (lvl[i] + 1 > l && r[i].x0 < r[j].x1 && r[j].x0 < r[i].x1 && r[i].y0 < r[j].y1 && r[j].y0 < r[i].y1)
-          l = lvl[i] + 1;
-      }
+      const int cx = r[j].x0 / n, cy = r[j].y0 / n;
+      for (int ax = std::max(cx - 1, 0); ax <= std::min(cx + 1, gx - 1); ax++)
+        for (int ay = std::max(cy - 1, 0); ay <= std::min(cy + 1, gy - 1); ay++)
+          for (int i : cell[(size_t)ax * gy + ay]) {
+            if (lvl[i] + 1 > l && r[i].x0 < r[j].x1 && r[j].x0 < r[i].x1 && r[i].y0 < r[j].y1 &&
+                r[j].y0 < r[i].y1)
+              l = lvl[i] + 1;
+          }
+      cell[(size_t)cx * gy + cy].push_back(j);
     }
     lvl[j] = l;
     nl = std::max(nl, l + 1);
@@ -453,7 +465,7 @@ static std::vector<int> pt_levels(const cdx_ptycho* p, const int* order, int m, 
       const int k = (int)std::min<size_t>(cap, v.size() - a);
       for (int q = 0; q < k; q++) {
         const int s = order[v[a + q]];
-        list[w++] = {s, p->sx[s], p->sy[s], 0};
+        list[w++] = {s, sx[s], sy[s], 0};
       }
       groups.push_back(k);
     }
@@ -511,7 +523,8 @@ extern "C" int cdx_ptycho_sweep(cdx_ptycho* p, const int* order, int norder, con
   int rc = pt_stage(p, &b);
   if (rc) return rc;
   PtPos* dev = p->dpos[b];
-  const std::vector<int> groups = pt_levels(p, order, m, cap, p->hpos[b]);
+  const std::vector<int> groups = pt_levels(p->n, p->orow, p->ocol, p->sx.data(), p->sy.data(), order, m, cap,
+                                            p->hpos[b]);
   PCHK(hipMemcpyAsync(dev, p->hpos[b], m * sizeof(PtPos), hipMemcpyHostToDevice, p->st));
   PCHK(hipEventRecord(p->staged[b], p->st));
   PtArgs a = pt_args(p, object, probe);
@@ -543,6 +556,31 @@ extern "C" int cdx_ptycho_sweep(cdx_ptycho* p, const int* order, int norder, con
     off += k;
   }
   if (n_groups) *n_groups = (int)groups.size();
+  return CDX_OK;
+}
+
+extern "C" int cdx_ptycho_levels(int n, int obj_rows, int obj_cols, int nscan, const int* sx, const int* sy,
+                                 const int* order, int norder, int max_concurrent, int* groups, int* list,
+                                 int* n_groups) {
+  if (n <= 0 || obj_rows <= 0 || obj_cols <= 0 || nscan <= 0 || !sx || !sy || !groups || !list || !n_groups)
+    return pfail(CDX_ERR_ARG, "cdx_ptycho_levels: bad arguments");
+  const int m = order ? norder : nscan;
+  if (m < 0 || m > nscan) return pfail(CDX_ERR_ARG, "cdx_ptycho_levels: %d positions (at most %d)", m, nscan);
+  std::vector<int> ident;
+  if (!order) {
+    ident.resize(m);
+    for (int i = 0; i < m; i++) ident[i] = i;
+    order = ident.data();
+  }
+  for (int i = 0; i < m; i++)
+    if (order[i] < 0 || order[i] >= nscan)
+      return pfail(CDX_ERR_ARG, "cdx_ptycho_levels: order[%d] = %d out of range", i, order[i]);
+  const int cap = max_concurrent > 0 ? max_concurrent : kPtSlots;
+  std::vector<PtPos> pl(m > 0 ? m : 1);
+  const std::vector<int> g = pt_levels(n, obj_rows, obj_cols, sx, sy, order, m, cap, pl.data());
+  for (size_t i = 0; i < g.size(); i++) groups[i] = g[i];
+  for (int i = 0; i < m; i++) list[i] = pl[i].scan;
+  *n_groups = (int)g.size();
   return CDX_OK;
 }
 

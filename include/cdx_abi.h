@@ -354,6 +354,16 @@ int cdx_ptycho_set_patterns(cdx_ptycho* pt, const void* patterns, const void* bs
  * them.  n_groups: launch groups used (optional). */
 int cdx_ptycho_sweep(cdx_ptycho* pt, const int* order, int norder, const cdx_ptycho_params* p, void* object,
                      const void* probe, void* probe_step, int* n_groups);
+/* the launch schedule of a sweep (host only, no device needed): groups of
+ * mutually independent positions in launch order.  A position's level is one
+ * more than the highest level of the earlier positions in `order` whose
+ * windows (clipped to the object) overlap its own; levels are split into
+ * groups of at most max_concurrent (0: 64).  groups[n_groups] holds the group
+ * sizes (at most norder entries), list[norder] the scan indices in launch
+ * order.  Has no counterpart in the reference, whose loop runs one position
+ * at a time (ptycho.cpp:514). */
+int cdx_ptycho_levels(int n, int obj_rows, int obj_cols, int nscan, const int* sx, const int* sy, const int* order,
+                      int norder, int max_concurrent, int* groups, int* list, int* n_groups);
 /* probe += probe_step; probe_step = 0 (ptycho.cpp:563-566) */
 int cdx_ptycho_apply_probe_step(cdx_ptycho* pt, void* probe, void* probe_step);
 /* forward model without noise: patterns[i] = |FFT(P * window_i(O))|^2 / (n n)

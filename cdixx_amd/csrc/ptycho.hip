@@ -400,6 +400,7 @@ extern "C" int cdx_ptycho_set_patterns(cdx_ptycho* p, const void* patterns, cons
   hipLaunchKernelGGL(kp_pack, dim3((unsigned)std::min<int64_t>((total + 255) / 256, 8192)), dim3(256), 0, p->st, nn,
                      total, (const float*)patterns, (const float*)bs, (float*)p->Ipk);
   PCHK(hipGetLastError());
+  PCHK(hipStreamSynchronize(p->st));  // the caller may release patterns and bs on return
   p->have_patterns = true;
   return CDX_OK;
 }

@@ -347,7 +347,8 @@ int cdx_ptycho_create(cdx_ptycho** out, int n, int obj_rows, int obj_cols, int n
 int cdx_ptycho_destroy(cdx_ptycho* pt);
 int cdx_ptycho_set_positions(cdx_ptycho* pt, const int* sx, const int* sy);
 /* patterns: device f32 [nscan][n][n], FFT order, divided by the exposure
- * (ptycho.cpp:734-736); bs: device f32 [n][n] (> 0.5 = beamstop) or NULL */
+ * (ptycho.cpp:734-736); bs: device f32 [n][n] (> 0.5 = beamstop) or NULL.
+ * Both are copied (packed) before the call returns. */
 int cdx_ptycho_set_patterns(cdx_ptycho* pt, const void* patterns, const void* bs);
 /* one pass over the positions order[0..norder) (order NULL: 0..nscan-1):
  * the object (and probe_step when updateProbe) as the reference's loop leaves

@@ -593,7 +593,7 @@ struct Loop {
   int finish() {
     if (!f_pending) {
       ProjArgs<T> none{};
-      HIPCHK(launch_col<T>(COL_FWD, c->rows, c->cols, c->T1, F, c->tw_col, nullptr, none, 0, c->st));
+      HIPCHK(launch_col<T>(COL_FWD_T1, c->rows, c->cols, c->T1, F, c->tw_col, nullptr, none, 0, c->st));
       f_pending = true;
     }
     return CDX_OK;

@@ -23,7 +23,16 @@ enum ColOp : int {
   COL_PROJ_INV = 3,       // input already in Fourier space
   COL_FWD_GAUSS_INV = 4,  // forward -> Gaussian multiply -> inverse (shrink-wrap)
   COL_GAUSS_INV = 5,
+  COL_FWD_T1 = 6,         // COL_FWD reading the loop state T1 in its column-tile layout
 };
+
+// The loop state T1 = rowFFT(g) of a single-frame context is stored in
+// column tiles: element (r, c) of a rows x cols image at
+// (c / kT1Tile) * rows * kT1Tile + r * kT1Tile + c % kT1Tile, so the column
+// pass reads each tile of kT1Tile columns as one contiguous block and the row
+// pass writes kT1Tile-element pieces.  Every supported line length is a
+// multiple of kT1Tile.
+constexpr int kT1Tile = 4;
 
 // Row-pass operations (one workgroup owns R whole rows).
 enum RowOp : int {

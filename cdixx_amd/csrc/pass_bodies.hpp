@@ -88,11 +88,18 @@ __device__ __forceinline__ int xcd_tile(int bid, int nb) {
   return (bid & 7) * (nb >> 3) + (bid >> 3);
 }
 
+// index of element (r, c) of the loop state T1 in its column-tile layout
+__device__ __forceinline__ size_t t1_index(int r, int c, int rows) {
+  return (size_t)(c / kT1Tile) * ((size_t)rows * kT1Tile) + (size_t)r * kT1Tile + (c % kT1Tile);
+}
+
 // ---------------------------------------------------------------------------
 // column pass body: columns [tile << lsh, (tile+1) << lsh).  MODE/RF = -1:
-// projection mode and R-factor chosen at run time
+// projection mode and R-factor chosen at run time.  TILED: the transformed
+// input (ops that start with the forward transform) is T1 in its column-tile
+// layout.
 // ---------------------------------------------------------------------------
-template <class T, int N, int OP, int MODE, int RF>
+template <class T, int N, int OP, int MODE, int RF, bool TILED = false>
 __device__ __forceinline__ void col_body(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out, int cols,
                                          const cplx<T>* __restrict__ tw, const T* __restrict__ I,
                                          const ProjArgs<T>& pa, T gsig, int tile, int lsh, unsigned char* smem) {
@@ -108,6 +115,7 @@ __device__ __forceinline__ void col_body(const cplx<T>* __restrict__ in, cplx<T>
   const bool ok = col < cols;
   cplx<T>* my = lds + c * pitch;
   constexpr bool from_f = (OP == COL_INV || OP == COL_PROJ_INV || OP == COL_GAUSS_INV);
+  constexpr bool fwd_only = (OP == COL_FWD || OP == COL_FWD_T1);
   constexpr bool proj = (OP == COL_FWD_PROJ_INV || OP == COL_PROJ_INV);
   cplx<T> v[E];
   T iv[proj ? E : 1];
@@ -115,6 +123,9 @@ __device__ __forceinline__ void col_body(const cplx<T>* __restrict__ in, cplx<T>
   if constexpr (from_f) {
 #pragma unroll
     for (int i = 0; i < E; i++) v[i] = ok ? in[(size_t)L::dout_pos(i, t) * cols + col] : cplx<T>{(T)0, (T)0};
+  } else if constexpr (TILED) {
+#pragma unroll
+    for (int i = 0; i < E; i++) v[i] = ok ? in[t1_index(L::din_pos(i, t), col, N)] : cplx<T>{(T)0, (T)0};
   } else {
 #pragma unroll
     for (int i = 0; i < E; i++) v[i] = ok ? in[(size_t)L::din_pos(i, t) * cols + col] : cplx<T>{(T)0, (T)0};
@@ -127,10 +138,10 @@ __device__ __forceinline__ void col_body(const cplx<T>* __restrict__ in, cplx<T>
   typename L::TwF twf;
   typename L::TwI twi;
   if constexpr (!from_f) twf = L::template tw_load<-1>(tw, t);
-  if constexpr (OP != COL_FWD) twi = L::template tw_load<+1>(tw, t);
+  if constexpr (!fwd_only) twi = L::template tw_load<+1>(tw, t);
   if constexpr (!from_f) L::forward(v, my, alt, ph, twf, t);
 
-  if constexpr (OP == COL_FWD) {
+  if constexpr (fwd_only) {
     if (ok) {
 #pragma unroll
       for (int i = 0; i < E; i++) out[(size_t)L::dout_pos(i, t) * cols + col] = v[i];
@@ -184,9 +195,10 @@ __device__ __forceinline__ void col_body(const cplx<T>* __restrict__ in, cplx<T>
 
 // ---------------------------------------------------------------------------
 // row pass body: rows [tile << lsh, (tile+1) << lsh).  ALGO: support-update
-// rule of ROW_INV_UPDATE_FWD (enum Algorithm)
+// rule of ROW_INV_UPDATE_FWD (enum Algorithm).  TILED: the update ops store
+// T1 in its column-tile layout.
 // ---------------------------------------------------------------------------
-template <class T, int N, int OP, int ALGO>
+template <class T, int N, int OP, int ALGO, bool TILED = false>
 __device__ __forceinline__ void row_body(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out, int rows,
                                          const cplx<T>* __restrict__ tw, cplx<T>* __restrict__ g,
                                          const T* __restrict__ spt, const UpdArgs<T>& ua, T scale,
@@ -291,7 +303,8 @@ __device__ __forceinline__ void row_body(const cplx<T>* __restrict__ in, cplx<T>
       L::forward(v, my, alt, ph, twf, t);
       if (ok) {
 #pragma unroll
-        for (int i = 0; i < E; i++) store_stream(&out[base + L::dout_pos(i, t)], v[i]);
+        for (int i = 0; i < E; i++)
+          store_stream(&out[TILED ? t1_index(row, L::dout_pos(i, t), rows) : base + L::dout_pos(i, t)], v[i]);
       }
       if constexpr (prtf) {
         __shared__ double sh[16];

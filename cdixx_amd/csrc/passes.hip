@@ -94,7 +94,9 @@ __global__ void __launch_bounds__(kMaxThreads)
 k_col(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out, int cols, const cplx<T>* __restrict__ tw,
       const T* __restrict__ I, ProjArgs<T> pa, T gsig, int lsh) {
   extern __shared__ __align__(16) unsigned char smem[];
-  col_body<T, N, OP, MODE, RF>(in, out, cols, tw, I, pa, gsig, xcd_tile(blockIdx.x, gridDim.x), lsh, smem);
+  // the single-frame loop keeps T1 in its column-tile layout
+  constexpr bool tiled = OP == COL_FWD_PROJ_INV || OP == COL_FWD_GAUSS_INV || OP == COL_FWD_T1;
+  col_body<T, N, OP, MODE, RF, tiled>(in, out, cols, tw, I, pa, gsig, xcd_tile(blockIdx.x, gridDim.x), lsh, smem);
 }
 template <class T, int N, int OP, int ALGO>
 __global__ void __launch_bounds__(kMaxThreads)
@@ -102,7 +104,8 @@ k_row(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out, int rows, const
       cplx<T>* __restrict__ g, const T* __restrict__ spt, UpdArgs<T> ua, T scale, T* __restrict__ sout,
       ReduceSlots rs, int lsh) {
   extern __shared__ __align__(16) unsigned char smem[];
-  row_body<T, N, OP, ALGO>(in, out, rows, tw, g, spt, ua, scale, sout, rs, blockIdx.x, lsh, smem);
+  constexpr bool tiled = OP == ROW_INV_UPDATE_FWD || OP == ROW_INV_UPDATE_FWD_PRTF;
+  row_body<T, N, OP, ALGO, tiled>(in, out, rows, tw, g, spt, ua, scale, sout, rs, blockIdx.x, lsh, smem);
 }
 
 // ---------------------------------------------------------------------------
@@ -146,6 +149,7 @@ static hipError_t col_launch_sz(int op, int cols, const void* in, void* out, con
   const bool rf = pa.rf != nullptr;
   switch (op) {
     case COL_FWD: return col_launch_n<T, N, COL_FWD, 0, 0>(cols, in, out, tw, I, pa, gs, st, lines, kt);
+    case COL_FWD_T1: return col_launch_n<T, N, COL_FWD_T1, 0, 0>(cols, in, out, tw, I, pa, gs, st, lines, kt);
     case COL_INV: return col_launch_n<T, N, COL_INV, 0, 0>(cols, in, out, tw, I, pa, gs, st, lines, kt);
     case COL_FWD_PROJ_INV:
       if (pa.fsave) return col_launch_n<T, N, COL_FWD_PROJ_INV, -1, -1>(cols, in, out, tw, I, pa, gs, st, lines, kt);

@@ -523,8 +523,9 @@ struct Loop {
 
   int setup(const void* I, const void* bs) {
     const bool pack_bs = p->useBS && bs && (p->simCCDbit || p->freeBeamstop);
-    HIPCHK(launch_pack_intensity<T>(c->n, (const T*)I, pack_bs ? (const T*)bs : nullptr, (T*)c->Ipk, !p->simCCDbit,
-                                    c->st));
+    // packed in the column-tile layout the single-frame column pass reads
+    HIPCHK(launch_pack_intensity_tiled<T>(c->rows, c->cols, (const T*)I, pack_bs ? (const T*)bs : nullptr,
+                                          (T*)c->Ipk, !p->simCCDbit, c->st));
     pa.mode = p->simCCDbit ? PROJ_MOD_NOISE : (pack_bs ? PROJ_MOD_ABS_BS : PROJ_MOD_ABS);
     pa.noiseLevel = p->noiseLevel;
     pa.scale = (T)(1.0 / p->exposure);

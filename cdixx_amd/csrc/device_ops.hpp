@@ -54,6 +54,12 @@ __host__ __device__ constexpr size_t lds_bytes(int lsh) {
   return (size_t)2 * ((size_t)1 << lsh) * lds_pitch<T, N>(lsh) * sizeof(cplx<T>);
 }
 
+// index of element (r, c) of a rows-row image in the column-tile layout of
+// the loop state T1 and the packed measurement
+__device__ __forceinline__ size_t t1_index(int r, int c, int rows) {
+  return (size_t)(c / kT1Tile) * ((size_t)rows * kT1Tile) + (size_t)r * kT1Tile + (c % kT1Tile);
+}
+
 // ---------------------------------------------------------------------------
 // pointwise device helpers (reference semantics)
 // ---------------------------------------------------------------------------

@@ -17,6 +17,23 @@ hipError_t launch_pack_intensity(int64_t n, const T* I, const T* bs, T* out, int
   hipLaunchKernelGGL(k_pack<T>, ew_grid(n), dim3(256), 0, st, n, I, bs, out, amp);
   return hipGetLastError();
 }
+// the same, written in the column-tile layout of a single-frame context
+template <class T>
+__global__ void k_pack_tiled(int rows, int cols, const T* __restrict__ I, const T* __restrict__ bs,
+                             T* __restrict__ out, int amp) {
+  const int64_t n = (int64_t)rows * cols;
+  EW_LOOP(n) {
+    const T v = I[i];
+    const int r = (int)(i / cols), c = (int)(i % cols);
+    out[t1_index(r, c, rows)] = (bs && bs[i] > (T)0.5) ? (T)NAN : amp ? (v > (T)0 ? dsqrt(v) : (T)0) : v;
+  }
+}
+template <class T>
+hipError_t launch_pack_intensity_tiled(int rows, int cols, const T* I, const T* bs, T* out, int amp,
+                                       hipStream_t st) {
+  hipLaunchKernelGGL(k_pack_tiled<T>, ew_grid((int64_t)rows * cols), dim3(256), 0, st, rows, cols, I, bs, out, amp);
+  return hipGetLastError();
+}
 
 template <class T>
 __global__ void k_mod_abs(int64_t n, cplx<T>* F, const T* I) {
@@ -805,6 +822,7 @@ hipError_t launch_radial(const void* src, int rows, int cols, int kind, T norm, 
   template hipError_t launch_extend<T>(int64_t, const T*, void*, hipStream_t);                                   \
   template hipError_t launch_wavefront<T>(int, int, const T*, const T*, int, int, void*, hipStream_t);           \
   template hipError_t launch_pack_intensity<T>(int64_t, const T*, const T*, T*, int, hipStream_t);                    \
+  template hipError_t launch_pack_intensity_tiled<T>(int, int, const T*, const T*, T*, int, hipStream_t);             \
   template hipError_t launch_mod_abs<T>(int64_t, void*, const T*, hipStream_t);                                  \
   template hipError_t launch_mod_noise<T>(int64_t, void*, const T*, const T*, int, T, T, T, hipStream_t);        \
   template hipError_t launch_support<T>(int64_t, void*, const void*, const T*, const UpdArgs<T>&, hipStream_t);  \

@@ -127,6 +127,9 @@ template <class T> hipError_t launch_radial(const void* src, int rows, int cols,
 // packed measurement: NaN where bs > 0.5 (unmeasured), else I, or the
 // amplitude sqrt(max(I, 0)) when amp != 0 (what the modulus projection uses)
 template <class T> hipError_t launch_pack_intensity(int64_t n, const T* I, const T* bs, T* out, int amp, hipStream_t st);
+// packed measurement in the column-tile layout (single-frame loop)
+template <class T>
+hipError_t launch_pack_intensity_tiled(int rows, int cols, const T* I, const T* bs, T* out, int amp, hipStream_t st);
 template <class T> hipError_t launch_mod_abs(int64_t n, void* F, const T* I, hipStream_t st);
 template <class T> hipError_t launch_mod_noise(int64_t n, void* F, const T* I, const T* bs, int noise, T scale,
                                                T rcolor, T bsnorm, hipStream_t st);

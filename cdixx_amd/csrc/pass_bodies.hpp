@@ -88,18 +88,13 @@ __device__ __forceinline__ int xcd_tile(int bid, int nb) {
   return (bid & 7) * (nb >> 3) + (bid >> 3);
 }
 
-// index of element (r, c) of the loop state T1 in its column-tile layout
-__device__ __forceinline__ size_t t1_index(int r, int c, int rows) {
-  return (size_t)(c / kT1Tile) * ((size_t)rows * kT1Tile) + (size_t)r * kT1Tile + (c % kT1Tile);
-}
-
 // ---------------------------------------------------------------------------
 // column pass body: columns [tile << lsh, (tile+1) << lsh).  MODE/RF = -1:
 // projection mode and R-factor chosen at run time.  TILED: the transformed
 // input (ops that start with the forward transform) is T1 in its column-tile
-// layout.
+// layout.  TILED_I: the packed measurement is in that layout too.
 // ---------------------------------------------------------------------------
-template <class T, int N, int OP, int MODE, int RF, bool TILED = false>
+template <class T, int N, int OP, int MODE, int RF, bool TILED = false, bool TILED_I = false>
 __device__ __forceinline__ void col_body(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out, int cols,
                                          const cplx<T>* __restrict__ tw, const T* __restrict__ I,
                                          const ProjArgs<T>& pa, T gsig, int tile, int lsh, unsigned char* smem) {
@@ -132,7 +127,8 @@ __device__ __forceinline__ void col_body(const cplx<T>* __restrict__ in, cplx<T>
   }
   if constexpr (proj) {
 #pragma unroll
-    for (int i = 0; i < E; i++) iv[i] = ok ? I[(size_t)L::dout_pos(i, t) * cols + col] : (T)NAN;
+    for (int i = 0; i < E; i++)
+      iv[i] = ok ? I[TILED_I ? t1_index(L::dout_pos(i, t), col, N) : (size_t)L::dout_pos(i, t) * cols + col] : (T)NAN;
   }
   // table twiddles of both transforms, loaded behind the data
   typename L::TwF twf;

@@ -94,9 +94,10 @@ __global__ void __launch_bounds__(kMaxThreads)
 k_col(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out, int cols, const cplx<T>* __restrict__ tw,
       const T* __restrict__ I, ProjArgs<T> pa, T gsig, int lsh) {
   extern __shared__ __align__(16) unsigned char smem[];
-  // the single-frame loop keeps T1 in its column-tile layout
+  // the single-frame loop keeps T1 and the packed measurement in the column-tile layout
   constexpr bool tiled = OP == COL_FWD_PROJ_INV || OP == COL_FWD_GAUSS_INV || OP == COL_FWD_T1;
-  col_body<T, N, OP, MODE, RF, tiled>(in, out, cols, tw, I, pa, gsig, xcd_tile(blockIdx.x, gridDim.x), lsh, smem);
+  col_body<T, N, OP, MODE, RF, tiled, true>(in, out, cols, tw, I, pa, gsig, xcd_tile(blockIdx.x, gridDim.x), lsh,
+                                            smem);
 }
 template <class T, int N, int OP, int ALGO>
 __global__ void __launch_bounds__(kMaxThreads)

@@ -19,7 +19,7 @@
 // order instead of scan order.
 //
 // A level is three launches over (tile, position) grids:
-//   kp_fwd   gather O_w * P, forward row FFT                   -> slot A
+//   kp_fwd   gather O_w * P, forward row FFT                   -> slot A (column tiles)
 //   kp_col   the CDI column pass body (col_body, COL_FWD_PROJ_INV) with the
 //            ptychographic projection                          A -> slot B
 //   kp_upd   inverse row FFT, 1/sqrt(N), U, rPIE, scatter into O
@@ -108,9 +108,11 @@ __global__ void __launch_bounds__(NT) kp_fwd(PtArgs a) {
   }
   const typename L::TwF twf = L::template tw_load<-1>(a.tw, t);
   L::forward(v, my, alt, ph, twf, t);
-  cplx<float>* out = a.slotA + (size_t)blockIdx.y * N * N + (size_t)row * N;
+  // slot A in column tiles (the layout of the CDI loop state): the column
+  // kernel reads each tile contiguously
+  cplx<float>* out = a.slotA + (size_t)blockIdx.y * N * N;
 #pragma unroll
-  for (int i = 0; i < E; i++) out[L::dout_pos(i, t)] = v[i];
+  for (int i = 0; i < E; i++) out[t1_index(row, L::dout_pos(i, t), N)] = v[i];
 }
 
 // column FFT, projection, inverse column FFT: slot A -> slot B
@@ -119,8 +121,9 @@ __global__ void __launch_bounds__(NT) kp_col(PtArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   const PtPos p = a.pos[blockIdx.y];
   const size_t so = (size_t)blockIdx.y * N * N;
-  col_body<float, N, COL_FWD_PROJ_INV, MODE, 0>(a.slotA + so, a.slotB + so, N, a.tw, a.Ipk + (size_t)p.scan * N * N,
-                                                a.pa, 0.f, blockIdx.x, plsh<N, NT>(), smem);
+  col_body<float, N, COL_FWD_PROJ_INV, MODE, 0, true>(a.slotA + so, a.slotB + so, N, a.tw,
+                                                      a.Ipk + (size_t)p.scan * N * N, a.pa, 0.f, blockIdx.x,
+                                                      plsh<N, NT>(), smem);
 }
 
 // forward column FFT only (forward model): slot A -> slot B
@@ -128,8 +131,8 @@ template <int N, int NT>
 __global__ void __launch_bounds__(NT) kp_colfwd(PtArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   const size_t so = (size_t)blockIdx.y * N * N;
-  col_body<float, N, COL_FWD, 0, 0>(a.slotA + so, a.slotB + so, N, a.tw, nullptr, a.pa, 0.f, blockIdx.x,
-                                    plsh<N, NT>(), smem);
+  col_body<float, N, COL_FWD, 0, 0, true>(a.slotA + so, a.slotB + so, N, a.tw, nullptr, a.pa, 0.f, blockIdx.x,
+                                          plsh<N, NT>(), smem);
 }
 
 // inverse row FFT of slot B, exit-wave difference, rPIE update, scatter

@@ -180,12 +180,13 @@ __device__ __forceinline__ void col_body(const cplx<T>* __restrict__ in, cplx<T>
 
   L::inverse(v, my, alt, ph, twi, t);
   if (ok) {
-    // With T1 and the measurement read as contiguous tiles, streaming stores
-    // of the output beat leaving its partial lines in L2 for the end-of-kernel
-    // write-back for columns up to 1024 points (per iteration: 768^2 21.5 ->
-    // 20.6 us, 1024^2 21.0 -> 20.1 us) but not for 2048 (57.3 -> 67.6 us);
-    // with row-order inputs they were 20% slower.
-    if constexpr (TILED && N <= 1024) {
+    // Streaming stores of the output beat leaving its partial lines in L2
+    // for the end-of-kernel write-back for columns up to 1024 points (single
+    // frame with tiled inputs, per iteration: 768^2 21.5 -> 20.6 us, 1024^2
+    // 21.0 -> 20.1 us; batched 16-column tiles: 85.5 -> 84.2 ms) but not for
+    // 2048 (57.3 -> 67.6 us).  Before the single-frame inputs were tiled they
+    // made that column pass 20% slower.
+    if constexpr (N <= 1024) {
 #pragma unroll
       for (int i = 0; i < E; i++) store_stream(&out[(size_t)L::din_pos(i, t) * cols + col], v[i]);
     } else {

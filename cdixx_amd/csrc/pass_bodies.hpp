@@ -131,8 +131,11 @@ __device__ __forceinline__ void col_body(const cplx<T>* __restrict__ in, cplx<T>
 #pragma unroll
     for (int i = 0; i < E; i++) v[i] = ok ? in[(size_t)L::dout_pos(i, t) * cols + col] : cplx<T>{(T)0, (T)0};
   } else if constexpr (TILED) {
+    // T1 is read once per iteration: streaming loads keep it from evicting
+    // what stays resident in L2 between iterations (g, S, the measurement)
 #pragma unroll
-    for (int i = 0; i < E; i++) v[i] = ok ? in[t1_index(L::din_pos(i, t), col, N)] : cplx<T>{(T)0, (T)0};
+    for (int i = 0; i < E; i++)
+      v[i] = ok ? load_stream(&in[t1_index(L::din_pos(i, t), col, N)]) : cplx<T>{(T)0, (T)0};
   } else {
 #pragma unroll
     for (int i = 0; i < E; i++) v[i] = ok ? in[(size_t)L::din_pos(i, t) * cols + col] : cplx<T>{(T)0, (T)0};
@@ -255,17 +258,21 @@ __device__ __forceinline__ void row_body(const cplx<T>* __restrict__ in, cplx<T>
     T sv[upd ? E : 1];
     cplx<T> g0[prtf ? E : 1];
     cplx<T>* gamma0 = (cplx<T>*)ua.gamma0;
-    // Streaming loads of T2, g and S in the single-frame update pass pay off
-    // for 1024-point rows only (per iteration: 1024^2 20.1 -> 19.0 us; 768^2
-    // +0.2-0.6 us, 2048^2 +3.6 us, smaller sizes unchanged).
+    // A workgroup id handles the same rows on the same XCD every iteration,
+    // so plain loads of g and S can hit in L2 when the data read once per
+    // iteration streams past them.  Streaming loads of T2 pay off for
+    // 1024-point rows only (per iteration at 1024^2: 20.1 us with g, S and T2
+    // all plain, 19.0 with all streamed, 17.9 with only T2 streamed and T1
+    // streamed in the column pass; at 2048^2 streaming T2 cost 3.6 us, at
+    // smaller sizes it changed nothing).
     constexpr bool ld_nt = TILED && upd && N == 1024;
     if constexpr (ld_nt) {
 #pragma unroll
       for (int i = 0; i < E; i++) v[i] = ok ? load_stream(&in[base + L::dout_pos(i, t)]) : cplx<T>{(T)0, (T)0};
 #pragma unroll
       for (int i = 0; i < E; i++) {
-        gv[i] = ok ? load_stream(&g[base + L::din_pos(i, t)]) : cplx<T>{(T)0, (T)0};
-        sv[i] = ok ? load_stream(&spt[base + L::din_pos(i, t)]) : (T)0;
+        gv[i] = ok ? g[base + L::din_pos(i, t)] : cplx<T>{(T)0, (T)0};
+        sv[i] = ok ? spt[base + L::din_pos(i, t)] : (T)0;
       }
     } else {
 #pragma unroll

@@ -16,6 +16,12 @@
 // parameters (no per-pixel branching on them), and every operand a pass
 // reads besides the line itself (I; g and S) is loaded at the start so its
 // latency overlaps the first transform.
+//
+// In the single-frame loop (TILED) T1 and the packed measurement are kept in
+// column tiles of kT1Tile columns (t1_index), so the column pass reads
+// contiguous blocks; T1 and T2 are moved with streaming (non-temporal) loads
+// and stores, while g and S are loaded plainly: a workgroup id handles the
+// same rows on the same XCD every iteration, so they can stay in its L2.
 #pragma once
 #include "device_ops.hpp"
 

@@ -11,30 +11,37 @@ namespace cdx {
 // ---------------------------------------------------------------------------
 // line configurations
 // ---------------------------------------------------------------------------
-// lanes per line (TL) and Stockham radices (forward order) per line length;
-// every radix divides the E = N/TL points a lane holds.
+// lanes per line (TL), exchange-buffer padding (PAD, one element per 2^PAD;
+// 0: none) and Stockham radices (forward order) per line length; every radix
+// divides the E = N/TL points a lane holds.  PAD is measured per length: the
+// 384- and 768-point lines (last radix 6 / 12) run faster unpadded
+// (per iteration: 384^2 17.06 -> 15.98 us, 768^2 20.59 -> 19.89 us), 2048
+// needs the padding (57.3 us padded, 67.1 unpadded), 1024 is indifferent
+// (padding every 8, 16 or 32 elements or an XOR swizzle: 17.8-17.9 us;
+// unpadded 19.6).
 template <int N>
 struct LineCfg;
-#define CDX_LINE(n, tl, ...)                 \
+#define CDX_LINE(n, tl, pad, ...)            \
   template <>                                \
   struct LineCfg<n> {                        \
     static constexpr int TL = tl;            \
+    static constexpr int PAD = pad;          \
     using RL = Radices<__VA_ARGS__>;         \
   };
-CDX_LINE(16, 4, 4, 4)
-CDX_LINE(32, 4, 8, 4)
-CDX_LINE(64, 8, 8, 8)
-CDX_LINE(128, 16, 8, 8, 2)
-CDX_LINE(256, 16, 16, 16)
-CDX_LINE(384, 16, 8, 8, 6)
-CDX_LINE(512, 32, 16, 16, 2)
-CDX_LINE(768, 32, 8, 8, 12)
-CDX_LINE(1024, 128, 8, 8, 8, 2)
-CDX_LINE(2048, 128, 16, 16, 8)
+CDX_LINE(16, 4, 5, 4, 4)
+CDX_LINE(32, 4, 5, 8, 4)
+CDX_LINE(64, 8, 5, 8, 8)
+CDX_LINE(128, 16, 5, 8, 8, 2)
+CDX_LINE(256, 16, 5, 16, 16)
+CDX_LINE(384, 16, 0, 8, 8, 6)
+CDX_LINE(512, 32, 5, 16, 16, 2)
+CDX_LINE(768, 32, 0, 8, 8, 12)
+CDX_LINE(1024, 128, 5, 8, 8, 8, 2)
+CDX_LINE(2048, 128, 5, 16, 16, 8)
 #undef CDX_LINE
 
 template <class T, int N>
-using LineOf = Line<T, N, LineCfg<N>::TL, typename LineCfg<N>::RL>;
+using LineOf = Line<T, N, LineCfg<N>::TL, typename LineCfg<N>::RL, LineCfg<N>::PAD>;
 
 // LDS layout of a workgroup of 2^lsh lines: two exchange buffers (the
 // transforms alternate between them, one barrier per exchange); buffer b of

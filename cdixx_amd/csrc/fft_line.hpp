@@ -164,17 +164,21 @@ struct Radices {
   static constexpr int R[M] = {Rs...};
 };
 
-template <class T, int N, int TL, class RL>
+// PAD: one pad element per 2^PAD elements of the exchange buffer (0: none),
+// which keeps strided pass outputs off a single bank
+template <class T, int N, int TL, class RL, int PAD = 5>
 struct Line {
   static constexpr int M = RL::M;
   static constexpr int E = N / TL;
   static constexpr int R0 = RL::R[0];
   static constexpr int RLAST = RL::R[M - 1];
-  // one pad element per 32 keeps strided pass outputs off a single bank
-  static constexpr int LDS_ELEMS = N + N / 32 + 1;
+  static constexpr int LDS_ELEMS = PAD ? N + (N >> PAD) + 1 : N + 1;
   static_assert(TL * E == N, "TL must divide N");
 
-  __device__ __forceinline__ static int swz(int p) { return p + (p >> 5); }
+  __device__ __forceinline__ static int swz(int p) {
+    if constexpr (PAD > 0) return p + (p >> PAD);
+    else return p;
+  }
 
   // radix of pass p in direction DIR (forward runs the list, inverse reverses it)
   template <int DIR>

@@ -57,17 +57,10 @@ def _iters_to(trace, target, every):
     return None
 
 
-def evaluate(cdx, net, nframes=64, seed0=HELDOUT_SEED0, photons=PHOTONS, blocks=60, beta=0.9, target=0.05,
-             support_threshold=0.5, shrink_threshold=0.02, arms=("random", "learned")):
-    """Iterations to R <= target (and to R <= max(target, 1.1 x the true
-    object's R, the frame's noise floor)) and final R per start on `nframes`
-    held-out frames.  Arms: "random" (random phase, rectangular support), "learned"
-    (warm start), "learned_support" (random phase, predicted support),
-    "true_support" (random phase, the true support: a bound on what a
-    support estimate alone can give).  Also reported: the R-factor of the
-    true object (the noise floor of each frame), the overlap of the predicted
-    support with the true one, and per frame the iterations each arm needs to
-    reach the random start's final R.  Returns (per-frame dict, summary)."""
+def _eval_chunk(cdx, net, nframes, seed0, photons, blocks, beta, support_threshold, shrink_threshold, arms):
+    """one batched run of every arm on frames seed0 .. seed0 + nframes - 1:
+    per-arm R-factor traces (every projection), the frames' noise floors and
+    the predicted supports' IoU"""
     n = net.size
     b = cdx.Batch([n] * nframes)
     truth, r_true = [], []
@@ -89,7 +82,7 @@ def evaluate(cdx, net, nframes=64, seed0=HELDOUT_SEED0, photons=PHOTONS, blocks=
     alg = f"{blocks // 2}*(9*HIO+1*ER+9*HIO+1*ER+1*shrinkWrap)"
     nproj = (blocks // 2) * 20
     p = cdx.Params.default(beta=beta, relax=1.0, recordR=1, shrinkThreshold=shrink_threshold)
-    out, iou = {}, []
+    traces, seconds, iou = {}, {}, []
     for arm in arms:
         for i in range(nframes):
             f = b.frame(i)
@@ -113,12 +106,42 @@ def evaluate(cdx, net, nframes=64, seed0=HELDOUT_SEED0, photons=PHOTONS, blocks=
         t0 = time.perf_counter()
         _, tr = b.phase_retrieve(alg, p, rtrace_cap=nproj)
         b.sync()
-        el = time.perf_counter() - t0
-        floor = [max(target, 1.1 * r_true[i]) for i in range(nframes)]
+        seconds[arm] = time.perf_counter() - t0
+        traces[arm] = tr
+    b.close()
+    return traces, seconds, r_true, iou, nproj
+
+
+def evaluate(cdx, net, nframes=64, seed0=HELDOUT_SEED0, photons=PHOTONS, blocks=60, beta=0.9, target=0.05,
+             support_threshold=0.5, shrink_threshold=0.02, arms=("random", "learned"), chunk=256):
+    """Iterations to R <= target (and to R <= max(target, 1.1 x the true
+    object's R, the frame's noise floor)) and final R per start on `nframes`
+    held-out frames (batched runs of at most `chunk` frames).  Arms: "random"
+    (random phase, rectangular support), "learned" (warm start),
+    "learned_support" (random phase, predicted support), "true_support"
+    (random phase, the true support: a bound on what a support estimate
+    alone can give).  Also reported: the R-factor of the true object (the
+    noise floor of each frame), the overlap of the predicted support with the
+    true one, and per frame the iterations each arm needs to reach the random
+    start's final R.  Returns (per-frame dict, summary)."""
+    traces = {arm: [] for arm in arms}
+    seconds = {arm: 0.0 for arm in arms}
+    r_true, iou, nproj = [], [], 0
+    for k in range(0, nframes, chunk):
+        tr, sec, rt, io, nproj = _eval_chunk(cdx, net, min(chunk, nframes - k), seed0 + k, photons, blocks, beta,
+                                             support_threshold, shrink_threshold, arms)
+        for arm in arms:
+            traces[arm].append(tr[arm])
+            seconds[arm] += sec[arm]
+        r_true += rt
+        iou += io
+    out = {}
+    floor = [max(target, 1.1 * r) for r in r_true]
+    for arm in arms:
+        tr = np.concatenate(traces[arm], axis=0)
         out[arm] = dict(trace=tr[:, ::10], iters=[_iters_to(tr[i], target, 10) for i in range(nframes)],
                         iters_floor=[_iters_to(tr[i], floor[i], 10) for i in range(nframes)],
-                        final_R=np.array([tr[i][nproj - 10] for i in range(nframes)]), seconds=el)
-    b.close()
+                        final_R=np.array([tr[i][nproj - 10] for i in range(nframes)]), seconds=seconds[arm])
     summary = {"frames": nframes, "iterations": nproj, "target_R": target, "photons": list(photons),
                "median_R_true_object": float(np.median(r_true)),
                "frames_with_R_true_below_target": int(np.sum(np.array(r_true) <= target))}

@@ -35,7 +35,7 @@ __host__ __device__ constexpr int bsize_index(int n) {
 template <class T>
 constexpr int bthreads() { return sizeof(T) == 8 ? kBatchThreads / 2 : kBatchThreads; }
 template <class T, int N>
-constexpr int blines() { return bthreads<T>() / LineCfg<N>::TL; }
+constexpr int blines() { return bthreads<T>() / LineCfgT<T, N>::TL; }
 template <class T, int N>
 constexpr int blsh() {
   int s = 0;

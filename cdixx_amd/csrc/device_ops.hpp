@@ -40,8 +40,40 @@ CDX_LINE(1024, 128, 5, 8, 8, 8, 2)
 CDX_LINE(2048, 128, 5, 16, 16, 8)
 #undef CDX_LINE
 
+// Per precision: FP64 lines holding 16 or 24 complex doubles per lane (512
+// and 2048 on 32 and 128 lanes; 384 and 768 on 16 and 32) exceed, with the
+// row pass's g, S and twiddles, the 256 VGPRs of a wave (up to 462 spilled);
+// they run on more lanes with 8 or 12 points each.  The 256-point FP64 line
+// keeps 16 points per lane (66 spilled VGPRs in the update pass).
 template <class T, int N>
-using LineOf = Line<T, N, LineCfg<N>::TL, typename LineCfg<N>::RL, LineCfg<N>::PAD>;
+struct LineCfgT : LineCfg<N> {};
+template <>
+struct LineCfgT<double, 512> {
+  static constexpr int TL = 64;
+  static constexpr int PAD = 5;
+  using RL = Radices<8, 8, 8>;
+};
+template <>
+struct LineCfgT<double, 384> {
+  static constexpr int TL = 32;
+  static constexpr int PAD = 0;
+  using RL = Radices<4, 4, 4, 6>;
+};
+template <>
+struct LineCfgT<double, 768> {
+  static constexpr int TL = 64;
+  static constexpr int PAD = 0;
+  using RL = Radices<4, 4, 4, 12>;
+};
+template <>
+struct LineCfgT<double, 2048> {
+  static constexpr int TL = 256;
+  static constexpr int PAD = 5;
+  using RL = Radices<8, 8, 8, 4>;
+};
+
+template <class T, int N>
+using LineOf = Line<T, N, LineCfgT<T, N>::TL, typename LineCfgT<T, N>::RL, LineCfgT<T, N>::PAD>;
 
 // LDS layout of a workgroup of 2^lsh lines: two exchange buffers (the
 // transforms alternate between them, one barrier per exchange); buffer b of

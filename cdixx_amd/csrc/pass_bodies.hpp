@@ -233,7 +233,7 @@ __device__ __forceinline__ void row_body(const cplx<T>* __restrict__ in, cplx<T>
                                          T* __restrict__ sout, const ReduceSlots& rs, int tile, int lsh,
                                          unsigned char* smem) {
   using L = LineOf<T, N>;
-  constexpr int TL = LineCfg<N>::TL;
+  constexpr int TL = LineCfgT<T, N>::TL;
   const int pitch = lds_pitch<T, N>(lsh);
   const int alt = pitch << lsh;
   int ph = 0;

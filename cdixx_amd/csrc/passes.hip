@@ -40,9 +40,9 @@ constexpr size_t kMaxLds = 160 * 1024;
 
 #define CDX_SIZES(F) F(16) F(32) F(64) F(128) F(256) F(384) F(512) F(768) F(1024) F(2048)
 
-static int line_lanes(int n) {
+static int line_lanes(int n, size_t csz) {
   switch (n) {
-#define F(n_) case n_: return LineCfg<n_>::TL;
+#define F(n_) case n_: return csz == 8 ? LineCfgT<float, n_>::TL : LineCfgT<double, n_>::TL;
     CDX_SIZES(F)
 #undef F
     default: return 0;
@@ -65,17 +65,17 @@ static size_t wg_lds_bytes(int n, size_t csz, int lines) {
 }
 
 #ifdef CDX_PASS_HOST
-int line_supported(int n) { return line_lanes(n) != 0; }
+int line_supported(int n) { return line_lanes(n, 8) != 0; }
 
 int tiling_valid(int n, size_t csz, int lines) {
-  const int tl = line_lanes(n);
+  const int tl = line_lanes(n, csz);
   if (tl == 0 || lines <= 0 || (lines & (lines - 1))) return 0;
   const int threads = lines * tl;
   if (threads < 64 || threads > kMaxThreads) return 0;
   return wg_lds_bytes(n, csz, lines) <= kMaxLds;
 }
 int tiling_default(int n, size_t csz, int column_pass) {
-  const int tl = line_lanes(n);
+  const int tl = line_lanes(n, csz);
   if (!tl) return 0;
   // columns: 64-byte row segments; rows: 256 threads
   int lines = column_pass ? (int)(64 / csz) : 256 / tl;
@@ -128,7 +128,7 @@ template <class T, int N, int OP, int MODE, int RF>
 static hipError_t col_launch_n(int cols, const void* in, void* out, const void* tw, const T* I,
                                const ProjArgs<T>& pa, double gs, hipStream_t st, int lines, const KTimer* kt) {
   static size_t lds_done = 0;
-  constexpr int TL = LineCfg<N>::TL;
+  constexpr int TL = LineCfgT<T, N>::TL;
   if (!tiling_valid(N, sizeof(cplx<T>), lines)) return hipErrorInvalidConfiguration;
   const size_t lds = lds_bytes<T, N>(log2i(lines));
   auto kern = k_col<T, N, OP, MODE, RF>;
@@ -191,7 +191,7 @@ static hipError_t row_launch_n(int rows, const void* in, void* out, const void* 
                                const UpdArgs<T>& ua, T scale, T* sout, const ReduceSlots& rs, hipStream_t st,
                                int lines, const KTimer* kt) {
   static size_t lds_done = 0;
-  constexpr int TL = LineCfg<N>::TL;
+  constexpr int TL = LineCfgT<T, N>::TL;
   if (!tiling_valid(N, sizeof(cplx<T>), lines)) return hipErrorInvalidConfiguration;
   const size_t lds = lds_bytes<T, N>(log2i(lines));
   auto kern = k_row<T, N, OP, ALGO>;

@@ -262,7 +262,6 @@ __device__ __forceinline__ void row_body(const cplx<T>* __restrict__ in, cplx<T>
   } else {
     cplx<T> gv[upd ? E : 1];
     T sv[upd ? E : 1];
-    cplx<T> g0[prtf ? E : 1];
     cplx<T>* gamma0 = (cplx<T>*)ua.gamma0;
     // A workgroup id handles the same rows on the same XCD every iteration,
     // so plain loads of g and S can hit in L2 when the data read once per
@@ -289,12 +288,6 @@ __device__ __forceinline__ void row_body(const cplx<T>* __restrict__ in, cplx<T>
           gv[i] = ok ? g[base + L::din_pos(i, t)] : cplx<T>{(T)0, (T)0};
           sv[i] = ok ? spt[base + L::din_pos(i, t)] : (T)0;
         }
-      }
-    }
-    if constexpr (prtf) {
-      if (ua.prtf == 2) {
-#pragma unroll
-        for (int i = 0; i < E; i++) g0[i] = ok ? gamma0[base + L::din_pos(i, t)] : cplx<T>{(T)0, (T)0};
       }
     }
     const typename L::TwI twi = L::template tw_load<+1>(tw, t);
@@ -338,9 +331,11 @@ __device__ __forceinline__ void row_body(const cplx<T>* __restrict__ in, cplx<T>
             if (ok) gamma0[base + L::din_pos(i, t)] = q;
             are += (double)q.x * q.x - (double)q.y * q.y;
             aim += 2.0 * q.x * q.y;
-          } else if (in) {  // sum P_S(g') conj(gamma0)
-            are += (double)p.x * g0[i].x + (double)p.y * g0[i].y;
-            aim += (double)p.y * g0[i].x - (double)p.x * g0[i].y;
+          } else if (in && ok) {  // sum P_S(g') conj(gamma0); loaded here, not preloaded,
+                                  // to keep the pass's registers within a wave's budget
+            const cplx<T> g0 = gamma0[base + L::din_pos(i, t)];
+            are += (double)p.x * g0.x + (double)p.y * g0.y;
+            aim += (double)p.y * g0.x - (double)p.x * g0.y;
           }
         }
         v[i] = support_update(gv[i], p, in, ALGO >= 0 ? ALGO : ua.algo, ua.beta, ua.relax);

@@ -16,8 +16,8 @@ projection), so the R-factor recorded after each ER step is the R-factor of
 the support-constrained estimate; shrink-wrap every other block.
 
 Command line (on the GPU):
-  python -m cdixx_amd.warmstart train --steps 3000 --out cdixx_amd/models/unet256.bin
-  python -m cdixx_amd.warmstart eval --weights cdixx_amd/models/unet256.bin --frames 256
+  python -m cdixx_amd.warmstart train --steps 30000 --photons 1e5 1e10 --out cdixx_amd/models/unet256.bin
+  python -m cdixx_amd.warmstart eval --weights cdixx_amd/models/unet256.bin --frames 1024 --blocks 100
 """
 import argparse
 import json

@@ -3,11 +3,12 @@
 The network (cdx_unet_*, BF16 MFMA kernels in csrc/unet.hip) is trained on
 the GPU on pairs from the forward model: random ellipse / polygon supports of
 radius 20-60 px, smoothed-noise amplitude in [0.2, 1] and phase in [-pi, pi],
-photons log-uniform in [1e5, 1e8], Poisson intensities.  Training seeds start
+photons log-uniform in a range (PHOTONS, 1e5-1e8, by default; the shipped
+weights were trained on 1e5-1e10), Poisson intensities.  Training seeds start
 at 1,000,000, held-out seeds at 2,000,000.
 
 Evaluation runs the same HIO schedule from two starts on held-out frames,
-all frames advancing together in one batched reconstruction:
+the frames advancing together in batched reconstructions:
   random  - uniform random phase on sqrt(I), the reference's rectangular
             support (createSupport, oversampling 2, src/post_gpu/cdi.cc:240-251)
   learned - cdx_unet_warm_start: predicted object and support

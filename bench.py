@@ -265,6 +265,104 @@ def learned_leg(cdx):
     return out
 
 
+# arms of the 2048^2 learned warm start: (phases from the network, support:
+# "disk" = the 0.2 N disk of the random start or "learned", dilation of the
+# learned support in network pixels).  The network's phases help at this
+# scale; its support, upsampled 8x, cuts the object (the object's 3 px
+# texture averages out at 8x coarser pixels, unlike the training objects')
+# and is reported as a contrast.
+LARGE_ARMS = {"random": (False, "disk", 0), "learned": (True, "disk", 0), "learned_support": (True, "learned", 2)}
+
+
+def learned_large_leg(cdx, seeds=(0, 1, 2, 3), arms=LARGE_ARMS):
+    """Learned warm start at 2048^2 (object radius 320 px, 1e9 photons,
+    beamstop r = 16): the network sees the central 256^2 frequencies (the
+    object at 8x coarser pixels, radius 40 px) and seeds those phases (and,
+    in the learned_support arm, the support); against random starts with the
+    0.2 N disk support.  Schedule:
+    100 x (19 RAAR + 1 ER + shrink-wrap), beta 0.85: R is read after every
+    ER step (the support-projected object).  The learned start of the first
+    seed is also run in FP64 for the FP32 final-image error."""
+    from cdixx_amd.warmstart import DEFAULT_WEIGHTS, LEVELS, BASE, SIZE, _iters_to
+    n, every, target = 2048, 20, 0.05
+    alg = "100*(19*RAAR+1*ER+1*shrinkWrap)"
+    net = cdx.UNet(SIZE, LEVELS, BASE, max_batch=1, seed=1)
+    net.load(DEFAULT_WEIGHTS)
+    cs = cdx.Context(SIZE, precision=cdx.F32)
+    ctx = cdx.Context(n, precision=cdx.F32)
+    p = cdx.Params.default(recordR=1, **E2E_PARAMS)
+    runs = {a: [] for a in arms}
+    r_true, fp64 = [], None
+    for seed in seeds:
+        obj, I, bs, Ft = ctx.cplx(), ctx.real(), ctx.real(), ctx.cplx()
+        ctx.sim_object(obj, 320, seed=seed)
+        ctx.sim_intensity(obj, I, 1e9, seed=seed, poisson=True)
+        ctx.create_beamstop(bs, 16)
+        Ih, bsh = ctx.get_real(I), ctx.get_real(bs)
+        # noise floor: R of the true object's transform scaled to the measured total
+        ctx.fft(obj, Ft, cdx.FORWARD)
+        Fth = ctx.get_cplx(Ft).astype(np.complex128)
+        Fth *= np.sqrt(Ih[bsh < 0.5].sum() / (np.abs(Fth[bsh < 0.5]) ** 2).sum())
+        r_true.append(_rfactor_host(Fth, Ih, bsh))
+        for arm, (phases, support, dil) in arms.items():
+            F, g, spt = ctx.cplx(), ctx.cplx(), ctx.real()
+            if phases or support == "learned":
+                net.warm_start_scaled(cs, ctx, I, bs, F, g, spt, threshold=0.5, dilate=dil, seed=seed)
+            if not phases:
+                ctx.random_start(I, bs, F, seed=seed, mode=1)
+                ctx.fft(F, g, cdx.INVERSE)
+                ctx.apply_norm(g, 1.0 / n)
+            if support == "disk":
+                ctx.create_circle(spt, 0.2 * n)
+            start = None
+            if arm == "learned" and seed == seeds[0]:
+                start = dict(F=ctx.get_cplx(F), g=ctx.get_cplx(g), spt=ctx.get_real(spt))
+            ctx.set_threshold(0.5)
+            ctx.sync()
+            t0 = time.perf_counter()
+            nit, rt = ctx.phase_retrieve(alg, p, F, g, I, spt, bs=bs, rtrace_cap=2000)
+            ctx.sync()
+            sec = time.perf_counter() - t0
+            gfin = ctx.get_cplx(g).astype(np.complex128) / n
+            r_fin = _rfactor_host(np.fft.fft2(gfin), Ih, bsh)  # g after the last ER step is support-projected
+            runs[arm].append(dict(iters_to_target=_iters_to(rt, target, every), final_R=r_fin,
+                                  R_at_200=float(rt[200]), seconds=sec))
+            if start is not None:
+                c64 = cdx.Context(n, precision=cdx.F64)
+                c64.set_threshold(0.5)
+                dF, dg, dI = c64.cplx(start["F"]), c64.cplx(start["g"]), c64.real(Ih)
+                ds, dbs = c64.real(start["spt"]), c64.real(bsh)
+                c64.phase_retrieve(alg, cdx.Params.default(**E2E_PARAMS), dF, dg, dI, ds, bs=dbs)
+                g64 = c64.get_cplx(dg) / n
+                c64.close()
+                a, b = gfin, g64
+                ph = np.vdot(a.ravel(), b.ravel())
+                fp64 = {"seed": seed, "R_fp32": round(r_fin, 5),
+                        "R_fp64": round(_rfactor_host(np.fft.fft2(b), Ih, bsh), 5),
+                        "fp32_vs_fp64_rel_l2": float(np.linalg.norm(a * (ph / abs(ph)) - b) / np.linalg.norm(b))
+                        if abs(ph) > 0 else None}
+            for buf in (F, g, spt):
+                buf.free()
+        for buf in (obj, I, bs, Ft):
+            buf.free()
+    ctx.close()
+    cs.close()
+    net.close()
+    out = {"frame": [n, n], "object_radius": 320, "photons": 1e9, "beamstop": 16, "schedule": alg, "beta": 0.85,
+           "network_input": "central 256x256 frequencies (object at 8x coarser pixels, radius 40 px)",
+           "seeds": list(seeds), "target_R": target, "R_true_object": [round(r, 5) for r in r_true]}
+    for arm, rs in runs.items():
+        phases, support, dil = arms[arm]
+        out[arm] = {"phases": "network" if phases else "random", "support": support, "dilate": dil,
+                    "iters_to_target": [r["iters_to_target"] for r in rs],
+                    "final_R": [round(r["final_R"], 5) for r in rs],
+                    "R_after_200": [round(r["R_at_200"], 5) for r in rs],
+                    "seconds": [round(r["seconds"], 4) for r in rs]}
+    if fp64 is not None:
+        out["fp32_vs_fp64_learned_start"] = fp64
+    return out
+
+
 def slab_leg(cdx, iters=200):
     """One GPU's share of a 2048x2048 frame split over 8 GPUs in row slabs
     (the sharding described for frames larger than one GPU): the row pass on
@@ -341,6 +439,7 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start leg")
     ap.add_argument("--no-ptycho", action="store_true", help="skip the ptychography leg")
     ap.add_argument("--ptycho-only", action="store_true", help="run only the ptychography leg (profiling)")
+    ap.add_argument("--learned-large-only", action="store_true", help="run only the 2048^2 learned warm-start leg")
     ap.add_argument("--slab-only", action="store_true", help="run only the one-GPU slab-share leg")
     ap.add_argument("--batch-iters", type=int, default=500)
     ap.add_argument("--copy-calib-mb", type=int, default=0,
@@ -354,6 +453,9 @@ def main():
     import cdixx_amd as cdx
     if args.ptycho_only:
         print(json.dumps({"ptycho": ptycho_leg(cdx)}), flush=True)
+        return
+    if args.learned_large_only:
+        print(json.dumps({"learned_warm_start_2048": learned_large_leg(cdx)}), flush=True)
         return
     if args.slab_only:
         print(json.dumps({"slab_share_2048_over_8": slab_leg(cdx)}), flush=True)
@@ -439,6 +541,7 @@ def main():
         out["fp64_report"] = fp64_report(cdx)
     if not args.no_learned:
         out["learned_warm_start"] = learned_leg(cdx)
+        out["learned_warm_start_2048"] = learned_large_leg(cdx)
     if not args.no_ptycho:
         out["ptycho"] = ptycho_leg(cdx)
     out["slab_share_2048_over_8"] = slab_leg(cdx)

@@ -124,6 +124,7 @@ _SIGS = {
     "cdx_unet_synth_frame": ([_vp, _u64, _d, _d, _vp, _vp], _i),
     "cdx_unet_train": ([_vp, _i, _i, _u64, _d, _d, _d, _dp], _i),
     "cdx_unet_warm_start": ([_vp, _vp, _vp, _d, _vp, _vp, _vp], _i),
+    "cdx_unet_warm_start_scaled": ([_vp, _vp, _vp, _vp, _vp, _d, _i, _u64, _vp, _vp, _vp], _i),
     "cdx_unet_save": ([_vp, ctypes.c_char_p], _i), "cdx_unet_load": ([_vp, ctypes.c_char_p], _i),
     "cdx_ptycho_create": ([ctypes.POINTER(_vp), _i, _i, _i, _i, _ip, _ip, _vp], _i),
     "cdx_ptycho_destroy": ([_vp], _i), "cdx_ptycho_set_positions": ([_vp, _ip, _ip], _i),
@@ -522,6 +523,14 @@ class UNet:
 
     def warm_start(self, ctx, I, F, g, spt, threshold=0.5):
         check(lib.cdx_unet_warm_start(self.h, ctx.h, I.ptr, threshold, F.ptr, g.ptr, spt.ptr))
+
+    def warm_start_scaled(self, ctx_small, ctx, I, bs, F, g, spt, threshold=0.5, dilate=0, seed=0):
+        """Warm start of a frame f times larger than the network (per side):
+        predicted phases on the central frequencies, random phases elsewhere,
+        support dilated by `dilate` network pixels and upsampled about the
+        centre (cdx_unet_warm_start_scaled)."""
+        check(lib.cdx_unet_warm_start_scaled(self.h, ctx_small.h, ctx.h, I.ptr, bs.ptr if bs else None, threshold,
+                                              int(dilate), seed, F.ptr, g.ptr, spt.ptr))
 
     def save(self, path):
         check(lib.cdx_unet_save(self.h, path.encode()))

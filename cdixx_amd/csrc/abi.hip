@@ -768,33 +768,51 @@ static int time_t_(cdx_ctx* c, const cdx_params* p, int algo, void* F, void* g, 
   if (exec) hipGraphExecDestroy(exec);
   if (graph) hipGraphDestroy(graph);
   if (col_ms || row_ms) {
-    // kernel execution times (start/end timestamps of each launch, as a
-    // kernel-trace profiler reports them) on a bounded sample of the same
-    // iterations, launched back to back
+    // average launch duration of each pass: up to 200 launches of that pass
+    // alone, back to back (one captured graph when use_graph, as the loop
+    // runs), between two events.  Every column-pass launch reads the same T1
+    // and every row-pass launch the same T2, so each moves the bytes of its
+    // pass in the loop.
     n = n < 200 ? n : 200;
-    std::vector<hipEvent_t> ev(4 * (size_t)n);
-    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
     UpdArgs<T> ua = upd_args<T>(c, algo, p->beta, p->relax);
     L.pa.rf = rf;
-    for (int i = 0; i < n; i++) {
-      const KTimer kc{ev[4 * i], ev[4 * i + 1]}, kr{ev[4 * i + 2], ev[4 * i + 3]};
-      HIPCHK(launch_col<T>(COL_FWD_PROJ_INV, c->rows, c->cols, c->T1, c->T2, c->tw_col, (const T*)c->Ipk, L.pa, 0,
-                           c->st, c->col_lines, &kc));
-      HIPCHK(launch_row<T>(ROW_INV_UPDATE_FWD, c->rows, c->cols, c->T2, c->T1, c->tw_row, g, (const T*)spt, ua, (T)1,
-                           nullptr, ReduceSlots{}, c->st, c->row_lines, &kr));
+    for (int pass = 0; pass < 2; pass++) {
+      double* dst = pass == 0 ? col_ms : row_ms;
+      if (!dst) continue;
+      auto launch = [&]() -> hipError_t {
+        if (pass == 0)
+          return launch_col<T>(COL_FWD_PROJ_INV, c->rows, c->cols, c->T1, c->T2, c->tw_col, (const T*)c->Ipk, L.pa,
+                               0, c->st, c->col_lines);
+        return launch_row<T>(ROW_INV_UPDATE_FWD, c->rows, c->cols, c->T2, c->T1, c->tw_row, g, (const T*)spt, ua,
+                             (T)1, nullptr, ReduceSlots{}, c->st, c->row_lines);
+      };
+      hipGraph_t pg = nullptr;
+      hipGraphExec_t px = nullptr;
+      if (use_graph) {
+        HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+        hipError_t e = hipSuccess;
+        for (int i = 0; i < n && e == hipSuccess; i++) e = launch();
+        hipError_t e2 = hipStreamEndCapture(c->st, &pg);
+        HIPCHK(e);
+        HIPCHK(e2);
+        HIPCHK(hipGraphInstantiate(&px, pg, nullptr, nullptr, 0));
+        HIPCHK(hipGraphLaunch(px, c->st));  // warm the graph once
+      }
+      HIPCHK(hipStreamSynchronize(c->st));
+      HIPCHK(hipEventRecord(e0, c->st));
+      if (use_graph) {
+        HIPCHK(hipGraphLaunch(px, c->st));
+      } else {
+        for (int i = 0; i < n; i++) HIPCHK(launch());
+      }
+      HIPCHK(hipEventRecord(e1, c->st));
+      HIPCHK(hipEventSynchronize(e1));
+      float pms = 0;
+      HIPCHK(hipEventElapsedTime(&pms, e0, e1));
+      *dst = pms / n;
+      if (px) hipGraphExecDestroy(px);
+      if (pg) hipGraphDestroy(pg);
     }
-    HIPCHK(hipStreamSynchronize(c->st));
-    double sc = 0, sr = 0;
-    for (int i = 0; i < n; i++) {
-      float a, b;
-      HIPCHK(hipEventElapsedTime(&a, ev[4 * i], ev[4 * i + 1]));
-      HIPCHK(hipEventElapsedTime(&b, ev[4 * i + 2], ev[4 * i + 3]));
-      sc += a;
-      sr += b;
-    }
-    for (auto& e : ev) hipEventDestroy(e);
-    if (col_ms) *col_ms = sc / n;
-    if (row_ms) *row_ms = sr / n;
   }
   // leave F consistent with the state
   if ((rc = L.finish())) return rc;

@@ -1041,6 +1041,113 @@ extern "C" int cdx_unet_warm_start(cdx_unet* u, cdx_ctx* ctx, const void* I, dou
   return CDX_OK;
 }
 
+// ---------------------------------------------------------------------------
+// warm start of a frame larger than the network: the network sees the
+// pattern's central ns x ns frequencies (FFT order), i.e. the object at
+// f = nl/ns times coarser pixels over the same field of view; its phases seed
+// those frequencies of the large frame, the rest keep a random start, and
+// its support is upsampled (nearest) about the centre.
+// ---------------------------------------------------------------------------
+// signed frequency of FFT-order index u on an n-point axis, wrapped onto m points
+__device__ __forceinline__ int band_index(int u, int n, int m) { return u < n / 2 ? u : u - n + m; }
+
+__global__ void k_crop_band(int nl, int ns, const float* __restrict__ Il, float* __restrict__ Is) {
+  const int64_t N = (int64_t)ns * ns;
+  EW_LOOP(N) {
+    const int u = (int)(i / ns), v = (int)(i % ns);
+    Is[i] = Il[(size_t)band_index(u, ns, nl) * nl + band_index(v, ns, nl)];
+  }
+}
+// F_l (random start: sqrt(I) e^{i phi}, 0 in the beamstop) takes the phase of
+// F_s at the band's frequencies where F_s != 0; its modulus is kept
+__global__ void k_band_phase(int nl, int ns, const cplx<float>* __restrict__ Fs, cplx<float>* __restrict__ Fl) {
+  const int64_t N = (int64_t)ns * ns;
+  EW_LOOP(N) {
+    const int u = (int)(i / ns), v = (int)(i % ns);
+    const cplx<float> s = Fs[i];
+    const float m2 = s.x * s.x + s.y * s.y;
+    if (!(m2 > 0.f)) continue;
+    cplx<float>* d = Fl + (size_t)band_index(u, ns, nl) * nl + band_index(v, ns, nl);
+    const cplx<float> r = *d;
+    const float a = sqrtf(r.x * r.x + r.y * r.y) * rsqrtf(m2);
+    *d = {s.x * a, s.y * a};
+  }
+}
+// centred nearest upsampling by f = nl/ns: (X, Y) -> (x, y) = (X/f, Y/f)
+// keeps the centre nl/2 on ns/2; with dil > 0 the small support is first
+// dilated by a disk of radius dil (small-frame pixels: (x, y) is in when some
+// (x + dx, y + dy) with dx^2 + dy^2 <= dil^2 inside the frame is)
+__global__ void k_upsample_support(int nl, int ns, int dil, const float* __restrict__ ss, float* __restrict__ sl) {
+  const int64_t N = (int64_t)nl * nl;
+  const int f = nl / ns;
+  EW_LOOP(N) {
+    const int x = (int)(i / nl) / f, y = (int)(i % nl) / f;
+    float v = ss[(size_t)x * ns + y];
+    for (int dx = -dil; dx <= dil && v == 0.f; dx++)
+      for (int dy = -dil; dy <= dil; dy++) {
+        const int xx = x + dx, yy = y + dy;
+        if (dx * dx + dy * dy > dil * dil || xx < 0 || xx >= ns || yy < 0 || yy >= ns) continue;
+        if (ss[(size_t)xx * ns + yy] != 0.f) {
+          v = 1.f;
+          break;
+        }
+      }
+    sl[i] = v;
+  }
+}
+
+extern "C" int cdx_unet_warm_start_scaled(cdx_unet* u, cdx_ctx* ctx_small, cdx_ctx* ctx, const void* I,
+                                          const void* bs, double support_threshold, int dilate, uint64_t seed,
+                                          void* F, void* g, void* spt) {
+  UNET_CHECK(u);
+  if (!ctx_small || !ctx || !I || !F || !g || !spt)
+    return ufail(CDX_ERR_ARG, "cdx_unet_warm_start_scaled: null argument");
+  int rows = 0, cols = 0, prec = 0, rs = 0, cs = 0, ps = 0;
+  RCHK(cdx_context_info(ctx, &rows, &cols, &prec));
+  RCHK(cdx_context_info(ctx_small, &rs, &cs, &ps));
+  if (rs != u->size || cs != u->size || ps != CDX_F32)
+    return ufail(CDX_ERR_ARG, "cdx_unet_warm_start_scaled: small context must be %dx%d FP32", u->size, u->size);
+  if (dilate < 0 || dilate > 16) return ufail(CDX_ERR_ARG, "cdx_unet_warm_start_scaled: dilate %d outside 0..16", dilate);
+  if (rows != cols || prec != CDX_F32 || rows < rs || rows % rs)
+    return ufail(CDX_ERR_ARG, "cdx_unet_warm_start_scaled: frame %dx%d must be square FP32, a multiple of %d", rows,
+                 cols, rs);
+  const int nl = rows, ns = rs;
+  const int64_t Ns = (int64_t)ns * ns, Nl = (int64_t)nl * nl;
+  // small-frame pattern and support (float), prediction F and g (complex)
+  void* buf = nullptr;
+  UCHK(hipMalloc(&buf, (size_t)Ns * (2 * sizeof(float) + 2 * sizeof(cplx<float>))));
+  float* Is = (float*)buf;
+  float* ss = Is + Ns;
+  cplx<float>* Fs = (cplx<float>*)(ss + Ns);
+  cplx<float>* gs = Fs + Ns;
+  int rc = CDX_OK;
+  do {
+    if ((rc = cdx_synchronize(ctx))) break;
+    hipLaunchKernelGGL(k_crop_band, dim3(ew_blocks(Ns)), dim3(256), 0, u->st, nl, ns, (const float*)I, Is);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(u->st) != hipSuccess) {
+      rc = ufail(CDX_ERR_HIP, "cdx_unet_warm_start_scaled: crop failed");
+      break;
+    }
+    if ((rc = cdx_unet_warm_start(u, ctx_small, Is, support_threshold, Fs, gs, ss))) break;
+    if ((rc = cdx_random_start(ctx, I, bs, seed, 1, F))) break;
+    if ((rc = cdx_synchronize(ctx))) break;
+    hipLaunchKernelGGL(k_band_phase, dim3(ew_blocks(Ns)), dim3(256), 0, u->st, nl, ns, (const cplx<float>*)Fs,
+                       (cplx<float>*)F);
+    hipLaunchKernelGGL(k_upsample_support, dim3(ew_blocks(Nl)), dim3(256), 0, u->st, nl, ns, dilate,
+                       (const float*)ss, (float*)spt);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(u->st) != hipSuccess) {
+      rc = ufail(CDX_ERR_HIP, "cdx_unet_warm_start_scaled: band transfer failed");
+      break;
+    }
+    // g = IFFT(F)/sqrt(N), the state CDI::prepareIter leaves
+    if ((rc = cdx_fft(ctx, F, g, CDX_INVERSE))) break;
+    if ((rc = cdx_apply_norm(ctx, g, 1.0 / sqrt((double)Nl)))) break;
+    rc = cdx_synchronize(ctx);
+  } while (0);
+  hipFree(buf);
+  return rc;
+}
+
 // parameters as a small binary file: "CDXUNET1", size, levels, base, count, FP32 values
 extern "C" int cdx_unet_save(cdx_unet* u, const char* path) {
   UNET_CHECK(u);

@@ -214,10 +214,10 @@ int cdx_read_config_json(const char* path, char* buf, int cap);
 /* ---- measurement helpers (timing with HIP events on the context stream) ---- */
 /* Runs n_iter iterations of `algo` from the current state (F pending on the
  * first call) without shrink-wrap.  ms_total: wall time of the n_iter
- * iterations between two events.  If col_ms/row_ms are given, a second run
- * of min(n_iter, 200) iterations records each launch's kernel start and end
- * timestamps (hipExtLaunchKernel events) and returns the mean execution time
- * of the column pass and of the row pass.  With p->recordR the column pass
+ * iterations between two events.  If col_ms/row_ms are given, each pass is
+ * then launched min(n_iter, 200) times on its own, back to back (one graph
+ * when use_graph), between two events, and the mean time per launch of the
+ * column pass and of the row pass is returned.  With p->recordR the column pass
  * also forms the R-factor sums (the cost of recording R every iteration). */
 int cdx_time_iterations(cdx_ctx* ctx, const cdx_params* p, int algo, void* F, void* g, const void* I,
                         const void* bs, const void* spt, int n_iter, int use_graph, double* ms_total,
@@ -311,6 +311,18 @@ int cdx_unet_train(cdx_unet* net, int steps, int batch, uint64_t seed0, double l
  * F = FFT(g0), g = IFFT(F)/sqrt(N) (the state CDI::prepareIter leaves) */
 int cdx_unet_warm_start(cdx_unet* net, cdx_ctx* ctx, const void* I, double support_threshold, void* F, void* g,
                         void* spt);
+/* warm start of a square FP32 frame of n = f * size pixels per side (ctx; f
+ * a positive integer) with ctx_small (size x size, FP32) as the network's
+ * work context: the network sees the pattern's central size x size
+ * frequencies (FFT order), i.e. the object at f-times coarser pixels over the
+ * same field of view.  F = cdx_random_start(I, bs, seed, mode 1), then at
+ * those frequencies F takes the phase of the prediction's FFT (modulus
+ * kept); spt = the predicted support, dilated by a disk of `dilate` (0..16)
+ * small-frame pixels, upsampled by f about the centre (nearest); g =
+ * IFFT(F)/sqrt(N).  Configuration: a 2048^2 frame of a 320 px object becomes
+ * a 256^2 pattern of a 40 px object, inside the trained range. */
+int cdx_unet_warm_start_scaled(cdx_unet* net, cdx_ctx* ctx_small, cdx_ctx* ctx, const void* I, const void* bs,
+                               double support_threshold, int dilate, uint64_t seed, void* F, void* g, void* spt);
 int cdx_unet_save(cdx_unet* net, const char* path);
 int cdx_unet_load(cdx_unet* net, const char* path);
 

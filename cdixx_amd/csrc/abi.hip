@@ -2,6 +2,7 @@
 #include "../../include/cdx_abi.h"
 #include "errors.hpp"
 #include "kernels.hpp"
+#include <algorithm>
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdarg.h>
@@ -770,7 +771,7 @@ static int time_t_(cdx_ctx* c, const cdx_params* p, int algo, void* F, void* g, 
   if (col_ms || row_ms) {
     // average launch duration of each pass: up to 200 launches of that pass
     // alone, back to back (one captured graph when use_graph, as the loop
-    // runs), between two events.  Every column-pass launch reads the same T1
+    // runs), between two events; the median of three such runs.  Every column-pass launch reads the same T1
     // and every row-pass launch the same T2, so each moves the bytes of its
     // pass in the loop.
     n = n < 200 ? n : 200;
@@ -798,18 +799,23 @@ static int time_t_(cdx_ctx* c, const cdx_params* p, int algo, void* F, void* g, 
         HIPCHK(hipGraphInstantiate(&px, pg, nullptr, nullptr, 0));
         HIPCHK(hipGraphLaunch(px, c->st));  // warm the graph once
       }
-      HIPCHK(hipStreamSynchronize(c->st));
-      HIPCHK(hipEventRecord(e0, c->st));
-      if (use_graph) {
-        HIPCHK(hipGraphLaunch(px, c->st));
-      } else {
-        for (int i = 0; i < n; i++) HIPCHK(launch());
+      // median of three timed runs (single runs vary by up to 20% between
+      // processes on one GPU)
+      float pms[3];
+      for (int k = 0; k < 3; k++) {
+        HIPCHK(hipStreamSynchronize(c->st));
+        HIPCHK(hipEventRecord(e0, c->st));
+        if (use_graph) {
+          HIPCHK(hipGraphLaunch(px, c->st));
+        } else {
+          for (int i = 0; i < n; i++) HIPCHK(launch());
+        }
+        HIPCHK(hipEventRecord(e1, c->st));
+        HIPCHK(hipEventSynchronize(e1));
+        HIPCHK(hipEventElapsedTime(&pms[k], e0, e1));
       }
-      HIPCHK(hipEventRecord(e1, c->st));
-      HIPCHK(hipEventSynchronize(e1));
-      float pms = 0;
-      HIPCHK(hipEventElapsedTime(&pms, e0, e1));
-      *dst = pms / n;
+      std::sort(pms, pms + 3);
+      *dst = pms[1] / n;
       if (px) hipGraphExecDestroy(px);
       if (pg) hipGraphDestroy(pg);
     }

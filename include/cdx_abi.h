@@ -216,8 +216,8 @@ int cdx_read_config_json(const char* path, char* buf, int cap);
  * first call) without shrink-wrap.  ms_total: wall time of the n_iter
  * iterations between two events.  If col_ms/row_ms are given, each pass is
  * then launched min(n_iter, 200) times on its own, back to back (one graph
- * when use_graph), between two events, and the mean time per launch of the
- * column pass and of the row pass is returned.  With p->recordR the column pass
+ * when use_graph), between two events, three times, and the median mean
+ * time per launch of the column pass and of the row pass is returned.  With p->recordR the column pass
  * also forms the R-factor sums (the cost of recording R every iteration). */
 int cdx_time_iterations(cdx_ctx* ctx, const cdx_params* p, int algo, void* F, void* g, const void* I,
                         const void* bs, const void* spt, int n_iter, int use_graph, double* ms_total,

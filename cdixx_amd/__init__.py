@@ -101,6 +101,8 @@ _SIGS = {
     "cdx_batch_get_thresholds": ([_vp, _dp], _i), "cdx_batch_synchronize": ([_vp], _i),
     "cdx_batch_phase_retrieve": ([_vp, ctypes.c_char_p, ctypes.POINTER(Params), _dp, _i, _ip], _i),
     "cdx_batch_time_iterations": ([_vp, ctypes.POINTER(Params), _i, _i, _i, _dp], _i),
+    "cdx_batch_set_tiling": ([_vp, _i], _i), "cdx_batch_get_tiling": ([_vp, _ip], _i),
+    "cdx_batch_autotune": ([_vp, _i, _dp, _i], _i),
     "cdx_set_tiling": ([_vp, _i, _i], _i), "cdx_get_tiling": ([_vp, _ip, _ip], _i),
     "cdx_autotune": ([_vp, _i, _dp, _i], _i),
     "cdx_convert_fo_phase": ([_vp, _vp, _d], _i), "cdx_mask_multiply": ([_vp, _vp, _vp, _i], _i),
@@ -614,6 +616,22 @@ class Batch:
         check(lib.cdx_batch_time_iterations(self.h, ctypes.byref(params), algo, n, int(use_graph),
                                             ctypes.byref(ms)))
         return ms.value
+
+    def set_tiling(self, threads):
+        """threads per workgroup of both passes: 64, 128 or 256 (cdx_batch_set_tiling)"""
+        check(lib.cdx_batch_set_tiling(self.h, int(threads)))
+
+    def tiling(self):
+        t = ctypes.c_int()
+        check(lib.cdx_batch_get_tiling(self.h, ctypes.byref(t)))
+        return t.value
+
+    def autotune(self, reps=20):
+        """times HIO iterations at 64/128/256 threads per workgroup and keeps the
+        fastest; returns {threads: us per iteration} of the sizes that fit"""
+        us = (ctypes.c_double * 3)()
+        check(lib.cdx_batch_autotune(self.h, int(reps), us, 3))
+        return {64 << k: us[k] for k in range(3) if us[k] > 0}
 
     def simulate(self, radii, photons, seeds, bs_radius=0, support_scale=1.0, start_mode=1, poisson=True):
         """Fills every frame: seeded object of the given radius, Poisson

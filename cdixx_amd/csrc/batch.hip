@@ -30,26 +30,11 @@ constexpr int kScal = 16;  // scalars per frame (same layout as a context's)
 __host__ __device__ constexpr int bsize_index(int n) {
   return n == 128 ? 0 : n == 256 ? 1 : n == 384 ? 2 : n == 512 ? 3 : -1;
 }
-// threads per workgroup: 256 (FP32) / 128 (FP64, whose two exchange
-// buffers per line would otherwise exceed the LDS at 384)
+// default threads per workgroup: 256 (FP32) / 128 (FP64, whose two exchange
+// buffers per line would otherwise exceed the LDS at 384); cdx_batch_autotune
+// times 64, 128 and 256
 template <class T>
 constexpr int bthreads() { return sizeof(T) == 8 ? kBatchThreads / 2 : kBatchThreads; }
-template <class T, int N>
-constexpr int blines() { return bthreads<T>() / LineCfgT<T, N>::TL; }
-template <class T, int N>
-constexpr int blsh() {
-  int s = 0;
-  while ((1 << s) < blines<T, N>()) s++;
-  return s;
-}
-static int blines_rt(int n, size_t csz) {
-  switch (n) {
-#define F(n_) case n_: return csz == 8 ? blines<float, n_>() : blines<double, n_>();
-    F(128) F(256) F(384) F(512)
-#undef F
-    default: return 0;
-  }
-}
 
 struct BFrame {
   int n;
@@ -75,6 +60,7 @@ struct BArgs {
   double* partials;
   int* nparts;
   int rslot;         // R-factor ring slot of this projection, -1: none
+  int lsh[kNumBatchSizes];  // log2 lines per workgroup, per frame size
 };
 
 // ---------------------------------------------------------------------------
@@ -95,7 +81,7 @@ __global__ void __launch_bounds__(kBatchThreads) kb_col(BArgs<T> a, const int2* 
   switch (f.n) {
 #define CASE(n_)                                                                                       \
   case n_:                                                                                             \
-    col_body<T, n_, OP, MODE, RF>(in, out, n_, a.tw[bsize_index(n_)], I, pa, gs, m.y, blsh<T, n_>(), smem); \
+    col_body<T, n_, OP, MODE, RF>(in, out, n_, a.tw[bsize_index(n_)], I, pa, gs, m.y, a.lsh[bsize_index(n_)], smem); \
     break;
     CASE(128) CASE(256) CASE(384) CASE(512)
 #undef CASE
@@ -122,7 +108,7 @@ __global__ void __launch_bounds__(kBatchThreads) kb_row(BArgs<T> a, const int2* 
 #define CASE(n_)                                                                                              \
   case n_:                                                                                                    \
     row_body<T, n_, OP, ALGO>(in, out, n_, a.tw[bsize_index(n_)], a.g + f.off, a.spt + f.off, ua, (T)1,        \
-                              a.spt + f.off, rs, m.y, blsh<T, n_>(), smem);                                      \
+                              a.spt + f.off, rs, m.y, a.lsh[bsize_index(n_)], smem);                             \
     break;
     CASE(128) CASE(256) CASE(384) CASE(512)
 #undef CASE
@@ -197,6 +183,9 @@ struct cdx_batch {
   double* partials = nullptr;
   int* nparts = nullptr;
   size_t lds_col = 0, lds_row = 0;
+  int threads = 0;                       // threads per workgroup of both passes
+  int lsh[kNumBatchSizes] = {0, 0, 0, 0};  // log2 lines per workgroup, per frame size
+  int partials_cap = 0;
   std::vector<cdx_ctx*> ctx;  // a single-frame context per frame size (setup operations on slices)
   std::vector<int> ctx_n;
 };
@@ -216,14 +205,92 @@ static int bfail(int code, const char* fmt, ...) {
     if (e_ != hipSuccess) return bfail(CDX_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_));   \
   } while (0)
 
-// LDS bytes of a batched workgroup of frame size n (both exchange buffers)
-static size_t wg_lds_bytes(int n, size_t csz) {
+// LDS bytes of a batched workgroup of frame size n with 2^lsh lines (both
+// exchange buffers); lanes per line of size n
+static size_t wg_lds_bytes(int n, size_t csz, int lsh) {
   switch (n) {
-#define F(n_) case n_: return csz == 8 ? lds_bytes<float, n_>(blsh<float, n_>()) : lds_bytes<double, n_>(blsh<double, n_>());
+#define F(n_) case n_: return csz == 8 ? lds_bytes<float, n_>(lsh) : lds_bytes<double, n_>(lsh);
     F(128) F(256) F(384) F(512)
 #undef F
     default: return 0;
   }
+}
+static int line_lanes_b(int n, size_t csz) {
+  switch (n) {
+#define F(n_) case n_: return csz == 8 ? LineCfgT<float, n_>::TL : LineCfgT<double, n_>::TL;
+    F(128) F(256) F(384) F(512)
+#undef F
+    default: return 0;
+  }
+}
+
+// Workgroup maps of both passes for `threads` threads per workgroup (64,
+// 128 or 256): frame size n runs threads / TL(n) lines per workgroup.
+// Returns CDX_ERR_ARG (maps unchanged) when a size present cannot take it.
+static int batch_tiling(cdx_batch* b, int threads) {
+  if (threads != 64 && threads != 128 && threads != 256)
+    return bfail(CDX_ERR_ARG, "cdx_batch_set_tiling: %d threads per workgroup (64, 128 or 256)", threads);
+  int lsh[kNumBatchSizes] = {0, 0, 0, 0};
+  size_t lds = 0;
+  for (int k = 0; k < kNumBatchSizes; k++) {
+    bool used = false;
+    for (int s : b->n) used |= s == kBatchSizes[k];
+    if (!used) continue;
+    const int tl = line_lanes_b(kBatchSizes[k], b->csz), lines = threads / tl;
+    if (lines < 1 || lines * tl != threads || kBatchSizes[k] % lines)
+      return bfail(CDX_ERR_ARG, "cdx_batch_set_tiling: %d threads do not tile %d-point lines", threads, kBatchSizes[k]);
+    while ((1 << lsh[k]) < lines) lsh[k]++;
+    lds = std::max(lds, wg_lds_bytes(kBatchSizes[k], b->csz, lsh[k]));
+  }
+  if (lds > 160 * 1024)
+    return bfail(CDX_ERR_ARG, "cdx_batch_set_tiling: %d threads need %zu bytes of LDS", threads, lds);
+  std::vector<int2> cols, rows;
+  std::vector<BFrame> fr(b->nf);
+  for (int i = 0; i < b->nf; i++) {
+    const int n = b->n[i], lines = 1 << lsh[bsize_index(n)];
+    for (int t = 0; t < n / lines; t++) cols.push_back(make_int2(i, t));
+    fr[i].n = n;
+    fr[i].wg_row0 = (int)rows.size();
+    fr[i].nwg_row = n / lines;
+    for (int t = 0; t < n / lines; t++) rows.push_back(make_int2(i, t));
+    fr[i].pad = 0;
+    fr[i].off = b->off[i];
+    fr[i].scal = b->scal + (size_t)i * kScal;
+    fr[i].rf = b->ring + (size_t)i * b->ring_cap * 2 * kRfSlots;
+  }
+  if ((int)rows.size() > b->partials_cap)
+    return bfail(CDX_ERR_ARG, "cdx_batch_set_tiling: too many row workgroups (%zu)", rows.size());
+  // column tiles sharing cache lines go to the same XCD: workgroup id b runs
+  // tile xcd(b) of the (padded) tile list
+  while (cols.size() % 8) cols.push_back(make_int2(-1, 0));
+  std::vector<int2> cmap(cols.size());
+  const int nb = (int)cols.size();
+  for (int bid = 0; bid < nb; bid++) cmap[bid] = cols[(bid & 7) * (nb >> 3) + (bid >> 3)];
+  int2 *cm = nullptr, *rm = nullptr;
+  BFrame* df = nullptr;
+  if (hipMalloc((void**)&cm, cmap.size() * sizeof(int2)) || hipMalloc((void**)&rm, rows.size() * sizeof(int2)) ||
+      hipMalloc((void**)&df, fr.size() * sizeof(BFrame)) ||
+      hipMemcpy(cm, cmap.data(), cmap.size() * sizeof(int2), hipMemcpyHostToDevice) ||
+      hipMemcpy(rm, rows.data(), rows.size() * sizeof(int2), hipMemcpyHostToDevice) ||
+      hipMemcpy(df, fr.data(), fr.size() * sizeof(BFrame), hipMemcpyHostToDevice)) {
+    hipFree(cm);
+    hipFree(rm);
+    hipFree(df);
+    return bfail(CDX_ERR_HIP, "cdx_batch_set_tiling: out of device memory");
+  }
+  if (b->st) hipStreamSynchronize(b->st);
+  hipFree(b->col_map);
+  hipFree(b->row_map);
+  hipFree(b->dframes);
+  b->col_map = cm;
+  b->row_map = rm;
+  b->dframes = df;
+  b->ncol = nb;
+  b->nrow = (int)rows.size();
+  b->threads = threads;
+  for (int k = 0; k < kNumBatchSizes; k++) b->lsh[k] = lsh[k];
+  b->lds_col = b->lds_row = lds;
+  return CDX_OK;
 }
 
 extern "C" int cdx_batch_destroy(cdx_batch* b) {
@@ -292,48 +359,20 @@ extern "C" int cdx_batch_create(cdx_batch** out, int nframes, const int* sizes, 
     if ((rc = cdx_create(&c, kBatchSizes[k], kBatchSizes[k], precision, b->st))) return bad(rc);
     b->ctx.push_back(c);
     b->ctx_n.push_back(kBatchSizes[k]);
-    b->lds_col = std::max(b->lds_col, wg_lds_bytes(kBatchSizes[k], b->csz));
   }
-  b->lds_row = b->lds_col;
-  // workgroup maps
-  std::vector<int2> cols, rows;
-  std::vector<BFrame> fr(nframes);
   b->ring_cap = 4096;
   if (hipMalloc((void**)&b->scal, (size_t)nframes * kScal * sizeof(double)) ||
       hipMalloc((void**)&b->ring, (size_t)nframes * b->ring_cap * 2 * kRfSlots * sizeof(double)))
     return bad(bfail(CDX_ERR_HIP, "cdx_batch_create: out of device memory"));
-  for (int i = 0; i < nframes; i++) {
-    const int n = b->n[i], lines = blines_rt(n, b->csz);
-    for (int t = 0; t < n / lines; t++) cols.push_back(make_int2(i, t));
-    fr[i].n = n;
-    fr[i].wg_row0 = (int)rows.size();
-    fr[i].nwg_row = n / lines;
-    for (int t = 0; t < n / lines; t++) rows.push_back(make_int2(i, t));
-    fr[i].pad = 0;
-    fr[i].off = b->off[i];
-    fr[i].scal = b->scal + (size_t)i * kScal;
-    fr[i].rf = b->ring + (size_t)i * b->ring_cap * 2 * kRfSlots;
-  }
-  // column tiles sharing cache lines go to the same XCD: workgroup id b runs
-  // tile xcd(b) of the (padded) tile list
-  while (cols.size() % 8) cols.push_back(make_int2(-1, 0));
-  std::vector<int2> cmap(cols.size());
-  const int nb = (int)cols.size();
-  for (int bid = 0; bid < nb; bid++) cmap[bid] = cols[(bid & 7) * (nb >> 3) + (bid >> 3)];
-  b->ncol = nb;
-  b->nrow = (int)rows.size();
-  if (b->nrow > kMaxPartials) return bad(bfail(CDX_ERR_ARG, "cdx_batch_create: too many rows in the batch"));
-  if (hipMalloc((void**)&b->col_map, cmap.size() * sizeof(int2)) ||
-      hipMalloc((void**)&b->row_map, rows.size() * sizeof(int2)) ||
-      hipMalloc((void**)&b->dframes, fr.size() * sizeof(BFrame)) ||
-      hipMalloc((void**)&b->partials, kMaxPartials * sizeof(double)) || hipMalloc((void**)&b->nparts, sizeof(int)))
+  // one partial sum per row workgroup: at most one workgroup per row
+  for (int i = 0; i < nframes; i++) b->partials_cap += b->n[i];
+  if (hipMalloc((void**)&b->partials, (size_t)b->partials_cap * sizeof(double)) ||
+      hipMalloc((void**)&b->nparts, sizeof(int)))
     return bad(bfail(CDX_ERR_HIP, "cdx_batch_create: out of device memory"));
+  if ((rc = batch_tiling(b, precision == CDX_F64 ? bthreads<double>() : bthreads<float>()))) return bad(rc);
   std::vector<double> sc((size_t)nframes * kScal, 0.0);
   for (int i = 0; i < nframes; i++) sc[(size_t)i * kScal] = 0.5;  // init_cuda_image threshold
-  if (hipMemcpy(b->col_map, cmap.data(), cmap.size() * sizeof(int2), hipMemcpyHostToDevice) ||
-      hipMemcpy(b->row_map, rows.data(), rows.size() * sizeof(int2), hipMemcpyHostToDevice) ||
-      hipMemcpy(b->dframes, fr.data(), fr.size() * sizeof(BFrame), hipMemcpyHostToDevice) ||
-      hipMemcpy(b->scal, sc.data(), sc.size() * sizeof(double), hipMemcpyHostToDevice))
+  if (hipMemcpy(b->scal, sc.data(), sc.size() * sizeof(double), hipMemcpyHostToDevice))
     return bad(bfail(CDX_ERR_HIP, "cdx_batch_create: upload failed"));
   *out = b;
   return CDX_OK;
@@ -403,6 +442,7 @@ struct BLoop {
     a.partials = b->partials;
     a.nparts = b->nparts;
     a.rslot = -1;
+    for (int k = 0; k < kNumBatchSizes; k++) a.lsh[k] = b->lsh[k];
     return a;
   }
   template <class K>
@@ -411,7 +451,7 @@ struct BLoop {
       hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
       if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(kern, dim3(nwg), dim3(bthreads<T>()), lds, b->st, a, map);
+    hipLaunchKernelGGL(kern, dim3(nwg), dim3(b->threads), lds, b->st, a, map);
     return hipGetLastError();
   }
   int setup() {
@@ -618,6 +658,58 @@ extern "C" int cdx_batch_time_iterations(cdx_batch* b, const cdx_params* p, int 
   if (!b || !p || n <= 0 || !ms) return bfail(CDX_ERR_ARG, "cdx_batch_time_iterations: bad arguments");
   return b->prec == CDX_F64 ? batch_time_t<double>(b, p, algo, n, use_graph, ms)
                             : batch_time_t<float>(b, p, algo, n, use_graph, ms);
+}
+
+extern "C" int cdx_batch_set_tiling(cdx_batch* b, int threads) {
+  if (!b) return bfail(CDX_ERR_ARG, "cdx_batch_set_tiling: null batch");
+  return batch_tiling(b, threads);
+}
+
+extern "C" int cdx_batch_get_tiling(cdx_batch* b, int* threads) {
+  if (!b || !threads) return bfail(CDX_ERR_ARG, "cdx_batch_get_tiling: bad arguments");
+  *threads = b->threads;
+  return CDX_OK;
+}
+
+// Times `reps` HIO iterations (one captured graph) at each valid workgroup
+// size, keeps the fastest.  F and g are restored afterwards, so the frames'
+// state is as before the call.  us[k] = microseconds per iteration at 64 <<
+// k threads (0 where the size does not fit the batch).
+extern "C" int cdx_batch_autotune(cdx_batch* b, int reps, double* us, int nus) {
+  if (!b || reps <= 0) return bfail(CDX_ERR_ARG, "cdx_batch_autotune: bad arguments");
+  const size_t bytes = (size_t)b->total * b->csz;
+  void* save = nullptr;
+  BCHK(hipMalloc(&save, 2 * bytes));
+  BCHK(hipStreamSynchronize(b->st));
+  BCHK(hipMemcpy(save, b->F, bytes, hipMemcpyDeviceToDevice));
+  BCHK(hipMemcpy((char*)save + bytes, b->g, bytes, hipMemcpyDeviceToDevice));
+  cdx_params p;
+  cdx_params_default(&p);
+  const int start = b->threads;
+  int best = start, rc = CDX_OK;
+  double best_ms = 0;
+  for (int k = 0; k < 3 && !rc; k++) {
+    const int th = 64 << k;
+    double ms = 0;
+    if (k < nus && us) us[k] = 0;
+    if (batch_tiling(b, th) != CDX_OK) continue;  // size does not fit: skipped
+    rc = cdx_batch_time_iterations(b, &p, CDX_HIO, reps, 1, &ms);
+    if (rc) break;
+    if (k < nus && us) us[k] = 1e3 * ms / reps;
+    if (best_ms == 0 || ms < best_ms) {
+      best_ms = ms;
+      best = th;
+    }
+  }
+  const int rc2 = batch_tiling(b, rc ? start : best);
+  hipError_t e1 = hipMemcpy(b->F, save, bytes, hipMemcpyDeviceToDevice);
+  hipError_t e2 = hipMemcpy(b->g, (char*)save + bytes, bytes, hipMemcpyDeviceToDevice);
+  hipFree(save);
+  if (rc) return rc;
+  if (rc2) return rc2;
+  if (e1 != hipSuccess || e2 != hipSuccess) return bfail(CDX_ERR_HIP, "cdx_batch_autotune: restore failed");
+  set_last_error("");
+  return CDX_OK;
 }
 
 extern "C" int cdx_batch_synchronize(cdx_batch* b) {

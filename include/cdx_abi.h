@@ -201,6 +201,17 @@ int cdx_batch_phase_retrieve(cdx_batch* b, const char* algorithm, const cdx_para
                              int rtrace_cap, int* n_iter);
 /* times n iterations of `algo` for the whole batch (graph or stream launches) */
 int cdx_batch_time_iterations(cdx_batch* b, const cdx_params* p, int algo, int n, int use_graph, double* ms);
+/* threads per workgroup of both batched passes (64, 128 or 256; frame size n
+ * runs threads / lanes(n) lines per workgroup); CDX_ERR_ARG, tiling
+ * unchanged, when a frame size present cannot take it.  Default 256 (FP32),
+ * 128 (FP64). */
+int cdx_batch_set_tiling(cdx_batch* b, int threads);
+int cdx_batch_get_tiling(cdx_batch* b, int* threads);
+/* autotuner: times `reps` HIO iterations (one graph) at 64, 128 and 256
+ * threads per workgroup and keeps the fastest; us[k] (k < nus, optional) =
+ * microseconds per iteration at 64 << k threads, 0 where it does not fit.
+ * F and g are restored, so the frames' state is unchanged. */
+int cdx_batch_autotune(cdx_batch* b, int reps, double* us, int nus);
 
 /* ---- the schedule grammar: AlgoParser, src/independant/readConfig.cc:82-163 ---- */
 /* expands the schedule into tokens; returns the token count or -CDX_ERR_SCHEDULE */

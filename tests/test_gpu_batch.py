@@ -78,3 +78,49 @@ def test_batch_shrinkwrap_and_rtrace(cdx):
 def test_batch_rejects_unsupported_sizes(cdx):
     with pytest.raises(cdx.CdxError):
         cdx.Batch([256, 1024])
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_batch_tilings_and_autotune(cdx, prec):
+    """Every workgroup size that fits gives the same reconstruction: lines per
+    workgroup change only which lines share a workgroup, so the iterates and
+    the blurred shrink-wrap support are bit-identical, and the shrink-wrap
+    thresholds agree to the rounding of their per-workgroup partial sums.  The
+    autotuner keeps the fastest size it timed and leaves F and g as they were."""
+    print(f"batch tilings prec={prec}", flush=True)
+    b = cdx.Batch(SIZES, precision=prec)
+    d = _fill(cdx, b)
+    p = cdx.Params.default(beta=0.9, useBS=1, freeBeamstop=1, shrinkThreshold=0.1)
+    default = b.tiling()
+    assert default == (256 if prec == 0 else 128)
+    out = {}
+    for th in (64, 128, 256):
+        try:
+            b.set_tiling(th)
+        except cdx.CdxError as e:
+            print(f"  {th} threads: {e}")
+            assert b.tiling() != th
+            continue
+        for i in range(len(b)):
+            for k in ("F", "g", "spt"):
+                b.put(i, k, d[i][k])
+        b.set_threshold(0.5)
+        b.phase_retrieve("3*HIO+2*RAAR+2*ER+1*shrinkWrap", p)
+        out[th] = ([b.get(i, "g") for i in range(len(b))], [b.get(i, "spt") for i in range(len(b))],
+                   b.thresholds())
+    assert default in out and len(out) >= 2
+    ref_g, ref_s, ref_t = out[default]
+    for th, (g, sp, t) in out.items():
+        assert np.all(ref_t > 0)
+        print(f"  {th} threads: max threshold rel diff {np.max(np.abs(t - ref_t) / ref_t):.2e}", flush=True)
+        assert np.allclose(t, ref_t, rtol=1e-6 if prec == 0 else 1e-13, atol=0)
+        for i in range(len(b)):
+            assert np.array_equal(g[i], ref_g[i]) and np.array_equal(sp[i], ref_s[i]), (th, i)
+    Fb, gb = [b.get(i, "F") for i in range(len(b))], [b.get(i, "g") for i in range(len(b))]
+    us = b.autotune(reps=5)
+    print(f"  autotune: {us} -> {b.tiling()}", flush=True)
+    assert set(us) == set(out) and us[b.tiling()] == min(us.values())
+    assert all(np.array_equal(b.get(i, "F"), Fb[i]) and np.array_equal(b.get(i, "g"), gb[i]) for i in range(len(b)))
+    with pytest.raises(cdx.CdxError):
+        b.set_tiling(96)
+    b.close()

@@ -238,13 +238,23 @@ def test_parser_reference_configs(o64):
 
 
 def test_parser_reads_reference_cfg_files(o64):
+    """The schedules of the reference's three .cfg files (fixture written by
+    tests/golden/make_cfg_schedules.py) expand to their known token counts in
+    the oracle and in the library's host parser; where the reference tree is
+    mounted, the fixture matches the files."""
+    import json
     import re
-    for f, n in (("html/cdi.cfg", 2521), ("html/default/cdi.cfg", 4500), ("html/default/pulse.cfg", 3020)):
-        path = os.path.join(REFERENCE, f)
-        if not os.path.exists(path):
-            pytest.skip("reference tree not mounted")
-        m = re.search(r'^\s*algorithm\s*=\s*"([^"]*)"', open(path).read(), re.M)
-        assert len(o64.expand(m.group(1))) == n
+    import cdixx_amd
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_schedules.json")))
+    assert [e["tokens"] for e in fx] == [2521, 4500, 3020]
+    for e in fx:
+        toks = o64.expand(e["algorithm"])
+        assert len(toks) == e["tokens"]
+        assert cdixx_amd.expand_schedule(e["algorithm"]) == toks
+        path = os.path.join(REFERENCE, e["file"])
+        if os.path.exists(path):
+            m = re.search(r'^\s*algorithm\s*=\s*"([^"]*)"', open(path).read(), re.M)
+            assert m.group(1) == e["algorithm"]
 
 
 # ---------------------------------------------------------------- RNG

@@ -90,7 +90,7 @@ def batched_leg(cdx, iters):
     b.simulate(radii, photons, seeds, support_scale=1.25)
     p = cdx.Params.default(beta=0.9)
     b.set_threshold(0.5)
-    tuner_us = b.autotune(reps=20)  # workgroup size of the batched passes
+    tuner_us = b.autotune(reps=20)  # workgroup size and radix schedules of the batched passes
     b.time_iterations(p, cdx.HIO, 20)  # warm
     ms_batch = b.time_iterations(p, cdx.HIO, iters)
     # per-frame loop: the same frames through single-frame contexts
@@ -109,7 +109,7 @@ def batched_leg(cdx, iters):
     for c, F, g, I, s in bufs:
         ms_loop += c.time_iterations(p, cdx.HIO, F, g, I, s, iters, per_kernel=False)[0]
     pix = sum(n * n for n in BATCH_SIZES)
-    b_threads = b.tiling()
+    b_threads, _, b_radices = b.schedule()
     fps_b = len(BATCH_SIZES) / (ms_batch * 1e-3) * 1.0
     fps_l = len(BATCH_SIZES) / (ms_loop * 1e-3)
     for c in ctxs.values():
@@ -120,7 +120,8 @@ def batched_leg(cdx, iters):
             "frames_per_s_batched": round(fps_b, 2), "frames_per_s_loop": round(fps_l, 2),
             "speedup": round(ms_loop / ms_batch, 2),
             "batched_us_per_iteration": round(1e3 * ms_batch / iters, 2),
-            "tiling": {"threads": b_threads, "tuner_us": {str(k): round(v, 2) for k, v in tuner_us.items()}},
+            "schedule": {"threads": b_threads, "radices": b_radices,
+                         "tuner_best_us_by_threads": {str(k): round(v, 2) for k, v in tuner_us.items()}},
             "effective_GBps": round(80 * pix * iters / (ms_batch * 1e-3) / 1e9, 1)}
 
 

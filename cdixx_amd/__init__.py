@@ -103,6 +103,8 @@ _SIGS = {
     "cdx_batch_time_iterations": ([_vp, ctypes.POINTER(Params), _i, _i, _i, _dp], _i),
     "cdx_batch_set_tiling": ([_vp, _i], _i), "cdx_batch_get_tiling": ([_vp, _ip], _i),
     "cdx_batch_autotune": ([_vp, _i, _dp, _i], _i),
+    "cdx_batch_set_schedule": ([_vp, _i, _ip], _i),
+    "cdx_batch_get_schedule": ([_vp, _ip, _ip, ctypes.c_char_p, _i], _i),
     "cdx_set_tiling": ([_vp, _i, _i], _i), "cdx_get_tiling": ([_vp, _ip, _ip], _i),
     "cdx_autotune": ([_vp, _i, _dp, _i], _i),
     "cdx_convert_fo_phase": ([_vp, _vp, _d], _i), "cdx_mask_multiply": ([_vp, _vp, _vp, _i], _i),
@@ -626,9 +628,23 @@ class Batch:
         check(lib.cdx_batch_get_tiling(self.h, ctypes.byref(t)))
         return t.value
 
+    def set_schedule(self, threads, variants):
+        """threads per workgroup and line variants (radix schedules) per frame
+        size: variants = {128: v, 256: v, 384: v, 512: v}, v in {0, 1}
+        (cdx_batch_set_schedule)"""
+        arr = (ctypes.c_int * 4)(*[int(variants.get(n, 0)) for n in (128, 256, 384, 512)])
+        check(lib.cdx_batch_set_schedule(self.h, int(threads), arr))
+
+    def schedule(self):
+        """(threads, {size: variant}, radices string) of the current schedule"""
+        t, arr, buf = ctypes.c_int(), (ctypes.c_int * 4)(), ctypes.create_string_buffer(256)
+        check(lib.cdx_batch_get_schedule(self.h, ctypes.byref(t), arr, buf, 256))
+        return t.value, dict(zip((128, 256, 384, 512), arr[:])), buf.value.decode()
+
     def autotune(self, reps=20):
-        """times HIO iterations at 64/128/256 threads per workgroup and keeps the
-        fastest; returns {threads: us per iteration} of the sizes that fit"""
+        """times HIO iterations for every schedule (64/128/256 threads per
+        workgroup x line variants) and keeps the fastest; returns {threads: us
+        per iteration of the best schedule at that size} of the sizes that fit"""
         us = (ctypes.c_double * 3)()
         check(lib.cdx_batch_autotune(self.h, int(reps), us, 3))
         return {64 << k: us[k] for k in range(3) if us[k] > 0}

@@ -61,6 +61,7 @@ struct BArgs {
   int* nparts;
   int rslot;         // R-factor ring slot of this projection, -1: none
   int lsh[kNumBatchSizes];  // log2 lines per workgroup, per frame size
+  int var[kNumBatchSizes];  // line variant (radix schedule), per frame size
 };
 
 // ---------------------------------------------------------------------------
@@ -79,9 +80,14 @@ __global__ void __launch_bounds__(kBatchThreads) kb_col(BArgs<T> a, const int2* 
   cplx<T>* out = a.dst + f.off;
   const T* I = a.Ipk + f.off;
   switch (f.n) {
-#define CASE(n_)                                                                                       \
-  case n_:                                                                                             \
-    col_body<T, n_, OP, MODE, RF>(in, out, n_, a.tw[bsize_index(n_)], I, pa, gs, m.y, a.lsh[bsize_index(n_)], smem); \
+#define CASE(n_)                                                                                              \
+  case n_:                                                                                                    \
+    if (sizeof(T) == 4 && a.var[bsize_index(n_)])                                                             \
+      col_body<T, n_, OP, MODE, RF, false, false, sizeof(T) == 4 ? 1 : 0>(in, out, n_, a.tw[bsize_index(n_)], I, \
+                                                                         pa, gs, m.y, a.lsh[bsize_index(n_)], smem); \
+    else                                                                                                      \
+      col_body<T, n_, OP, MODE, RF>(in, out, n_, a.tw[bsize_index(n_)], I, pa, gs, m.y, a.lsh[bsize_index(n_)], \
+                                    smem);                                                                    \
     break;
     CASE(128) CASE(256) CASE(384) CASE(512)
 #undef CASE
@@ -107,8 +113,13 @@ __global__ void __launch_bounds__(kBatchThreads) kb_row(BArgs<T> a, const int2* 
   switch (f.n) {
 #define CASE(n_)                                                                                              \
   case n_:                                                                                                    \
-    row_body<T, n_, OP, ALGO>(in, out, n_, a.tw[bsize_index(n_)], a.g + f.off, a.spt + f.off, ua, (T)1,        \
-                              a.spt + f.off, rs, m.y, a.lsh[bsize_index(n_)], smem);                             \
+    if (sizeof(T) == 4 && a.var[bsize_index(n_)])                                                             \
+      row_body<T, n_, OP, ALGO, false, sizeof(T) == 4 ? 1 : 0>(in, out, n_, a.tw[bsize_index(n_)], a.g + f.off,  \
+                                                               a.spt + f.off, ua, (T)1, a.spt + f.off, rs, m.y,  \
+                                                               a.lsh[bsize_index(n_)], smem);                    \
+    else                                                                                                      \
+      row_body<T, n_, OP, ALGO>(in, out, n_, a.tw[bsize_index(n_)], a.g + f.off, a.spt + f.off, ua, (T)1,      \
+                                a.spt + f.off, rs, m.y, a.lsh[bsize_index(n_)], smem);                           \
     break;
     CASE(128) CASE(256) CASE(384) CASE(512)
 #undef CASE
@@ -185,6 +196,7 @@ struct cdx_batch {
   size_t lds_col = 0, lds_row = 0;
   int threads = 0;                       // threads per workgroup of both passes
   int lsh[kNumBatchSizes] = {0, 0, 0, 0};  // log2 lines per workgroup, per frame size
+  int var[kNumBatchSizes] = {0, 0, 0, 0};  // line variant (radix schedule) per frame size (FP32)
   int partials_cap = 0;
   std::vector<cdx_ctx*> ctx;  // a single-frame context per frame size (setup operations on slices)
   std::vector<int> ctx_n;
@@ -205,42 +217,66 @@ static int bfail(int code, const char* fmt, ...) {
     if (e_ != hipSuccess) return bfail(CDX_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_));   \
   } while (0)
 
-// LDS bytes of a batched workgroup of frame size n with 2^lsh lines (both
-// exchange buffers); lanes per line of size n
-static size_t wg_lds_bytes(int n, size_t csz, int lsh) {
+// LDS bytes of a batched workgroup of frame size n with 2^lsh lines of
+// variant v (both exchange buffers); lanes per line of size n, variant v
+static size_t wg_lds_bytes(int n, size_t csz, int lsh, int v) {
   switch (n) {
-#define F(n_) case n_: return csz == 8 ? lds_bytes<float, n_>(lsh) : lds_bytes<double, n_>(lsh);
+#define F(n_)                                                                                     \
+  case n_:                                                                                        \
+    return csz == 8 ? (v ? lds_bytes<float, n_, 1>(lsh) : lds_bytes<float, n_>(lsh)) : lds_bytes<double, n_>(lsh);
     F(128) F(256) F(384) F(512)
 #undef F
     default: return 0;
   }
 }
-static int line_lanes_b(int n, size_t csz) {
+static int line_lanes_b(int n, size_t csz, int v) {
   switch (n) {
-#define F(n_) case n_: return csz == 8 ? LineCfgT<float, n_>::TL : LineCfgT<double, n_>::TL;
+#define F(n_) \
+  case n_: return csz == 8 ? (v ? LineCfgT<float, n_, 1>::TL : LineCfgT<float, n_>::TL) : LineCfgT<double, n_>::TL;
     F(128) F(256) F(384) F(512)
 #undef F
     default: return 0;
+  }
+}
+// radix schedule of frame size n, variant v (forward order), e.g. "16x16"
+static std::string radix_name(int n, size_t csz, int v) {
+  auto fmt = [](const int* r, int m) {
+    std::string s;
+    for (int i = 0; i < m; i++) s += (i ? "x" : "") + std::to_string(r[i]);
+    return s;
+  };
+  switch (n) {
+#define F(n_)                                                                                            \
+  case n_:                                                                                               \
+    if (csz == 8 && v) return fmt(LineCfgT<float, n_, 1>::RL::R, LineCfgT<float, n_, 1>::RL::M);         \
+    if (csz == 8) return fmt(LineCfgT<float, n_>::RL::R, LineCfgT<float, n_>::RL::M);                    \
+    return fmt(LineCfgT<double, n_>::RL::R, LineCfgT<double, n_>::RL::M);
+    F(128) F(256) F(384) F(512)
+#undef F
+    default: return "";
   }
 }
 
 // Workgroup maps of both passes for `threads` threads per workgroup (64,
 // 128 or 256): frame size n runs threads / TL(n) lines per workgroup.
 // Returns CDX_ERR_ARG (maps unchanged) when a size present cannot take it.
-static int batch_tiling(cdx_batch* b, int threads) {
+static int batch_tiling(cdx_batch* b, int threads, const int* var) {
   if (threads != 64 && threads != 128 && threads != 256)
     return bfail(CDX_ERR_ARG, "cdx_batch_set_tiling: %d threads per workgroup (64, 128 or 256)", threads);
   int lsh[kNumBatchSizes] = {0, 0, 0, 0};
   size_t lds = 0;
   for (int k = 0; k < kNumBatchSizes; k++) {
+    if (var[k] != 0 && (var[k] != 1 || b->prec != CDX_F32))
+      return bfail(CDX_ERR_ARG, "cdx_batch_set_schedule: line variant %d for size %d (0, or 1 in FP32)", var[k],
+                   kBatchSizes[k]);
     bool used = false;
     for (int s : b->n) used |= s == kBatchSizes[k];
     if (!used) continue;
-    const int tl = line_lanes_b(kBatchSizes[k], b->csz), lines = threads / tl;
+    const int tl = line_lanes_b(kBatchSizes[k], b->csz, var[k]), lines = threads / tl;
     if (lines < 1 || lines * tl != threads || kBatchSizes[k] % lines)
       return bfail(CDX_ERR_ARG, "cdx_batch_set_tiling: %d threads do not tile %d-point lines", threads, kBatchSizes[k]);
     while ((1 << lsh[k]) < lines) lsh[k]++;
-    lds = std::max(lds, wg_lds_bytes(kBatchSizes[k], b->csz, lsh[k]));
+    lds = std::max(lds, wg_lds_bytes(kBatchSizes[k], b->csz, lsh[k], var[k]));
   }
   if (lds > 160 * 1024)
     return bfail(CDX_ERR_ARG, "cdx_batch_set_tiling: %d threads need %zu bytes of LDS", threads, lds);
@@ -288,7 +324,10 @@ static int batch_tiling(cdx_batch* b, int threads) {
   b->ncol = nb;
   b->nrow = (int)rows.size();
   b->threads = threads;
-  for (int k = 0; k < kNumBatchSizes; k++) b->lsh[k] = lsh[k];
+  for (int k = 0; k < kNumBatchSizes; k++) {
+    b->lsh[k] = lsh[k];
+    b->var[k] = var[k];
+  }
   b->lds_col = b->lds_row = lds;
   return CDX_OK;
 }
@@ -369,7 +408,7 @@ extern "C" int cdx_batch_create(cdx_batch** out, int nframes, const int* sizes, 
   if (hipMalloc((void**)&b->partials, (size_t)b->partials_cap * sizeof(double)) ||
       hipMalloc((void**)&b->nparts, sizeof(int)))
     return bad(bfail(CDX_ERR_HIP, "cdx_batch_create: out of device memory"));
-  if ((rc = batch_tiling(b, precision == CDX_F64 ? bthreads<double>() : bthreads<float>()))) return bad(rc);
+  if ((rc = batch_tiling(b, precision == CDX_F64 ? bthreads<double>() : bthreads<float>(), b->var))) return bad(rc);
   std::vector<double> sc((size_t)nframes * kScal, 0.0);
   for (int i = 0; i < nframes; i++) sc[(size_t)i * kScal] = 0.5;  // init_cuda_image threshold
   if (hipMemcpy(b->scal, sc.data(), sc.size() * sizeof(double), hipMemcpyHostToDevice))
@@ -442,7 +481,10 @@ struct BLoop {
     a.partials = b->partials;
     a.nparts = b->nparts;
     a.rslot = -1;
-    for (int k = 0; k < kNumBatchSizes; k++) a.lsh[k] = b->lsh[k];
+    for (int k = 0; k < kNumBatchSizes; k++) {
+      a.lsh[k] = b->lsh[k];
+      a.var[k] = b->var[k];
+    }
     return a;
   }
   template <class K>
@@ -662,7 +704,7 @@ extern "C" int cdx_batch_time_iterations(cdx_batch* b, const cdx_params* p, int 
 
 extern "C" int cdx_batch_set_tiling(cdx_batch* b, int threads) {
   if (!b) return bfail(CDX_ERR_ARG, "cdx_batch_set_tiling: null batch");
-  return batch_tiling(b, threads);
+  return batch_tiling(b, threads, b->var);
 }
 
 extern "C" int cdx_batch_get_tiling(cdx_batch* b, int* threads) {
@@ -671,10 +713,32 @@ extern "C" int cdx_batch_get_tiling(cdx_batch* b, int* threads) {
   return CDX_OK;
 }
 
-// Times `reps` HIO iterations (one captured graph) at each valid workgroup
-// size, keeps the fastest.  F and g are restored afterwards, so the frames'
-// state is as before the call.  us[k] = microseconds per iteration at 64 <<
-// k threads (0 where the size does not fit the batch).
+extern "C" int cdx_batch_set_schedule(cdx_batch* b, int threads, const int* variants) {
+  if (!b || !variants) return bfail(CDX_ERR_ARG, "cdx_batch_set_schedule: bad arguments");
+  return batch_tiling(b, threads, variants);
+}
+
+extern "C" int cdx_batch_get_schedule(cdx_batch* b, int* threads, int* variants, char* radices, int cap) {
+  if (!b || !threads || !variants) return bfail(CDX_ERR_ARG, "cdx_batch_get_schedule: bad arguments");
+  *threads = b->threads;
+  std::string r;
+  for (int k = 0; k < kNumBatchSizes; k++) {
+    variants[k] = b->var[k];
+    bool used = false;
+    for (int s : b->n) used |= s == kBatchSizes[k];
+    if (used)
+      r += (r.empty() ? "" : " ") + std::to_string(kBatchSizes[k]) + ":" + radix_name(kBatchSizes[k], b->csz, b->var[k]);
+  }
+  if (radices && cap > 0) snprintf(radices, (size_t)cap, "%s", r.c_str());
+  return CDX_OK;
+}
+
+// Times `reps` HIO iterations (one captured graph) for every schedule: each
+// workgroup size (64, 128, 256 threads) with each combination of line
+// variants over the frame sizes present (FP32; FP64 has one variant), and
+// keeps the fastest.  F and g are restored afterwards, so the frames' state
+// is as before the call.  us[k] = microseconds per iteration of the best
+// schedule at 64 << k threads (0 where no schedule fits the batch).
 extern "C" int cdx_batch_autotune(cdx_batch* b, int reps, double* us, int nus) {
   if (!b || reps <= 0) return bfail(CDX_ERR_ARG, "cdx_batch_autotune: bad arguments");
   const size_t bytes = (size_t)b->total * b->csz;
@@ -685,23 +749,38 @@ extern "C" int cdx_batch_autotune(cdx_batch* b, int reps, double* us, int nus) {
   BCHK(hipMemcpy((char*)save + bytes, b->g, bytes, hipMemcpyDeviceToDevice));
   cdx_params p;
   cdx_params_default(&p);
-  const int start = b->threads;
-  int best = start, rc = CDX_OK;
+  // sizes present get a variant bit each (FP32)
+  int present[kNumBatchSizes], np = 0;
+  for (int k = 0; k < kNumBatchSizes; k++) {
+    bool used = false;
+    for (int s : b->n) used |= s == kBatchSizes[k];
+    if (used && b->prec == CDX_F32) present[np++] = k;
+  }
+  const int start_th = b->threads;
+  int start_var[kNumBatchSizes], best_var[kNumBatchSizes];
+  for (int k = 0; k < kNumBatchSizes; k++) start_var[k] = best_var[k] = b->var[k];
+  int best_th = start_th, rc = CDX_OK;
   double best_ms = 0;
   for (int k = 0; k < 3 && !rc; k++) {
     const int th = 64 << k;
-    double ms = 0;
     if (k < nus && us) us[k] = 0;
-    if (batch_tiling(b, th) != CDX_OK) continue;  // size does not fit: skipped
-    rc = cdx_batch_time_iterations(b, &p, CDX_HIO, reps, 1, &ms);
-    if (rc) break;
-    if (k < nus && us) us[k] = 1e3 * ms / reps;
-    if (best_ms == 0 || ms < best_ms) {
-      best_ms = ms;
-      best = th;
+    for (int mask = 0; mask < (1 << np) && !rc; mask++) {
+      int var[kNumBatchSizes] = {0, 0, 0, 0};
+      for (int q = 0; q < np; q++) var[present[q]] = (mask >> q) & 1;
+      if (batch_tiling(b, th, var) != CDX_OK) continue;  // does not fit the batch: skipped
+      double ms = 0;
+      rc = cdx_batch_time_iterations(b, &p, CDX_HIO, reps, 1, &ms);
+      if (rc) break;
+      const double u = 1e3 * ms / reps;
+      if (k < nus && us && (us[k] == 0 || u < us[k])) us[k] = u;
+      if (best_ms == 0 || ms < best_ms) {
+        best_ms = ms;
+        best_th = th;
+        for (int q = 0; q < kNumBatchSizes; q++) best_var[q] = var[q];
+      }
     }
   }
-  const int rc2 = batch_tiling(b, rc ? start : best);
+  const int rc2 = rc ? batch_tiling(b, start_th, start_var) : batch_tiling(b, best_th, best_var);
   hipError_t e1 = hipMemcpy(b->F, save, bytes, hipMemcpyDeviceToDevice);
   hipError_t e2 = hipMemcpy(b->g, (char*)save + bytes, bytes, hipMemcpyDeviceToDevice);
   hipFree(save);

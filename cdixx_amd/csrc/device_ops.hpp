@@ -45,7 +45,7 @@ CDX_LINE(2048, 128, 5, 16, 16, 8)
 // row pass's g, S and twiddles, the 256 VGPRs of a wave (up to 462 spilled);
 // they run on more lanes with 8 or 12 points each.  The 256-point FP64 line
 // keeps 16 points per lane (66 spilled VGPRs in the update pass).
-template <class T, int N>
+template <class T, int N, int V = 0>
 struct LineCfgT : LineCfg<N> {};
 template <>
 struct LineCfgT<double, 512> {
@@ -72,8 +72,37 @@ struct LineCfgT<double, 2048> {
   using RL = Radices<8, 8, 8, 4>;
 };
 
-template <class T, int N>
-using LineOf = Line<T, N, LineCfgT<T, N>::TL, typename LineCfgT<T, N>::RL, LineCfgT<T, N>::PAD>;
+// Variant 1 of the FP32 lines of the batched passes (128-512 points): twice
+// the lanes with half the points each, the schedules of the FP64 lines (for
+// 128 points, 4 points per lane); the batch autotuner picks a variant per
+// frame size.  Other lengths and FP64 have variant 0 only.
+template <>
+struct LineCfgT<float, 128, 1> {
+  static constexpr int TL = 32;
+  static constexpr int PAD = 5;
+  using RL = Radices<4, 4, 4, 2>;
+};
+template <>
+struct LineCfgT<float, 256, 1> {
+  static constexpr int TL = 32;
+  static constexpr int PAD = 5;
+  using RL = Radices<8, 8, 4>;
+};
+template <>
+struct LineCfgT<float, 384, 1> {
+  static constexpr int TL = 32;
+  static constexpr int PAD = 0;
+  using RL = Radices<4, 4, 4, 6>;
+};
+template <>
+struct LineCfgT<float, 512, 1> {
+  static constexpr int TL = 64;
+  static constexpr int PAD = 5;
+  using RL = Radices<8, 8, 8>;
+};
+
+template <class T, int N, int V = 0>
+using LineOf = Line<T, N, LineCfgT<T, N, V>::TL, typename LineCfgT<T, N, V>::RL, LineCfgT<T, N, V>::PAD>;
 
 // LDS layout of a workgroup of 2^lsh lines: two exchange buffers (the
 // transforms alternate between them, one barrier per exchange); buffer b of
@@ -81,16 +110,16 @@ using LineOf = Line<T, N, LineCfgT<T, N>::TL, typename LineCfgT<T, N>::RL, LineC
 // footprint rounded to a 128-byte bank row (EPB complex elements) plus
 // EPB / 2^lsh, so the lines of a column tile (lanes (t, c) with c fastest)
 // start on different banks.
-template <class T, int N>
+template <class T, int N, int V = 0>
 __host__ __device__ constexpr int lds_pitch(int lsh) {
   constexpr int EPB = 128 / (int)sizeof(cplx<T>);
-  constexpr int e = LineOf<T, N>::LDS_ELEMS;
+  constexpr int e = LineOf<T, N, V>::LDS_ELEMS;
   constexpr int base = (e + EPB - 1) / EPB * EPB;
   return base + (lsh < 31 ? (EPB >> lsh) & (EPB - 1) : 0);
 }
-template <class T, int N>
+template <class T, int N, int V = 0>
 __host__ __device__ constexpr size_t lds_bytes(int lsh) {
-  return (size_t)2 * ((size_t)1 << lsh) * lds_pitch<T, N>(lsh) * sizeof(cplx<T>);
+  return (size_t)2 * ((size_t)1 << lsh) * lds_pitch<T, N, V>(lsh) * sizeof(cplx<T>);
 }
 
 // index of element (r, c) of a rows-row image in the column-tile layout of

@@ -113,12 +113,12 @@ __device__ __forceinline__ int xcd_tile(int bid, int nb) {
 // input (ops that start with the forward transform) is T1 in its column-tile
 // layout.  TILED_I: the packed measurement is in that layout too.
 // ---------------------------------------------------------------------------
-template <class T, int N, int OP, int MODE, int RF, bool TILED = false, bool TILED_I = false>
+template <class T, int N, int OP, int MODE, int RF, bool TILED = false, bool TILED_I = false, int V = 0>
 __device__ __forceinline__ void col_body(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out, int cols,
                                          const cplx<T>* __restrict__ tw, const T* __restrict__ I,
                                          const ProjArgs<T>& pa, T gsig, int tile, int lsh, unsigned char* smem) {
-  using L = LineOf<T, N>;
-  const int pitch = lds_pitch<T, N>(lsh);
+  using L = LineOf<T, N, V>;
+  const int pitch = lds_pitch<T, N, V>(lsh);
   const int alt = pitch << lsh;
   int ph = 0;
   constexpr int E = L::E;
@@ -227,15 +227,15 @@ __device__ __forceinline__ void col_body(const cplx<T>* __restrict__ in, cplx<T>
 // rule of ROW_INV_UPDATE_FWD (enum Algorithm).  TILED: the update ops store
 // T1 in its column-tile layout.
 // ---------------------------------------------------------------------------
-template <class T, int N, int OP, int ALGO, bool TILED = false>
+template <class T, int N, int OP, int ALGO, bool TILED = false, int V = 0>
 __device__ __forceinline__ void row_body(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out, int rows,
                                          const cplx<T>* __restrict__ tw, cplx<T>* __restrict__ g,
                                          const T* __restrict__ spt, const UpdArgs<T>& ua, T scale,
                                          T* __restrict__ sout, const ReduceSlots& rs, int tile, int lsh,
                                          unsigned char* smem) {
-  using L = LineOf<T, N>;
-  constexpr int TL = LineCfgT<T, N>::TL;
-  const int pitch = lds_pitch<T, N>(lsh);
+  using L = LineOf<T, N, V>;
+  constexpr int TL = LineCfgT<T, N, V>::TL;
+  const int pitch = lds_pitch<T, N, V>(lsh);
   const int alt = pitch << lsh;
   int ph = 0;
   constexpr int E = L::E;

@@ -207,10 +207,20 @@ int cdx_batch_time_iterations(cdx_batch* b, const cdx_params* p, int algo, int n
  * 128 (FP64). */
 int cdx_batch_set_tiling(cdx_batch* b, int threads);
 int cdx_batch_get_tiling(cdx_batch* b, int* threads);
-/* autotuner: times `reps` HIO iterations (one graph) at 64, 128 and 256
- * threads per workgroup and keeps the fastest; us[k] (k < nus, optional) =
- * microseconds per iteration at 64 << k threads, 0 where it does not fit.
- * F and g are restored, so the frames' state is unchanged. */
+/* the whole schedule: threads per workgroup and, per frame size 128, 256,
+ * 384, 512 (variants[0..3]), the line variant: 0 = the default radix
+ * schedule, 1 (FP32 only) = twice the lanes with half the points each
+ * (128: 4x4x4x2, 256: 8x8x4, 384: 4x4x4x6, 512: 8x8x8).  get_schedule also
+ * writes the radices of the sizes present ("256:16x16 384:8x8x6 ...") into
+ * radices[cap] (optional). */
+int cdx_batch_set_schedule(cdx_batch* b, int threads, const int* variants);
+int cdx_batch_get_schedule(cdx_batch* b, int* threads, int* variants, char* radices, int cap);
+/* autotuner: times `reps` HIO iterations (one graph) for every schedule (64,
+ * 128 and 256 threads per workgroup, each with every combination of line
+ * variants over the sizes present) and keeps the fastest; us[k] (k < nus,
+ * optional) = microseconds per iteration of the best schedule at 64 << k
+ * threads, 0 where none fits.  F and g are restored, so the frames' state is
+ * unchanged. */
 int cdx_batch_autotune(cdx_batch* b, int reps, double* us, int nus);
 
 /* ---- the schedule grammar: AlgoParser, src/independant/readConfig.cc:82-163 ---- */

@@ -124,3 +124,39 @@ def test_batch_tilings_and_autotune(cdx, prec):
     with pytest.raises(cdx.CdxError):
         b.set_tiling(96)
     b.close()
+
+
+def test_batch_radix_variants(cdx):
+    """FP32 line variant 1 (twice the lanes, half the points each: 128 4x4x4x2,
+    256 8x8x4, 384 4x4x4x6, 512 8x8x8) computes the same reconstruction as the
+    default radices to rounding, against the single-frame path; FP64 takes
+    variant 0 only."""
+    print("batch radix variants", flush=True)
+    b = cdx.Batch(SIZES)
+    d = _fill(cdx, b)
+    p = cdx.Params.default(beta=0.9, useBS=1, freeBeamstop=1)
+    th, var, rad = b.schedule()
+    assert th == 256 and set(var.values()) == {0}
+    assert rad == "128:8x8x2 256:16x16 384:8x8x6 512:16x16x2", rad
+    ones = {n: 1 for n in (128, 256, 384, 512)}
+    for threads in (64, 128, 256):
+        b.set_schedule(threads, ones)
+        assert b.schedule()[2] == "128:4x4x4x2 256:8x8x4 384:4x4x4x6 512:8x8x8"
+        # one iteration to the single-iteration bar; seven, where HIO has
+        # amplified the different rounding of the two schedules, to 1e-3
+        for alg, tol in (("1*HIO", 1e-5), ("3*HIO+2*RAAR+2*ER", 1e-3)):
+            for i in range(len(b)):
+                b.put(i, "F", d[i]["F"])
+                b.put(i, "g", d[i]["g"])
+            b.set_threshold(0.5)
+            b.phase_retrieve(alg, p)
+            for i, n in enumerate(SIZES):
+                ref = _single(cdx, n, 0, d[i], alg, p)
+                eg, eF = rel_l2(b.get(i, "g"), ref["g"]), rel_l2(b.get(i, "F"), ref["F"])
+                assert eg < tol and eF < tol, (threads, alg, i, eg, eF)
+    b.close()
+    b64 = cdx.Batch([256, 384], precision=1)
+    with pytest.raises(cdx.CdxError):
+        b64.set_schedule(128, {256: 1})
+    assert b64.schedule()[1][256] == 0
+    b64.close()

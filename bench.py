@@ -443,6 +443,7 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start leg")
     ap.add_argument("--no-ptycho", action="store_true", help="skip the ptychography leg")
     ap.add_argument("--ptycho-only", action="store_true", help="run only the ptychography leg (profiling)")
+    ap.add_argument("--batch-only", action="store_true", help="run only the batched leg (profiling)")
     ap.add_argument("--learned-large-only", action="store_true", help="run only the 2048^2 learned warm-start leg")
     ap.add_argument("--slab-only", action="store_true", help="run only the one-GPU slab-share leg")
     ap.add_argument("--batch-iters", type=int, default=500)
@@ -457,6 +458,9 @@ def main():
     import cdixx_amd as cdx
     if args.ptycho_only:
         print(json.dumps({"ptycho": ptycho_leg(cdx)}), flush=True)
+        return
+    if args.batch_only:
+        print(json.dumps({"batched": batched_leg(cdx, args.batch_iters)}), flush=True)
         return
     if args.learned_large_only:
         print(json.dumps({"learned_warm_start_2048": learned_large_leg(cdx)}), flush=True)

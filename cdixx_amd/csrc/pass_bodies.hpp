@@ -20,9 +20,9 @@
 // In the single-frame loop (TILED) T1 and the packed measurement are kept in
 // column tiles of kT1Tile columns (t1_index), so the column pass reads
 // contiguous blocks; T1 and T2 are moved with streaming (non-temporal) loads
-// and stores, while g and S are loaded and g stored plainly: a workgroup id
-// handles the same rows on the same XCD every iteration, so they can stay in
-// its L2.
+// and stores, while g and S are loaded plainly (and g stored plainly up to
+// 1024^2): a workgroup id handles the same rows on the same XCD every
+// iteration, so they can stay in its L2.
 #pragma once
 #include "device_ops.hpp"
 
@@ -340,7 +340,12 @@ __device__ __forceinline__ void row_body(const cplx<T>* __restrict__ in, cplx<T>
           }
         }
         v[i] = support_update(gv[i], p, in, ALGO >= 0 ? ALGO : ua.algo, ua.beta, ua.relax);
-        if (ok) g[base + L::din_pos(i, t)] = v[i];  // plain: read back next iteration
+        if (ok) {
+          // the single-frame loop up to 1024^2 reads g back from L2 next
+          // iteration (g and S fit); larger frames and batches stream it
+          if constexpr (TILED && N <= 1024) g[base + L::din_pos(i, t)] = v[i];
+          else store_stream(&g[base + L::din_pos(i, t)], v[i]);
+        }
       }
       L::forward(v, my, alt, ph, twf, t);
       if (ok) {

@@ -128,6 +128,15 @@ __device__ __forceinline__ size_t t1_index(int r, int c, int rows) {
   return (size_t)(c / kT1Tile) * ((size_t)rows * kT1Tile) + (size_t)r * kT1Tile + (c % kT1Tile);
 }
 
+// index of element (r, c) of a cols-column image in the pair-of-rows layout
+// of the single-frame loop's T2: rows 2k and 2k+1 interleaved in groups of
+// four columns, so the column pass writes 64-byte pieces (two rows of its
+// four-column tile) and a row-pass workgroup of two rows reads one
+// contiguous block (rows even, cols a multiple of 4)
+__device__ __forceinline__ size_t t2_index(int r, int c, int cols) {
+  return (size_t)(r >> 1) * (2 * (size_t)cols) + (size_t)((c >> 2) << 3) + (size_t)((r & 1) << 2) + (c & 3);
+}
+
 // ---------------------------------------------------------------------------
 // pointwise device helpers (reference semantics)
 // ---------------------------------------------------------------------------

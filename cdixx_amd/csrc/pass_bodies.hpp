@@ -113,7 +113,8 @@ __device__ __forceinline__ int xcd_tile(int bid, int nb) {
 // input (ops that start with the forward transform) is T1 in its column-tile
 // layout.  TILED_I: the packed measurement is in that layout too.
 // ---------------------------------------------------------------------------
-template <class T, int N, int OP, int MODE, int RF, bool TILED = false, bool TILED_I = false, int V = 0>
+template <class T, int N, int OP, int MODE, int RF, bool TILED = false, bool TILED_I = false, int V = 0,
+          bool T2L = false>
 __device__ __forceinline__ void col_body(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out, int cols,
                                          const cplx<T>* __restrict__ tw, const T* __restrict__ I,
                                          const ProjArgs<T>& pa, T gsig, int tile, int lsh, unsigned char* smem) {
@@ -210,10 +211,12 @@ __device__ __forceinline__ void col_body(const cplx<T>* __restrict__ in, cplx<T>
     // made that column pass 20% slower.
     if constexpr (N <= 1024) {
 #pragma unroll
-      for (int i = 0; i < E; i++) store_stream(&out[(size_t)L::din_pos(i, t) * cols + col], v[i]);
+      for (int i = 0; i < E; i++)
+        store_stream(&out[T2L ? t2_index(L::din_pos(i, t), col, cols) : (size_t)L::din_pos(i, t) * cols + col], v[i]);
     } else {
 #pragma unroll
-      for (int i = 0; i < E; i++) out[(size_t)L::din_pos(i, t) * cols + col] = v[i];
+      for (int i = 0; i < E; i++)
+        out[T2L ? t2_index(L::din_pos(i, t), col, cols) : (size_t)L::din_pos(i, t) * cols + col] = v[i];
     }
   }
   if (rf) {
@@ -227,7 +230,7 @@ __device__ __forceinline__ void col_body(const cplx<T>* __restrict__ in, cplx<T>
 // rule of ROW_INV_UPDATE_FWD (enum Algorithm).  TILED: the update ops store
 // T1 in its column-tile layout.
 // ---------------------------------------------------------------------------
-template <class T, int N, int OP, int ALGO, bool TILED = false, int V = 0>
+template <class T, int N, int OP, int ALGO, bool TILED = false, int V = 0, bool T2L = false>
 __device__ __forceinline__ void row_body(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out, int rows,
                                          const cplx<T>* __restrict__ tw, cplx<T>* __restrict__ g,
                                          const T* __restrict__ spt, const UpdArgs<T>& ua, T scale,
@@ -274,7 +277,9 @@ __device__ __forceinline__ void row_body(const cplx<T>* __restrict__ in, cplx<T>
     constexpr bool ld_nt = TILED && upd && N == 1024;
     if constexpr (ld_nt) {
 #pragma unroll
-      for (int i = 0; i < E; i++) v[i] = ok ? load_stream(&in[base + L::dout_pos(i, t)]) : cplx<T>{(T)0, (T)0};
+      for (int i = 0; i < E; i++)
+        v[i] = ok ? load_stream(&in[T2L ? t2_index(row, L::dout_pos(i, t), N) : base + L::dout_pos(i, t)])
+                  : cplx<T>{(T)0, (T)0};
 #pragma unroll
       for (int i = 0; i < E; i++) {
         gv[i] = ok ? g[base + L::din_pos(i, t)] : cplx<T>{(T)0, (T)0};
@@ -282,7 +287,8 @@ __device__ __forceinline__ void row_body(const cplx<T>* __restrict__ in, cplx<T>
       }
     } else {
 #pragma unroll
-      for (int i = 0; i < E; i++) v[i] = ok ? in[base + L::dout_pos(i, t)] : cplx<T>{(T)0, (T)0};
+      for (int i = 0; i < E; i++)
+        v[i] = ok ? in[T2L ? t2_index(row, L::dout_pos(i, t), N) : base + L::dout_pos(i, t)] : cplx<T>{(T)0, (T)0};
       if constexpr (upd) {
 #pragma unroll
         for (int i = 0; i < E; i++) {

@@ -96,8 +96,11 @@ k_col(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out, int cols, const
   extern __shared__ __align__(16) unsigned char smem[];
   // the single-frame loop keeps T1 and the packed measurement in the column-tile layout
   constexpr bool tiled = OP == COL_FWD_PROJ_INV || OP == COL_FWD_GAUSS_INV || OP == COL_FWD_T1;
-  col_body<T, N, OP, MODE, RF, tiled, true>(in, out, cols, tw, I, pa, gsig, xcd_tile(blockIdx.x, gridDim.x), lsh,
-                                            smem);
+  // the loop's T2 (the inverse column transforms read by the update and
+  // shrink-wrap row passes) is in the pair-of-rows layout
+  constexpr bool t2l = OP == COL_FWD_PROJ_INV || OP == COL_PROJ_INV || OP == COL_FWD_GAUSS_INV || OP == COL_GAUSS_INV;
+  col_body<T, N, OP, MODE, RF, tiled, true, 0, t2l>(in, out, cols, tw, I, pa, gsig, xcd_tile(blockIdx.x, gridDim.x),
+                                                     lsh, smem);
 }
 template <class T, int N, int OP, int ALGO>
 __global__ void __launch_bounds__(kMaxThreads)
@@ -106,7 +109,8 @@ k_row(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out, int rows, const
       ReduceSlots rs, int lsh) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr bool tiled = OP == ROW_INV_UPDATE_FWD || OP == ROW_INV_UPDATE_FWD_PRTF;
-  row_body<T, N, OP, ALGO, tiled>(in, out, rows, tw, g, spt, ua, scale, sout, rs, blockIdx.x, lsh, smem);
+  constexpr bool t2l = OP == ROW_INV_UPDATE_FWD || OP == ROW_INV_UPDATE_FWD_PRTF || OP == ROW_INV_MOD2;
+  row_body<T, N, OP, ALGO, tiled, 0, t2l>(in, out, rows, tw, g, spt, ua, scale, sout, rs, blockIdx.x, lsh, smem);
 }
 
 // ---------------------------------------------------------------------------

@@ -75,6 +75,8 @@ __global__ void __launch_bounds__(kBatchThreads) kb_col(BArgs<T> a, const int2* 
   // the batch's T2 (read by the update and shrink-wrap row passes) is in the
   // pair-of-rows layout, per frame
   constexpr bool t2l = OP == COL_FWD_PROJ_INV || OP == COL_PROJ_INV || OP == COL_FWD_GAUSS_INV || OP == COL_GAUSS_INV;
+  // T1 (written by the update row pass) is in column tiles per frame
+  constexpr bool t1l = OP == COL_FWD_PROJ_INV || OP == COL_FWD_GAUSS_INV || OP == COL_FWD;
   const BFrame f = a.frames[m.x];
   ProjArgs<T> pa = a.pa;
   pa.rf = (RF && a.rslot >= 0) ? f.rf + 2 * (size_t)a.rslot : nullptr;
@@ -86,10 +88,10 @@ __global__ void __launch_bounds__(kBatchThreads) kb_col(BArgs<T> a, const int2* 
 #define CASE(n_)                                                                                              \
   case n_:                                                                                                    \
     if (sizeof(T) == 4 && a.var[bsize_index(n_)])                                                             \
-      col_body<T, n_, OP, MODE, RF, false, false, sizeof(T) == 4 ? 1 : 0, t2l>(                               \
+      col_body<T, n_, OP, MODE, RF, t1l, false, sizeof(T) == 4 ? 1 : 0, t2l>(                                 \
           in, out, n_, a.tw[bsize_index(n_)], I, pa, gs, m.y, a.lsh[bsize_index(n_)], smem);                  \
     else                                                                                                      \
-      col_body<T, n_, OP, MODE, RF, false, false, 0, t2l>(in, out, n_, a.tw[bsize_index(n_)], I, pa, gs, m.y,  \
+      col_body<T, n_, OP, MODE, RF, t1l, false, 0, t2l>(in, out, n_, a.tw[bsize_index(n_)], I, pa, gs, m.y,    \
                                                           a.lsh[bsize_index(n_)], smem);                      \
     break;
     CASE(128) CASE(256) CASE(384) CASE(512)
@@ -104,6 +106,7 @@ __global__ void __launch_bounds__(kBatchThreads) kb_row(BArgs<T> a, const int2* 
   const int2 m = map[blockIdx.x];
   if (m.x < 0) return;
   constexpr bool t2l = OP == ROW_INV_UPDATE_FWD || OP == ROW_INV_MOD2;
+  constexpr bool t1l = OP == ROW_INV_UPDATE_FWD;
   const BFrame f = a.frames[m.x];
   UpdArgs<T> ua;
   ua.algo = ALGO;
@@ -118,12 +121,12 @@ __global__ void __launch_bounds__(kBatchThreads) kb_row(BArgs<T> a, const int2* 
 #define CASE(n_)                                                                                              \
   case n_:                                                                                                    \
     if (sizeof(T) == 4 && a.var[bsize_index(n_)])                                                             \
-      row_body<T, n_, OP, ALGO, false, sizeof(T) == 4 ? 1 : 0, t2l>(in, out, n_, a.tw[bsize_index(n_)],        \
+      row_body<T, n_, OP, ALGO, t1l, sizeof(T) == 4 ? 1 : 0, t2l>(in, out, n_, a.tw[bsize_index(n_)],          \
                                                                     a.g + f.off, a.spt + f.off, ua, (T)1,       \
                                                                     a.spt + f.off, rs, m.y,                     \
                                                                     a.lsh[bsize_index(n_)], smem);              \
     else                                                                                                      \
-      row_body<T, n_, OP, ALGO, false, 0, t2l>(in, out, n_, a.tw[bsize_index(n_)], a.g + f.off, a.spt + f.off, \
+      row_body<T, n_, OP, ALGO, t1l, 0, t2l>(in, out, n_, a.tw[bsize_index(n_)], a.g + f.off, a.spt + f.off,   \
                                                ua, (T)1, a.spt + f.off, rs, m.y, a.lsh[bsize_index(n_)], smem); \
     break;
     CASE(128) CASE(256) CASE(384) CASE(512)

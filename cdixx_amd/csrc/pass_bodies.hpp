@@ -230,7 +230,7 @@ __device__ __forceinline__ void col_body(const cplx<T>* __restrict__ in, cplx<T>
 // rule of ROW_INV_UPDATE_FWD (enum Algorithm).  TILED: the update ops store
 // T1 in its column-tile layout.
 // ---------------------------------------------------------------------------
-template <class T, int N, int OP, int ALGO, bool TILED = false, int V = 0, bool T2L = false>
+template <class T, int N, int OP, int ALGO, bool TILED = false, int V = 0, bool T2L = false, bool GPLAIN = false>
 __device__ __forceinline__ void row_body(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out, int rows,
                                          const cplx<T>* __restrict__ tw, cplx<T>* __restrict__ g,
                                          const T* __restrict__ spt, const UpdArgs<T>& ua, T scale,
@@ -347,9 +347,9 @@ __device__ __forceinline__ void row_body(const cplx<T>* __restrict__ in, cplx<T>
         }
         v[i] = support_update(gv[i], p, in, ALGO >= 0 ? ALGO : ua.algo, ua.beta, ua.relax);
         if (ok) {
-          // the single-frame loop up to 1024^2 reads g back from L2 next
-          // iteration (g and S fit); larger frames and batches stream it
-          if constexpr (TILED && N <= 1024) g[base + L::din_pos(i, t)] = v[i];
+          // GPLAIN: the single-frame loop up to 1024^2 reads g back from L2
+          // next iteration (g and S fit); larger frames and batches stream it
+          if constexpr (GPLAIN) g[base + L::din_pos(i, t)] = v[i];
           else store_stream(&g[base + L::din_pos(i, t)], v[i]);
         }
       }

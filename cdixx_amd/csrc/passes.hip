@@ -114,7 +114,7 @@ k_row(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out, int rows, const
   // two 64-byte halves of a 128-byte T1 tile line leave from the same L2
   // (1024^2: 54.8k -> 55.3k iterations/s; at 2048^2 it cost 3.5 us, so not there)
   constexpr bool xcd_rows = tiled && N == 1024;
-  row_body<T, N, OP, ALGO, tiled, 0, t2l>(in, out, rows, tw, g, spt, ua, scale, sout, rs,
+  row_body<T, N, OP, ALGO, tiled, 0, t2l, tiled && N <= 1024>(in, out, rows, tw, g, spt, ua, scale, sout, rs,
                                           xcd_rows ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x, lsh, smem);
 }
 

@@ -88,10 +88,10 @@ __global__ void __launch_bounds__(kBatchThreads) kb_col(BArgs<T> a, const int2* 
 #define CASE(n_)                                                                                              \
   case n_:                                                                                                    \
     if (sizeof(T) == 4 && a.var[bsize_index(n_)])                                                             \
-      col_body<T, n_, OP, MODE, RF, t1l, false, sizeof(T) == 4 ? 1 : 0, t2l>(                                 \
+      col_body<T, n_, OP, MODE, RF, t1l, true, sizeof(T) == 4 ? 1 : 0, t2l>(                                  \
           in, out, n_, a.tw[bsize_index(n_)], I, pa, gs, m.y, a.lsh[bsize_index(n_)], smem);                  \
     else                                                                                                      \
-      col_body<T, n_, OP, MODE, RF, t1l, false, 0, t2l>(in, out, n_, a.tw[bsize_index(n_)], I, pa, gs, m.y,    \
+      col_body<T, n_, OP, MODE, RF, t1l, true, 0, t2l>(in, out, n_, a.tw[bsize_index(n_)], I, pa, gs, m.y,     \
                                                           a.lsh[bsize_index(n_)], smem);                      \
     break;
     CASE(128) CASE(256) CASE(384) CASE(512)
@@ -506,8 +506,12 @@ struct BLoop {
   }
   int setup() {
     const bool pack_bs = p->useBS && (p->simCCDbit || p->freeBeamstop);
-    BCHK(launch_pack_intensity<T>(b->total, (const T*)b->I, pack_bs ? (const T*)b->bs : nullptr, (T*)b->Ipk,
-                                  !p->simCCDbit, b->st));
+    // packed per frame in the column-tile layout the batched column pass reads
+    for (int i = 0; i < b->nf; i++) {
+      const int64_t o = b->off[i];
+      BCHK(launch_pack_intensity_tiled<T>(b->n[i], b->n[i], (const T*)b->I + o, pack_bs ? (const T*)b->bs + o : nullptr,
+                                          (T*)b->Ipk + o, !p->simCCDbit, b->st));
+    }
     pa = ProjArgs<T>{};
     pa.mode = p->simCCDbit ? PROJ_MOD_NOISE : (pack_bs ? PROJ_MOD_ABS_BS : PROJ_MOD_ABS);
     pa.noiseLevel = p->noiseLevel;

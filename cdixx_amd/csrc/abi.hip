@@ -619,6 +619,9 @@ static int phase_retrieve_t(cdx_ctx* c, const char* algorithm, const cdx_params*
   if (rc) return rc;
   const T rootN = (T)sqrt((double)c->n);
   HIPCHK(launch_norm_c<T>(c->n, g, (T)1 / rootN, c->st));  // cdi.cc:313
+  // the shrink-wrap average is taken on the first shrink-wrap of each call
+  // (a local of CDI::phaseRetrieve, cdi.cc:315,331)
+  HIPCHK(launch_fill_f64(1, c->scal + S_AVG, 0.0, c->st));
   const int nrec = p->recordR ? (rcap < c->ring_cap ? rcap : c->ring_cap) : 0;
   if (nrec > 0) HIPCHK(launch_fill_f64(2 * kRfSlots * (int64_t)c->ring_cap, c->ring, 0.0, c->st));
 

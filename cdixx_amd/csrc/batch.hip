@@ -135,6 +135,11 @@ __global__ void __launch_bounds__(kBatchThreads) kb_row(BArgs<T> a, const int2* 
   }
 }
 
+__global__ void kb_reset_avg(double* scal, int nf) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nf) scal[(size_t)i * kScal + 1] = 0.0;
+}
+
 // per-frame shrink-wrap threshold (frequency form, cdi.cc:330-333; real form
 // holo.cc:141): one wavefront per frame adds its partials in a fixed order
 template <class T>
@@ -602,6 +607,10 @@ int batch_retrieve_t(cdx_batch* b, const char* algorithm, const cdx_params* p, d
   BLoop<T> L{b, p};
   int rc = L.setup();
   if (rc) return rc;
+  // each frame's shrink-wrap average is taken on the first shrink-wrap of
+  // each call (a local of CDI::phaseRetrieve, cdi.cc:315,331)
+  hipLaunchKernelGGL(kb_reset_avg, dim3((b->nf + 63) / 64), dim3(64), 0, b->st, b->scal, b->nf);
+  BCHK(hipGetLastError());
   const int nrec = p->recordR ? std::min(rcap, b->ring_cap) : 0;
   if (nrec > 0) BCHK(launch_fill_f64((int64_t)b->nf * b->ring_cap * 2 * kRfSlots, b->ring, 0.0, b->st));
   if ((rc = L.scale_g(0))) return rc;  // cdi.cc:313

@@ -160,3 +160,31 @@ def test_batch_radix_variants(cdx):
         b64.set_schedule(128, {256: 1})
     assert b64.schedule()[1][256] == 0
     b64.close()
+
+
+def test_batch_shrinkwrap_average_is_per_call(cdx):
+    """Each frame's shrink-wrap average is taken on the first shrink-wrap of
+    each call (cdi.cc:315,331): after a run on 1e4-times fainter patterns, a
+    batch reconstructs the frames bit for bit as a fresh batch does."""
+    print("batch shrink-wrap average per call", flush=True)
+    p = cdx.Params.default(beta=0.9, useBS=1, freeBeamstop=1, shrinkThreshold=0.1)
+    alg = "10*HIO+1*shrinkWrap+3*ER"
+    fresh = cdx.Batch(SIZES)
+    d = _fill(cdx, fresh)
+    fresh.set_threshold(0.5)
+    fresh.phase_retrieve(alg, p)
+    want = [fresh.get(i, "g") for i in range(len(fresh))], fresh.thresholds()
+    fresh.close()
+    b = cdx.Batch(SIZES)
+    for scale in (1e-2, 1.0):
+        for i in range(len(b)):
+            b.put(i, "F", d[i]["F"] * scale)
+            b.put(i, "g", d[i]["g"] * scale)
+            b.put(i, "I", d[i]["I"] * scale ** 2)
+            b.put(i, "bs", d[i]["bs"])
+            b.put(i, "spt", d[i]["spt"])
+        b.set_threshold(0.5)
+        b.phase_retrieve(alg, p)
+    assert np.array_equal(b.thresholds(), want[1])
+    assert all(np.array_equal(b.get(i, "g"), want[0][i]) for i in range(len(b)))
+    b.close()

@@ -216,6 +216,31 @@ def test_loop_with_shrinkwrap_matches_oracle(cdx, o64, mode):
         ctx.close()
 
 
+def test_shrinkwrap_average_is_per_call(cdx, o64):
+    """CDI::phaseRetrieve takes the shrink-wrap average on the first
+    shrink-wrap of each call (a local, cdi.cc:315,331).  A context that has
+    already run a reconstruction of a 1e4-times fainter pattern (whose
+    average would clamp this one's threshold) reconstructs the pattern as the
+    oracle does from a fresh state."""
+    n = 128
+    F, g, I, _ = _injected(o64, n)
+    spt = o64.support_rect(n, n, 2.0)
+    kw = dict(beta=0.9, shrinkThreshold=0.1, shrinkMode=0, sigmaMin=2.0)
+    p = cdx.Params.default(beta=0.9, shrinkThreshold=0.1, shrinkMode=0, sigmaMin=2.0)
+    alg = "10*HIO+1*shrinkWrap+3*ER"
+    ref = o64.phase_retrieve(alg, F, g, I, spt, 0.5, **kw)
+    ctx = cdx.Context(n, precision=1)
+    for scale in (1e-2, 1.0):
+        print(f"shrink-wrap average per call, pattern scale {scale ** 2:g}", flush=True)
+        ctx.set_threshold(0.5)
+        dF, dg, dI, ds = ctx.cplx(F * scale), ctx.cplx(g * scale), ctx.real(I * scale ** 2), ctx.real(spt)
+        ctx.phase_retrieve(alg, p, dF, dg, dI, ds)
+    assert abs(ctx.threshold() - ref["threshold"]) <= 1e-12 * abs(ref["threshold"])
+    assert rel_l2(ctx.get_real(ds), ref["spt"]) < 1e-10
+    assert rel_l2(ctx.get_cplx(dg), ref["g"]) < 1e-10
+    ctx.close()
+
+
 def test_graph_and_stream_launches_agree(cdx, o64):
     n = 256
     F, g, I, spt = _injected(o64, n)

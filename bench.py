@@ -271,10 +271,10 @@ def learned_leg(cdx):
 
 # arms of the 2048^2 learned warm start: (phases from the network, support:
 # "disk" = the 0.2 N disk of the random start or "learned", dilation of the
-# learned support in network pixels).  The network's phases help at this
-# scale; its support, upsampled 8x, cuts the object (the object's 3 px
-# texture averages out at 8x coarser pixels, unlike the training objects')
-# and is reported as a contrast.
+# learned support in network pixels).  Both learned arms are measured
+# against random phases; the upsampled support cuts the object (the object's
+# 3 px texture averages out at 8x coarser pixels, unlike the training
+# objects').
 LARGE_ARMS = {"random": (False, "disk", 0), "learned": (True, "disk", 0), "learned_support": (True, "learned", 2)}
 
 

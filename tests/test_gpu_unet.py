@@ -206,16 +206,17 @@ def test_scaled_warm_start_band_transfer(cdx, nl, dil):
     net.close()
 
 
-def test_scaled_warm_start_gains_at_2048(cdx):
-    """What the network's phases gain on a frame 8x its size per side: a
-    2048^2 frame (object radius 320 px, 1e9 photons, beamstop r = 16), 100 x
-    (19 RAAR + 1 ER + shrink-wrap) from the learned phases with the random
-    start's disk support ends at a lower R-factor than from random phases,
-    and both stay above the true object's noise floor."""
+def test_scaled_warm_start_runs_at_2048(cdx):
+    """The scaled warm start on a frame 8x the network's size per side: a
+    2048^2 frame (object radius 320 px, 1e9 photons, beamstop r = 16) runs
+    100 x (19 RAAR + 1 ER + shrink-wrap) from the network's phases with the
+    random start's disk support, and the support-projected result lies above
+    the true object's noise floor with a finite R-factor.  At this scale the
+    network's phases do not beat random phases under the reference's
+    shrink-wrap rule (final R 0.28-0.32 against 0.23-0.25 on seeds 0-7)."""
     import bench
-    arms = {"random": bench.LARGE_ARMS["random"], "learned": bench.LARGE_ARMS["learned"]}
     print("learned warm start 2048^2 seed 0", flush=True)
-    out = bench.learned_large_leg(cdx, seeds=(0,), arms=arms)
-    r_rand, r_learn, floor = out["random"]["final_R"][0], out["learned"]["final_R"][0], out["R_true_object"][0]
-    print(f"final R random {r_rand} learned {r_learn} floor {floor}", flush=True)
-    assert floor < r_learn < r_rand
+    out = bench.learned_large_leg(cdx, seeds=(0,), arms={"learned": bench.LARGE_ARMS["learned"]})
+    r_learn, floor = out["learned"]["final_R"][0], out["R_true_object"][0]
+    print(f"final R learned {r_learn} floor {floor}", flush=True)
+    assert np.isfinite(r_learn) and floor < r_learn < 0.5

@@ -186,6 +186,15 @@ extern "C" int cdx_create(cdx_ctx** out, int rows, int cols, int precision, void
     cdx_destroy(c);
     return fail(CDX_ERR_HIP, "cdx_create: out of device memory for %dx%d", rows, cols);
   }
+  // raise the LDS caps of every pass this context can launch now, outside
+  // any stream capture
+  hipError_t pe = precision == CDX_F64 ? prepare_col<double>(rows) : prepare_col<float>(rows);
+  if (pe == hipSuccess) pe = precision == CDX_F64 ? prepare_row<double>(cols) : prepare_row<float>(cols);
+  if (pe == hipSuccess) pe = precision == CDX_F64 ? prepare_blur<double>() : prepare_blur<float>();
+  if (pe != hipSuccess) {
+    cdx_destroy(c);
+    return fail(CDX_ERR_HIP, "cdx_create: raising the LDS limits of the passes failed: %s", hipGetErrorString(pe));
+  }
   double init[S_COUNT] = {0};
   init[S_THR] = 0.5;  // init_cuda_image, src/gpu/cdi.cu:20
   if (hipMemcpy(c->scal, init, sizeof init, hipMemcpyHostToDevice) != hipSuccess) {
@@ -732,51 +741,86 @@ extern "C" int cdx_prtf(cdx_ctx* c, double* prtf, double* weights, int cap, int*
 // ---------------------------------------------------------------------------
 // timing
 // ---------------------------------------------------------------------------
+// Events and graphs of a timing run, released on every exit path.
+struct TimingRes {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+  ~TimingRes() {
+    drop_graph();
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  }
+};
+
+// Captures `body` (a sequence of launches on st) into res.graph and
+// instantiates it.  A capture that fails part-way is still ended, so the
+// stream is never left capturing.
+template <class Body>
+static int capture(hipStream_t st, TimingRes& res, Body body) {
+  res.drop_graph();
+  HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  int rc = body();
+  hipError_t e = hipStreamEndCapture(st, &res.graph);
+  if (rc) return rc;
+  HIPCHK(e);
+  HIPCHK(hipGraphInstantiate(&res.exec, res.graph, nullptr, nullptr, 0));
+  return CDX_OK;
+}
+
 template <class T>
 static int time_t_(cdx_ctx* c, const cdx_params* p, int algo, void* F, void* g, const void* I, const void* bs,
                    const void* spt, int n, int use_graph, double* ms_total, double* col_ms, double* row_ms) {
   Loop<T> L{c, p, {}, true, F, g, (T*)spt};
   int rc = L.setup(I, bs);
   if (rc) return rc;
+  TimingRes res;
+  HIPCHK(hipEventCreate(&res.e0));
+  HIPCHK(hipEventCreate(&res.e1));
+  // g as CDI::prepareIter leaves it: the loop carries IFFT(F)/N (cdi.cc:313)
+  const T rootN = (T)sqrt((double)c->n);
+  HIPCHK(launch_norm_c<T>(c->n, g, (T)1 / rootN, c->st));
   // R-factor sums when p->recordR (accumulated into the first ring entry)
   double* rf = p->recordR ? c->ring : nullptr;
   // first iteration leaves the F-pending state; it is not timed
   if ((rc = L.iterate(algo, rf))) return rc;
-  hipEvent_t e0, e1;
-  HIPCHK(hipEventCreate(&e0));
-  HIPCHK(hipEventCreate(&e1));
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
   if (use_graph) {
-    HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-    for (int i = 0; i < n && !rc; i++) rc = L.iterate(algo, rf);
-    hipError_t e = hipStreamEndCapture(c->st, &graph);
+    rc = capture(c->st, res, [&]() {
+      int r = CDX_OK;
+      for (int i = 0; i < n && !r; i++) r = L.iterate(algo, rf);
+      return r;
+    });
     if (rc) return rc;
-    HIPCHK(e);
-    HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-    HIPCHK(hipGraphLaunch(exec, c->st));  // warm the graph once
+    HIPCHK(hipGraphLaunch(res.exec, c->st));  // warm the graph once
   }
   HIPCHK(hipStreamSynchronize(c->st));
-  HIPCHK(hipEventRecord(e0, c->st));
+  HIPCHK(hipEventRecord(res.e0, c->st));
   if (use_graph) {
-    HIPCHK(hipGraphLaunch(exec, c->st));
+    HIPCHK(hipGraphLaunch(res.exec, c->st));
   } else {
     for (int i = 0; i < n && !rc; i++) rc = L.iterate(algo, rf);
     if (rc) return rc;
   }
-  HIPCHK(hipEventRecord(e1, c->st));
-  HIPCHK(hipEventSynchronize(e1));
+  HIPCHK(hipEventRecord(res.e1, c->st));
+  HIPCHK(hipEventSynchronize(res.e1));
   float ms = 0;
-  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  HIPCHK(hipEventElapsedTime(&ms, res.e0, res.e1));
   if (ms_total) *ms_total = ms;
-  if (exec) hipGraphExecDestroy(exec);
-  if (graph) hipGraphDestroy(graph);
+  res.drop_graph();
   if (col_ms || row_ms) {
     // average launch duration of each pass: up to 200 launches of that pass
     // alone, back to back (one captured graph when use_graph, as the loop
-    // runs), between two events; the median of three such runs.  Every column-pass launch reads the same T1
-    // and every row-pass launch the same T2, so each moves the bytes of its
-    // pass in the loop.
+    // runs), between two events; the median of three such runs.  Every
+    // column-pass launch reads the same T1 and every row-pass launch the same
+    // T2, so each moves the bytes of its pass in the loop.  The row-pass
+    // launches re-apply the update to g, so the state afterwards is no
+    // iterate of the loop.
     n = n < 200 ? n : 200;
     UpdArgs<T> ua = upd_args<T>(c, algo, p->beta, p->relax);
     L.pa.rf = rf;
@@ -790,44 +834,38 @@ static int time_t_(cdx_ctx* c, const cdx_params* p, int algo, void* F, void* g, 
         return launch_row<T>(ROW_INV_UPDATE_FWD, c->rows, c->cols, c->T2, c->T1, c->tw_row, g, (const T*)spt, ua,
                              (T)1, nullptr, ReduceSlots{}, c->st, c->row_lines);
       };
-      hipGraph_t pg = nullptr;
-      hipGraphExec_t px = nullptr;
       if (use_graph) {
-        HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-        hipError_t e = hipSuccess;
-        for (int i = 0; i < n && e == hipSuccess; i++) e = launch();
-        hipError_t e2 = hipStreamEndCapture(c->st, &pg);
-        HIPCHK(e);
-        HIPCHK(e2);
-        HIPCHK(hipGraphInstantiate(&px, pg, nullptr, nullptr, 0));
-        HIPCHK(hipGraphLaunch(px, c->st));  // warm the graph once
+        rc = capture(c->st, res, [&]() {
+          for (int i = 0; i < n; i++) HIPCHK(launch());
+          return (int)CDX_OK;
+        });
+        if (rc) return rc;
+        HIPCHK(hipGraphLaunch(res.exec, c->st));  // warm the graph once
       }
       // median of three timed runs (single runs vary by up to 20% between
       // processes on one GPU)
       float pms[3];
       for (int k = 0; k < 3; k++) {
         HIPCHK(hipStreamSynchronize(c->st));
-        HIPCHK(hipEventRecord(e0, c->st));
+        HIPCHK(hipEventRecord(res.e0, c->st));
         if (use_graph) {
-          HIPCHK(hipGraphLaunch(px, c->st));
+          HIPCHK(hipGraphLaunch(res.exec, c->st));
         } else {
           for (int i = 0; i < n; i++) HIPCHK(launch());
         }
-        HIPCHK(hipEventRecord(e1, c->st));
-        HIPCHK(hipEventSynchronize(e1));
-        HIPCHK(hipEventElapsedTime(&pms[k], e0, e1));
+        HIPCHK(hipEventRecord(res.e1, c->st));
+        HIPCHK(hipEventSynchronize(res.e1));
+        HIPCHK(hipEventElapsedTime(&pms[k], res.e0, res.e1));
       }
       std::sort(pms, pms + 3);
       *dst = pms[1] / n;
-      if (px) hipGraphExecDestroy(px);
-      if (pg) hipGraphDestroy(pg);
+      res.drop_graph();
     }
   }
-  // leave F consistent with the state
+  // leave F = FFT(g) and g in the caller's scaling (cdi.cc:406)
   if ((rc = L.finish())) return rc;
+  HIPCHK(launch_norm_c<T>(c->n, g, rootN, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
   return CDX_OK;
 }
 

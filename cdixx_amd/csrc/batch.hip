@@ -363,6 +363,32 @@ extern "C" int cdx_batch_destroy(cdx_batch* b) {
   return CDX_OK;
 }
 
+// raises the dynamic-LDS cap of every batched pass instantiation the loop
+// can launch, before any stream capture
+template <class T>
+static hipError_t batch_prepare() {
+  const void* ks[] = {
+      (const void*)kb_col<T, COL_PROJ_INV, PROJ_MOD_ABS, 0>,    (const void*)kb_col<T, COL_PROJ_INV, PROJ_MOD_ABS, 1>,
+      (const void*)kb_col<T, COL_PROJ_INV, PROJ_MOD_NOISE, 0>,  (const void*)kb_col<T, COL_PROJ_INV, PROJ_MOD_NOISE, 1>,
+      (const void*)kb_col<T, COL_PROJ_INV, PROJ_MOD_ABS_BS, 0>, (const void*)kb_col<T, COL_PROJ_INV, PROJ_MOD_ABS_BS, 1>,
+      (const void*)kb_col<T, COL_FWD_PROJ_INV, PROJ_MOD_ABS, 0>,
+      (const void*)kb_col<T, COL_FWD_PROJ_INV, PROJ_MOD_ABS, 1>,
+      (const void*)kb_col<T, COL_FWD_PROJ_INV, PROJ_MOD_NOISE, 0>,
+      (const void*)kb_col<T, COL_FWD_PROJ_INV, PROJ_MOD_NOISE, 1>,
+      (const void*)kb_col<T, COL_FWD_PROJ_INV, PROJ_MOD_ABS_BS, 0>,
+      (const void*)kb_col<T, COL_FWD_PROJ_INV, PROJ_MOD_ABS_BS, 1>,
+      (const void*)kb_col<T, COL_GAUSS_INV, 0, 0>,              (const void*)kb_col<T, COL_FWD_GAUSS_INV, 0, 0>,
+      (const void*)kb_col<T, COL_FWD, 0, 0>,                    (const void*)kb_row<T, ROW_INV_UPDATE_FWD, 0>,
+      (const void*)kb_row<T, ROW_INV_UPDATE_FWD, 1>,            (const void*)kb_row<T, ROW_INV_UPDATE_FWD, 2>,
+      (const void*)kb_row<T, ROW_INV_UPDATE_FWD, 3>,            (const void*)kb_row<T, ROW_INV_UPDATE_FWD, 4>,
+      (const void*)kb_row<T, ROW_INV_UPDATE_FWD, 5>,            (const void*)kb_row<T, ROW_INV_MOD2, 0>};
+  for (const void* k : ks) {
+    hipError_t e = lds_cap(k);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 extern "C" int cdx_batch_create(cdx_batch** out, int nframes, const int* sizes, int precision, void* stream) {
   if (!out || nframes <= 0 || !sizes) return bfail(CDX_ERR_ARG, "cdx_batch_create: bad arguments");
   if (precision != CDX_F32 && precision != CDX_F64) return bfail(CDX_ERR_ARG, "cdx_batch_create: bad precision");
@@ -422,6 +448,8 @@ extern "C" int cdx_batch_create(cdx_batch** out, int nframes, const int* sizes, 
       hipMalloc((void**)&b->nparts, sizeof(int)))
     return bad(bfail(CDX_ERR_HIP, "cdx_batch_create: out of device memory"));
   if ((rc = batch_tiling(b, precision == CDX_F64 ? bthreads<double>() : bthreads<float>(), b->var))) return bad(rc);
+  if ((precision == CDX_F64 ? batch_prepare<double>() : batch_prepare<float>()) != hipSuccess)
+    return bad(bfail(CDX_ERR_HIP, "cdx_batch_create: raising the LDS limits of the batched passes failed"));
   std::vector<double> sc((size_t)nframes * kScal, 0.0);
   for (int i = 0; i < nframes; i++) sc[(size_t)i * kScal] = 0.5;  // init_cuda_image threshold
   if (hipMemcpy(b->scal, sc.data(), sc.size() * sizeof(double), hipMemcpyHostToDevice))
@@ -453,10 +481,10 @@ extern "C" int cdx_batch_set_threshold(cdx_batch* b, double thr) {
   if (!b) return bfail(CDX_ERR_ARG, "cdx_batch_set_threshold: null batch");
   std::vector<double> h((size_t)b->nf * kScal);
   BCHK(hipMemcpy(h.data(), b->scal, h.size() * sizeof(double), hipMemcpyDeviceToHost));
-  for (int i = 0; i < b->nf; i++) {
-    h[(size_t)i * kScal] = b->prec == CDX_F64 ? thr : (double)(float)thr;
-    h[(size_t)i * kScal + 1] = 0;  // shrink-wrap avg: taken again on the next first call
-  }
+  // the shrink-wrap average is left alone: each cdx_batch_phase_retrieve
+  // takes it again on its first shrink-wrap (kb_reset_avg), as the
+  // single-frame cdx_set_threshold leaves the context's
+  for (int i = 0; i < b->nf; i++) h[(size_t)i * kScal] = b->prec == CDX_F64 ? thr : (double)(float)thr;
   BCHK(hipMemcpy(b->scal, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
   return CDX_OK;
 }
@@ -502,8 +530,10 @@ struct BLoop {
   }
   template <class K>
   hipError_t launch(K kern, int nwg, const int2* map, const BArgs<T>& a, size_t lds) {
+    // the caps are raised at batch creation (batch_prepare); lds_cap only
+    // looks them up here
     if (lds > 64 * 1024) {
-      hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      hipError_t e = lds_cap((const void*)kern);
       if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL(kern, dim3(nwg), dim3(b->threads), lds, b->st, a, map);
@@ -663,44 +693,56 @@ int batch_retrieve_t(cdx_batch* b, const char* algorithm, const cdx_params* p, d
   return CDX_OK;
 }
 
+// events and graph of a batched timing run, released on every exit path
+struct BTimingRes {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~BTimingRes() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  }
+};
+
+// one untimed iteration, with use_graph one untimed launch of the captured
+// n iterations, then n timed iterations; g enters and leaves in the scaling
+// of cdx_batch_phase_retrieve (cdi.cc:313,406)
 template <class T>
 int batch_time_t(cdx_batch* b, const cdx_params* p, int algo, int n, int use_graph, double* ms_out) {
   BLoop<T> L{b, p};
   int rc = L.setup();
   if (rc) return rc;
+  BTimingRes res;
+  BCHK(hipEventCreate(&res.e0));
+  BCHK(hipEventCreate(&res.e1));
+  if ((rc = L.scale_g(0))) return rc;
   if ((rc = L.iterate(algo, -1))) return rc;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
   if (use_graph) {
     BCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
     for (int i = 0; i < n && !rc; i++) rc = L.iterate(algo, -1);
-    hipError_t e = hipStreamEndCapture(b->st, &graph);
+    hipError_t e = hipStreamEndCapture(b->st, &res.graph);
     if (rc) return rc;
     BCHK(e);
-    BCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-    BCHK(hipGraphLaunch(exec, b->st));
+    BCHK(hipGraphInstantiate(&res.exec, res.graph, nullptr, nullptr, 0));
+    BCHK(hipGraphLaunch(res.exec, b->st));
   }
-  hipEvent_t e0, e1;
-  BCHK(hipEventCreate(&e0));
-  BCHK(hipEventCreate(&e1));
   BCHK(hipStreamSynchronize(b->st));
-  BCHK(hipEventRecord(e0, b->st));
+  BCHK(hipEventRecord(res.e0, b->st));
   if (use_graph) {
-    BCHK(hipGraphLaunch(exec, b->st));
+    BCHK(hipGraphLaunch(res.exec, b->st));
   } else {
     for (int i = 0; i < n && !rc; i++) rc = L.iterate(algo, -1);
     if (rc) return rc;
   }
-  BCHK(hipEventRecord(e1, b->st));
-  BCHK(hipEventSynchronize(e1));
+  BCHK(hipEventRecord(res.e1, b->st));
+  BCHK(hipEventSynchronize(res.e1));
   float ms = 0;
-  BCHK(hipEventElapsedTime(&ms, e0, e1));
+  BCHK(hipEventElapsedTime(&ms, res.e0, res.e1));
   *ms_out = ms;
-  if (exec) hipGraphExecDestroy(exec);
-  if (graph) hipGraphDestroy(graph);
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
   if ((rc = L.finish())) return rc;
+  if ((rc = L.scale_g(1))) return rc;
   BCHK(hipStreamSynchronize(b->st));
   return CDX_OK;
 }

@@ -94,22 +94,26 @@ namespace cdx {
 int tiling_valid(int n, size_t csz, int lines);
 int tiling_default(int n, size_t csz, int column_pass);
 
-// Events that receive the start and end timestamps of one kernel's
-// execution (hipExtLaunchKernel), the figure a kernel-trace profiler reports.
-struct KTimer {
-  hipEvent_t start, stop;
-};
-
 // The passes; tw = twiddle table of the transformed line length (2n
-// entries); lines = lines per workgroup (0: default); kt = optional timer.
+// entries); lines = lines per workgroup (0: default).
 template <class T>
 hipError_t launch_col(int op, int rows, int cols, const void* in, void* out, const void* tw,
-                      const T* I, const ProjArgs<T>& pa, double gauss_sigma, hipStream_t st, int lines = 0,
-                      const KTimer* kt = nullptr);
+                      const T* I, const ProjArgs<T>& pa, double gauss_sigma, hipStream_t st, int lines = 0);
 template <class T>
 hipError_t launch_row(int op, int rows, int cols, const void* in, void* out, const void* tw,
                       void* g, const T* spt, const UpdArgs<T>& ua, T scale, T* sout,
-                      const ReduceSlots& rs, hipStream_t st, int lines = 0, const KTimer* kt = nullptr);
+                      const ReduceSlots& rs, hipStream_t st, int lines = 0);
+// Raise the dynamic-LDS cap (160 KiB) of every column-pass instantiation for
+// rows-point columns / row-pass instantiation for cols-point rows, so no
+// attribute call is left for a launch inside a stream capture.
+template <class T> hipError_t prepare_col(int rows);
+template <class T> hipError_t prepare_row(int cols);
+// hipFuncSetAttribute(MaxDynamicSharedMemorySize, 160 KiB) once per kernel
+// and device (thread-safe)
+hipError_t lds_cap(const void* kern);
+// the real-space shrink-wrap blur's column kernel (k_blur_cols<T>), which
+// can also exceed 64 KiB
+template <class T> hipError_t prepare_blur();
 
 // PRTF (cdi.cc:355-380,423-440).  prtf_phase sums the complex partials of
 // ROW_INV_UPDATE_FWD_PRTF: first step phi0 = -arg(sum)/2, theta = phi0;

@@ -22,7 +22,7 @@
 // This file is compiled four times (CDX_PASS_T = float/double, CDX_PASS_ROW =
 // 0/1) so the instantiations build in parallel; CDX_PASS_HOST marks the
 // translation unit that also defines the host-side helpers.
-#include <hip/hip_ext.h>
+#include <algorithm>
 
 #include "pass_bodies.hpp"
 
@@ -121,51 +121,58 @@ k_row(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out, int rows, const
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-// Each kernel instantiation raises its own dynamic-LDS limit (above the
-// 64 KiB default) once, to the largest tiling it is launched with.
-template <class K>
-static hipError_t lds_limit(K kern, size_t bytes, size_t& done) {
-  if (bytes > 64 * 1024 && bytes > done) {
-    hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+// A kernel instantiation whose tilings can exceed the default 64 KiB of
+// dynamic LDS gets the 160 KiB cap once (lds_cap).  prepare_col/prepare_row
+// do that for every instantiation of a line length, so a context raises the
+// caps at creation and no attribute call happens inside a stream capture.
+template <class T, int N>
+static size_t max_lds() {
+  size_t m = 0;
+  for (int lines = 1; lines <= kMaxThreads; lines *= 2)
+    if (tiling_valid(N, sizeof(cplx<T>), lines)) m = std::max(m, lds_bytes<T, N>(log2i(lines)));
+  return m;
+}
+template <class K, class T, int N>
+static hipError_t lds_prepare(K kern, bool& capped) {
+  if (capped) return hipSuccess;
+  if (max_lds<T, N>() > 64 * 1024) {
+    hipError_t e = lds_cap((const void*)kern);
     if (e != hipSuccess) return e;
-    done = bytes;
   }
+  capped = true;
   return hipSuccess;
 }
 #if CDX_PASS_ROW == 0
 template <class T, int N, int OP, int MODE, int RF>
 static hipError_t col_launch_n(int cols, const void* in, void* out, const void* tw, const T* I,
-                               const ProjArgs<T>& pa, double gs, hipStream_t st, int lines, const KTimer* kt) {
-  static size_t lds_done = 0;
+                               const ProjArgs<T>& pa, double gs, hipStream_t st, int lines, bool prep) {
+  static bool capped = false;
   constexpr int TL = LineCfgT<T, N>::TL;
+  auto kern = k_col<T, N, OP, MODE, RF>;
+  hipError_t e = lds_prepare<decltype(kern), T, N>(kern, capped);
+  if (e != hipSuccess || prep) return e;
   if (!tiling_valid(N, sizeof(cplx<T>), lines)) return hipErrorInvalidConfiguration;
   const size_t lds = lds_bytes<T, N>(log2i(lines));
-  auto kern = k_col<T, N, OP, MODE, RF>;
-  hipError_t e = lds_limit(kern, lds, lds_done);
-  if (e != hipSuccess) return e;
   dim3 grid((cols + lines - 1) / lines), block(lines * TL);
-  if (kt)
-    hipExtLaunchKernelGGL(kern, grid, block, (uint32_t)lds, st, kt->start, kt->stop, 0, (const cplx<T>*)in,
-                          (cplx<T>*)out, cols, (const cplx<T>*)tw, I, pa, (T)gs, log2i(lines));
-  else
-    hipLaunchKernelGGL(kern, grid, block, lds, st, (const cplx<T>*)in, (cplx<T>*)out, cols, (const cplx<T>*)tw, I,
-                       pa, (T)gs, log2i(lines));
+  hipLaunchKernelGGL(kern, grid, block, lds, st, (const cplx<T>*)in, (cplx<T>*)out, cols, (const cplx<T>*)tw, I, pa,
+                     (T)gs, log2i(lines));
   return hipGetLastError();
 }
 
 template <class T, int N>
 static hipError_t col_launch_sz(int op, int cols, const void* in, void* out, const void* tw, const T* I,
-                                const ProjArgs<T>& pa, double gs, hipStream_t st, int lines, const KTimer* kt) {
+                                const ProjArgs<T>& pa, double gs, hipStream_t st, int lines, bool prep) {
   const bool rf = pa.rf != nullptr;
   switch (op) {
-    case COL_FWD: return col_launch_n<T, N, COL_FWD, 0, 0>(cols, in, out, tw, I, pa, gs, st, lines, kt);
-    case COL_FWD_T1: return col_launch_n<T, N, COL_FWD_T1, 0, 0>(cols, in, out, tw, I, pa, gs, st, lines, kt);
-    case COL_INV: return col_launch_n<T, N, COL_INV, 0, 0>(cols, in, out, tw, I, pa, gs, st, lines, kt);
+    case COL_FWD: return col_launch_n<T, N, COL_FWD, 0, 0>(cols, in, out, tw, I, pa, gs, st, lines, prep);
+    case COL_FWD_T1: return col_launch_n<T, N, COL_FWD_T1, 0, 0>(cols, in, out, tw, I, pa, gs, st, lines, prep);
+    case COL_INV: return col_launch_n<T, N, COL_INV, 0, 0>(cols, in, out, tw, I, pa, gs, st, lines, prep);
     case COL_FWD_PROJ_INV:
-      if (pa.fsave) return col_launch_n<T, N, COL_FWD_PROJ_INV, -1, -1>(cols, in, out, tw, I, pa, gs, st, lines, kt);
-#define M(mode)                                                                                     \
-  return rf ? col_launch_n<T, N, COL_FWD_PROJ_INV, mode, 1>(cols, in, out, tw, I, pa, gs, st, lines, kt) \
-            : col_launch_n<T, N, COL_FWD_PROJ_INV, mode, 0>(cols, in, out, tw, I, pa, gs, st, lines, kt);
+      if (pa.fsave)
+        return col_launch_n<T, N, COL_FWD_PROJ_INV, -1, -1>(cols, in, out, tw, I, pa, gs, st, lines, prep);
+#define M(mode)                                                                                           \
+  return rf ? col_launch_n<T, N, COL_FWD_PROJ_INV, mode, 1>(cols, in, out, tw, I, pa, gs, st, lines, prep) \
+            : col_launch_n<T, N, COL_FWD_PROJ_INV, mode, 0>(cols, in, out, tw, I, pa, gs, st, lines, prep);
       switch (pa.mode) {
         case PROJ_MOD_ABS: M(PROJ_MOD_ABS)
         case PROJ_MOD_NOISE: M(PROJ_MOD_NOISE)
@@ -173,94 +180,134 @@ static hipError_t col_launch_sz(int op, int cols, const void* in, void* out, con
         default: return hipErrorInvalidValue;
       }
 #undef M
-    case COL_PROJ_INV: return col_launch_n<T, N, COL_PROJ_INV, -1, -1>(cols, in, out, tw, I, pa, gs, st, lines, kt);
+    case COL_PROJ_INV:
+      return col_launch_n<T, N, COL_PROJ_INV, -1, -1>(cols, in, out, tw, I, pa, gs, st, lines, prep);
     case COL_FWD_GAUSS_INV:
-      return col_launch_n<T, N, COL_FWD_GAUSS_INV, 0, 0>(cols, in, out, tw, I, pa, gs, st, lines, kt);
-    case COL_GAUSS_INV: return col_launch_n<T, N, COL_GAUSS_INV, 0, 0>(cols, in, out, tw, I, pa, gs, st, lines, kt);
+      return col_launch_n<T, N, COL_FWD_GAUSS_INV, 0, 0>(cols, in, out, tw, I, pa, gs, st, lines, prep);
+    case COL_GAUSS_INV:
+      return col_launch_n<T, N, COL_GAUSS_INV, 0, 0>(cols, in, out, tw, I, pa, gs, st, lines, prep);
     default: return hipErrorInvalidValue;
   }
 }
 
 template <class T>
-hipError_t launch_col(int op, int rows, int cols, const void* in, void* out, const void* tw, const T* I,
-                      const ProjArgs<T>& pa, double gs, hipStream_t st, int lines, const KTimer* kt) {
+static hipError_t col_dispatch(int op, int rows, int cols, const void* in, void* out, const void* tw, const T* I,
+                               const ProjArgs<T>& pa, double gs, hipStream_t st, int lines, bool prep) {
   if (lines == 0) lines = tiling_default(rows, sizeof(cplx<T>), 1);
   switch (rows) {
-#define F(n) case n: return col_launch_sz<T, n>(op, cols, in, out, tw, I, pa, gs, st, lines, kt);
+#define F(n) case n: return col_launch_sz<T, n>(op, cols, in, out, tw, I, pa, gs, st, lines, prep);
     CDX_SIZES(F)
 #undef F
     default: return hipErrorInvalidValue;
   }
 }
+template <class T>
+hipError_t launch_col(int op, int rows, int cols, const void* in, void* out, const void* tw, const T* I,
+                      const ProjArgs<T>& pa, double gs, hipStream_t st, int lines) {
+  return col_dispatch<T>(op, rows, cols, in, out, tw, I, pa, gs, st, lines, false);
+}
+template <class T>
+hipError_t prepare_col(int rows) {
+  // every instantiation col_launch_sz can pick: each op, and for the
+  // projection ops each mode with and without R-factor sums and the
+  // run-time-mode (PRTF) variant
+  double dummy = 0;
+  for (int op : {COL_FWD, COL_INV, COL_FWD_PROJ_INV, COL_PROJ_INV, COL_FWD_GAUSS_INV, COL_GAUSS_INV, COL_FWD_T1})
+    for (int mode : {PROJ_MOD_ABS, PROJ_MOD_NOISE, PROJ_MOD_ABS_BS})
+      for (int v = 0; v < 3; v++) {
+        ProjArgs<T> pa{};
+        pa.mode = mode;
+        pa.rf = v == 1 ? &dummy : nullptr;
+        pa.fsave = v == 2 ? (void*)&dummy : nullptr;
+        hipError_t e = col_dispatch<T>(op, rows, 0, nullptr, nullptr, nullptr, nullptr, pa, 0, nullptr, 0, true);
+        if (e != hipSuccess) return e;
+      }
+  return hipSuccess;
+}
 template hipError_t launch_col<CDX_PASS_T>(int, int, int, const void*, void*, const void*, const CDX_PASS_T*,
-                                           const ProjArgs<CDX_PASS_T>&, double, hipStream_t, int, const KTimer*);
+                                           const ProjArgs<CDX_PASS_T>&, double, hipStream_t, int);
+template hipError_t prepare_col<CDX_PASS_T>(int);
 #else
 template <class T, int N, int OP, int ALGO>
 static hipError_t row_launch_n(int rows, const void* in, void* out, const void* tw, void* g, const T* spt,
                                const UpdArgs<T>& ua, T scale, T* sout, const ReduceSlots& rs, hipStream_t st,
-                               int lines, const KTimer* kt) {
-  static size_t lds_done = 0;
+                               int lines, bool prep) {
+  static bool capped = false;
   constexpr int TL = LineCfgT<T, N>::TL;
+  auto kern = k_row<T, N, OP, ALGO>;
+  hipError_t e = lds_prepare<decltype(kern), T, N>(kern, capped);
+  if (e != hipSuccess || prep) return e;
   if (!tiling_valid(N, sizeof(cplx<T>), lines)) return hipErrorInvalidConfiguration;
   const size_t lds = lds_bytes<T, N>(log2i(lines));
-  auto kern = k_row<T, N, OP, ALGO>;
-  hipError_t e = lds_limit(kern, lds, lds_done);
-  if (e != hipSuccess) return e;
   dim3 grid((rows + lines - 1) / lines), block(lines * TL);
-  if (kt)
-    hipExtLaunchKernelGGL(kern, grid, block, (uint32_t)lds, st, kt->start, kt->stop, 0, (const cplx<T>*)in,
-                          (cplx<T>*)out, rows, (const cplx<T>*)tw, (cplx<T>*)g, spt, ua, scale, sout, rs,
-                          log2i(lines));
-  else
-    hipLaunchKernelGGL(kern, grid, block, lds, st, (const cplx<T>*)in, (cplx<T>*)out, rows, (const cplx<T>*)tw,
-                       (cplx<T>*)g, spt, ua, scale, sout, rs, log2i(lines));
+  hipLaunchKernelGGL(kern, grid, block, lds, st, (const cplx<T>*)in, (cplx<T>*)out, rows, (const cplx<T>*)tw,
+                     (cplx<T>*)g, spt, ua, scale, sout, rs, log2i(lines));
   return hipGetLastError();
 }
 
 template <class T, int N>
 static hipError_t row_launch_sz(int op, int rows, const void* in, void* out, const void* tw, void* g,
                                 const T* spt, const UpdArgs<T>& ua, T scale, T* sout, const ReduceSlots& rs,
-                                hipStream_t st, int lines, const KTimer* kt) {
+                                hipStream_t st, int lines, bool prep) {
+#define R(op_, a_) \
+  return row_launch_n<T, N, op_, a_>(rows, in, out, tw, g, spt, ua, scale, sout, rs, st, lines, prep);
   switch (op) {
-    case ROW_FWD: return row_launch_n<T, N, ROW_FWD, 0>(rows, in, out, tw, g, spt, ua, scale, sout, rs, st, lines, kt);
-    case ROW_INV: return row_launch_n<T, N, ROW_INV, 0>(rows, in, out, tw, g, spt, ua, scale, sout, rs, st, lines, kt);
-    case ROW_INV_MOD2:
-      return row_launch_n<T, N, ROW_INV_MOD2, 0>(rows, in, out, tw, g, spt, ua, scale, sout, rs, st, lines, kt);
+    case ROW_FWD: R(ROW_FWD, 0)
+    case ROW_INV: R(ROW_INV, 0)
+    case ROW_INV_MOD2: R(ROW_INV_MOD2, 0)
     case ROW_INV_UPDATE_FWD:
-#define A(a) \
-  return row_launch_n<T, N, ROW_INV_UPDATE_FWD, a>(rows, in, out, tw, g, spt, ua, scale, sout, rs, st, lines, kt);
       switch (ua.algo) {
-        case 0: A(0)
-        case 1: A(1)
-        case 2: A(2)
-        case 3: A(3)
-        case 4: A(4)
-        default: A(5)  // FHIO and every other token (the dispatcher's default)
+        case 0: R(ROW_INV_UPDATE_FWD, 0)
+        case 1: R(ROW_INV_UPDATE_FWD, 1)
+        case 2: R(ROW_INV_UPDATE_FWD, 2)
+        case 3: R(ROW_INV_UPDATE_FWD, 3)
+        case 4: R(ROW_INV_UPDATE_FWD, 4)
+        default: R(ROW_INV_UPDATE_FWD, 5)  // FHIO and every other token (the dispatcher's default)
       }
-#undef A
     case ROW_INV_UPDATE_FWD_PRTF:
-      if (2 * (size_t)((rows + lines - 1) / lines) > (size_t)kMaxPartials) return hipErrorInvalidConfiguration;
-      return row_launch_n<T, N, ROW_INV_UPDATE_FWD_PRTF, -1>(rows, in, out, tw, g, spt, ua, scale, sout, rs, st,
-                                                             lines, kt);
+      if (!prep && 2 * (size_t)((rows + lines - 1) / lines) > (size_t)kMaxPartials)
+        return hipErrorInvalidConfiguration;
+      R(ROW_INV_UPDATE_FWD_PRTF, -1)
     default: return hipErrorInvalidValue;
   }
+#undef R
 }
 
 template <class T>
-hipError_t launch_row(int op, int rows, int cols, const void* in, void* out, const void* tw, void* g,
-                      const T* spt, const UpdArgs<T>& ua, T scale, T* sout, const ReduceSlots& rs,
-                      hipStream_t st, int lines, const KTimer* kt) {
+static hipError_t row_dispatch(int op, int rows, int cols, const void* in, void* out, const void* tw, void* g,
+                               const T* spt, const UpdArgs<T>& ua, T scale, T* sout, const ReduceSlots& rs,
+                               hipStream_t st, int lines, bool prep) {
   if (lines == 0) lines = tiling_default(cols, sizeof(cplx<T>), 0);
   switch (cols) {
-#define F(n) case n: return row_launch_sz<T, n>(op, rows, in, out, tw, g, spt, ua, scale, sout, rs, st, lines, kt);
+#define F(n) \
+  case n: return row_launch_sz<T, n>(op, rows, in, out, tw, g, spt, ua, scale, sout, rs, st, lines, prep);
     CDX_SIZES(F)
 #undef F
     default: return hipErrorInvalidValue;
   }
 }
+template <class T>
+hipError_t launch_row(int op, int rows, int cols, const void* in, void* out, const void* tw, void* g,
+                      const T* spt, const UpdArgs<T>& ua, T scale, T* sout, const ReduceSlots& rs,
+                      hipStream_t st, int lines) {
+  return row_dispatch<T>(op, rows, cols, in, out, tw, g, spt, ua, scale, sout, rs, st, lines, false);
+}
+template <class T>
+hipError_t prepare_row(int cols) {
+  for (int op : {ROW_FWD, ROW_INV, ROW_INV_MOD2, ROW_INV_UPDATE_FWD, ROW_INV_UPDATE_FWD_PRTF})
+    for (int algo = 0; algo <= 5; algo++) {
+      UpdArgs<T> ua{};
+      ua.algo = algo;
+      hipError_t e = row_dispatch<T>(op, 0, cols, nullptr, nullptr, nullptr, nullptr, nullptr, ua, (T)1, nullptr,
+                                     ReduceSlots{}, nullptr, 0, true);
+      if (e != hipSuccess) return e;
+    }
+  return hipSuccess;
+}
 template hipError_t launch_row<CDX_PASS_T>(int, int, int, const void*, void*, const void*, void*,
                                            const CDX_PASS_T*, const UpdArgs<CDX_PASS_T>&, CDX_PASS_T, CDX_PASS_T*,
-                                           const ReduceSlots&, hipStream_t, int, const KTimer*);
+                                           const ReduceSlots&, hipStream_t, int);
+template hipError_t prepare_row<CDX_PASS_T>(int);
 #endif
 
 }  // namespace cdx

@@ -199,7 +199,10 @@ int cdx_batch_synchronize(cdx_batch* b);
  * nframes x rtrace_cap R-factors, frame-major. */
 int cdx_batch_phase_retrieve(cdx_batch* b, const char* algorithm, const cdx_params* p, double* rtrace,
                              int rtrace_cap, int* n_iter);
-/* times n iterations of `algo` for the whole batch (graph or stream launches) */
+/* times n iterations of `algo` for the whole batch (graph or stream launches):
+ * one untimed iteration, with use_graph one untimed launch of the captured n
+ * iterations, then n timed ones; F and g enter and leave as
+ * cdx_batch_phase_retrieve takes and leaves them */
 int cdx_batch_time_iterations(cdx_batch* b, const cdx_params* p, int algo, int n, int use_graph, double* ms);
 /* threads per workgroup of both batched passes (64, 128 or 256; frame size n
  * runs threads / lanes(n) lines per workgroup); CDX_ERR_ARG, tiling
@@ -233,13 +236,18 @@ int cdx_expand_schedule(const char* algorithm, int* out, int cap);
 int cdx_read_config_json(const char* path, char* buf, int cap);
 
 /* ---- measurement helpers (timing with HIP events on the context stream) ---- */
-/* Runs n_iter iterations of `algo` from the current state (F pending on the
- * first call) without shrink-wrap.  ms_total: wall time of the n_iter
- * iterations between two events.  If col_ms/row_ms are given, each pass is
- * then launched min(n_iter, 200) times on its own, back to back (one graph
- * when use_graph), between two events, three times, and the median mean
- * time per launch of the column pass and of the row pass is returned.  With p->recordR the column pass
- * also forms the R-factor sums (the cost of recording R every iteration). */
+/* Runs iterations of `algo` from the state F, g (g as cdx_phase_retrieve
+ * takes it, IFFT(F)/sqrt(N)) without shrink-wrap: one untimed iteration, with
+ * use_graph one untimed launch of the captured n_iter iterations, then n_iter
+ * timed iterations (so 1 + n_iter, or 1 + 2 n_iter with use_graph, in all).
+ * ms_total: wall time of the n_iter timed iterations between two events.
+ * F and g are left as cdx_phase_retrieve leaves them.  If col_ms/row_ms are
+ * given, each pass is then launched min(n_iter, 200) times on its own, back
+ * to back (one graph when use_graph), between two events, three times, and
+ * the median mean time per launch of the column pass and of the row pass is
+ * returned; those launches re-apply the update to the same input, so F and g
+ * are then no iterate of the loop.  With p->recordR the column pass also
+ * forms the R-factor sums (the cost of recording R every iteration). */
 int cdx_time_iterations(cdx_ctx* ctx, const cdx_params* p, int algo, void* F, void* g, const void* I,
                         const void* bs, const void* spt, int n_iter, int use_graph, double* ms_total,
                         double* col_ms, double* row_ms);

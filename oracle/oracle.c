@@ -544,11 +544,24 @@ double orc_residual(int64_t n, const ocpx* F, const REAL* I, const REAL* bs) {
 /* One iteration of the loop body, src/post_gpu/cdi.cc:349-354,381:           */
 /* projection -> IFFT -> 1/N -> support update -> FFT.  gp is scratch.        */
 /* ------------------------------------------------------------------------ */
+/* The projection of cdi.cc:349-350: applyMod (simCCDbit) or applyModAbs;
+ * with freeBeamstop (this build's mode, applyModAbs path) pixels with
+ * bs > 0.5 keep F. */
+static void orc_project(int64_t n, ocpx* F, const REAL* I, const REAL* bs, const orc_params* p) {
+  if (p->simCCDbit) {
+    orc_apply_mod(n, F, I, p->useBS ? bs : NULL, p->noiseLevel, p->scale, p->rcolor, 1);
+  } else if (p->freeBeamstop && p->useBS && bs) {
+    for (int64_t i = 0; i < n; i++)
+      if (!(bs[i] > (REAL)0.5)) orc_apply_mod_abs(1, F + i, I + i);
+  } else {
+    orc_apply_mod_abs(n, F, I);
+  }
+}
+
 void orc_iteration(int rows, int cols, ocpx* F, ocpx* g, const REAL* I, const REAL* bs,
                    const REAL* spt, int algo, const orc_params* p, REAL threshold, ocpx* gp) {
   int64_t n = (int64_t)rows * cols;
-  if (p->simCCDbit) orc_apply_mod(n, F, I, p->useBS ? bs : NULL, p->noiseLevel, p->scale, p->rcolor, 1);
-  else orc_apply_mod_abs(n, F, I);
+  orc_project(n, F, I, bs, p);
   orc_fft2d(rows, cols, F, gp, +1);
   orc_apply_norm(n, gp, (REAL)(1.0 / (double)n));
   orc_apply_support(n, g, gp, algo, spt, threshold, p->beta, p->relax);
@@ -644,8 +657,11 @@ int orc_phase_retrieve_prtf(int rows, int cols, const char* algorithm, ocpx* F, 
     }
     if (a == O_TV) { rc = -1; break; }
     if (rtrace && nr < rtrace_cap) {
+      /* beamstop pixels are left out where the projection leaves them
+       * unconstrained (applyMod with useBS, or the free-beamstop mode) */
+      const int bs_free = p->useBS && (p->simCCDbit || p->freeBeamstop);
       double num, den;
-      orc_rfactor(n, F, I, p->useBS ? bs : NULL, &num, &den);
+      orc_rfactor(n, F, I, bs_free ? bs : NULL, &num, &den);
       rtrace[nr++] = den > 0 ? num / den : 0;
     }
     if (prtfIter < 0 || iter < prtfIter) {
@@ -653,8 +669,7 @@ int orc_phase_retrieve_prtf(int rows, int cols, const char* algorithm, ocpx* F, 
       continue;
     }
     /* orc_iteration split around the PRTF accumulation */
-    if (p->simCCDbit) orc_apply_mod(n, F, I, p->useBS ? bs : NULL, p->noiseLevel, p->scale, p->rcolor, 1);
-    else orc_apply_mod_abs(n, F, I);
+    orc_project(n, F, I, bs, p);
     orc_fft2d(rows, cols, F, gp, +1);
     orc_apply_norm(n, gp, (REAL)(1.0 / (double)n));
     orc_apply_support(n, g, gp, a, spt, st.threshold, p->beta, p->relax);

@@ -78,6 +78,9 @@ double orc_residual(int64_t n, const ocpx* F, const REAL* I, const REAL* bs);
 typedef struct {
   int simCCDbit, useBS, noiseLevel, rcolor, shrinkMode, startSign;
   REAL scale, beta, relax, shrinkThreshold, sigmaStart, sigmaMin, sigmaDecay;
+  /* applyModAbs path only: pixels with bs > 0.5 keep F (the beamstop is left
+   * unconstrained); a mode of this build, the reference has no such switch */
+  int freeBeamstop;
 } orc_params;
 void orc_iteration(int rows, int cols, ocpx* F, ocpx* g, const REAL* I, const REAL* bs,
                    const REAL* spt, int algo, const orc_params* p, REAL threshold, ocpx* gp);

@@ -26,7 +26,8 @@ def _params_type(real):
         _fields_ = [("simCCDbit", ctypes.c_int), ("useBS", ctypes.c_int), ("noiseLevel", ctypes.c_int),
                     ("rcolor", ctypes.c_int), ("shrinkMode", ctypes.c_int), ("startSign", ctypes.c_int),
                     ("scale", real), ("beta", real), ("relax", real), ("shrinkThreshold", real),
-                    ("sigmaStart", real), ("sigmaMin", real), ("sigmaDecay", real)]
+                    ("sigmaStart", real), ("sigmaMin", real), ("sigmaDecay", real),
+                    ("freeBeamstop", ctypes.c_int)]
     return P
 
 
@@ -62,6 +63,7 @@ class Oracle:
             "orc_phase_retrieve": [i, i, ctypes.c_char_p, vp, vp, vp, vp, vp, vp, vp, vp, i, vp],
             "orc_phase_retrieve_prtf": [i, i, ctypes.c_char_p, vp, vp, vp, vp, vp, vp, vp, vp, i, vp, i, vp, vp, vp],
             "orc_add_phase": [i64, vp, vp, r], "orc_radial_sum": [i, i, vp, vp],
+            "orc_shrink_wrap_freq": [i, i, vp, vp, r, vp, vp],
         }
         for name, args in sig.items():
             getattr(L, name).argtypes = args
@@ -236,6 +238,20 @@ class Oracle:
         if prtf_iter >= 0:
             out.update(prtf=prtf, weights=wts, prtf_count=cnt.value)
         return out
+
+    def sw_state(self, sigma=3.0, avg=0.0, threshold=0.5, decay=0.99):
+        """orc_sw_state {sigma, avg, threshold, decay} of the frequency-form shrink-wrap"""
+        return (self.creal * 4)(sigma, avg, threshold, decay)
+
+    def shrink_wrap_freq(self, F, shrink_threshold, state):
+        """one frequency-form shrink-wrap step on the Fourier estimate F:
+        returns the new (non-binary) support; `state` (sw_state) is updated"""
+        F = self._c(F)
+        spt = np.empty(F.shape, self.real)
+        scratch = np.empty_like(F)
+        self.lib.orc_shrink_wrap_freq(F.shape[0], F.shape[1], F.ctypes.data, spt.ctypes.data, shrink_threshold,
+                                      ctypes.cast(state, ctypes.c_void_p), scratch.ctypes.data)
+        return spt
 
     def add_phase(self, a, b, phi):
         a, b = self._c(a).copy(), self._c(b)

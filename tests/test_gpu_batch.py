@@ -188,3 +188,92 @@ def test_batch_shrinkwrap_average_is_per_call(cdx):
     assert np.array_equal(b.thresholds(), want[1])
     assert all(np.array_equal(b.get(i, "g"), want[0][i]) for i in range(len(b)))
     b.close()
+
+
+# the 64-frame mix of the benchmark's batched workload
+CFG3_SIZES = [256] * 24 + [384] * 24 + [512] * 16
+
+
+def _cfg3_batch(cdx, seed=3):
+    """The benchmark's batched workload: 24x256^2 + 24x384^2 + 16x512^2,
+    object radius 0.16 N, photons log-uniform in [1e6, 1e8], per-frame seeds,
+    disk support 1.25x the object radius, random-phase starts."""
+    rs = np.random.default_rng(seed)
+    photons = 10 ** rs.uniform(6, 8, len(CFG3_SIZES))
+    b = cdx.Batch(CFG3_SIZES)
+    b.simulate([0.16 * n for n in CFG3_SIZES], photons, list(range(len(CFG3_SIZES))), support_scale=1.25)
+    return b
+
+
+def test_batch_cfg3_mix_matches_oracle_per_frame(cdx, o64):
+    """Every frame of the 64-frame mix after three HIO steps (beta 0.9) in one
+    batch, against the FP64 oracle run on that frame alone (CDI::phaseRetrieve,
+    cdi.cc:297-449): 1e-5 relative L2 on g and F, under the autotuned
+    schedule and under the variant-1 radices at 128 threads."""
+    print("cfg3 mix vs oracle", flush=True)
+    b = _cfg3_batch(cdx)
+    d = {i: {k: b.get(i, k) for k in ("F", "g", "I", "spt")} for i in range(len(b))}
+    p = cdx.Params.default(beta=0.9)
+    refs = [o64.phase_retrieve("3*HIO", d[i]["F"], d[i]["g"], d[i]["I"], d[i]["spt"], 0.5, beta=0.9)
+            for i in range(len(b))]
+    b.autotune(reps=5)
+    for sched in ("autotuned", "variant1"):
+        if sched == "variant1":
+            b.set_schedule(128, {n: 1 for n in (128, 256, 384, 512)})
+        for i in range(len(b)):
+            b.put(i, "F", d[i]["F"])
+            b.put(i, "g", d[i]["g"])
+        b.set_threshold(0.5)
+        nit, _ = b.phase_retrieve("3*HIO", p)
+        assert nit == 3
+        worst = 0.0
+        for i, n in enumerate(CFG3_SIZES):
+            eg, eF = rel_l2(b.get(i, "g"), refs[i]["g"]), rel_l2(b.get(i, "F"), refs[i]["F"])
+            worst = max(worst, eg, eF)
+            assert eg < 1e-5 and eF < 1e-5, (sched, i, n, eg, eF)
+        print(f"  {sched} {b.schedule()[2]}: worst rel-L2 {worst:.2e}", flush=True)
+    b.close()
+
+
+def test_batch_cfg3_mix_500_hio_properties(cdx, o64):
+    """The full 64-frame mix through the benchmark's 500 HIO iterations, held
+    to properties that do not depend on a CPU run of that length:
+      - from the exact solution (F = FFT(obj), I = |F|^2, the object's disk
+        as support) HIO is at a fixed point, so every frame's object stays
+        put: within 1e-4 relative L2 after 500 steps (the FP32 oracle drifts
+        8e-6 at 256^2 and 512^2) and R stays below 1e-4;
+      - from random starts every frame's R-factor trace is finite and ends
+        below where it started."""
+    print("cfg3 mix 500 HIO: fixed point", flush=True)
+    b = cdx.Batch(CFG3_SIZES)
+    objs = []
+    for i, n in enumerate(CFG3_SIZES):
+        ctx = b.context(i)
+        f = b.frame(i)
+        ctx.sim_object(f["F"], 0.16 * n, seed=i)
+        obj = ctx.get_cplx(f["F"]).astype(np.complex128)
+        objs.append(obj)
+        F = np.fft.fft2(obj)
+        b.put(i, "F", F)
+        b.put(i, "g", np.fft.ifft2(F) * n)
+        b.put(i, "I", np.abs(F) ** 2)
+        ctx.create_circle(f["spt"], 0.16 * n + 1)
+    b.set_threshold(0.5)
+    p = cdx.Params.default(beta=0.9, recordR=1)
+    nit, rt = b.phase_retrieve("500*HIO", p, rtrace_cap=500)
+    assert nit == 500
+    worst = 0.0
+    for i, n in enumerate(CFG3_SIZES):
+        e = rel_l2(b.get(i, "g") / n, objs[i])
+        worst = max(worst, e)
+        assert e < 1e-4 and rt[i].max() < 1e-4, (i, n, e, rt[i].max())
+    print(f"  worst drift {worst:.2e}", flush=True)
+    b.close()
+    print("cfg3 mix 500 HIO: random starts", flush=True)
+    b = _cfg3_batch(cdx)
+    b.set_threshold(0.5)
+    nit, rt = b.phase_retrieve("500*HIO", p, rtrace_cap=500)
+    assert np.isfinite(rt).all()
+    assert (rt[:, -1] < rt[:, 0]).all(), rt[:, [0, -1]]
+    print(f"  R start {np.median(rt[:, 0]):.3f} -> end {np.median(rt[:, -1]):.3f} (medians)", flush=True)
+    b.close()

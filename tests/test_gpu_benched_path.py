@@ -1,0 +1,173 @@
+"""GPU parity of the path as the benchmark runs it, against the CPU oracle.
+
+The benchmark's timed loop runs the steady-state column pass on the tiled
+loop state T1 (forward column FFT, modulus projection leaving the beamstop
+free, inverse column FFT) and the RAAR row pass, at the tiling the autotuner
+picks, launched from a captured graph.  Only iterations after the first run
+that column pass (the first projects F directly), so every test here runs
+three or more steps.
+
+Tolerances (relative L2 against the FP64 oracle unless stated):
+  - three RAAR steps from an identical state: 1e-5 in FP32 (the FP32 oracle
+    itself is 1.3e-6 away at 1024^2);
+  - shrink-wrap steps from an identical injected state (the state the FP64
+    oracle reaches after 20 RAAR steps): threshold 1e-6 relative, support
+    1e-5, g and F 1e-5 (the FP32 oracle: 1e-8, 1e-7, 4e-7);
+  - a schedule from a random start, where 20 RAAR steps amplify rounding
+    ~1000-fold (the FP32 oracle ends 1-3% away from the FP64 one in g): the
+    FP32 GPU path must land no further from the FP64 oracle than three times
+    the FP32 oracle does, plus 1e-5.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+BENCH = dict(beta=0.85, useBS=1, freeBeamstop=1)
+
+
+@pytest.fixture(scope="module")
+def cdx():
+    import cdixx_amd
+    return cdixx_amd
+
+
+def bench_inputs(cdx, n, seed=0):
+    """The benchmark's inputs at n^2 (cdx generator, FP32): object radius
+    0.16 n, 1e8 photons (1e9 at 2048^2) with Poisson noise, beamstop radius
+    n/128, disk support 0.2 n, random-phase start, g = IFFT(F)/sqrt(N)."""
+    ctx = cdx.Context(n)
+    obj, I, bs, spt, F, g = ctx.cplx(), ctx.real(), ctx.real(), ctx.real(), ctx.cplx(), ctx.cplx()
+    ctx.sim_object(obj, 0.16 * n, seed=seed)
+    ctx.sim_intensity(obj, I, 1e9 if n >= 2048 else 1e8, seed=seed, poisson=True)
+    ctx.create_beamstop(bs, n / 128)
+    ctx.create_circle(spt, 0.2 * n)
+    ctx.random_start(I, bs, F, seed=seed, mode=1)
+    ctx.fft(F, g, cdx.INVERSE)
+    ctx.apply_norm(g, 1.0 / n)
+    out = dict(I=ctx.get_real(I), bs=ctx.get_real(bs), spt=ctx.get_real(spt), F=ctx.get_cplx(F), g=ctx.get_cplx(g))
+    ctx.close()
+    return out
+
+
+def _upload(ctx, d):
+    return ctx.cplx(d["F"]), ctx.cplx(d["g"]), ctx.real(d["I"]), ctx.real(d["spt"]), ctx.real(d["bs"])
+
+
+@pytest.mark.parametrize("n", [1024, 2048])
+@pytest.mark.parametrize("record", [0, 1])
+def test_benched_column_pass_three_steps(cdx, o64, o32, n, record):
+    """'3*RAAR' with the benchmark's parameters (beta 0.85, useBS, free
+    beamstop, autotuned tiling, graph launch) from the benchmark's inputs:
+    steps 2 and 3 run k_col<float, n, COL_FWD_PROJ_INV, PROJ_MOD_ABS_BS, RF>
+    on tiled T1, the benchmark's kernel (RF = 1 when recording R)."""
+    print(f"benched path n={n} recordR={record}", flush=True)
+    d = bench_inputs(cdx, n)
+    ref = o64.phase_retrieve("3*RAAR", d["F"], d["g"], d["I"], d["spt"], 0.5, bs=d["bs"], rtrace_cap=3, **BENCH)
+    r32 = o32.phase_retrieve("3*RAAR", d["F"], d["g"], d["I"], d["spt"], 0.5, bs=d["bs"], **BENCH)
+    ctx = cdx.Context(n)
+    ctx.autotune(reps=10)
+    ctx.set_threshold(0.5)
+    dF, dg, dI, ds, dbs = _upload(ctx, d)
+    p = cdx.Params.default(recordR=record, useGraph=1, **BENCH)
+    nit, rt = ctx.phase_retrieve("3*RAAR", p, dF, dg, dI, ds, bs=dbs, rtrace_cap=3)
+    assert nit == 3
+    eg, eF = rel_l2(ctx.get_cplx(dg), ref["g"]), rel_l2(ctx.get_cplx(dF), ref["F"])
+    e32 = rel_l2(r32["g"], ref["g"])
+    print(f"  tiling {ctx.tiling()}: g {eg:.2e} F {eF:.2e} (FP32 oracle {e32:.2e})", flush=True)
+    assert eg < 1e-5 and eF < 1e-5, (eg, eF)
+    if record:
+        assert np.allclose(rt, ref["rtrace"], rtol=1e-5, atol=0), (rt, ref["rtrace"])
+    ctx.close()
+
+
+@pytest.mark.parametrize("n", [1024, 2048])
+def test_benched_timing_entry_point_iterates(cdx, o64, n):
+    """cdx_time_iterations, the benchmark's timed entry point, advances the
+    state: with one graph iteration and no per-kernel timing it runs the
+    untimed first step, the graph's warm-up launch and the timed launch, i.e.
+    three RAAR steps, and leaves F = FFT(g) of the last one."""
+    print(f"timing entry point n={n}", flush=True)
+    d = bench_inputs(cdx, n, seed=1)
+    ref = o64.phase_retrieve("3*RAAR", d["F"], d["g"], d["I"], d["spt"], 0.5, bs=d["bs"], **BENCH)
+    ctx = cdx.Context(n)
+    ctx.autotune(reps=10)
+    ctx.set_threshold(0.5)
+    dF, dg, dI, ds, dbs = _upload(ctx, d)
+    p = cdx.Params.default(**BENCH)
+    ctx.time_iterations(p, cdx.RAAR, dF, dg, dI, ds, 1, bs=dbs, use_graph=True, per_kernel=False)
+    eg = rel_l2(ctx.get_cplx(dg), ref["g"])
+    eF = rel_l2(ctx.get_cplx(dF), ref["F"])
+    print(f"  g {eg:.2e} F {eF:.2e}", flush=True)
+    assert eg < 1e-5 and eF < 1e-5, (eg, eF)
+    ctx.close()
+
+
+def _pre_shrinkwrap_state(o64, d):
+    """the state the FP64 oracle reaches after 20 RAAR steps, as
+    cdx_phase_retrieve takes it (g = IFFT(F)/sqrt(N) scaled, F = FFT(g))"""
+    s = o64.phase_retrieve("20*RAAR", d["F"], d["g"], d["I"], d["spt"], 0.5, bs=d["bs"], **BENCH)
+    return s["F"], s["g"]
+
+
+@pytest.mark.parametrize("n", [256, 1024])
+@pytest.mark.parametrize("alg", ["1*shrinkWrap+3*ER", "1*RAAR+1*shrinkWrap+3*ER", "2*(3*RAAR+1*shrinkWrap)+2*ER"])
+def test_fp32_shrinkwrap_from_injected_state(cdx, o64, o32, n, alg):
+    """FP32 frequency-form shrink-wrap (cdi.cc:326-341) and the steps after it
+    from an identical state: the first-call average, the threshold
+    min(max, 5 avg) * t, the blurred support S and the iterates.  The three
+    schedules cover the shrink-wrap column pass from F (first token) and from
+    the tiled loop state T1, and a second shrink-wrap that keeps the first
+    call's average."""
+    print(f"fp32 shrink-wrap n={n} {alg}", flush=True)
+    d = bench_inputs(cdx, n, seed=2)
+    F0, g0 = _pre_shrinkwrap_state(o64, d)
+    kw = dict(shrinkThreshold=0.02, **BENCH)
+    ref = o64.phase_retrieve(alg, F0, g0, d["I"], d["spt"], 0.5, bs=d["bs"], **kw)
+    r32 = o32.phase_retrieve(alg, F0, g0, d["I"], d["spt"], 0.5, bs=d["bs"], **kw)
+    ctx = cdx.Context(n)
+    ctx.set_threshold(0.5)
+    dF, dg, dI, ds, dbs = ctx.cplx(F0), ctx.cplx(g0), ctx.real(d["I"]), ctx.real(d["spt"]), ctx.real(d["bs"])
+    ctx.phase_retrieve(alg, cdx.Params.default(**kw), dF, dg, dI, ds, bs=dbs)
+    thr, S = ctx.threshold(), ctx.get_real(ds)
+    et = abs(thr - ref["threshold"]) / ref["threshold"]
+    es, eg, eF = rel_l2(S, ref["spt"]), rel_l2(ctx.get_cplx(dg), ref["g"]), rel_l2(ctx.get_cplx(dF), ref["F"])
+    flips = int(((S > thr) != (ref["spt"] > ref["threshold"])).sum())
+    print(f"  threshold {et:.2e} S {es:.2e} g {eg:.2e} F {eF:.2e} mask flips {flips}; FP32 oracle: "
+          f"threshold {abs(r32['threshold'] - ref['threshold']) / ref['threshold']:.2e} "
+          f"S {rel_l2(r32['spt'], ref['spt']):.2e} g {rel_l2(r32['g'], ref['g']):.2e}", flush=True)
+    assert et < 1e-6 and es < 1e-5, (et, es)
+    assert flips == 0
+    assert eg < 1e-5 and eF < 1e-5, (eg, eF)
+    ctx.close()
+
+
+@pytest.mark.parametrize("n", [256, 1024])
+def test_fp32_shrinkwrap_schedule_from_random_start(cdx, o64, o32, n):
+    """'20*RAAR+1*shrinkWrap+3*ER' from the benchmark's random start.  The
+    RAAR steps amplify rounding by ~1000 here, so the FP32 oracle itself ends
+    1-3% from the FP64 oracle; the FP32 GPU path is held to the same
+    distance: threshold, support and iterate each within 3x the FP32
+    oracle's distance + 1e-5."""
+    print(f"fp32 shrink-wrap schedule n={n}", flush=True)
+    d = bench_inputs(cdx, n, seed=0)
+    alg = "20*RAAR+1*shrinkWrap+3*ER"
+    kw = dict(shrinkThreshold=0.02, **BENCH)
+    ref = o64.phase_retrieve(alg, d["F"], d["g"], d["I"], d["spt"], 0.5, bs=d["bs"], **kw)
+    r32 = o32.phase_retrieve(alg, d["F"], d["g"], d["I"], d["spt"], 0.5, bs=d["bs"], **kw)
+    ctx = cdx.Context(n)
+    ctx.set_threshold(0.5)
+    dF, dg, dI, ds, dbs = _upload(ctx, d)
+    ctx.phase_retrieve(alg, cdx.Params.default(**kw), dF, dg, dI, ds, bs=dbs)
+    got = dict(threshold=ctx.threshold(), spt=ctx.get_real(ds), g=ctx.get_cplx(dg))
+    for k in ("threshold", "spt", "g"):
+        if k == "threshold":
+            e_gpu = abs(got[k] - ref[k]) / ref[k]
+            e_o32 = abs(r32[k] - ref[k]) / ref[k]
+        else:
+            e_gpu, e_o32 = rel_l2(got[k], ref[k]), rel_l2(r32[k], ref[k])
+        print(f"  {k}: GPU FP32 {e_gpu:.2e}, oracle FP32 {e_o32:.2e}", flush=True)
+        assert e_gpu <= 3 * e_o32 + 1e-5, (k, e_gpu, e_o32)
+    ctx.close()

@@ -17,7 +17,11 @@ term, wrong sign or index, transposed operand) fails:
                         projection; POSER output Re >= 0 and zero outside; HIO
                         fixed point; each branch against its defining identity.
   shrink-wrap (a8)      frequency Gaussian == periodic spatial convolution with
-                        sigma_x = rows/(2 pi sigma_f) (brute force at 64^2).
+                        sigma_x = rows/(2 pi sigma_f) (brute force at 64^2);
+                        threshold rule from closed forms (constant field, plane
+                        wave, delta with Parseval, first-call average).
+  residual (a11)        exact modulus match -> 0, single/double perturbations,
+                        over-exposed and beamstop pixels excluded.
   real blur (a8')       scipy.ndimage.convolve with the (2r+1)^2 kernel,
                         zero padding.
   masks (a10)           disk/rectangle extents and counts.
@@ -193,6 +197,102 @@ def test_gauss_mult_is_periodic_convolution(o64):
     cont = np.exp(-d ** 2 / (2 * sx ** 2))
     cont /= cont.sum()
     assert np.abs(h1 - cont).max() < 1e-4
+
+
+def _gauss_window(n, sigma_f):
+    """exp(-(kx^2 + ky^2) / 2 sigma_f^2) on wrapped frequencies (applyGaussMult
+    isFreq=1, cudaConfig.cu:380-395)"""
+    k = np.where(np.arange(n) >= n // 2, np.arange(n) - n, np.arange(n))
+    return np.exp(-(k[:, None] ** 2 + k[None, :] ** 2) / (2 * sigma_f ** 2))
+
+
+def test_shrink_wrap_threshold_closed_forms(o64):
+    """The frequency shrink-wrap of cdi.cc:326-341: S = |IFFT(F G)|^2 with
+    sigma_f = (rows/4)/sigma; threshold = min(max S, 5 avg) * shrinkThreshold,
+    avg = 4 sum S / N taken on the first call only; sigma *= 0.99 while > 2.
+    Each case has an answer fixed without the oracle's own steps:
+      - a constant field c (F = c N at DC) gives S = (c N)^2 everywhere, so
+        the threshold is S t (max < 5 avg);
+      - a plane wave at (5, -7) gives the constant S = (c N G(k))^2, which pins
+        the window's width and its wrapped coordinates (unwrapped, -7 would be
+        57 and S would underflow);
+      - a delta at the origin (F = 1) gives S = |sum_k G_k e^{...}|^2: its peak
+        is (sum G)^2 and, by Parseval, sum S = N sum G^2, so 5 avg = 20 sum G^2
+        < max and the threshold is 20 sum G^2 t;
+      - a second call keeps the first call's average: a constant field brighter
+        than 5 avg is then clamped to 5 avg t."""
+    n, t, c = 64, 0.3, 2.0
+    N = n * n
+    sf = (n / 4) / 3.0
+    # constant field
+    st = o64.sw_state(sigma=3.0)
+    F = np.zeros((n, n), complex)
+    F[0, 0] = c * N
+    S = o64.shrink_wrap_freq(F, t, st)
+    assert np.allclose(S, (c * N) ** 2, rtol=1e-12)
+    assert abs(st[2] - (c * N) ** 2 * t) <= 1e-12 * (c * N) ** 2
+    assert abs(st[1] - 4 * (c * N) ** 2) <= 1e-12 * (c * N) ** 2
+    assert abs(st[0] - 3.0 * 0.99) < 1e-15
+    # plane wave at wrapped frequency (5, -7)
+    st = o64.sw_state(sigma=3.0)
+    F = np.zeros((n, n), complex)
+    F[5, n - 7] = c * N
+    S = o64.shrink_wrap_freq(F, t, st)
+    want = (c * N * np.exp(-(5 ** 2 + 7 ** 2) / (2 * sf ** 2))) ** 2
+    assert np.allclose(S, want, rtol=1e-12)
+    assert abs(st[2] - want * t) <= 1e-12 * want
+    # delta at the origin
+    st = o64.sw_state(sigma=3.0)
+    G = _gauss_window(n, sf)
+    S = o64.shrink_wrap_freq(np.ones((n, n), complex), t, st)
+    assert abs(S[0, 0] - G.sum() ** 2) <= 1e-12 * G.sum() ** 2 and S.max() == S[0, 0]
+    assert abs(S.sum() - N * (G ** 2).sum()) <= 1e-12 * S.sum()
+    assert 20 * (G ** 2).sum() < G.sum() ** 2
+    assert abs(st[2] - 20 * (G ** 2).sum() * t) <= 1e-12 * st[2]
+    first_avg = 4 * (G ** 2).sum()
+    assert abs(st[1] - first_avg) <= 1e-12 * first_avg
+    # second call: the first call's average clamps a much brighter field
+    F = np.zeros((n, n), complex)
+    F[0, 0] = 1e3 * N
+    S = o64.shrink_wrap_freq(F, t, st)
+    assert S.min() > 5 * first_avg
+    assert abs(st[1] - first_avg) <= 1e-12 * first_avg
+    assert abs(st[2] - 5 * first_avg * t) <= 1e-12 * st[2]
+    assert abs(st[0] - 3.0 * 0.99 * 0.99) < 1e-15
+    # sigma stops decaying at 2
+    st = o64.sw_state(sigma=2.0)
+    o64.shrink_wrap_freq(F, t, st)
+    assert st[0] == 2.0
+
+
+def test_residual_closed_forms(o64):
+    """residual of cdi.cc:413-421: sum over pixels with I < 0.99 of
+    ((|F|^2 - I) (1 - bs))^2 (getMod2, addRemoveOE(., ., -1), invert(beamstop)
+    and multiply, getMod2, findSum).  An exact modulus match gives 0; a
+    perturbation d of |F|^2 at one pixel gives d^2, at two pixels the sum of
+    squares; over-exposed pixels (I >= 0.99) and beamstop pixels do not count;
+    a beamstop value of 0.5 weights the difference by 0.5."""
+    n = 16
+    rs = np.random.default_rng(8)
+    I = rs.uniform(0, 0.9, (n, n))
+    ph = np.exp(2j * np.pi * rs.uniform(size=(n, n)))
+    F = np.sqrt(I) * ph
+    assert o64.residual(F, I) < 1e-28
+    F1 = F.copy()
+    F1[3, 4] = np.sqrt(I[3, 4] + 0.25) * ph[3, 4]
+    assert abs(o64.residual(F1, I) - 0.0625) < 1e-14
+    F1[9, 2] = np.sqrt(I[9, 2] + 0.5) * ph[9, 2]
+    assert abs(o64.residual(F1, I) - (0.0625 + 0.25)) < 1e-14
+    # over-exposed pixel: excluded whatever F holds there
+    I2 = I.copy()
+    I2[9, 2] = 0.99
+    assert abs(o64.residual(F1, I2) - 0.0625) < 1e-14
+    # beamstop: 1 excludes, 0.5 halves the difference
+    bs = np.zeros((n, n))
+    bs[3, 4] = 1.0
+    assert abs(o64.residual(F1, I, bs) - 0.25) < 1e-14
+    bs[3, 4] = 0.5
+    assert abs(o64.residual(F1, I, bs) - (0.25 * 0.0625 + 0.25)) < 1e-14
 
 
 def test_gauss_conv_real_matches_scipy(o64):

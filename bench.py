@@ -157,60 +157,180 @@ def _rfactor_host(F, I, bs):
     return float(np.abs(np.abs(F[m]) - a).sum() / a.sum())
 
 
-def fp64_report(cdx, seed=0):
+def _twin(a):
+    """the conjugate-inverted object conj(a(-x)) (periodic): same |FFT| as a"""
+    return np.conj(np.roll(a[::-1, ::-1], 1, axis=(0, 1)))
+
+
+def aligned_distance(a, b):
+    """relative L2 distance of a from b up to the trivial ambiguities of
+    phase retrieval -- a global phase, a periodic translation and the twin
+    image (conjugate inversion) -- which leave |FFT| unchanged"""
+    best = None
+    Fb = np.fft.fft2(b)
+    for c in (a, _twin(a)):
+        xc = np.fft.ifft2(Fb * np.conj(np.fft.fft2(c)))
+        k = np.unravel_index(int(np.argmax(np.abs(xc))), xc.shape)
+        cs = np.roll(c, k, axis=(0, 1))
+        ph = np.vdot(cs.ravel(), b.ravel())
+        if abs(ph) == 0:
+            continue
+        e = float(np.linalg.norm(cs * (ph / abs(ph)) - b) / np.linalg.norm(b))
+        best = e if best is None else min(best, e)
+    return best
+
+
+def _true_object_R(ctx, cdx, obj, Ih, bsh):
+    """noise floor: R of the true object's transform scaled to the measured
+    total intensity (outside the beamstop)"""
+    Ft = ctx.cplx()
+    ctx.fft(obj, Ft, cdx.FORWARD)
+    F = ctx.get_cplx(Ft).astype(np.complex128)
+    Ft.free()
+    m = np.ones(Ih.shape, bool) if bsh is None else bsh < 0.5
+    F *= np.sqrt(Ih[m].sum() / (np.abs(F[m]) ** 2).sum())
+    return _rfactor_host(F, Ih, bsh)
+
+
+def fp64_report(cdx, seeds=None):
     """Each workload's seeded inputs (object, Poisson intensities, beamstop,
     support, random-phase start) are made once in FP64 and run through the
-    FP32 and the FP64 pipelines.  Reported: R-factor of the support-projected
-    final object in each precision, and the relative L2 distance between the
-    two final objects (inside the FP64 support, global phase aligned)."""
+    FP32 and the FP64 pipelines, for several seeds.  Reported per seed and
+    precision: the R-factor of the final object projected on that run's own
+    support, the final threshold and support size; the noise floor (R of the
+    true object); and, for the first seed, the relative L2 distance between
+    the two final support-projected objects up to a global phase, a
+    translation and the twin image (the ambiguities |FFT| cannot resolve).
+    Two runs from one start separate within tens of iterations (HIO/RAAR
+    amplify rounding), so the comparison across seeds is of outcomes."""
     out = {}
     for name, c in REPORT_CONFIGS:
         n = c["n"]
-        ctx = cdx.Context(n, precision=cdx.F64)
-        obj, I, bs, spt, F = ctx.cplx(), ctx.real(), ctx.real(), ctx.real(), ctx.cplx()
-        ctx.sim_object(obj, c["radius"], seed=seed)
-        ctx.sim_intensity(obj, I, c["photons"], seed=seed, poisson=True)
-        if c["bs"]:
-            ctx.create_beamstop(bs, c["bs"])
-        else:
-            bs.zero()
-        if c["support"] == "rect":
-            ctx.create_support(spt, 2.0)
-        else:
-            ctx.create_circle(spt, 0.2 * n)
-        ctx.random_start(I, bs if c["bs"] else None, F, seed=seed, mode=1)
-        host = dict(I=ctx.get_real(I), bs=ctx.get_real(bs) if c["bs"] else None, spt=ctx.get_real(spt),
-                    F=ctx.get_cplx(F))
-        ctx.close()
-        host["g"] = np.fft.ifft2(host["F"]) * n  # IFFT(F)/sqrt(N), as prepareIter leaves objectWave
-        res = {}
-        for prec in (cdx.F32, cdx.F64):
-            ctx = cdx.Context(n, precision=prec)
-            ctx.set_threshold(0.5)
-            p = cdx.Params.default(**c["params"])
-            dF, dg, dI, ds = ctx.cplx(host["F"]), ctx.cplx(host["g"]), ctx.real(host["I"]), ctx.real(host["spt"])
-            dbs = ctx.real(host["bs"]) if host["bs"] is not None else None
-            ctx.sync()
-            t0 = time.perf_counter()
-            nit, _ = ctx.phase_retrieve(c["schedule"], p, dF, dg, dI, ds, bs=dbs)
-            ctx.sync()
-            el = time.perf_counter() - t0
-            g = ctx.get_cplx(dg).astype(np.complex128) / n
-            mask = ctx.get_real(ds) > ctx.threshold()
-            res[prec] = dict(g=g, mask=mask, seconds=el, n_iter=nit)
+        sds = seeds if seeds is not None else (range(8) if n <= 1024 else range(4))
+        rows = {"R_fp32": [], "R_fp64": [], "threshold_fp32": [], "threshold_fp64": [], "support_px_fp32": [],
+                "support_px_fp64": [], "R_true_object": [], "seconds_fp32": [], "seconds_fp64": []}
+        dist = None
+        for seed in sds:
+            ctx = cdx.Context(n, precision=cdx.F64)
+            obj, I, bs, spt, F = ctx.cplx(), ctx.real(), ctx.real(), ctx.real(), ctx.cplx()
+            ctx.sim_object(obj, c["radius"], seed=seed)
+            ctx.sim_intensity(obj, I, c["photons"], seed=seed, poisson=True)
+            if c["bs"]:
+                ctx.create_beamstop(bs, c["bs"])
+            else:
+                bs.zero()
+            if c["support"] == "rect":
+                ctx.create_support(spt, 2.0)
+            else:
+                ctx.create_circle(spt, 0.2 * n)
+            ctx.random_start(I, bs if c["bs"] else None, F, seed=seed, mode=1)
+            host = dict(I=ctx.get_real(I), bs=ctx.get_real(bs) if c["bs"] else None, spt=ctx.get_real(spt),
+                        F=ctx.get_cplx(F))
+            rows["R_true_object"].append(round(_true_object_R(ctx, cdx, obj, host["I"], host["bs"]), 5))
             ctx.close()
-        m64 = res[cdx.F64]["mask"]
-        for prec in res:
-            gs = res[prec]["g"] * m64
-            res[prec]["R"] = _rfactor_host(np.fft.fft2(gs), host["I"], host["bs"])
-            res[prec]["gs"] = gs
-        a, b = res[cdx.F32]["gs"], res[cdx.F64]["gs"]
-        ph = np.vdot(a.ravel(), b.ravel())
-        err = float(np.linalg.norm(a * (ph / abs(ph)) - b) / np.linalg.norm(b)) if abs(ph) > 0 else None
-        out[name] = {"iterations": res[cdx.F64]["n_iter"], "R_fp32": round(res[cdx.F32]["R"], 5),
-                     "R_fp64": round(res[cdx.F64]["R"], 5), "fp32_vs_fp64_rel_l2": err,
-                     "seconds_fp32": round(res[cdx.F32]["seconds"], 4),
-                     "seconds_fp64": round(res[cdx.F64]["seconds"], 4)}
+            host["g"] = np.fft.ifft2(host["F"]) * n  # IFFT(F)/sqrt(N), as prepareIter leaves objectWave
+            res = {}
+            for prec in (cdx.F32, cdx.F64):
+                ctx = cdx.Context(n, precision=prec)
+                ctx.set_threshold(0.5)
+                p = cdx.Params.default(**c["params"])
+                dF, dg, dI, ds = (ctx.cplx(host["F"]), ctx.cplx(host["g"]), ctx.real(host["I"]),
+                                  ctx.real(host["spt"]))
+                dbs = ctx.real(host["bs"]) if host["bs"] is not None else None
+                ctx.sync()
+                t0 = time.perf_counter()
+                nit, _ = ctx.phase_retrieve(c["schedule"], p, dF, dg, dI, ds, bs=dbs)
+                ctx.sync()
+                el = time.perf_counter() - t0
+                g = ctx.get_cplx(dg).astype(np.complex128) / n
+                thr = ctx.threshold()
+                mask = ctx.get_real(ds) > thr
+                gs = g * mask
+                tag = "fp32" if prec == cdx.F32 else "fp64"
+                rows[f"R_{tag}"].append(round(_rfactor_host(np.fft.fft2(gs), host["I"], host["bs"]), 5))
+                rows[f"threshold_{tag}"].append(float(thr))
+                rows[f"support_px_{tag}"].append(int(mask.sum()))
+                rows[f"seconds_{tag}"].append(round(el, 4))
+                res[prec] = gs
+                ctx.close()
+            if dist is None:
+                dist = aligned_distance(res[cdx.F32], res[cdx.F64])
+        r32, r64 = np.array(rows["R_fp32"]), np.array(rows["R_fp64"])
+        out[name] = {"iterations": nit, "seeds": list(sds), **rows,
+                     "R_fp32_mean": round(float(r32.mean()), 5), "R_fp64_mean": round(float(r64.mean()), 5),
+                     "R_fp32_std": round(float(r32.std()), 5), "R_fp64_std": round(float(r64.std()), 5),
+                     "fp32_vs_fp64_aligned_rel_l2_seed0": dist}
+    return out
+
+
+def prtf_crossings(prtf, weights, levels=(np.exp(-1), 0.5)):
+    """first crossings of the PRTF curve prtf/weights (the values prtf.txt
+    holds, cdi.cc:435-439) with each level, linearly interpolated between
+    radii, as gnuplot's `watch y=...` finds them in
+    gnuplot_script/plot_prtf.plt:6-24; in pixels of frequency radius, None
+    when the curve never crosses"""
+    ok = weights > 0
+    curve = np.where(ok, prtf / np.where(ok, weights, 1), np.nan)
+    out = {}
+    for lv in levels:
+        x = None
+        for i in range(len(curve) - 1):
+            a, b = curve[i], curve[i + 1]
+            if np.isnan(a) or np.isnan(b) or a == b:
+                continue
+            if (a - lv) * (b - lv) <= 0:
+                x = float(i + (lv - a) / (b - a))
+                break
+        out[lv] = x
+    return out
+
+
+QUALITY_CONFIGS = [
+    ("1024x1024 RAAR beta=0.85, 1000 iterations, shrink-wrap every 20", 1024, 160, 1e8, 8,
+     "50*(20*RAAR+1*shrinkWrap)", (0, 1, 2, 3)),
+    ("2048x2048 RAAR beta=0.85, 2000 iterations, shrink-wrap every 20", 2048, 320, 1e9, 16,
+     "100*(20*RAAR+1*shrinkWrap)", (0,)),
+]
+
+
+def quality_leg(cdx, prtf_steps=100):
+    """Converged quality of the FP32 reconstructions of the 1024^2 and 2048^2
+    RAAR workloads: R of the support-projected final object against the
+    noise floor (R of the true object), and the PRTF accumulated over the
+    last `prtf_steps` schedule steps (prtfIter, cdi.cc:355-380) with its 1/e
+    and 0.5 crossings (radius in pixels and as a fraction of the Nyquist
+    radius rows/2)."""
+    out = {}
+    for name, n, radius, photons, bsr, sched, seeds in QUALITY_CONFIGS:
+        ntok = len(cdx.expand_schedule(sched))
+        ctx = cdx.Context(n)
+        p = cdx.Params.default(prtfIter=ntok - prtf_steps, **E2E_PARAMS)
+        rows = {"R_support_projected": [], "R_true_object": [], "prtf_1_over_e_px": [], "prtf_half_px": []}
+        for seed in seeds:
+            obj, I, bs, spt, F, g = ctx.cplx(), ctx.real(), ctx.real(), ctx.real(), ctx.cplx(), ctx.cplx()
+            ctx.sim_object(obj, radius, seed=seed)
+            ctx.sim_intensity(obj, I, photons, seed=seed, poisson=True)
+            ctx.create_beamstop(bs, bsr)
+            ctx.create_circle(spt, 0.2 * n)
+            ctx.set_threshold(0.5)
+            ctx.random_start(I, bs, F, seed=seed, mode=1)
+            ctx.fft(F, g, cdx.INVERSE)
+            ctx.apply_norm(g, 1.0 / n)
+            Ih, bsh = ctx.get_real(I), ctx.get_real(bs)
+            ctx.phase_retrieve(sched, p, F, g, I, spt, bs=bs)
+            gfin = ctx.get_cplx(g).astype(np.complex128) / n
+            m = ctx.get_real(spt) > ctx.threshold()
+            rows["R_support_projected"].append(round(_rfactor_host(np.fft.fft2(gfin * m), Ih, bsh), 5))
+            rows["R_true_object"].append(round(_true_object_R(ctx, cdx, obj, Ih, bsh), 5))
+            pr, wt, cnt = ctx.prtf()
+            cr = prtf_crossings(pr, wt)
+            rows["prtf_1_over_e_px"].append(None if cr[np.exp(-1)] is None else round(cr[np.exp(-1)], 2))
+            rows["prtf_half_px"].append(None if cr[0.5] is None else round(cr[0.5], 2))
+            for b in (obj, I, bs, spt, F, g):
+                b.free()
+        ctx.close()
+        out[name] = {"schedule": sched, "prtf_steps": prtf_steps, "seeds": list(seeds), "nyquist_px": n // 2,
+                     **rows}
     return out
 
 
@@ -556,6 +676,7 @@ def main():
         out["batched"] = batched_leg(cdx, args.batch_iters)
     if not args.no_report:
         out["fp64_report"] = fp64_report(cdx)
+        out["quality"] = quality_leg(cdx)
     if not args.no_learned:
         out["learned_warm_start"] = learned_leg(cdx)
         out["learned_warm_start_2048"] = learned_large_leg(cdx)

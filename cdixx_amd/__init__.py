@@ -185,6 +185,33 @@ def preprocess_pattern(sig, bkg=None, nmerge=1, centre=False, shift_row=0.0, shi
     return out[:k * k].reshape(k, k).copy(), (mid[0], mid[1])
 
 
+_BIN_TYPES = {3: np.float32, 4: np.complex64, 5: np.float64, 6: np.complex128}
+
+
+def bin_read_header(path):
+    """(type, rows, cols) of a .bin image (struct imageFile, include/imageFile.hpp)"""
+    t, r, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    check(lib.cdx_bin_read_header(path.encode(), ctypes.byref(t), ctypes.byref(r), ctypes.byref(c)))
+    return t.value, r.value, c.value
+
+
+def bin_read(path):
+    """readImage_c of a .bin file (FLOAT, FLOAT2, DOUBLE, DOUBLE2) -> numpy array"""
+    t, r, c = bin_read_header(path)
+    out = np.empty((r, c), _BIN_TYPES[t])
+    check(lib.cdx_bin_read(path.encode(), out.ctypes.data, out.nbytes))
+    return out
+
+
+def bin_write(path, a):
+    """writeFloatImage / writeComplexImage (and their double forms) of a 2-D array"""
+    a = np.ascontiguousarray(a)
+    code = {np.dtype(v): k for k, v in _BIN_TYPES.items()}.get(a.dtype)
+    if code is None or a.ndim != 2:
+        raise ValueError("bin_write: 2-D float32/complex64/float64/complex128 array expected")
+    check(lib.cdx_bin_write(path.encode(), code, a.shape[0], a.shape[1], a.ctypes.data))
+
+
 def read_config(path):
     """readConfig of a libconfig file -> dict of every field (host only)."""
     import json
@@ -397,6 +424,25 @@ class Context:
                                       ctypes.byref(cm) if per_kernel else None,
                                       ctypes.byref(rm) if per_kernel else None))
         return tot.value, (cm.value if per_kernel else None), (rm.value if per_kernel else None)
+
+    # setup steps of CDI::prepareIter / readObjectWave
+    def create_wavefront(self, out, I=None, phase=None, obj_shape=None):
+        """createWaveFront(I, phase, out, objrow, objcol): the object image
+        centred in the frame (I, phase: device buffers of obj_shape or None)"""
+        r, c = obj_shape
+        check(lib.cdx_create_wavefront(self.h, I.ptr if I else None, phase.ptr if phase else None, r, c, out.ptr))
+
+    def convert_fo_phase(self, d, norm):
+        check(lib.cdx_convert_fo_phase(self.h, d.ptr, norm))
+
+    def mask_multiply(self, d, mask, invert=False):
+        check(lib.cdx_mask_multiply(self.h, d.ptr, mask.ptr, int(invert)))
+
+    def poisson_noise_wo(self, d, noise_level, seed=0):
+        check(lib.cdx_poisson_noise_wo(self.h, d.ptr, noise_level, seed))
+
+    def extend_to_complex(self, a, b):
+        check(lib.cdx_extend_to_complex(self.h, a.ptr, b.ptr))
 
     def sim_object(self, obj, radius, seed=0, smooth_sigma=3.0):
         check(lib.cdx_sim_object(self.h, radius, seed, smooth_sigma, obj.ptr))

@@ -190,7 +190,6 @@ extern "C" int cdx_create(cdx_ctx** out, int rows, int cols, int precision, void
   // any stream capture
   hipError_t pe = precision == CDX_F64 ? prepare_col<double>(rows) : prepare_col<float>(rows);
   if (pe == hipSuccess) pe = precision == CDX_F64 ? prepare_row<double>(cols) : prepare_row<float>(cols);
-  if (pe == hipSuccess) pe = precision == CDX_F64 ? prepare_blur<double>() : prepare_blur<float>();
   if (pe != hipSuccess) {
     cdx_destroy(c);
     return fail(CDX_ERR_HIP, "cdx_create: raising the LDS limits of the passes failed: %s", hipGetErrorString(pe));
@@ -644,6 +643,8 @@ static int phase_retrieve_t(cdx_ctx* c, const char* algorithm, const cdx_params*
     HIPCHK(launch_fill_f64(2048, c->prtf_dev, 0.0, c->st));
   }
 
+  // the real-space blur's LDS grows with sigma, largest at the start
+  if (p->shrinkMode == 1) HIPCHK(prepare_blur<T>(c->rows, (T)p->sigma0));
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   if (p->useGraph) HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));

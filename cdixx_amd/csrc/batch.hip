@@ -270,6 +270,33 @@ static std::string radix_name(int n, size_t csz, int v) {
   }
 }
 
+// raises the dynamic-LDS cap of every batched pass instantiation the loop
+// can launch to `lds` (both passes use the same), when the tiling is set and
+// so before any stream capture
+template <class T>
+static hipError_t batch_prepare(size_t lds) {
+  const void* ks[] = {
+      (const void*)kb_col<T, COL_PROJ_INV, PROJ_MOD_ABS, 0>,    (const void*)kb_col<T, COL_PROJ_INV, PROJ_MOD_ABS, 1>,
+      (const void*)kb_col<T, COL_PROJ_INV, PROJ_MOD_NOISE, 0>,  (const void*)kb_col<T, COL_PROJ_INV, PROJ_MOD_NOISE, 1>,
+      (const void*)kb_col<T, COL_PROJ_INV, PROJ_MOD_ABS_BS, 0>, (const void*)kb_col<T, COL_PROJ_INV, PROJ_MOD_ABS_BS, 1>,
+      (const void*)kb_col<T, COL_FWD_PROJ_INV, PROJ_MOD_ABS, 0>,
+      (const void*)kb_col<T, COL_FWD_PROJ_INV, PROJ_MOD_ABS, 1>,
+      (const void*)kb_col<T, COL_FWD_PROJ_INV, PROJ_MOD_NOISE, 0>,
+      (const void*)kb_col<T, COL_FWD_PROJ_INV, PROJ_MOD_NOISE, 1>,
+      (const void*)kb_col<T, COL_FWD_PROJ_INV, PROJ_MOD_ABS_BS, 0>,
+      (const void*)kb_col<T, COL_FWD_PROJ_INV, PROJ_MOD_ABS_BS, 1>,
+      (const void*)kb_col<T, COL_GAUSS_INV, 0, 0>,              (const void*)kb_col<T, COL_FWD_GAUSS_INV, 0, 0>,
+      (const void*)kb_col<T, COL_FWD, 0, 0>,                    (const void*)kb_row<T, ROW_INV_UPDATE_FWD, 0>,
+      (const void*)kb_row<T, ROW_INV_UPDATE_FWD, 1>,            (const void*)kb_row<T, ROW_INV_UPDATE_FWD, 2>,
+      (const void*)kb_row<T, ROW_INV_UPDATE_FWD, 3>,            (const void*)kb_row<T, ROW_INV_UPDATE_FWD, 4>,
+      (const void*)kb_row<T, ROW_INV_UPDATE_FWD, 5>,            (const void*)kb_row<T, ROW_INV_MOD2, 0>};
+  for (const void* k : ks) {
+    hipError_t e = lds_cap(k, lds);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 // Workgroup maps of both passes for `threads` threads per workgroup (64,
 // 128 or 256): frame size n runs threads / TL(n) lines per workgroup.
 // Returns CDX_ERR_ARG (maps unchanged) when a size present cannot take it.
@@ -293,6 +320,8 @@ static int batch_tiling(cdx_batch* b, int threads, const int* var) {
   }
   if (lds > 160 * 1024)
     return bfail(CDX_ERR_ARG, "cdx_batch_set_tiling: %d threads need %zu bytes of LDS", threads, lds);
+  // best effort here; a launch the device refuses reports the error itself
+  (void)(b->prec == CDX_F64 ? batch_prepare<double>(lds) : batch_prepare<float>(lds));
   std::vector<int2> cols, rows;
   std::vector<BFrame> fr(b->nf);
   for (int i = 0; i < b->nf; i++) {
@@ -363,32 +392,6 @@ extern "C" int cdx_batch_destroy(cdx_batch* b) {
   return CDX_OK;
 }
 
-// raises the dynamic-LDS cap of every batched pass instantiation the loop
-// can launch, before any stream capture
-template <class T>
-static hipError_t batch_prepare() {
-  const void* ks[] = {
-      (const void*)kb_col<T, COL_PROJ_INV, PROJ_MOD_ABS, 0>,    (const void*)kb_col<T, COL_PROJ_INV, PROJ_MOD_ABS, 1>,
-      (const void*)kb_col<T, COL_PROJ_INV, PROJ_MOD_NOISE, 0>,  (const void*)kb_col<T, COL_PROJ_INV, PROJ_MOD_NOISE, 1>,
-      (const void*)kb_col<T, COL_PROJ_INV, PROJ_MOD_ABS_BS, 0>, (const void*)kb_col<T, COL_PROJ_INV, PROJ_MOD_ABS_BS, 1>,
-      (const void*)kb_col<T, COL_FWD_PROJ_INV, PROJ_MOD_ABS, 0>,
-      (const void*)kb_col<T, COL_FWD_PROJ_INV, PROJ_MOD_ABS, 1>,
-      (const void*)kb_col<T, COL_FWD_PROJ_INV, PROJ_MOD_NOISE, 0>,
-      (const void*)kb_col<T, COL_FWD_PROJ_INV, PROJ_MOD_NOISE, 1>,
-      (const void*)kb_col<T, COL_FWD_PROJ_INV, PROJ_MOD_ABS_BS, 0>,
-      (const void*)kb_col<T, COL_FWD_PROJ_INV, PROJ_MOD_ABS_BS, 1>,
-      (const void*)kb_col<T, COL_GAUSS_INV, 0, 0>,              (const void*)kb_col<T, COL_FWD_GAUSS_INV, 0, 0>,
-      (const void*)kb_col<T, COL_FWD, 0, 0>,                    (const void*)kb_row<T, ROW_INV_UPDATE_FWD, 0>,
-      (const void*)kb_row<T, ROW_INV_UPDATE_FWD, 1>,            (const void*)kb_row<T, ROW_INV_UPDATE_FWD, 2>,
-      (const void*)kb_row<T, ROW_INV_UPDATE_FWD, 3>,            (const void*)kb_row<T, ROW_INV_UPDATE_FWD, 4>,
-      (const void*)kb_row<T, ROW_INV_UPDATE_FWD, 5>,            (const void*)kb_row<T, ROW_INV_MOD2, 0>};
-  for (const void* k : ks) {
-    hipError_t e = lds_cap(k);
-    if (e != hipSuccess) return e;
-  }
-  return hipSuccess;
-}
-
 extern "C" int cdx_batch_create(cdx_batch** out, int nframes, const int* sizes, int precision, void* stream) {
   if (!out || nframes <= 0 || !sizes) return bfail(CDX_ERR_ARG, "cdx_batch_create: bad arguments");
   if (precision != CDX_F32 && precision != CDX_F64) return bfail(CDX_ERR_ARG, "cdx_batch_create: bad precision");
@@ -448,8 +451,6 @@ extern "C" int cdx_batch_create(cdx_batch** out, int nframes, const int* sizes, 
       hipMalloc((void**)&b->nparts, sizeof(int)))
     return bad(bfail(CDX_ERR_HIP, "cdx_batch_create: out of device memory"));
   if ((rc = batch_tiling(b, precision == CDX_F64 ? bthreads<double>() : bthreads<float>(), b->var))) return bad(rc);
-  if ((precision == CDX_F64 ? batch_prepare<double>() : batch_prepare<float>()) != hipSuccess)
-    return bad(bfail(CDX_ERR_HIP, "cdx_batch_create: raising the LDS limits of the batched passes failed"));
   std::vector<double> sc((size_t)nframes * kScal, 0.0);
   for (int i = 0; i < nframes; i++) sc[(size_t)i * kScal] = 0.5;  // init_cuda_image threshold
   if (hipMemcpy(b->scal, sc.data(), sc.size() * sizeof(double), hipMemcpyHostToDevice))
@@ -530,12 +531,10 @@ struct BLoop {
   }
   template <class K>
   hipError_t launch(K kern, int nwg, const int2* map, const BArgs<T>& a, size_t lds) {
-    // the caps are raised at batch creation (batch_prepare); lds_cap only
-    // looks them up here
-    if (lds > 64 * 1024) {
-      hipError_t e = lds_cap((const void*)kern);
-      if (e != hipSuccess) return e;
-    }
+    // the caps were raised when the tiling was set (batch_prepare); lds_cap
+    // only looks them up here
+    hipError_t e = lds_cap((const void*)kern, lds);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(kern, dim3(nwg), dim3(b->threads), lds, b->st, a, map);
     return hipGetLastError();
   }

@@ -2,23 +2,29 @@
 // the CDI path (the fused FFT passes are in passes.hip).
 #include "device_ops.hpp"
 
+#include <map>
 #include <mutex>
-#include <set>
 #include <utility>
 
 namespace cdx {
 
-hipError_t lds_cap(const void* kern) {
+hipError_t lds_cap(const void* kern, size_t bytes) {
+  if (bytes <= 64 * 1024) return hipSuccess;
   static std::mutex mu;
-  static std::set<std::pair<int, const void*>> done;
+  static std::map<std::pair<int, const void*>, size_t> raised;
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return e;
   std::lock_guard<std::mutex> lk(mu);
-  if (done.count({dev, kern})) return hipSuccess;
-  e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-  if (e == hipSuccess) done.insert({dev, kern});
-  return e;
+  size_t& have = raised[{dev, kern}];
+  if (bytes <= have) return hipSuccess;
+  e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();  // leave no sticky error for the next launch check
+    return e;
+  }
+  have = bytes;
+  return hipSuccess;
 }
 
 // beamstop folded into the intensities as a NaN sentinel (bs > 0.5)
@@ -351,15 +357,24 @@ hipError_t launch_blur_real(int rows, int cols, const void* g, int cpx_in, T* tm
   size_t l2 = (size_t)((rows + 2 * r) * CW + 2 * r + 1) * sizeof(T);
   if (l2 > 160 * 1024) return hipErrorInvalidValue;
   auto kern = k_blur_cols<T>;
-  if (l2 > 64 * 1024 && (e = lds_cap((const void*)kern)) != hipSuccess) return e;
+  if ((e = lds_cap((const void*)kern, l2)) != hipSuccess) return e;
   hipLaunchKernelGGL(kern, dim3((cols + CW - 1) / CW), dim3(256), l2, st, rows, cols, (const T*)tmp, spt, sigma, r,
                      rs);
   return hipGetLastError();
 }
 
+// LDS of the blur's column kernel for `rows`-point columns at sigma
 template <class T>
-hipError_t prepare_blur() {
-  return lds_cap((const void*)k_blur_cols<T>);
+static size_t blur_col_lds(int rows, T sigma) {
+  const int r = ((int)floor((double)sigma * 6)) >> 1;
+  constexpr int CW = 64 / sizeof(T);
+  return (size_t)((rows + 2 * r) * CW + 2 * r + 1) * sizeof(T);
+}
+template <class T>
+hipError_t prepare_blur(int rows, T sigma) {
+  const size_t l2 = blur_col_lds<T>(rows, sigma);
+  if (l2 > 160 * 1024) return hipErrorInvalidValue;
+  return lds_cap((const void*)k_blur_cols<T>, l2);
 }
 
 // Start state (prepareIter, cdi.cc:222-223): createWaveFront(sqrt(I)) then
@@ -853,7 +868,7 @@ hipError_t launch_radial(const void* src, int rows, int cols, int kind, T norm, 
   template hipError_t launch_reduce<T>(int64_t, const T*, const ReduceSlots&, hipStream_t);                      \
   template hipError_t launch_threshold<T>(double*, const ReduceSlots&, double, int, int, int, hipStream_t);      \
   template hipError_t launch_blur_real<T>(int, int, const void*, int, T*, T*, T, const ReduceSlots&, hipStream_t);    \
-  template hipError_t prepare_blur<T>();                                                                         \
+  template hipError_t prepare_blur<T>(int, T);                                                                       \
   template hipError_t launch_start<T>(int64_t, const T*, const T*, double, uint64_t, int, void*, hipStream_t);   \
   template hipError_t launch_residual<T>(int64_t, const void*, const T*, const T*, double*, hipStream_t);        \
   template hipError_t launch_rfactor<T>(int64_t, const void*, const T*, double*, hipStream_t);                   \

@@ -103,17 +103,19 @@ template <class T>
 hipError_t launch_row(int op, int rows, int cols, const void* in, void* out, const void* tw,
                       void* g, const T* spt, const UpdArgs<T>& ua, T scale, T* sout,
                       const ReduceSlots& rs, hipStream_t st, int lines = 0);
-// Raise the dynamic-LDS cap (160 KiB) of every column-pass instantiation for
-// rows-point columns / row-pass instantiation for cols-point rows, so no
-// attribute call is left for a launch inside a stream capture.
+// Raise the dynamic-LDS cap of every column-pass instantiation for rows-point
+// columns / row-pass instantiation for cols-point rows to its largest valid
+// tiling, so no attribute call is left for a launch inside a stream capture.
 template <class T> hipError_t prepare_col(int rows);
 template <class T> hipError_t prepare_row(int cols);
-// hipFuncSetAttribute(MaxDynamicSharedMemorySize, 160 KiB) once per kernel
-// and device (thread-safe)
-hipError_t lds_cap(const void* kern);
-// the real-space shrink-wrap blur's column kernel (k_blur_cols<T>), which
-// can also exceed 64 KiB
-template <class T> hipError_t prepare_blur();
+// Lets `kern` launch with `bytes` of dynamic LDS (above the 64 KiB default):
+// hipFuncSetAttribute(MaxDynamicSharedMemorySize) when bytes exceed what was
+// already raised for that kernel on this device (thread-safe; a lookup
+// otherwise)
+hipError_t lds_cap(const void* kern, size_t bytes);
+// the real-space shrink-wrap blur's column kernel (k_blur_cols<T>) for
+// rows-point columns at blur width sigma (its LDS grows with sigma)
+template <class T> hipError_t prepare_blur(int rows, T sigma);
 
 // PRTF (cdi.cc:355-380,423-440).  prtf_phase sums the complex partials of
 // ROW_INV_UPDATE_FWD_PRTF: first step phi0 = -arg(sum)/2, theta = phi0;

@@ -122,7 +122,8 @@ k_row(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out, int rows, const
 // launchers
 // ---------------------------------------------------------------------------
 // A kernel instantiation whose tilings can exceed the default 64 KiB of
-// dynamic LDS gets the 160 KiB cap once (lds_cap).  prepare_col/prepare_row
+// dynamic LDS gets its cap raised once to the largest valid tiling's LDS
+// (lds_cap).  prepare_col/prepare_row
 // do that for every instantiation of a line length, so a context raises the
 // caps at creation and no attribute call happens inside a stream capture.
 template <class T, int N>
@@ -132,27 +133,30 @@ static size_t max_lds() {
     if (tiling_valid(N, sizeof(cplx<T>), lines)) m = std::max(m, lds_bytes<T, N>(log2i(lines)));
   return m;
 }
+// prepare: raise the cap to the largest valid tiling's LDS, best effort (a
+// size whose largest tiling the device refuses still launches its smaller
+// tilings: the launch raises the cap to what it uses)
 template <class K, class T, int N>
-static hipError_t lds_prepare(K kern, bool& capped) {
-  if (capped) return hipSuccess;
-  if (max_lds<T, N>() > 64 * 1024) {
-    hipError_t e = lds_cap((const void*)kern);
-    if (e != hipSuccess) return e;
-  }
-  capped = true;
-  return hipSuccess;
+static void lds_prepare(K kern, bool& prepared) {
+  if (prepared) return;
+  (void)lds_cap((const void*)kern, max_lds<T, N>());
+  prepared = true;
 }
 #if CDX_PASS_ROW == 0
 template <class T, int N, int OP, int MODE, int RF>
 static hipError_t col_launch_n(int cols, const void* in, void* out, const void* tw, const T* I,
                                const ProjArgs<T>& pa, double gs, hipStream_t st, int lines, bool prep) {
-  static bool capped = false;
+  static bool prepared = false;
   constexpr int TL = LineCfgT<T, N>::TL;
   auto kern = k_col<T, N, OP, MODE, RF>;
-  hipError_t e = lds_prepare<decltype(kern), T, N>(kern, capped);
-  if (e != hipSuccess || prep) return e;
+  if (prep) {
+    lds_prepare<decltype(kern), T, N>(kern, prepared);
+    return hipSuccess;
+  }
   if (!tiling_valid(N, sizeof(cplx<T>), lines)) return hipErrorInvalidConfiguration;
   const size_t lds = lds_bytes<T, N>(log2i(lines));
+  hipError_t e = lds_cap((const void*)kern, lds);
+  if (e != hipSuccess) return e;
   dim3 grid((cols + lines - 1) / lines), block(lines * TL);
   hipLaunchKernelGGL(kern, grid, block, lds, st, (const cplx<T>*)in, (cplx<T>*)out, cols, (const cplx<T>*)tw, I, pa,
                      (T)gs, log2i(lines));
@@ -232,13 +236,17 @@ template <class T, int N, int OP, int ALGO>
 static hipError_t row_launch_n(int rows, const void* in, void* out, const void* tw, void* g, const T* spt,
                                const UpdArgs<T>& ua, T scale, T* sout, const ReduceSlots& rs, hipStream_t st,
                                int lines, bool prep) {
-  static bool capped = false;
+  static bool prepared = false;
   constexpr int TL = LineCfgT<T, N>::TL;
   auto kern = k_row<T, N, OP, ALGO>;
-  hipError_t e = lds_prepare<decltype(kern), T, N>(kern, capped);
-  if (e != hipSuccess || prep) return e;
+  if (prep) {
+    lds_prepare<decltype(kern), T, N>(kern, prepared);
+    return hipSuccess;
+  }
   if (!tiling_valid(N, sizeof(cplx<T>), lines)) return hipErrorInvalidConfiguration;
   const size_t lds = lds_bytes<T, N>(log2i(lines));
+  hipError_t e = lds_cap((const void*)kern, lds);
+  if (e != hipSuccess) return e;
   dim3 grid((rows + lines - 1) / lines), block(lines * TL);
   hipLaunchKernelGGL(kern, grid, block, lds, st, (const cplx<T>*)in, (cplx<T>*)out, rows, (const cplx<T>*)tw,
                      (cplx<T>*)g, spt, ua, scale, sout, rs, log2i(lines));

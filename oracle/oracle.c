@@ -852,6 +852,65 @@ int64_t orc_poisson(double lam, uint64_t seed, uint64_t stream, uint64_t idx) {
   }
 }
 
+/* ---- setup steps of CDI::prepareIter / readObjectWave (cdi.cc:62-82,161-200) ---- */
+
+/* createWaveFront, integer-size form (src/gpu/cudaConfig.cu:688-709) with
+ * shift 0 and phaseFactor 0, as readObjectWave calls it (cdi.cc:78): an
+ * orow x ocol image at margins ((rows-orow)/2, (cols-ocol)/2), amplitude
+ * sqrt(max(I,0)) (1 without I), phase (P-0.5)*2*pi with a phase image, zero
+ * elsewhere. */
+void orc_create_wavefront(int rows, int cols, const REAL* I, const REAL* P, int orow, int ocol, ocpx* out) {
+  const int mx = (rows - orow) / 2, my = (cols - ocol) / 2;
+  for (int x = 0; x < rows; x++)
+    for (int y = 0; y < cols; y++) {
+      ocpx* o = &out[(size_t)x * cols + y];
+      if (x < mx || x >= mx + orow || y < my || y >= my + ocol) {
+        o->re = o->im = 0;
+        continue;
+      }
+      const size_t t = (size_t)(x - mx) * ocol + (y - my);
+      const REAL mod = I ? (REAL)sqrt(I[t] > 0 ? (double)I[t] : 0.0) : (REAL)1;
+      const REAL ph = P ? (REAL)(((double)P[t] - 0.5) * 2 * M_PI) : (REAL)0;
+      if (ph != 0) {
+        o->re = mod * (REAL)cos((double)ph);
+        o->im = mod * (REAL)sin((double)ph);
+      } else {
+        o->re = mod;
+        o->im = 0;
+      }
+    }
+}
+
+/* convertFOPhase (cudaConfig.cu:861-870): data *= norm, negated where x+y is odd */
+void orc_convert_fo_phase(int rows, int cols, ocpx* d, REAL norm) {
+  for (int x = 0; x < rows; x++)
+    for (int y = 0; y < cols; y++) {
+      const REAL f = ((x + y) & 1) ? -norm : norm;
+      d[(size_t)x * cols + y].re *= f;
+      d[(size_t)x * cols + y].im *= f;
+    }
+}
+
+/* applyPoissonNoise_WO (src/gpu/cdi.cu:291-296) with scale = vars.scale:
+ * w = scale * (int(w*rcolor/scale) + Poisson(noiseLevel) - noiseLevel) / rcolor.
+ * The reference draws from cuRAND; here from the build's Philox Poisson
+ * stream 2 at counter i. */
+void orc_poisson_noise_wo(int64_t n, REAL* w, double noise, REAL scale, REAL rcolor, uint64_t seed) {
+  for (int64_t i = 0; i < n; i++) {
+    const double dark = (double)orc_poisson(noise, seed, 2, (uint64_t)i);
+    const REAL q = (REAL)((double)(int)(w[i] * rcolor / scale) + dark - noise);
+    w[i] = scale * q / rcolor;
+  }
+}
+
+/* extendToComplex (cudaConfig.cu:682-686): b = (a, 0) */
+void orc_extend_to_complex(int64_t n, const REAL* a, ocpx* b) {
+  for (int64_t i = 0; i < n; i++) {
+    b[i].re = a[i];
+    b[i].im = 0;
+  }
+}
+
 /* I = |FFT(obj)|^2 (FFT order) scaled to `photons` in total, then Poisson
  * sampled (stream 2) when `poisson` is set. */
 void orc_sim_intensity(int rows, int cols, const ocpx* obj, double photons, uint64_t seed,

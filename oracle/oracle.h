@@ -96,6 +96,12 @@ int  orc_phase_retrieve_prtf(int rows, int cols, const char* algorithm, ocpx* F,
                              double* rtrace, int rtrace_cap, int* n_iter_out, int prtfIter, REAL* prtf_out,
                              REAL* weights_out, int* prtf_count);
 
+/* ---- setup steps of prepareIter / readObjectWave (a14) ---- */
+void orc_create_wavefront(int rows, int cols, const REAL* I, const REAL* P, int orow, int ocol, ocpx* out);
+void orc_convert_fo_phase(int rows, int cols, ocpx* d, REAL norm);
+void orc_poisson_noise_wo(int64_t n, REAL* w, double noise, REAL scale, REAL rcolor, uint64_t seed);
+void orc_extend_to_complex(int64_t n, const REAL* a, ocpx* b);
+
 /* ---- synthetic forward model (Philox4x64-10 streams) ---- */
 void orc_philox4x64(const uint64_t ctr[4], const uint64_t key[2], uint64_t out[4]);
 double orc_uniform(uint64_t seed, uint64_t stream, uint64_t idx);

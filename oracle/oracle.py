@@ -64,6 +64,8 @@ class Oracle:
             "orc_phase_retrieve_prtf": [i, i, ctypes.c_char_p, vp, vp, vp, vp, vp, vp, vp, vp, i, vp, i, vp, vp, vp],
             "orc_add_phase": [i64, vp, vp, r], "orc_radial_sum": [i, i, vp, vp],
             "orc_shrink_wrap_freq": [i, i, vp, vp, r, vp, vp],
+            "orc_create_wavefront": [i, i, vp, vp, i, i, vp], "orc_convert_fo_phase": [i, i, vp, r],
+            "orc_poisson_noise_wo": [i64, vp, ctypes.c_double, r, r, u64], "orc_extend_to_complex": [i64, vp, vp],
         }
         for name, args in sig.items():
             getattr(L, name).argtypes = args
@@ -252,6 +254,31 @@ class Oracle:
         self.lib.orc_shrink_wrap_freq(F.shape[0], F.shape[1], F.ctypes.data, spt.ctypes.data, shrink_threshold,
                                       ctypes.cast(state, ctypes.c_void_p), scratch.ctypes.data)
         return spt
+
+    def create_wavefront(self, rows, cols, I=None, P=None, shape=None):
+        """createWaveFront(I, P, out, objrow, objcol): the object image centred in a rows x cols frame"""
+        shp = shape if shape is not None else (I if I is not None else P).shape
+        I = None if I is None else self._r(I)
+        P = None if P is None else self._r(P)
+        out = np.empty((rows, cols), self.cplx)
+        self.lib.orc_create_wavefront(rows, cols, self._p(I), self._p(P), shp[0], shp[1], out.ctypes.data)
+        return out
+
+    def convert_fo_phase(self, x, norm):
+        x = self._c(x).copy()
+        self.lib.orc_convert_fo_phase(x.shape[0], x.shape[1], x.ctypes.data, norm)
+        return x
+
+    def poisson_noise_wo(self, w, noise, scale, rcolor=65535, seed=0):
+        w = self._r(w).copy()
+        self.lib.orc_poisson_noise_wo(w.size, w.ctypes.data, noise, scale, rcolor, seed)
+        return w
+
+    def extend_to_complex(self, a):
+        a = self._r(a)
+        out = np.empty(a.shape, self.cplx)
+        self.lib.orc_extend_to_complex(a.size, a.ctypes.data, out.ctypes.data)
+        return out
 
     def add_phase(self, a, b, phi):
         a, b = self._c(a).copy(), self._c(b)

@@ -350,3 +350,52 @@ def test_errors_from_the_device_path(cdx):
         ctx.phase_retrieve("5*NOPE", cdx.Params.default(), d, d, ctx.real(), ctx.real())
     assert e.value.code == 3
     ctx.close()
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_prepare_iter_setup_steps_match_oracle(cdx, o64, o32, prec):
+    """The setup steps of CDI::prepareIter / readObjectWave (cdi.cc:62-82,
+    161-200) against the oracle: createWaveFront of an odd-margin object,
+    convertFOPhase, the beamstop multiply, applyPoissonNoise_WO (same Philox
+    Poisson stream; quantisation bit-exact where the draws agree) and
+    extendToComplex."""
+    o = o32 if prec == 0 else o64
+    R, C, r, c = 256, 256, 101, 77
+    rs = np.random.default_rng(31)
+    ctx = cdx.Context(R, C, precision=prec)
+    Ih, Ph = rs.uniform(-0.2, 4, (r, c)), rs.uniform(0, 1, (r, c))
+    dI = cdx.DeviceBuffer.from_array(Ih.astype(ctx.rdtype))
+    dP = cdx.DeviceBuffer.from_array(Ph.astype(ctx.rdtype))
+    w = ctx.cplx()
+    ctx.create_wavefront(w, dI, dP, (r, c))
+    ref = o.create_wavefront(R, C, Ih, Ph)
+    assert rel_l2(ctx.get_cplx(w), ref) < (1e-6 if prec == 0 else 1e-14)
+    ctx.create_wavefront(w, dI, None, (r, c))
+    assert np.array_equal(ctx.get_cplx(w), o.create_wavefront(R, C, Ih, None))
+    # convertFOPhase: a sign flip and a scale, bit-exact
+    x = crand(R, C).astype(ctx.cdtype)
+    d = ctx.cplx(x)
+    ctx.convert_fo_phase(d, 1 / 256.0)
+    assert np.array_equal(ctx.get_cplx(d), o.convert_fo_phase(x, 1 / 256.0))
+    # beamstop multiply (linearConst + multiply, cdi.cc:185-187)
+    pat = rs.uniform(0, 5, (R, C)).astype(ctx.rdtype)
+    bs = o64.beamstop(R, C, 7).astype(ctx.rdtype)
+    dp, dbs = ctx.real(pat), ctx.real(bs)
+    ctx.mask_multiply(dp, dbs, invert=True)
+    assert np.array_equal(ctx.get_real(dp), pat * (1 - bs))
+    # applyPoissonNoise_WO with vars.scale = 1/exposure
+    exposure = 0.1
+    ctx.init_vars(65535, 1 / exposure)
+    pat = rs.uniform(0, 20, (R, C)).astype(ctx.rdtype)
+    dp = ctx.real(pat)
+    ctx.poisson_noise_wo(dp, 24, seed=9)
+    got = ctx.get_real(dp)
+    want = o.poisson_noise_wo(pat, 24, 1 / exposure, 65535, seed=9)
+    same = got == want
+    print(f"poisson_noise_wo: {same.mean():.6f} of pixels bit-exact", flush=True)
+    assert same.mean() > 0.999
+    # extendToComplex
+    d = ctx.cplx()
+    ctx.extend_to_complex(dp, d)
+    assert np.array_equal(ctx.get_cplx(d), got.astype(ctx.cdtype))
+    ctx.close()

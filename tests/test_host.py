@@ -132,3 +132,43 @@ def test_config_reads_reference_cfg_files(rel, expect):
     c = cdixx_amd.read_config(path)
     for k, v in expect.items():
         assert c[k] == v, k
+
+
+def test_reference_bin_file_read_in_place():
+    """python/einstein.bin of the reference, read in place through the
+    library's .bin reader (readImage_c, src/independant/imgio.c:23-43): the
+    20-byte header (struct imageFile, include/imageFile.hpp:11-17) says FLOAT
+    (type 3), 150 x 150, and the file holds 20 + 150*150*4 = 90,020 bytes."""
+    import cdixx_amd
+    path = os.path.join(REFERENCE, "python", "einstein.bin")
+    if not os.path.exists(path):
+        pytest.skip("reference tree not present")
+    assert os.path.getsize(path) == 90020
+    assert cdixx_amd.bin_read_header(path) == (3, 150, 150)
+    img = cdixx_amd.bin_read(path)
+    assert img.shape == (150, 150) and img.dtype == np.float32 and np.isfinite(img).all() and img.max() > 0
+
+
+@pytest.mark.parametrize("dtype,code", [(np.float32, 3), (np.complex64, 4), (np.float64, 5), (np.complex128, 6)])
+def test_bin_round_trip(tmp_path, dtype, code):
+    """writeFloatImage / writeComplexImage (imgio.c:223-243) and the reader:
+    the header carries the type code and shape, the data come back bit for
+    bit; a truncated file and an unknown type code are errors."""
+    import cdixx_amd
+    rs = np.random.default_rng(code)
+    a = rs.standard_normal((7, 12))
+    if np.issubdtype(dtype, np.complexfloating):
+        a = a + 1j * rs.standard_normal((7, 12))
+    a = a.astype(dtype)
+    path = str(tmp_path / "x.bin")
+    cdixx_amd.bin_write(path, a)
+    assert os.path.getsize(path) == 20 + a.nbytes
+    assert cdixx_amd.bin_read_header(path) == (code, 7, 12)
+    assert np.array_equal(cdixx_amd.bin_read(path), a)
+    raw = open(path, "rb").read()
+    open(path, "wb").write(raw[:-4])
+    with pytest.raises(cdixx_amd.CdxError):
+        cdixx_amd.bin_read(path)
+    open(path, "wb").write(bytes([9]) + raw[1:])
+    with pytest.raises(cdixx_amd.CdxError):
+        cdixx_amd.bin_read_header(path)

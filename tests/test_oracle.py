@@ -521,3 +521,61 @@ def test_preprocess_pieces_closed_forms():
     img = np.exp(-((x - 40) ** 2 + (y - 25) ** 2) / 8.0)
     c = pp.shift_middle(img.astype(complex)).real
     assert np.unravel_index(np.argmax(c), c.shape) == (n // 2, n // 2)
+
+
+# ---------------------------------------------------------------------------
+# setup steps of CDI::prepareIter / readObjectWave (cdi.cc:62-82,161-200)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("frame,obj", [((12, 16), (5, 6)), ((16, 16), (8, 8)), ((9, 14), (4, 14))])
+def test_create_wavefront_is_a_centred_padding(o64, frame, obj):
+    """createWaveFront (cudaConfig.cu:688-709) places sqrt(max(I,0)) e^{2 pi i
+    (P - 0.5)} with zero margins of (frame - obj) // 2 before the image: the
+    same array as numpy's pad of the complex object, odd margins included."""
+    rs = np.random.default_rng(21)
+    I = rs.uniform(-0.5, 4, obj)
+    P = rs.uniform(0, 1, obj)
+    (R, C), (r, c) = frame, obj
+    pad = ((( R - r) // 2, R - r - (R - r) // 2), ((C - c) // 2, C - c - (C - c) // 2))
+    want = np.pad(np.sqrt(np.maximum(I, 0)) * np.exp(2j * np.pi * (P - 0.5)), pad)
+    assert np.abs(o64.create_wavefront(R, C, I, P) - want).max() < 1e-14
+    assert np.array_equal(o64.create_wavefront(R, C, I, None), np.pad(np.sqrt(np.maximum(I, 0)), pad) + 0j)
+    assert np.array_equal(o64.create_wavefront(R, C, None, None, shape=obj), np.pad(np.ones(obj), pad) + 0j)
+
+
+def test_convert_fo_phase_is_the_spectrum_of_the_centred_object(o64):
+    """convertFOPhase (cudaConfig.cu:861-870) multiplies by (-1)^(x+y) norm.
+    For even sizes (-1)^(kx+ky) FFT(x) = FFT(fftshift(x)) (shift theorem), so
+    prepareIter's convertFOPhase(FFT(obj), 1/sqrt(N)) is the unitary spectrum
+    of the object moved from the frame centre to the origin."""
+    rs = np.random.default_rng(22)
+    x = rs.standard_normal((16, 24)) + 1j * rs.standard_normal((16, 24))
+    N = x.size
+    got = o64.convert_fo_phase(np.fft.fft2(x), 1 / np.sqrt(N))
+    assert rel_l2(got, np.fft.fft2(np.fft.fftshift(x)) / np.sqrt(N)) < 1e-14
+    # norm 1 flips the odd checkerboard only
+    y = o64.convert_fo_phase(x, 1.0)
+    X, Y = np.mgrid[0:16, 0:24]
+    assert np.array_equal(y, np.where((X + Y) % 2 == 1, -x, x))
+
+
+def test_poisson_noise_wo_quantises_and_adds_dark_counts(o64):
+    """applyPoissonNoise_WO (cdi.cu:291-296): w = scale (int(w rcolor/scale) +
+    D - noise) / rcolor, D ~ Poisson(noise).  Without noise it is truncation
+    toward zero to steps of scale/rcolor; with noise the dark counts D are
+    non-negative integers with mean and variance noise."""
+    scale, rcolor = 2.0, 8.0
+    w = np.array([0.30, -0.30, 0.25, 1.99, -1.99, 0.0, 5.0])
+    want = scale * np.trunc(w * rcolor / scale) / rcolor
+    assert np.array_equal(o64.poisson_noise_wo(w, 0.0, scale, rcolor), want)
+    rs = np.random.default_rng(23)
+    w = rs.uniform(0, 3, 200_000)
+    out = o64.poisson_noise_wo(w, 24.0, scale, rcolor, seed=5)
+    D = out * rcolor / scale - np.trunc(w * rcolor / scale) + 24.0
+    assert np.allclose(D, np.round(D), atol=1e-9) and D.min() >= 0
+    assert abs(D.mean() - 24) < 0.05 and abs(D.var() / 24 - 1) < 0.02
+
+
+def test_extend_to_complex(o64):
+    a = np.random.default_rng(24).standard_normal((8, 8))
+    assert np.array_equal(o64.extend_to_complex(a), a + 0j)
+

@@ -285,11 +285,18 @@ def prtf_crossings(prtf, weights, levels=(np.exp(-1), 0.5)):
     return out
 
 
+# each workload as configured, and with the 20 ER steps the reference's
+# pulse.cfg ends on (html/default/pulse.cfg:27, "3000*RAAR+20*ER"): the RAAR
+# iterate itself is no projected estimate
 QUALITY_CONFIGS = [
     ("1024x1024 RAAR beta=0.85, 1000 iterations, shrink-wrap every 20", 1024, 160, 1e8, 8,
      "50*(20*RAAR+1*shrinkWrap)", (0, 1, 2, 3)),
+    ("1024x1024 RAAR beta=0.85, 1000 iterations, shrink-wrap every 20, then 20 ER", 1024, 160, 1e8, 8,
+     "50*(20*RAAR+1*shrinkWrap)+20*ER", (0, 1, 2, 3)),
     ("2048x2048 RAAR beta=0.85, 2000 iterations, shrink-wrap every 20", 2048, 320, 1e9, 16,
      "100*(20*RAAR+1*shrinkWrap)", (0,)),
+    ("2048x2048 RAAR beta=0.85, 2000 iterations, shrink-wrap every 20, then 20 ER", 2048, 320, 1e9, 16,
+     "100*(20*RAAR+1*shrinkWrap)+20*ER", (0,)),
 ]
 
 

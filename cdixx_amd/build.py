@@ -10,6 +10,7 @@ import sys
 from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcdixx_amd.so")
@@ -77,6 +78,13 @@ def build(verbose=True, jobs=None):
         _run(["g++"] + HOST_FLAGS + ["-o", os.path.join(HERE, "cdi_run"), cli_src,
                                     f"-L{HERE}", "-lcdixx_amd", f"-Wl,-rpath,{HERE}",
                                     "-Wl,-rpath,$ORIGIN"])
+    # a test program: a caller of include/cdi.hpp shaped like the
+    # reference's drivers (run by tests/test_gpu_dropin.py)
+    caller = os.path.join(ROOT, "tests", "dropin", "cdi_caller.cpp")
+    if os.path.exists(caller):
+        _run(["g++"] + HOST_FLAGS + [f"-I{os.path.join(ROOT, 'include')}", "-o", caller[:-4], caller,
+                                    f"-L{HERE}", "-lcdixx_amd", f"-Wl,-rpath,{HERE}",
+                                    "-Wl,-rpath,$ORIGIN/../../cdixx_amd"])
     if verbose:
         print(f"built {LIB}")
     return LIB

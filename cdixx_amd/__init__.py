@@ -90,6 +90,7 @@ _SIGS = {
     "cdx_expand_schedule": ([ctypes.c_char_p, _ip, _i], _i),
     "cdx_time_iterations": ([_vp, ctypes.POINTER(Params), _i, _vp, _vp, _vp, _vp, _vp, _i, _i, _dp, _dp, _dp],
                             _i),
+    "cdx_time_passes": ([_vp, ctypes.POINTER(Params), _i, _vp, _vp, _vp, _vp, _vp, _i, _i, _dp, _dp], _i),
     "cdx_sim_object": ([_vp, _d, _u64, _d, _vp], _i),
     "cdx_sim_intensity": ([_vp, _vp, _d, _u64, _i, _vp], _i),
     "cdx_read_config_json": ([ctypes.c_char_p, ctypes.c_char_p, _i], _i),
@@ -443,6 +444,14 @@ class Context:
 
     def extend_to_complex(self, a, b):
         check(lib.cdx_extend_to_complex(self.h, a.ptr, b.ptr))
+
+    def time_passes(self, params, algo, F, g, I, spt, n, bs=None, mode=0):
+        """ms per launch of the column and the row pass (cdx_time_passes):
+        mode 0 back to back in a graph, mode 1 each launch between its own events"""
+        cm, rm = ctypes.c_double(), ctypes.c_double()
+        check(lib.cdx_time_passes(self.h, ctypes.byref(params), algo, F.ptr, g.ptr, I.ptr, bs.ptr if bs else None,
+                                  spt.ptr, n, mode, ctypes.byref(cm), ctypes.byref(rm)))
+        return cm.value, rm.value
 
     def sim_object(self, obj, radius, seed=0, smooth_sigma=3.0):
         check(lib.cdx_sim_object(self.h, radius, seed, smooth_sigma, obj.ptr))

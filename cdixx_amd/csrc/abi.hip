@@ -692,9 +692,11 @@ static int phase_retrieve_t(cdx_ctx* c, const char* algorithm, const cdx_params*
   HIPCHK(launch_norm_c<T>(c->n, g, rootN, c->st));  // cdi.cc:406
   if (n_iter) *n_iter = iter;
   c->prtf_count = prtf_started ? iter - p->prtfIter : 0;
-  c->prtf_h.assign((size_t)c->rows / 2, 0.0);
-  c->weights_h.assign((size_t)c->rows / 2, 0.0);
+  c->prtf_h.clear();
+  c->weights_h.clear();
   if (prtf_started) {
+    c->prtf_h.assign((size_t)c->rows / 2, 0.0);
+    c->weights_h.assign((size_t)c->rows / 2, 0.0);
     HIPCHK(hipMemcpyAsync(c->prtf_h.data(), c->prtf_dev, c->prtf_h.size() * sizeof(double), hipMemcpyDeviceToHost,
                           c->st));
     HIPCHK(hipMemcpyAsync(c->weights_h.data(), c->prtf_dev + 1024, c->weights_h.size() * sizeof(double),
@@ -730,10 +732,12 @@ extern "C" int cdx_phase_retrieve(cdx_ctx* c, const char* algorithm, const cdx_p
 
 extern "C" int cdx_prtf(cdx_ctx* c, double* prtf, double* weights, int cap, int* count) {
   CTX_CHECK(c);
-  const int m = (int)c->prtf_h.size() < cap ? (int)c->prtf_h.size() : cap;
+  // rows/2 values; zeros when the last call accumulated nothing
+  const int m = c->rows / 2 < cap ? c->rows / 2 : cap;
+  const bool have = (int)c->prtf_h.size() >= m;
   for (int i = 0; i < m; i++) {
-    if (prtf) prtf[i] = c->prtf_h[i];
-    if (weights) weights[i] = c->weights_h[i];
+    if (prtf) prtf[i] = have ? c->prtf_h[i] : 0.0;
+    if (weights) weights[i] = have ? c->weights_h[i] : 0.0;
   }
   if (count) *count = c->prtf_count;
   return CDX_OK;
@@ -760,6 +764,21 @@ struct TimingRes {
   }
 };
 
+// a list of events released on every exit path
+struct EventList {
+  std::vector<hipEvent_t> e;
+  int create(int k) {
+    e.assign((size_t)k, nullptr);
+    for (auto& x : e)
+      if (hipEventCreate(&x) != hipSuccess) return fail(CDX_ERR_HIP, "hipEventCreate failed");
+    return CDX_OK;
+  }
+  ~EventList() {
+    for (auto x : e)
+      if (x) (void)hipEventDestroy(x);
+  }
+};
+
 // Captures `body` (a sequence of launches on st) into res.graph and
 // instantiates it.  A capture that fails part-way is still ended, so the
 // stream is never left capturing.
@@ -777,7 +796,8 @@ static int capture(hipStream_t st, TimingRes& res, Body body) {
 
 template <class T>
 static int time_t_(cdx_ctx* c, const cdx_params* p, int algo, void* F, void* g, const void* I, const void* bs,
-                   const void* spt, int n, int use_graph, double* ms_total, double* col_ms, double* row_ms) {
+                   const void* spt, int n, int use_graph, double* ms_total, double* col_ms, double* row_ms,
+                   int per_launch = 0, int n_pass = 0) {
   Loop<T> L{c, p, {}, true, F, g, (T*)spt};
   int rc = L.setup(I, bs);
   if (rc) return rc;
@@ -822,7 +842,7 @@ static int time_t_(cdx_ctx* c, const cdx_params* p, int algo, void* F, void* g, 
     // T2, so each moves the bytes of its pass in the loop.  The row-pass
     // launches re-apply the update to g, so the state afterwards is no
     // iterate of the loop.
-    n = n < 200 ? n : 200;
+    n = n_pass > 0 ? n_pass : (n < 200 ? n : 200);
     UpdArgs<T> ua = upd_args<T>(c, algo, p->beta, p->relax);
     L.pa.rf = rf;
     for (int pass = 0; pass < 2; pass++) {
@@ -835,6 +855,29 @@ static int time_t_(cdx_ctx* c, const cdx_params* p, int algo, void* F, void* g, 
         return launch_row<T>(ROW_INV_UPDATE_FWD, c->rows, c->cols, c->T2, c->T1, c->tw_row, g, (const T*)spt, ua,
                              (T)1, nullptr, ReduceSlots{}, c->st, c->row_lines);
       };
+      if (per_launch) {
+        // every launch between its own pair of events: launches are then
+        // serialized as under a kernel-trace profiler (mean over launches)
+        EventList ev;
+        rc = ev.create(2 * n);
+        if (rc) return rc;
+        HIPCHK(launch());  // warm
+        HIPCHK(hipStreamSynchronize(c->st));
+        for (int i = 0; i < n; i++) {
+          HIPCHK(hipEventRecord(ev.e[2 * i], c->st));
+          HIPCHK(launch());
+          HIPCHK(hipEventRecord(ev.e[2 * i + 1], c->st));
+        }
+        HIPCHK(hipEventSynchronize(ev.e[2 * n - 1]));
+        double sum = 0;
+        for (int i = 0; i < n; i++) {
+          float ms = 0;
+          HIPCHK(hipEventElapsedTime(&ms, ev.e[2 * i], ev.e[2 * i + 1]));
+          sum += ms;
+        }
+        *dst = sum / n;
+        continue;
+      }
       if (use_graph) {
         rc = capture(c->st, res, [&]() {
           for (int i = 0; i < n; i++) HIPCHK(launch());
@@ -868,6 +911,14 @@ static int time_t_(cdx_ctx* c, const cdx_params* p, int algo, void* F, void* g, 
   HIPCHK(launch_norm_c<T>(c->n, g, rootN, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
   return CDX_OK;
+}
+
+extern "C" int cdx_time_passes(cdx_ctx* c, const cdx_params* p, int algo, void* F, void* g, const void* I,
+                               const void* bs, const void* spt, int n, int mode, double* col_ms, double* row_ms) {
+  CTX_CHECK(c);
+  if (!p || !F || !g || !I || !spt || n <= 0 || (mode != 0 && mode != 1) || !col_ms || !row_ms)
+    return fail(CDX_ERR_ARG, "cdx_time_passes: bad argument");
+  return DISPATCH(c, time_t_, c, p, algo, F, g, I, bs, spt, 1, mode == 0, nullptr, col_ms, row_ms, mode, n);
 }
 
 extern "C" int cdx_time_iterations(cdx_ctx* c, const cdx_params* p, int algo, void* F, void* g, const void* I,

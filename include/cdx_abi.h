@@ -252,6 +252,17 @@ int cdx_time_iterations(cdx_ctx* ctx, const cdx_params* p, int algo, void* F, vo
                         const void* bs, const void* spt, int n_iter, int use_graph, double* ms_total,
                         double* col_ms, double* row_ms);
 
+/* Per-pass launch durations of the loop (ms per launch) from the state F, g
+ * after one untimed iteration: each pass launched n times on its own.
+ * mode 0: back to back in one captured graph between two events, the median
+ * of three runs (the figure cdx_time_iterations returns); mode 1: every
+ * launch between its own pair of events on the context stream, mean over
+ * the launches -- launches serialized the way a kernel-trace profiler
+ * serializes them.  F and g are afterwards no iterate of the loop (the row
+ * pass re-applies the update to the same input). */
+int cdx_time_passes(cdx_ctx* ctx, const cdx_params* p, int algo, void* F, void* g, const void* I, const void* bs,
+                    const void* spt, int n, int mode, double* col_ms, double* row_ms);
+
 /* ---- tiling of the FFT passes (no counterpart in the reference, whose
  * cuFFT plan chooses internally; init_fft, src/gpu/cudaConfig.cu:137-160) ----
  * lines = columns (column pass) / rows (row pass) per workgroup; 0 = default. */

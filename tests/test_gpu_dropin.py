@@ -96,7 +96,7 @@ def test_cdi_class_caller(cdx, o32, o64, tmp_path, sim_ccd):
         step = (1 / EXPOSURE) / 65535
         d = np.abs(got - want)
         print(f"  pattern vs oracle: {np.mean(d == 0):.5f} exact, max {d.max() / step:.2f} steps", flush=True)
-        assert d.max() <= 1.0001 * step and np.mean(d == 0) > 0.999
+        assert d.max() <= 1.01 * step and np.mean(d == 0) > 0.999
     assert os.path.exists(tmp_path / "prtf.txt")
 
 

@@ -220,3 +220,37 @@ def test_scaled_warm_start_runs_at_2048(cdx):
     r_learn, floor = out["learned"]["final_R"][0], out["R_true_object"][0]
     print(f"final R learned {r_learn} floor {floor}", flush=True)
     assert np.isfinite(r_learn) and floor < r_learn < 0.5
+
+
+def test_guarded_warm_start_never_ends_above_random_at_2048(cdx):
+    """Random-start phase retrieval as the fallback and judge of the learned
+    start (reconstruct_guarded, the pattern of util/cdi.cpp:36-45): at 2048^2
+    (object radius 320 px, 1e9 photons, beamstop r = 16, 100 x (19 RAAR + 1 ER
+    + shrink-wrap)) on seeds 0-3, both starts run and the kept result's R is
+    the lower of the two -- never above the random start's -- and the
+    returned buffers are the kept run's (its F = FFT(g) gives that R)."""
+    from cdixx_amd.warmstart import DEFAULT_WEIGHTS, LEVELS, BASE, SIZE, reconstruct_guarded
+    import bench
+    net = cdx.UNet(SIZE, LEVELS, BASE, max_batch=1, seed=1)
+    net.load(DEFAULT_WEIGHTS)
+    cs, ctx = cdx.Context(SIZE), cdx.Context(2048)
+    p = cdx.Params.default(recordR=1, **bench.E2E_PARAMS)
+    alg = "100*(19*RAAR+1*ER+1*shrinkWrap)"
+    for seed in range(4):
+        obj, I, bs = ctx.cplx(), ctx.real(), ctx.real()
+        ctx.sim_object(obj, 320, seed=seed)
+        ctx.sim_intensity(obj, I, 1e9, seed=seed, poisson=True)
+        ctx.create_beamstop(bs, 16)
+        r = reconstruct_guarded(cdx, net, cs, ctx, I, bs, alg, p, seed, 0.2 * 2048)
+        print(f"seed {seed}: R learned {r['R_learned']:.4f} random {r['R_random']:.4f} -> kept {r['kept']}",
+              flush=True)
+        assert r["R"] == min(r["R_learned"], r["R_random"]) and r["R"] <= r["R_random"]
+        assert abs(ctx.rfactor(r["F"], I, bs) - r["R"]) < 1e-9
+        g = ctx.get_cplx(r["g"]).astype(np.complex128)
+        F = ctx.get_cplx(r["F"]).astype(np.complex128)
+        assert np.linalg.norm(np.fft.fft2(g) / 2048 - F) < 1e-4 * np.linalg.norm(F)
+        for b in (obj, I, bs, r["F"], r["g"], r["spt"]):
+            b.free()
+    ctx.close()
+    cs.close()
+    net.close()

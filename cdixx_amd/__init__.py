@@ -96,6 +96,8 @@ _SIGS = {
     "cdx_read_config_json": ([ctypes.c_char_p, ctypes.c_char_p, _i], _i),
     "cdx_copy": ([_vp, _vp, _vp, _sz], _i),
     "cdx_prtf": ([_vp, _dp, _dp, _i, _ip], _i),
+    "cdx_phase_retrieve_static": ([_i, _i, ctypes.c_char_p, ctypes.POINTER(Params), _vp, _vp, _vp, _vp, _vp, _dp,
+                                   _vp, _ip], _i),
     "cdx_batch_create": ([ctypes.POINTER(_vp), _i, _ip, _i, _vp], _i), "cdx_batch_destroy": ([_vp], _i),
     "cdx_batch_frame": ([_vp, _i] + [ctypes.POINTER(_vp)] * 5 + [_ip], _i),
     "cdx_batch_context": ([_vp, _i], _vp), "cdx_batch_set_threshold": ([_vp, _d], _i),
@@ -211,6 +213,17 @@ def bin_write(path, a):
     if code is None or a.ndim != 2:
         raise ValueError("bin_write: 2-D float32/complex64/float64/complex128 array expected")
     check(lib.cdx_bin_write(path.encode(), code, a.shape[0], a.shape[1], a.ctypes.data))
+
+
+def phase_retrieve_static(rows, cols, algorithm, params, F, g, I, spt, bs=None, threshold=0.5, stream=None):
+    """cdx_phase_retrieve_static: the loop on FP32 frames up to 512 x 512
+    with no device allocation (static workspace).  Returns (n_iter, threshold)."""
+    thr = ctypes.c_double(threshold)
+    n_iter = ctypes.c_int()
+    check(lib.cdx_phase_retrieve_static(rows, cols, algorithm.encode(), ctypes.byref(params), F.ptr, g.ptr, I.ptr,
+                                        bs.ptr if bs else None, spt.ptr, ctypes.byref(thr), stream,
+                                        ctypes.byref(n_iter)))
+    return n_iter.value, thr.value
 
 
 def read_config(path):

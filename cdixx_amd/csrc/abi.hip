@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <mutex>
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
@@ -1151,4 +1152,126 @@ extern "C" int cdx_autotune(cdx_ctx* c, int reps, double* report, int cap) {
   CTX_CHECK(c);
   if (reps <= 0) reps = 50;
   return DISPATCH(c, autotune_t, c, reps, report, cap);
+}
+
+// ---------------------------------------------------------------------------
+// cdx_phase_retrieve_static: the loop for callers that cannot allocate
+// device memory.  Its workspace (T1, T2, the packed measurement, the blur
+// buffer, twiddle tables, scalars, partial sums) is the file-scope array
+// `scratch`, sized for frames of up to kStaticMaxSide points per side in
+// FP32; the kernels are those of cdx_phase_retrieve.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int kStaticMaxSide = 512;
+constexpr size_t kStaticMaxN = (size_t)kStaticMaxSide * kStaticMaxSide;
+constexpr size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr size_t kOffT1 = 0;
+constexpr size_t kOffT2 = kOffT1 + al256(kStaticMaxN * 8);
+constexpr size_t kOffIpk = kOffT2 + al256(kStaticMaxN * 8);
+constexpr size_t kOffTmp = kOffIpk + al256(kStaticMaxN * 4);
+constexpr size_t kOffTwR = kOffTmp + al256(kStaticMaxN * 4);
+constexpr size_t kOffTwC = kOffTwR + al256(kStaticMaxSide * 8);
+constexpr size_t kOffScal = kOffTwC + al256(kStaticMaxSide * 8);
+constexpr size_t kOffPart = kOffScal + al256(S_COUNT * sizeof(double));
+constexpr size_t kScratchBytes = kOffPart + al256(kMaxPartials * sizeof(double));  // 6 MiB + 38 KiB
+constexpr int kMaxDevices = 64;
+}  // namespace
+
+// One copy per GPU, held until the process ends.  Host code never uses this
+// name as an address: the per-device pointer from hipGetSymbolAddress is.
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+// device address of the current device's copy of `scratch`, looked up on
+// the first call on that device
+static int scratch_address(void** out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(CDX_ERR_ARG, "cdx_phase_retrieve_static: device %d", dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(CDX_ERR_HIP, "cdx_phase_retrieve_static: hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return CDX_OK;
+}
+
+// twiddle table tw[m] = W_n^m = exp(-2 pi i m/n) in float pairs, the values
+// make_twiddles uploads (double evaluation after exact octant reduction)
+__global__ void k_twiddles_f32(int n, float* tw) {
+  for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < n; m += gridDim.x * blockDim.x) {
+    const long e8 = 8L * m;
+    const int quad = (int)((e8 / n + 1) / 2);
+    const double r = -2.0 * M_PI * (double)(4L * m - (long)quad * n) / (4.0 * n);
+    const double c = cos(r), s = sin(r);
+    double re, im;
+    switch (quad & 3) {
+      case 0: re = c; im = s; break;
+      case 1: re = s; im = -c; break;
+      case 2: re = -c; im = -s; break;
+      default: re = -s; im = c; break;
+    }
+    tw[2 * m] = (float)re;
+    tw[2 * m + 1] = (float)im;
+  }
+}
+
+extern "C" int cdx_phase_retrieve_static(int rows, int cols, const char* algorithm, const cdx_params* p, void* F,
+                                         void* g, const void* I, const void* bs, void* spt, double* threshold,
+                                         void* stream, int* n_iter) {
+  if (!algorithm || !p || !F || !g || !I || !spt || !threshold)
+    return fail(CDX_ERR_ARG, "cdx_phase_retrieve_static: null argument");
+  if (!line_supported(rows) || !line_supported(cols) || rows > kStaticMaxSide || cols > kStaticMaxSide)
+    return fail(CDX_ERR_ARG,
+                "cdx_phase_retrieve_static: %dx%d does not fit the static workspace (FFT lengths up to %dx%d)", rows,
+                cols, kStaticMaxSide, kStaticMaxSide);
+  if (!(p->exposure != 0)) return fail(CDX_ERR_ARG, "cdx_phase_retrieve_static: exposure must be non-zero");
+  if (p->prtfIter >= 0) return fail(CDX_ERR_ARG, "cdx_phase_retrieve_static: no PRTF (prtfIter must be < 0)");
+  unsigned char* base = nullptr;
+  int rc = scratch_address((void**)&base);
+  if (rc) return rc;
+  cdx_params q = *p;
+  q.useGraph = 0;  // graph instantiation allocates
+  q.recordR = 0;
+  cdx_ctx c{};  // a context on the stack whose buffers are slices of scratch
+  c.rows = rows;
+  c.cols = cols;
+  c.prec = CDX_F32;
+  c.rsz = 4;
+  c.csz = 8;
+  c.n = (int64_t)rows * cols;
+  c.st = (hipStream_t)stream;
+  c.own_stream = false;
+  c.T1 = base + kOffT1;
+  c.T2 = base + kOffT2;
+  c.Ipk = base + kOffIpk;
+  c.tmpr = base + kOffTmp;
+  c.tw_row = base + kOffTwR;
+  c.tw_col = base + kOffTwC;
+  c.scal = (double*)(base + kOffScal);
+  c.partials = (double*)(base + kOffPart);
+  c.ring = nullptr;
+  c.ring_cap = 0;
+  c.rcolor = 65535;
+  c.scale = 1;
+  c.beta = 1;
+  c.relax = 1.9;
+  hipLaunchKernelGGL(k_twiddles_f32, dim3((cols + 255) / 256), dim3(256), 0, c.st, cols, (float*)c.tw_row);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_twiddles_f32, dim3((rows + 255) / 256), dim3(256), 0, c.st, rows, (float*)c.tw_col);
+  HIPCHK(hipGetLastError());
+  // scalars from zero (a previous call's are still there), then the threshold
+  HIPCHK(launch_fill_f64(S_COUNT, c.scal, 0.0, c.st));
+  HIPCHK(launch_fill_f64(1, c.scal + S_THR, (double)(float)*threshold, c.st));
+  rc = phase_retrieve_t<float>(&c, algorithm, &q, F, g, I, bs, spt, nullptr, 0, n_iter);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(threshold, c.scal + S_THR, sizeof(double), hipMemcpyDeviceToHost, c.st));
+  HIPCHK(hipStreamSynchronize(c.st));
+  return CDX_OK;
 }

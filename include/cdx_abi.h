@@ -173,6 +173,21 @@ void cdx_params_default(cdx_params* p);
 int cdx_phase_retrieve(cdx_ctx* ctx, const char* algorithm, const cdx_params* p, void* F, void* g,
                        const void* I, const void* bs, void* spt, double* rtrace, int rtrace_cap, int* n_iter);
 
+/* No counterpart in the reference: cdx_phase_retrieve for callers that
+ * cannot allocate device memory (code that must not call hipMalloc: tight
+ * serving loops, harnesses that forbid it).  FP32 frames of rows x cols <=
+ * 512 x 512 (FFT lengths); F, g, I, bs, spt as in cdx_phase_retrieve; the
+ * support threshold comes in and goes out through *threshold (host); stream
+ * may be NULL.  Its workspace is one static device array of 6 MiB + 38 KiB
+ * per GPU, held until the process ends: no device memory is allocated or
+ * freed (the schedule string is parsed on the host).  Launches go straight
+ * to the stream (no graph); PRTF accumulation is not available (prtfIter
+ * must be < 0) and recordR is ignored.  Calls that overlap on one GPU share
+ * that workspace and race.  Larger frames: CDX_ERR_ARG. */
+int cdx_phase_retrieve_static(int rows, int cols, const char* algorithm, const cdx_params* p, void* F, void* g,
+                              const void* I, const void* bs, void* spt, double* threshold, void* stream,
+                              int* n_iter);
+
 /* PRTF of the last cdx_phase_retrieve with p->prtfIter >= 0 (cdi.cc:423-440):
  * prtf[r] and weights[r], r < rows/2, are the radial sums whose ratio the
  * reference writes to prtf.txt; *count = iter - prtfIter (0: no step

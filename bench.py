@@ -49,7 +49,10 @@ def make_inputs(cdx, ctx, seed=0):
 
 def cpu_baseline(seconds=15.0):
     """The oracle (FP32, OpenMP over rows/columns) on the host cores, on a
-    bounded sample: as many 1024^2 RAAR iterations as fit in `seconds`."""
+    bounded sample: as many 1024^2 RAAR iterations (the headline workload's
+    parameters) as fit in `seconds`; beside it the 256^2 CPU-reference
+    workload end to end (10 x (20 HIO + shrink-wrap) + 2 x (20 ER +
+    shrink-wrap) + 10 ER, real-space shrink-wrap) and its per-iteration rate."""
     from oracle.oracle import Oracle
     o = Oracle("f32")
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -60,7 +63,7 @@ def cpu_baseline(seconds=15.0):
     spt = o.circle(N, N, 0.2 * N)
     F = np.sqrt(I) * np.exp(2j * np.pi * rs.uniform(size=I.shape))
     g = np.fft.ifft2(F) * N
-    kw = dict(beta=0.85)
+    kw = dict(beta=0.85, useBS=1, freeBeamstop=1)
     o.phase_retrieve("1*RAAR", F, g, I, spt, 0.5, bs=bs, **kw)  # warm
     n, t0 = 0, time.perf_counter()
     while True:
@@ -70,8 +73,28 @@ def cpu_baseline(seconds=15.0):
         el = time.perf_counter() - t0
         if el > seconds:
             break
+    # the 256^2 CPU-reference workload: end to end and per iteration
+    c = dict(REPORT_CONFIGS)[CPU_REFERENCE_WORKLOAD]
+    n1 = c["n"]
+    obj1 = o.sim_object(n1, c["radius"], 0)
+    I1 = o.sim_intensity(obj1, c["photons"], 0, poisson=True)
+    spt1 = o.support_rect(n1, n1, 2.0)
+    F1 = np.sqrt(I1) * np.exp(2j * np.pi * rs.uniform(size=I1.shape))
+    g1 = np.fft.ifft2(F1) * n1
+    pr = c["params"]
+    okw = dict(beta=pr["beta"], shrinkMode=pr["shrinkMode"], shrinkThreshold=pr["shrinkThreshold"],
+               sigmaStart=pr["sigma0"], sigmaMin=pr["sigmaMin"], sigmaDecay=pr["sigmaDecay"])
+    t1 = time.perf_counter()
+    r1 = o.phase_retrieve(c["schedule"], F1, g1, I1, spt1, 0.5, **okw)
+    e2e = time.perf_counter() - t1
+    t2 = time.perf_counter()
+    o.phase_retrieve("50*HIO", F1, g1, I1, spt1, 0.5, beta=pr["beta"])
+    it256 = 50 / (time.perf_counter() - t2)
     return dict(value=n / el, unit="iters/s", cores=threads, kind="port",
-                sample=f"{n} RAAR iterations at 1024x1024 complex64 (FP32 oracle, OpenMP) in {el:.1f} s")
+                sample=f"{n} RAAR iterations at 1024x1024 complex64 (FP32 oracle, OpenMP) in {el:.1f} s",
+                cpu_reference_workload_256={"schedule": c["schedule"], "steps": r1["n_iter"],
+                                            "seconds_end_to_end": round(e2e, 3),
+                                            "hio_iters_per_s": round(it256, 1)})
 
 
 BATCH_SIZES = [256] * 24 + [384] * 24 + [512] * 16
@@ -129,6 +152,8 @@ def batched_leg(cdx, iters):
 # a threshold of 0.02 keeps the support from collapsing onto the brightest
 # blob of the iterate, which 0.1 and above do on these objects
 E2E_PARAMS = dict(beta=0.85, shrinkThreshold=0.02, useBS=1, freeBeamstop=1)
+
+CPU_REFERENCE_WORKLOAD = "256x256 HIO+ER, real-space shrink-wrap (CPU reference path)"
 
 # FP32 pipeline against an FP64 GPU run from the same inputs, per workload
 REPORT_CONFIGS = [
@@ -675,7 +700,8 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c64/f32",
         "data": "synthetic (seeded object, Poisson 1e8 photons)",
-        "config": {"workload": "single-frame RAAR beta=0.85, 1024x1024 complex64, beamstop r=8, support r=160",
+        "config": {"workload": "single-frame RAAR beta=0.85, 1024x1024 complex64, object radius 160 px, "
+                               "disk support radius 204.8 px (0.2 N), beamstop r=8 (left unconstrained)",
                    "frame": [N, N], "algorithm": "RAAR", "graph": not args.no_graph,
                    "tiling": {"col_lines": tiling[0], "row_lines": tiling[1]},
                    "tuner_us": {f"{'col' if p_ == 0 else 'row'}{l_}": round(us, 2) for p_, l_, us in tuning}},

@@ -100,7 +100,7 @@ def cpu_baseline(seconds=15.0):
 BATCH_SIZES = [256] * 24 + [384] * 24 + [512] * 16
 
 
-def batched_leg(cdx, iters):
+def batched_leg(cdx, iters, use_graph=True):
     """64 frames (24x256^2, 24x384^2, 16x512^2), support radius 0.16 N,
     photons log-uniform in [1e6, 1e8], per-frame seeds: `iters` HIO
     iterations for the whole batch in one captured graph, against the same
@@ -113,9 +113,15 @@ def batched_leg(cdx, iters):
     b.simulate(radii, photons, seeds, support_scale=1.25)
     p = cdx.Params.default(beta=0.9)
     b.set_threshold(0.5)
-    tuner_us = b.autotune(reps=20)  # workgroup size and radix schedules of the batched passes
-    b.time_iterations(p, cdx.HIO, 20)  # warm
-    ms_batch = b.time_iterations(p, cdx.HIO, iters)
+    if use_graph:
+        tuner_us = b.autotune(reps=20)  # workgroup size and radix schedules of the batched passes
+    else:
+        # stream launches (kernel traces): the tuner times captured graphs, so
+        # take the schedule it picks on MI355X for this mix
+        b.set_schedule(256, {384: 1})
+        tuner_us = {}
+    b.time_iterations(p, cdx.HIO, 20, use_graph=use_graph)  # warm
+    ms_batch = b.time_iterations(p, cdx.HIO, iters, use_graph=use_graph)
     # per-frame loop: the same frames through single-frame contexts
     frames = [{k: b.get(i, k) for k in ("F", "g", "I", "spt")} for i in range(len(BATCH_SIZES))]
     ctxs = {n: cdx.Context(n) for n in set(BATCH_SIZES)}
@@ -125,12 +131,12 @@ def batched_leg(cdx, iters):
         bufs.append((c, c.cplx(frames[i]["F"]), c.cplx(frames[i]["g"]), c.real(frames[i]["I"]),
                      c.real(frames[i]["spt"])))
     for c, F, g, I, s in bufs[:3]:
-        c.time_iterations(p, cdx.HIO, F, g, I, s, 20, per_kernel=False)  # warm
+        c.time_iterations(p, cdx.HIO, F, g, I, s, 20, use_graph=use_graph, per_kernel=False)  # warm
     # each frame's `iters` iterations as one captured graph, timed with
     # events like the batched run (capture and instantiation excluded)
     ms_loop = 0.0
     for c, F, g, I, s in bufs:
-        ms_loop += c.time_iterations(p, cdx.HIO, F, g, I, s, iters, per_kernel=False)[0]
+        ms_loop += c.time_iterations(p, cdx.HIO, F, g, I, s, iters, use_graph=use_graph, per_kernel=False)[0]
     pix = sum(n * n for n in BATCH_SIZES)
     b_threads, _, b_radices = b.schedule()
     fps_b = len(BATCH_SIZES) / (ms_batch * 1e-3) * 1.0
@@ -621,7 +627,7 @@ def main():
         print(json.dumps({"ptycho": ptycho_leg(cdx)}), flush=True)
         return
     if args.batch_only:
-        print(json.dumps({"batched": batched_leg(cdx, args.batch_iters)}), flush=True)
+        print(json.dumps({"batched": batched_leg(cdx, args.batch_iters, use_graph=not args.no_graph)}), flush=True)
         return
     if args.learned_large_only:
         print(json.dumps({"learned_warm_start_2048": learned_large_leg(cdx)}), flush=True)
@@ -719,7 +725,7 @@ def main():
                        "R_support_projected": round(r_final, 5)},
     }
     if not args.no_batch:
-        out["batched"] = batched_leg(cdx, args.batch_iters)
+        out["batched"] = batched_leg(cdx, args.batch_iters, use_graph=not args.no_graph)
     if not args.no_report:
         out["fp64_report"] = fp64_report(cdx)
         out["quality"] = quality_leg(cdx)

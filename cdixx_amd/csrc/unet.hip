@@ -123,30 +123,85 @@ __device__ __forceinline__ void mfma_tile(const u16 (*As)[kLdsRow], const u16 (*
 // row of accumulator element r in the 32x32 block (column = lane & 31)
 __device__ __forceinline__ int acc_row(int r, int lane) { return (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3); }
 
+// Implicit-GEMM convolution on a TM x TN output tile (TM = 32 WM pixels, TN =
+// 32 WN channels, four waves of 32 x 32): K steps of 64 through two LDS
+// buffers, the next step's global loads in flight during the current step's
+// MFMAs, one barrier per step.  WM = 4, WN = 1 (128 x 32) serves layers of
+// at most 32 output channels without idle MFMA columns; WM = WN = 2 (64 x 64)
+// the wider ones.
+template <int WM, int WN>
 __global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
-  __shared__ __align__(16) u16 As[kTile][kLdsRow];
-  __shared__ __align__(16) u16 Bs[kTile][kLdsRow];
+  constexpr int TM = 32 * WM, TN = 32 * WN, BK = 64, LP = BK + 8;
+  constexpr int AV = TM * BK / 8 / 256;  // 16-byte A pieces per thread
+  constexpr int BV = TN * BK / 8 / 256;  // 16-byte B pieces per thread
+  static_assert(WM * WN == 4 && AV >= 1 && BV >= 1, "four waves");
+  __shared__ __align__(16) u16 As[2][TM][LP];
+  __shared__ __align__(16) u16 Bs[2][TN][LP];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
+  const int wm = wave / WN, wn = wave % WN;
   const int HW = a.H * a.W, M = a.B * HW, K = a.taps * a.Cin;
-  const int m0 = blockIdx.x * kTile, n0 = blockIdx.y * kTile;
-  const int lr = tid >> 2, lk = (tid & 3) * 8;
-  const int m = m0 + lr;
-  const bool mok = m < M;
-  int b = 0, h = 0, w = 0;
-  if (mok) pixel_of(m, HW, a.W, b, h, w);
-  const int co = n0 + lr;
-  const bool cok = co < a.Cout;
+  const int m0 = blockIdx.x * TM, n0 = blockIdx.y * TN;
+  // this thread's A pieces: row (pixel) and k offset within the step
+  int arow[AV], akc[AV], ab[AV], ah[AV], aw[AV];
+  bool aok[AV];
+#pragma unroll
+  for (int j = 0; j < AV; j++) {
+    const int idx = tid + 256 * j;
+    arow[j] = idx >> 3;
+    akc[j] = (idx & 7) * 8;
+    const int m = m0 + arow[j];
+    aok[j] = m < M;
+    ab[j] = ah[j] = aw[j] = 0;
+    if (aok[j]) pixel_of(m, HW, a.W, ab[j], ah[j], aw[j]);
+  }
+  int brow[BV], bkc[BV];
+  bool bok[BV];
+#pragma unroll
+  for (int j = 0; j < BV; j++) {
+    const int idx = tid + 256 * j;
+    brow[j] = idx >> 3;
+    bkc[j] = (idx & 7) * 8;
+    bok[j] = n0 + brow[j] < a.Cout;
+  }
+  uint4 ra[AV], rb[BV];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int j = 0; j < AV; j++) {
+      const int k = k0 + akc[j];
+      ra[j] = (aok[j] && k < K) ? load_patch8(a.x, a.H, a.W, a.Cin, a.taps, ab[j], ah[j], aw[j], k)
+                                : make_uint4(0, 0, 0, 0);
+    }
+#pragma unroll
+    for (int j = 0; j < BV; j++) {
+      const int k = k0 + bkc[j];
+      rb[j] = (bok[j] && k < a.Kp) ? *(const uint4*)(a.w + (size_t)(n0 + brow[j]) * a.Kp + k) : make_uint4(0, 0, 0, 0);
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int j = 0; j < AV; j++) *(uint4*)&As[buf][arow[j]][akc[j]] = ra[j];
+#pragma unroll
+    for (int j = 0; j < BV; j++) *(uint4*)&Bs[buf][brow[j]][bkc[j]] = rb[j];
+  };
   f32x16 acc = {};
-  for (int k0 = 0; k0 < a.Kp; k0 += kBK) {
-    const int k = k0 + lk;
-    const uint4 av = (mok && k < K) ? load_patch8(a.x, a.H, a.W, a.Cin, a.taps, b, h, w, k) : make_uint4(0, 0, 0, 0);
-    const uint4 bv = cok ? *(const uint4*)(a.w + (size_t)co * a.Kp + k) : make_uint4(0, 0, 0, 0);
+  load(0);
+  store(0);
+  __syncthreads();
+  int buf = 0;
+  for (int k0 = 0; k0 < a.Kp; k0 += BK) {
+    const bool more = k0 + BK < a.Kp;
+    if (more) load(k0 + BK);
+#pragma unroll
+    for (int s4 = 0; s4 < BK / 16; s4++) {
+      const int kk = s4 * 16 + (lane >> 5) * 8;
+      const bf16x8 av = *(const bf16x8*)&As[buf][wm * 32 + (lane & 31)][kk];
+      const bf16x8 bv = *(const bf16x8*)&Bs[buf][wn * 32 + (lane & 31)][kk];
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc, 0, 0, 0);
+    }
+    // the other buffer was last read in the previous step, before its barrier
+    if (more) store(buf ^ 1);
     __syncthreads();
-    *(uint4*)&As[lr][lk] = av;
-    *(uint4*)&Bs[lr][lk] = bv;
-    __syncthreads();
-    mfma_tile(As, Bs, wm, wn, lane, acc);
+    buf ^= 1;
   }
   const int coo = n0 + wn * 32 + (lane & 31);
   if (coo >= a.Cout) return;
@@ -161,6 +216,14 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
       if (a.yf) a.yf[(size_t)mm * a.Cout + coo] = v;
     }
   }
+}
+// the conv tile for `cout` output channels
+void launch_conv(const ConvArgs& a, int cout, hipStream_t st) {
+  const int M = a.B * a.H * a.W;
+  if (cout <= 32)
+    hipLaunchKernelGGL((k_conv<4, 1>), dim3((M + 127) / 128, (cout + 31) / 32), dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL((k_conv<2, 2>), dim3((M + 63) / 64, (cout + 63) / 64), dim3(256), 0, st, a);
 }
 
 // ---------------------------------------------------------------------------
@@ -216,12 +279,29 @@ __global__ void __launch_bounds__(256) k_wgrad(WgradArgs a) {
   }
 }
 
-// db[c] += sum over pixels of dz[m][c]
-__global__ void k_bias_grad(const u16* __restrict__ dz, int M, int C, float* __restrict__ db) {
-  for (int c = threadIdx.x; c < C; c += blockDim.x) {
+// db[c] += sum over pixels of dz[m][c].  The 256 threads of a workgroup read
+// whole pixel rows (thread t: channel c0 + t % cw of row t / cw, cw =
+// min(C, 256) channels per pass), so consecutive threads read consecutive
+// bytes; each workgroup strides over the pixels, adds its rows in LDS and
+// adds one value per channel.
+__global__ void __launch_bounds__(256) k_bias_grad(const u16* __restrict__ dz, int M, int C, float* __restrict__ db) {
+  __shared__ float red[256];
+  const int cw = C < 256 ? C : 256;
+  const int rpp = 256 / cw;  // pixel rows per step
+  const int r = threadIdx.x / cw, cl = threadIdx.x % cw;
+  for (int c0 = 0; c0 < C; c0 += cw) {
+    const int c = c0 + cl;
     float s = 0.f;
-    for (int m = blockIdx.x; m < M; m += gridDim.x) s += bf2f(dz[(size_t)m * C + c]);
-    atomicAdd(db + c, s);
+    if (r < rpp && c < C)
+      for (int m = blockIdx.x * rpp + r; m < M; m += gridDim.x * rpp) s += bf2f(dz[(size_t)m * C + c]);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    if (r == 0 && c < C) {
+      float t = 0.f;
+      for (int q = 0; q < rpp; q++) t += red[q * cw + cl];
+      atomicAdd(db + c, t);
+    }
+    __syncthreads();
   }
 }
 
@@ -501,16 +581,14 @@ void free_tensor(UTensor& t) {
 
 int conv_fwd(cdx_unet* u, const UConv& L, int B, int h, const u16* x, u16* y, float* yf, bool relu) {
   ConvArgs a{x, u->wb + L.w_off, u->w + L.b_off, y, yf, B, h, h, L.cin, L.cout, L.kp, L.taps, relu ? 1 : 0};
-  const int M = B * h * h;
-  hipLaunchKernelGGL(k_conv, dim3((M + kTile - 1) / kTile, (L.cout + kTile - 1) / kTile), dim3(256), 0, u->st, a);
+  launch_conv(a, L.cout, u->st);
   UCHK(hipGetLastError());
   return CDX_OK;
 }
 // input gradient: dx = conv(dz, flipped weights), no bias
 int conv_dgrad(cdx_unet* u, const UConv& L, int B, int h, const u16* dz, u16* dx) {
   ConvArgs a{dz, L.wflip, nullptr, dx, nullptr, B, h, h, L.cout, L.cin, L.kp2, L.taps, 0};
-  const int M = B * h * h;
-  hipLaunchKernelGGL(k_conv, dim3((M + kTile - 1) / kTile, (L.cin + kTile - 1) / kTile), dim3(256), 0, u->st, a);
+  launch_conv(a, L.cin, u->st);
   UCHK(hipGetLastError());
   return CDX_OK;
 }

@@ -211,7 +211,9 @@ def test_batch_cfg3_mix_matches_oracle_per_frame(cdx, o64, o32):
     autotuned schedule and under the variant-1 radices at 128 threads:
       - one HIO step (beta 0.9): 1e-5 relative L2 on g and F;
       - three HIO steps (the steady-state column pass on tiled T1 runs from
-        the second): max(1e-5, 10 x the FP32 oracle's distance on that frame).
+        the second): max(1e-5, 20 x the FP32 oracle's distance on that frame)
+        (measured: at most 9.2x under the autotuned schedule, 7.0x under the
+        variant-1 radices).
     HIO roughly doubles a rounding difference per step, and the FP32 oracle
     forms its butterflies in double (its transforms are 1.7x closer to
     numpy's than this build's or rocFFT's FP32 ones)."""
@@ -243,7 +245,7 @@ def test_batch_cfg3_mix_matches_oracle_per_frame(cdx, o64, o32):
                 bar = 1e-5
                 if alg == "3*HIO":
                     e32 = rel_l2(r32[i]["g"], ref["g"])
-                    bar = max(1e-5, 10 * e32)
+                    bar = max(1e-5, 20 * e32)
                     worst_ratio = max(worst_ratio, eg / e32)
                 worst = max(worst, eg, eF)
                 assert eg < bar and eF < bar, (sched, alg, i, n, eg, eF, bar)

@@ -33,8 +33,14 @@ def _rand_net(cdx, size, levels, base, B, seed=0):
     return net, ref, flat.astype(np.float64)
 
 
-@pytest.mark.parametrize("size,levels,base,B", [(32, 3, 8, 2), (64, 4, 16, 1)])
+@pytest.mark.parametrize("size,levels,base,B", [(32, 3, 8, 2), (64, 4, 16, 1), (128, 4, 32, 1)])
 def test_forward_and_gradients_match_oracle(cdx, size, levels, base, B):
+    """Forward pass, loss and every layer's weight and bias gradients against
+    the numpy oracle (same BF16 rounding points, float64 arithmetic), up to
+    the shipped geometry (4 levels x 32 base channels, here at 128^2).  The
+    residual differences come from FP32 accumulation order flipping BF16
+    roundings of activations: relative L2 bounds 1e-2 (forward) and 3e-2
+    (gradients); the measured values are printed."""
     print(f"unet parity size={size} levels={levels} base={base} B={B}", flush=True)
     net, ref, flat = _rand_net(cdx, size, levels, base, B)
     rs = np.random.default_rng(5)
@@ -44,6 +50,7 @@ def test_forward_and_gradients_match_oracle(cdx, size, levels, base, B):
     net.set_batch(x, tgt)
     y = net.forward(B)
     yr, cache = ref.forward(x, ref.unpack(flat))
+    print(f"  forward rel-L2 {rel_l2(y, yr[..., :3]):.2e}", flush=True)
     assert rel_l2(y, yr[..., :3]) < 1e-2, rel_l2(y, yr[..., :3])
     net.zero_grads()
     loss = net.backward(B)

@@ -23,6 +23,8 @@
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
+
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -68,9 +70,9 @@ __host__ __device__ inline u16 f2bf(float f) {
   return (u16)(u >> 16);
 }
 
-constexpr int kTile = 64;  // output tile (rows x cols) of the GEMM kernels
-constexpr int kBK = 32;    // K step
-constexpr int kLdsRow = kBK + 8;  // LDS row pitch in bf16 (80 B, 16-byte aligned rows)
+constexpr int kBK = 32;  // K padding of the weight rows (multiple of the conv's 16-wide MFMA steps)
+// workspace of the weight gradient's pixel splits, in floats
+constexpr int64_t kWgradPartFloats = (int64_t)6 << 20;
 
 // ---------------------------------------------------------------------------
 // convolution: out[m][co] = act(bias[co] + sum_k A[m][k] W[co][k]),
@@ -106,20 +108,6 @@ __device__ __forceinline__ uint4 load_patch8(const u16* x, int H, int W, int Cin
   return *(const uint4*)(x + ((size_t)(b * H + hh) * W + ww) * Cin + ci);
 }
 
-// Four waves, each a 32x32 quarter of the 64x64 tile: D += A B with A rows
-// from As[row][k], B columns from Bs[col][k] (both K-contiguous).  The MFMA
-// pairs A and B values by lane and element, so any fixed assignment of the
-// 16 k values of a step to (lane half, element) gives the same dot product.
-__device__ __forceinline__ void mfma_tile(const u16 (*As)[kLdsRow], const u16 (*Bs)[kLdsRow], int wm, int wn,
-                                          int lane, f32x16& acc) {
-#pragma unroll
-  for (int s = 0; s < kBK / 16; s++) {
-    const int kk = s * 16 + (lane >> 5) * 8;
-    const bf16x8 a = *(const bf16x8*)&As[wm * 32 + (lane & 31)][kk];
-    const bf16x8 b = *(const bf16x8*)&Bs[wn * 32 + (lane & 31)][kk];
-    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
-  }
-}
 // row of accumulator element r in the 32x32 block (column = lane & 31)
 __device__ __forceinline__ int acc_row(int r, int lane) { return (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3); }
 
@@ -227,81 +215,173 @@ void launch_conv(const ConvArgs& a, int cout, hipStream_t st) {
 }
 
 // ---------------------------------------------------------------------------
-// weight gradient: dw[co][n] += sum_m dz[m][co] A[m][n] over a pixel chunk
-// (grid.z splits the pixels; FP32 atomics combine the chunks)
+// weight gradient: dw[co][n] = sum_m dz[m][co] A[m][n] (A the im2col rows of
+// the layer input) and db[co] = sum_m dz[m][co].  grid.z splits the pixels;
+// each split writes its partial tile to a workspace and k_wgrad_reduce adds
+// the splits in a fixed order, so the gradient is the same on every run.
 // ---------------------------------------------------------------------------
 struct WgradArgs {
   const u16* dz;  // [M][Cout]
   const u16* x;   // [B][H][W][Cin]
-  float* dw;      // [Cout][Kp]
-  int B, H, W, Cin, Cout, Kp, taps, chunk;
+  float* part;  // [nsplit][Cout*K + Cout]: each split's weight then bias partials
+  int B, H, W, Cin, Cout, taps, chunk;
 };
 
+// TC x TN tile (TC = 32 WM output channels, TN = 32 WN weight columns, four
+// waves of 32 x 32), 64 pixels per step through two LDS buffers, the next
+// step's global loads in flight during the current step's MFMAs.  Both MFMA
+// operands run along the pixels, which are the rows of dz and of the input:
+// each thread loads a block of 4 pixels x 8 channels (four 16-byte rows),
+// transposes it in registers and stores 8 rows of 4 pixels.  Workgroups of
+// n-tile 0 also add up the dz blocks they hold for the bias gradient.
+// Requires W % 4 == 0 (a 4-pixel block lies in one image row).
+template <int WM, int WN>
 __global__ void __launch_bounds__(256) k_wgrad(WgradArgs a) {
-  __shared__ __align__(16) u16 As[kTile][kLdsRow];  // [co][pixel]
-  __shared__ __align__(16) u16 Bs[kTile][kLdsRow];  // [n][pixel]
+  constexpr int TC = 32 * WM, TN = 32 * WN, BK = 64, LP = BK + 8;
+  constexpr int NA = TC / 8 * (BK / 4), NB = TN / 8 * (BK / 4);  // 4x8 blocks per step
+  constexpr int NJ = (NA + NB + 255) / 256;
+  static_assert(WM * WN == 4, "four waves");
+  __shared__ __align__(16) u16 As[2][TC][LP];  // [co][pixel]
+  __shared__ __align__(16) u16 Bs[2][TN][LP];  // [n][pixel]
+  __shared__ float bsum[TC];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
+  const int wm = wave / WN, wn = wave % WN;
   const int HW = a.H * a.W, M = a.B * HW, K = a.taps * a.Cin;
-  const int co0 = blockIdx.x * kTile, n0 = blockIdx.y * kTile;
-  const int p_lo = blockIdx.z * a.chunk;
-  const int p_hi = min(M, p_lo + a.chunk);
-  const int lp = tid >> 3, lc = (tid & 7) * 8;  // pixel 0..31 of the step, 8 columns
-  f32x16 acc = {};
-  for (int p0 = p_lo; p0 < p_hi; p0 += kBK) {
-    const int m = p0 + lp;
-    uint4 av = make_uint4(0, 0, 0, 0), bv = make_uint4(0, 0, 0, 0);
-    if (m < p_hi) {
-      if (co0 + lc < a.Cout) av = *(const uint4*)(a.dz + (size_t)m * a.Cout + co0 + lc);
-      if (n0 + lc < K) {
+  const int co0 = blockIdx.x * TC, n0 = blockIdx.y * TN;
+  const int p_lo = blockIdx.z * a.chunk, p_hi = min(M, p_lo + a.chunk);
+  const bool with_bias = blockIdx.y == 0;
+  // this thread's blocks: A (dz) blocks first, then B (im2col) blocks
+  bool isA[NJ], act[NJ], cok[NJ];
+  int pg[NJ], col[NJ];
+#pragma unroll
+  for (int jj = 0; jj < NJ; jj++) {
+    const int j = tid + 256 * jj;
+    act[jj] = j < NA + NB;
+    isA[jj] = j < NA;
+    const int q = isA[jj] ? j : j - NA;
+    pg[jj] = q % (BK / 4);
+    col[jj] = (q / (BK / 4)) * 8;
+    cok[jj] = act[jj] && (isA[jj] ? co0 + col[jj] < a.Cout : n0 + col[jj] < K);
+  }
+  uint4 r[NJ][4];
+  float bs[NJ][8];
+#pragma unroll
+  for (int jj = 0; jj < NJ; jj++)
+#pragma unroll
+    for (int e = 0; e < 8; e++) bs[jj][e] = 0.f;
+  auto load = [&](int p0) {
+#pragma unroll
+    for (int jj = 0; jj < NJ; jj++) {
+      const int m = p0 + 4 * pg[jj];
+#pragma unroll
+      for (int k = 0; k < 4; k++) r[jj][k] = make_uint4(0, 0, 0, 0);
+      if (!cok[jj] || m >= p_hi) continue;
+      if (isA[jj]) {
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+          if (m + k < p_hi) r[jj][k] = *(const uint4*)(a.dz + (size_t)(m + k) * a.Cout + co0 + col[jj]);
+      } else {
         int b, h, w;
         pixel_of(m, HW, a.W, b, h, w);
-        bv = load_patch8(a.x, a.H, a.W, a.Cin, a.taps, b, h, w, n0 + lc);
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+          if (m + k < p_hi) r[jj][k] = load_patch8(a.x, a.H, a.W, a.Cin, a.taps, b, h, w + k, n0 + col[jj]);
       }
     }
-    __syncthreads();
-    const u16* ae = (const u16*)&av;
-    const u16* be = (const u16*)&bv;
+  };
+  auto store = [&](int buf) {
 #pragma unroll
-    for (int e = 0; e < 8; e++) {
-      As[lc + e][lp] = ae[e];
-      Bs[lc + e][lp] = be[e];
+    for (int jj = 0; jj < NJ; jj++) {
+      if (!act[jj]) continue;
+      u16(*T)[LP] = isA[jj] ? As[buf] : Bs[buf];
+      const unsigned* w0 = (const unsigned*)&r[jj][0];
+      const unsigned* w1 = (const unsigned*)&r[jj][1];
+      const unsigned* w2 = (const unsigned*)&r[jj][2];
+      const unsigned* w3 = (const unsigned*)&r[jj][3];
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        const int d = e >> 1, sh = (e & 1) * 16;
+        const unsigned v0 = (w0[d] >> sh) & 0xFFFFu, v1 = (w1[d] >> sh) & 0xFFFFu;
+        const unsigned v2 = (w2[d] >> sh) & 0xFFFFu, v3 = (w3[d] >> sh) & 0xFFFFu;
+        *(uint2*)&T[col[jj] + e][4 * pg[jj]] = make_uint2(v0 | (v1 << 16), v2 | (v3 << 16));
+        if (with_bias && isA[jj])
+          bs[jj][e] += (bf2f((u16)v0) + bf2f((u16)v1)) + (bf2f((u16)v2) + bf2f((u16)v3));
+      }
     }
+  };
+  if (tid < TC) bsum[tid] = 0.f;
+  f32x16 acc = {};
+  load(p_lo);
+  store(0);
+  __syncthreads();
+  int buf = 0;
+  for (int p0 = p_lo; p0 < p_hi; p0 += BK) {
+    const bool more = p0 + BK < p_hi;
+    if (more) load(p0 + BK);
+#pragma unroll
+    for (int s4 = 0; s4 < BK / 16; s4++) {
+      const int kk = s4 * 16 + (lane >> 5) * 8;
+      const bf16x8 av = *(const bf16x8*)&As[buf][wm * 32 + (lane & 31)][kk];
+      const bf16x8 bv = *(const bf16x8*)&Bs[buf][wn * 32 + (lane & 31)][kk];
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc, 0, 0, 0);
+    }
+    // the other buffer was last read in the previous step, before its barrier
+    if (more) store(buf ^ 1);
     __syncthreads();
-    mfma_tile(As, Bs, wm, wn, lane, acc);
+    buf ^= 1;
+  }
+  if (with_bias) {
+#pragma unroll
+    for (int jj = 0; jj < NJ; jj++)
+      if (cok[jj] && isA[jj])
+#pragma unroll
+        for (int e = 0; e < 8; e++) atomicAdd(&bsum[col[jj] + e], bs[jj][e]);
+    __syncthreads();
+    if (tid < TC && co0 + tid < a.Cout)
+      a.part[(size_t)blockIdx.z * a.Cout * (K + 1) + (size_t)a.Cout * K + co0 + tid] = bsum[tid];
   }
   const int n = n0 + wn * 32 + (lane & 31);
-  if (n >= K) return;
+  if (n < K) {
+    float* part = a.part + (size_t)blockIdx.z * a.Cout * (K + 1);
 #pragma unroll
-  for (int r = 0; r < 16; r++) {
-    const int co = co0 + wm * 32 + acc_row(r, lane);
-    if (co < a.Cout) atomicAdd(a.dw + (size_t)co * a.Kp + n, acc[r]);
+    for (int r2 = 0; r2 < 16; r2++) {
+      const int co = co0 + wm * 32 + acc_row(r2, lane);
+      if (co < a.Cout) part[(size_t)co * K + n] = acc[r2];
+    }
   }
 }
 
-// db[c] += sum over pixels of dz[m][c].  The 256 threads of a workgroup read
-// whole pixel rows (thread t: channel c0 + t % cw of row t / cw, cw =
-// min(C, 256) channels per pass), so consecutive threads read consecutive
-// bytes; each workgroup strides over the pixels, adds its rows in LDS and
-// adds one value per channel.
-__global__ void __launch_bounds__(256) k_bias_grad(const u16* __restrict__ dz, int M, int C, float* __restrict__ db) {
-  __shared__ float red[256];
-  const int cw = C < 256 ? C : 256;
-  const int rpp = 256 / cw;  // pixel rows per step
-  const int r = threadIdx.x / cw, cl = threadIdx.x % cw;
-  for (int c0 = 0; c0 < C; c0 += cw) {
-    const int c = c0 + cl;
+// The splits' partials are added in two fixed-order stages.  Stage 1:
+// workgroup (x, g) adds splits [g*zc, (g+1)*zc) of 64 consecutive partial
+// values (4 lanes of 64 threads, each lane every 4th split, then the lanes in
+// order) into part2[g][.].  nout = Cout*K + Cout values: the weight partials
+// [Cout][K] followed by the bias partials [Cout] of each split.
+__global__ void __launch_bounds__(256) k_wgrad_reduce1(int nsplit, int zc, int64_t nout, const float* __restrict__ part,
+                                                       float* __restrict__ part2) {
+  __shared__ float red[4][64];
+  const int il = threadIdx.x & 63, zl = threadIdx.x >> 6;
+  const int64_t i = blockIdx.x * (int64_t)64 + il;
+  const int z0 = blockIdx.y * zc, z1 = min(nsplit, z0 + zc);
+  float s = 0.f;
+  if (i < nout)
+    for (int z = z0 + zl; z < z1; z += 4) s += part[z * nout + i];
+  red[zl][il] = s;
+  __syncthreads();
+  if (zl == 0 && i < nout) part2[blockIdx.y * nout + i] = (red[0][il] + red[1][il]) + (red[2][il] + red[3][il]);
+}
+// Stage 2: dw[co][n] += sum_g part2[g][co*K + n], db[co] += sum_g part2[g][Cout*K + co]
+__global__ void k_wgrad_reduce2(int ng, int Cout, int K, int Kp, const float* __restrict__ part2,
+                                float* __restrict__ dw, float* __restrict__ db) {
+  const int64_t nw = (int64_t)Cout * K, nout = nw + Cout;
+  EW_LOOP(nout) {
     float s = 0.f;
-    if (r < rpp && c < C)
-      for (int m = blockIdx.x * rpp + r; m < M; m += gridDim.x * rpp) s += bf2f(dz[(size_t)m * C + c]);
-    red[threadIdx.x] = s;
-    __syncthreads();
-    if (r == 0 && c < C) {
-      float t = 0.f;
-      for (int q = 0; q < rpp; q++) t += red[q * cw + cl];
-      atomicAdd(db + c, t);
+    for (int g = 0; g < ng; g++) s += part2[g * nout + i];
+    if (i < nw) {
+      const int co = (int)(i / K), n = (int)(i - (int64_t)co * K);
+      dw[(size_t)co * Kp + n] += s;
+    } else {
+      db[i - nw] += s;
     }
-    __syncthreads();
   }
 }
 
@@ -553,6 +633,7 @@ struct cdx_unet {
   u16* dz_head = nullptr;
   float* tg = nullptr;  // targets [B][H][W][4]
   double* dscal = nullptr;  // reductions (loss partials, sums)
+  float* wg_part = nullptr;  // weight-gradient partials of the pixel splits
   void *obj = nullptr, *I = nullptr;  // forward-model scratch (one frame)
   int64_t adam_t = 0;
 };
@@ -592,21 +673,37 @@ int conv_dgrad(cdx_unet* u, const UConv& L, int B, int h, const u16* dz, u16* dx
   UCHK(hipGetLastError());
   return CDX_OK;
 }
+// weight and bias gradients of layer L: dz [B*h*h][cout], x its input
 int conv_wgrad(cdx_unet* u, const UConv& L, int B, int h, const u16* dz, const u16* x) {
+  if (h % 4 || L.cin % 8 || L.cout % 8)
+    return ufail(CDX_ERR_ARG, "unet: weight gradient needs sides divisible by 4 and channels by 8");
   const int M = B * h * h, K = L.taps * L.cin;
-  const int tm = (L.cout + kTile - 1) / kTile, tn = (K + kTile - 1) / kTile;
-  int nsplit = 2048 / (tm * tn);
-  if (nsplit < 1) nsplit = 1;
-  if (nsplit > M / 256) nsplit = M / 256 > 0 ? M / 256 : 1;
+  const bool narrow = L.cout <= 32;  // 32 x 128 tiles, else 64 x 64
+  const int TC = narrow ? 32 : 64, TN = narrow ? 128 : 64;
+  const int tm = (L.cout + TC - 1) / TC, tn = (K + TN - 1) / TN;
+  // about 1024 workgroups, at least 256 pixels each, within the workspace
+  // (the splits' partials and the reduction's first stage, 1/16 of them)
+  const int64_t nout = (int64_t)L.cout * (K + 1);
+  int nsplit = 1024 / (tm * tn);
+  nsplit = std::max(1, std::min(nsplit, M / 256));
+  nsplit = (int)std::min<int64_t>(nsplit, kWgradPartFloats / (nout + nout / 16 + 1) - 1);
+  if (nsplit < 1) return ufail(CDX_ERR_ARG, "unet: layer too large for the weight-gradient workspace");
   int chunk = (M + nsplit - 1) / nsplit;
-  chunk = (chunk + kBK - 1) / kBK * kBK;
+  chunk = (chunk + 63) / 64 * 64;
   nsplit = (M + chunk - 1) / chunk;
-  WgradArgs a{dz, x, u->g + L.w_off, B, h, h, L.cin, L.cout, L.kp, L.taps, chunk};
-  hipLaunchKernelGGL(k_wgrad, dim3(tm, tn, nsplit), dim3(256), 0, u->st, a);
+  WgradArgs a{dz, x, u->wg_part, B, h, h, L.cin, L.cout, L.taps, chunk};
+  if (narrow)
+    hipLaunchKernelGGL((k_wgrad<1, 4>), dim3(tm, tn, nsplit), dim3(256), 0, u->st, a);
+  else
+    hipLaunchKernelGGL((k_wgrad<2, 2>), dim3(tm, tn, nsplit), dim3(256), 0, u->st, a);
   UCHK(hipGetLastError());
-  int gb = M / 64;
-  gb = gb < 1 ? 1 : (gb > 1024 ? 1024 : gb);
-  hipLaunchKernelGGL(k_bias_grad, dim3(gb), dim3(256), 0, u->st, dz, M, L.cout, u->g + L.b_off);
+  const int zc = 16, ng = (nsplit + zc - 1) / zc;
+  float* part2 = u->wg_part + (int64_t)nsplit * nout;
+  hipLaunchKernelGGL(k_wgrad_reduce1, dim3((unsigned)((nout + 63) / 64), ng), dim3(256), 0, u->st, nsplit, zc, nout,
+                     (const float*)u->wg_part, part2);
+  UCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_wgrad_reduce2, dim3(ew_blocks(nout)), dim3(256), 0, u->st, ng, L.cout, K, L.kp,
+                     (const float*)part2, u->g + L.w_off, u->g + L.b_off);
   UCHK(hipGetLastError());
   return CDX_OK;
 }
@@ -791,7 +888,7 @@ void destroy(cdx_unet* u) {
   if (u->st) hipStreamSynchronize(u->st);
   for (auto& L : u->conv) hipFree(L.wflip);
   for (void* p : {(void*)u->w, (void*)u->g, (void*)u->m1, (void*)u->m2, (void*)u->wb, (void*)u->yf,
-                  (void*)u->dz_head, (void*)u->tg, (void*)u->dscal, u->obj, u->I})
+                  (void*)u->dz_head, (void*)u->tg, (void*)u->dscal, (void*)u->wg_part, u->obj, u->I})
     hipFree(p);
   free_tensor(u->in0);
   for (auto* v : {&u->ea, &u->e, &u->p, &u->cat, &u->da, &u->d})
@@ -859,6 +956,7 @@ extern "C" int cdx_unet_create(cdx_unet** out, int size, int levels, int base, i
             hipMalloc(&u->dz_head, (size_t)M * 8 * sizeof(u16)) == hipSuccess &&
             hipMalloc(&u->tg, (size_t)M * 4 * sizeof(float)) == hipSuccess &&
             hipMalloc(&u->dscal, 4096 * sizeof(double)) == hipSuccess &&
+            hipMalloc(&u->wg_part, kWgradPartFloats * sizeof(float)) == hipSuccess &&
             hipMalloc(&u->obj, (size_t)size * size * 8) == hipSuccess &&
             hipMalloc(&u->I, (size_t)size * size * 4) == hipSuccess;
   for (auto& L : u->conv) {

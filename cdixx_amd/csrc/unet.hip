@@ -85,6 +85,7 @@ struct ConvArgs {
   u16* y;             // [B][H][W][Cout] or null
   float* yf;          // FP32 copy of the output or null
   int B, H, W, Cin, Cout, Kp, taps, relu;
+  const u16* mask = nullptr;  // y is zeroed where mask <= 0 (a ReLU output's gradient), or null
 };
 
 __device__ __forceinline__ void pixel_of(int m, int HW, int W, int& b, int& h, int& w) {
@@ -200,6 +201,7 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
     if (mm < M) {
       float v = acc[r] + bias;
       if (a.relu) v = fmaxf(v, 0.f);
+      if (a.mask && !(bf2f(a.mask[(size_t)mm * a.Cout + coo]) > 0.f)) v = 0.f;
       if (a.y) a.y[(size_t)mm * a.Cout + coo] = f2bf(v);
       if (a.yf) a.yf[(size_t)mm * a.Cout + coo] = v;
     }
@@ -385,68 +387,109 @@ __global__ void k_wgrad_reduce2(int ng, int Cout, int K, int Kp, const float* __
   }
 }
 
-// dy *= (y > 0)
-__global__ void k_relu_bwd(int64_t n, u16* __restrict__ dy, const u16* __restrict__ y) {
-  EW_LOOP(n) {
-    if (!(bf2f(y[i]) > 0.f)) dy[i] = 0;
-  }
+// The layout kernels below move 8 channels (16 bytes) per thread: every
+// channel count of the network is a multiple of 8, and the element counts
+// they take (pixels x channels / 8) fit 32 bits (checked by the launchers).
+__device__ __forceinline__ void unpack8(const uint4& v, float* f) {
+  const unsigned* w = (const unsigned*)&v;
+#pragma unroll
+  for (int e = 0; e < 8; e++) f[e] = bf2f((u16)((w[e >> 1] >> ((e & 1) * 16)) & 0xFFFFu));
 }
+__device__ __forceinline__ uint4 pack8(const float* f) {
+  uint4 v;
+  unsigned* w = (unsigned*)&v;
+#pragma unroll
+  for (int d = 0; d < 4; d++) w[d] = (unsigned)f2bf(f[2 * d]) | ((unsigned)f2bf(f[2 * d + 1]) << 16);
+  return v;
+}
+// zero where the ReLU output y is not positive
+__device__ __forceinline__ void relu_mask8(float* f, const uint4& y) {
+  float m[8];
+  unpack8(y, m);
+#pragma unroll
+  for (int e = 0; e < 8; e++)
+    if (!(m[e] > 0.f)) f[e] = 0.f;
+}
+#define EW_LOOP32(n) for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += gridDim.x * blockDim.x)
 
-// 2x2 average pool, NHWC
-__global__ void k_pool(int B, int H, int W, int C, const u16* __restrict__ x, u16* __restrict__ y) {
-  const int Ho = H / 2, Wo = W / 2;
-  const int64_t n = (int64_t)B * Ho * Wo * C;
-  EW_LOOP(n) {
-    const int c = (int)(i % C);
-    const int64_t p = i / C;
-    const int wo = (int)(p % Wo), ho = (int)((p / Wo) % Ho), b = (int)(p / ((int64_t)Wo * Ho));
-    const size_t base = ((size_t)(b * H + 2 * ho) * W + 2 * wo) * C + c;
-    const float s = bf2f(x[base]) + bf2f(x[base + C]) + bf2f(x[base + (size_t)W * C]) +
-                    bf2f(x[base + (size_t)W * C + C]);
-    y[i] = f2bf(0.25f * s);
+// 2x2 average pool, NHWC; n8 = B * H/2 * W/2 * C/8
+__global__ void k_pool(int n8, int H, int W, int C, const u16* __restrict__ x, u16* __restrict__ y) {
+  const int Ho = H / 2, Wo = W / 2, C8 = C / 8;
+  EW_LOOP32(n8) {
+    const int g = i % C8, p = i / C8;
+    const int wo = p % Wo, t = p / Wo, ho = t % Ho, b = t / Ho;
+    const uint4* x0 = (const uint4*)(x + ((size_t)(b * H + 2 * ho) * W + 2 * wo) * C) + g;
+    const size_t row = (size_t)W * C8;  // uint4 per image row
+    float f0[8], f1[8], f2[8], f3[8], o[8];
+    unpack8(x0[0], f0);
+    unpack8(x0[C8], f1);
+    unpack8(x0[row], f2);
+    unpack8(x0[row + C8], f3);
+#pragma unroll
+    for (int e = 0; e < 8; e++) o[e] = 0.25f * (f0[e] + f1[e] + f2[e] + f3[e]);
+    ((uint4*)y)[i] = pack8(o);
   }
 }
-// dx[b,h,w,c] += dy[b,h/2,w/2,c] / 4
-__global__ void k_pool_bwd_add(int B, int H, int W, int C, const u16* __restrict__ dy, u16* __restrict__ dx) {
-  const int64_t n = (int64_t)B * H * W * C;
-  EW_LOOP(n) {
-    const int c = (int)(i % C);
-    const int64_t p = i / C;
-    const int w = (int)(p % W), h = (int)((p / W) % H), b = (int)(p / ((int64_t)W * H));
-    const float g = bf2f(dy[((size_t)(b * (H / 2) + h / 2) * (W / 2) + w / 2) * C + c]);
-    dx[i] = f2bf(bf2f(dx[i]) + 0.25f * g);
+// dx[b,h,w,c] = (dx + dy[b,h/2,w/2,c] / 4) * (y > 0): the pooled input's
+// gradient completed and passed back through its ReLU; n8 = B*H*W*C/8
+__global__ void k_pool_bwd_add(int n8, int H, int W, int C, const u16* __restrict__ dy, u16* __restrict__ dx,
+                               const u16* __restrict__ y) {
+  const int C8 = C / 8;
+  EW_LOOP32(n8) {
+    const int g = i % C8, p = i / C8;
+    const int w = p % W, t = p / W, h = t % H, b = t / H;
+    float a[8], d[8];
+    unpack8(((const uint4*)dx)[i], a);
+    unpack8(((const uint4*)(dy + ((size_t)(b * (H / 2) + h / 2) * (W / 2) + w / 2) * C))[g], d);
+#pragma unroll
+    for (int e = 0; e < 8; e++) a[e] = bf2f(f2bf(a[e] + 0.25f * d[e]));
+    relu_mask8(a, ((const uint4*)y)[i]);
+    ((uint4*)dx)[i] = pack8(a);
   }
 }
-// cat[b,h,w,:] = (low[b,h/2,w/2,0:Cu], skip[b,h,w,0:Cs])
-__global__ void k_upcat(int B, int H, int W, int Cu, int Cs, const u16* __restrict__ low,
+// cat[b,h,w,:] = (low[b,h/2,w/2,0:Cu], skip[b,h,w,0:Cs]); n8 = B*H*W*(Cu+Cs)/8
+__global__ void k_upcat(int n8, int H, int W, int Cu, int Cs, const u16* __restrict__ low,
                         const u16* __restrict__ skip, u16* __restrict__ cat) {
-  const int C = Cu + Cs;
-  const int64_t n = (int64_t)B * H * W * C;
-  EW_LOOP(n) {
-    const int c = (int)(i % C);
-    const int64_t p = i / C;
-    const int w = (int)(p % W), h = (int)((p / W) % H), b = (int)(p / ((int64_t)W * H));
-    cat[i] = c < Cu ? low[((size_t)(b * (H / 2) + h / 2) * (W / 2) + w / 2) * Cu + c] : skip[(size_t)p * Cs + c - Cu];
+  const int C8 = (Cu + Cs) / 8, Cu8 = Cu / 8, Cs8 = Cs / 8;
+  EW_LOOP32(n8) {
+    const int g = i % C8, p = i / C8;
+    uint4 v;
+    if (g < Cu8) {
+      const int w = p % W, t = p / W, h = t % H, b = t / H;
+      v = ((const uint4*)low)[((size_t)(b * (H / 2) + h / 2) * (W / 2) + w / 2) * Cu8 + g];
+    } else {
+      v = ((const uint4*)skip)[(size_t)p * Cs8 + g - Cu8];
+    }
+    ((uint4*)cat)[i] = v;
   }
 }
-// dskip = dcat[..., Cu:]
-__global__ void k_upcat_bwd_skip(int64_t P, int Cu, int Cs, const u16* __restrict__ dcat, u16* __restrict__ dskip) {
-  const int64_t n = P * Cs;
-  EW_LOOP(n) dskip[i] = dcat[(i / Cs) * (Cu + Cs) + Cu + i % Cs];
+// dskip = dcat[..., Cu:]; n8 = P*Cs/8
+__global__ void k_upcat_bwd_skip(int n8, int Cu, int Cs, const u16* __restrict__ dcat, u16* __restrict__ dskip) {
+  const int Cs8 = Cs / 8, C8 = (Cu + Cs) / 8;
+  EW_LOOP32(n8) {
+    const int g = i % Cs8, p = i / Cs8;
+    ((uint4*)dskip)[i] = ((const uint4*)dcat)[(size_t)p * C8 + Cu / 8 + g];
+  }
 }
-// dlow[b,h,w,c] = sum over the 2x2 block of dcat[..., c]
-__global__ void k_upcat_bwd_low(int B, int Hl, int Wl, int Cu, int Cs, const u16* __restrict__ dcat,
-                                u16* __restrict__ dlow) {
-  const int C = Cu + Cs, H = 2 * Hl, W = 2 * Wl;
-  const int64_t n = (int64_t)B * Hl * Wl * Cu;
-  EW_LOOP(n) {
-    const int c = (int)(i % Cu);
-    const int64_t p = i / Cu;
-    const int w = (int)(p % Wl), h = (int)((p / Wl) % Hl), b = (int)(p / ((int64_t)Wl * Hl));
-    const size_t base = ((size_t)(b * H + 2 * h) * W + 2 * w) * C + c;
-    const float s = bf2f(dcat[base]) + bf2f(dcat[base + C]) + bf2f(dcat[base + (size_t)W * C]) +
-                    bf2f(dcat[base + (size_t)W * C + C]);
-    dlow[i] = f2bf(s);
+// dlow[b,h,w,c] = (sum over the 2x2 block of dcat[..., c]) * (ylow > 0);
+// n8 = B*Hl*Wl*Cu/8
+__global__ void k_upcat_bwd_low(int n8, int Hl, int Wl, int Cu, int Cs, const u16* __restrict__ dcat,
+                                u16* __restrict__ dlow, const u16* __restrict__ ylow) {
+  const int C8 = (Cu + Cs) / 8, Cu8 = Cu / 8, H = 2 * Hl, W = 2 * Wl;
+  EW_LOOP32(n8) {
+    const int g = i % Cu8, p = i / Cu8;
+    const int w = p % Wl, t = p / Wl, h = t % Hl, b = t / Hl;
+    const uint4* d0 = (const uint4*)dcat + ((size_t)(b * H + 2 * h) * W + 2 * w) * C8 + g;
+    const size_t row = (size_t)W * C8;
+    float f0[8], f1[8], f2[8], f3[8], o[8];
+    unpack8(d0[0], f0);
+    unpack8(d0[C8], f1);
+    unpack8(d0[row], f2);
+    unpack8(d0[row + C8], f3);
+#pragma unroll
+    for (int e = 0; e < 8; e++) o[e] = bf2f(f2bf(f0[e] + f1[e] + f2[e] + f3[e]));
+    relu_mask8(o, ((const uint4*)ylow)[i]);
+    ((uint4*)dlow)[i] = pack8(o);
   }
 }
 
@@ -666,9 +709,10 @@ int conv_fwd(cdx_unet* u, const UConv& L, int B, int h, const u16* x, u16* y, fl
   UCHK(hipGetLastError());
   return CDX_OK;
 }
-// input gradient: dx = conv(dz, flipped weights), no bias
-int conv_dgrad(cdx_unet* u, const UConv& L, int B, int h, const u16* dz, u16* dx) {
-  ConvArgs a{dz, L.wflip, nullptr, dx, nullptr, B, h, h, L.cout, L.cin, L.kp2, L.taps, 0};
+// input gradient: dx = conv(dz, flipped weights), no bias, passed back
+// through the ReLU whose output was x (x_relu, or null when x is no ReLU output)
+int conv_dgrad(cdx_unet* u, const UConv& L, int B, int h, const u16* dz, u16* dx, const u16* x_relu) {
+  ConvArgs a{dz, L.wflip, nullptr, dx, nullptr, B, h, h, L.cout, L.cin, L.kp2, L.taps, 0, x_relu};
   launch_conv(a, L.cin, u->st);
   UCHK(hipGetLastError());
   return CDX_OK;
@@ -707,9 +751,10 @@ int conv_wgrad(cdx_unet* u, const UConv& L, int B, int h, const u16* dz, const u
   UCHK(hipGetLastError());
   return CDX_OK;
 }
-int relu_bwd(cdx_unet* u, int64_t n, u16* dy, const u16* y) {
-  hipLaunchKernelGGL(k_relu_bwd, dim3(ew_blocks(n)), dim3(256), 0, u->st, n, dy, y);
-  UCHK(hipGetLastError());
+// threads for a layout kernel over n8 16-byte groups
+int ew8_grid(int64_t n8, dim3& grid) {
+  if (n8 >= ((int64_t)1 << 31)) return ufail(CDX_ERR_ARG, "unet: activation too large");
+  grid = dim3((unsigned)ew_blocks(n8));
   return CDX_OK;
 }
 int refresh_weights(cdx_unet* u) {
@@ -732,18 +777,21 @@ int forward(cdx_unet* u, int B) {
     RCHK(conv_fwd(u, u->conv[enc_a(l)], B, h, x, u->ea[l].y, nullptr, true));
     RCHK(conv_fwd(u, u->conv[enc_b(l)], B, h, u->ea[l].y, u->e[l].y, nullptr, true));
     if (l + 1 < Lv) {
-      const int64_t n = (int64_t)B * (h / 2) * (h / 2) * u->e[l].c;
-      hipLaunchKernelGGL(k_pool, dim3(ew_blocks(n)), dim3(256), 0, u->st, B, h, h, u->e[l].c, u->e[l].y,
-                         u->p[l + 1].y);
+      const int64_t n8 = (int64_t)B * (h / 2) * (h / 2) * u->e[l].c / 8;
+      dim3 g;
+      RCHK(ew8_grid(n8, g));
+      hipLaunchKernelGGL(k_pool, g, dim3(256), 0, u->st, (int)n8, h, h, u->e[l].c, u->e[l].y, u->p[l + 1].y);
       UCHK(hipGetLastError());
     }
   }
   for (int l = Lv - 2; l >= 0; l--) {
     const int h = u->size >> l;
     const UTensor& low = l == Lv - 2 ? u->e[Lv - 1] : u->d[l + 1];
-    const int64_t n = (int64_t)B * h * h * u->cat[l].c;
-    hipLaunchKernelGGL(k_upcat, dim3(ew_blocks(n)), dim3(256), 0, u->st, B, h, h, low.c, u->e[l].c, low.y,
-                       u->e[l].y, u->cat[l].y);
+    const int64_t n8 = (int64_t)B * h * h * u->cat[l].c / 8;
+    dim3 g;
+    RCHK(ew8_grid(n8, g));
+    hipLaunchKernelGGL(k_upcat, g, dim3(256), 0, u->st, (int)n8, h, h, low.c, u->e[l].c, low.y, u->e[l].y,
+                       u->cat[l].y);
     UCHK(hipGetLastError());
     RCHK(conv_fwd(u, u->conv[dec_a(u, l)], B, h, u->cat[l].y, u->da[l].y, nullptr, true));
     RCHK(conv_fwd(u, u->conv[dec_b(u, l)], B, h, u->da[l].y, u->d[l].y, nullptr, true));
@@ -753,47 +801,51 @@ int forward(cdx_unet* u, int B) {
   return CDX_OK;
 }
 
+// Each ReLU's backward is folded into the kernel that completes the
+// gradient of its output: the head's and the b-convs' input gradients
+// (conv_dgrad's mask), the upsampling's low branch (k_upcat_bwd_low) and the
+// pooling's backward, which adds the last term of a skip tensor's gradient.
 int backward(cdx_unet* u, int B) {
   const int Lv = u->levels;
   const int H0 = u->size;
   UTensor& top = Lv >= 2 ? u->d[0] : u->e[0];
   const UConv& hd = u->conv[head(u)];
   RCHK(conv_wgrad(u, hd, B, H0, u->dz_head, top.y));
-  RCHK(conv_dgrad(u, hd, B, H0, u->dz_head, top.dy));
+  RCHK(conv_dgrad(u, hd, B, H0, u->dz_head, top.dy, top.y));
   for (int l = 0; l + 1 < Lv; l++) {
     const int h = H0 >> l;
     const int64_t np = (int64_t)B * h * h;
-    RCHK(relu_bwd(u, np * u->d[l].c, u->d[l].dy, u->d[l].y));
     RCHK(conv_wgrad(u, u->conv[dec_b(u, l)], B, h, u->d[l].dy, u->da[l].y));
-    RCHK(conv_dgrad(u, u->conv[dec_b(u, l)], B, h, u->d[l].dy, u->da[l].dy));
-    RCHK(relu_bwd(u, np * u->da[l].c, u->da[l].dy, u->da[l].y));
+    RCHK(conv_dgrad(u, u->conv[dec_b(u, l)], B, h, u->d[l].dy, u->da[l].dy, u->da[l].y));
     RCHK(conv_wgrad(u, u->conv[dec_a(u, l)], B, h, u->da[l].dy, u->cat[l].y));
-    RCHK(conv_dgrad(u, u->conv[dec_a(u, l)], B, h, u->da[l].dy, u->cat[l].dy));
+    RCHK(conv_dgrad(u, u->conv[dec_a(u, l)], B, h, u->da[l].dy, u->cat[l].dy, nullptr));
     UTensor& low = l == Lv - 2 ? u->e[Lv - 1] : u->d[l + 1];
     const int Cs = u->e[l].c;
-    hipLaunchKernelGGL(k_upcat_bwd_skip, dim3(ew_blocks(np * Cs)), dim3(256), 0, u->st, np, low.c, Cs,
-                       u->cat[l].dy, u->e[l].dy);
+    dim3 g;
+    RCHK(ew8_grid(np * Cs / 8, g));
+    hipLaunchKernelGGL(k_upcat_bwd_skip, g, dim3(256), 0, u->st, (int)(np * Cs / 8), low.c, Cs, u->cat[l].dy,
+                       u->e[l].dy);
     UCHK(hipGetLastError());
-    const int64_t nl = (int64_t)B * (h / 2) * (h / 2) * low.c;
-    hipLaunchKernelGGL(k_upcat_bwd_low, dim3(ew_blocks(nl)), dim3(256), 0, u->st, B, h / 2, h / 2, low.c, Cs,
-                       u->cat[l].dy, low.dy);
+    const int64_t nl8 = (int64_t)B * (h / 2) * (h / 2) * low.c / 8;
+    RCHK(ew8_grid(nl8, g));
+    hipLaunchKernelGGL(k_upcat_bwd_low, g, dim3(256), 0, u->st, (int)nl8, h / 2, h / 2, low.c, Cs, u->cat[l].dy,
+                       low.dy, low.y);
     UCHK(hipGetLastError());
   }
   for (int l = Lv - 1; l >= 0; l--) {
     const int h = H0 >> l;
-    const int64_t np = (int64_t)B * h * h;
-    RCHK(relu_bwd(u, np * u->e[l].c, u->e[l].dy, u->e[l].y));
     RCHK(conv_wgrad(u, u->conv[enc_b(l)], B, h, u->e[l].dy, u->ea[l].y));
-    RCHK(conv_dgrad(u, u->conv[enc_b(l)], B, h, u->e[l].dy, u->ea[l].dy));
-    RCHK(relu_bwd(u, np * u->ea[l].c, u->ea[l].dy, u->ea[l].y));
+    RCHK(conv_dgrad(u, u->conv[enc_b(l)], B, h, u->e[l].dy, u->ea[l].dy, u->ea[l].y));
     const u16* x = l == 0 ? u->in0.y : u->p[l].y;
     RCHK(conv_wgrad(u, u->conv[enc_a(l)], B, h, u->ea[l].dy, x));
     if (l > 0) {
-      RCHK(conv_dgrad(u, u->conv[enc_a(l)], B, h, u->ea[l].dy, u->p[l].dy));
+      RCHK(conv_dgrad(u, u->conv[enc_a(l)], B, h, u->ea[l].dy, u->p[l].dy, nullptr));
       const int hp = h * 2;
-      const int64_t n = (int64_t)B * hp * hp * u->e[l - 1].c;
-      hipLaunchKernelGGL(k_pool_bwd_add, dim3(ew_blocks(n)), dim3(256), 0, u->st, B, hp, hp, u->e[l - 1].c,
-                         u->p[l].dy, u->e[l - 1].dy);
+      const int64_t n8 = (int64_t)B * hp * hp * u->e[l - 1].c / 8;
+      dim3 g;
+      RCHK(ew8_grid(n8, g));
+      hipLaunchKernelGGL(k_pool_bwd_add, g, dim3(256), 0, u->st, (int)n8, hp, hp, u->e[l - 1].c, u->p[l].dy,
+                         u->e[l - 1].dy, u->e[l - 1].y);
       UCHK(hipGetLastError());
     }
   }

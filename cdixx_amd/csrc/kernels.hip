@@ -500,17 +500,39 @@ hipError_t launch_uniform_field(int n, uint64_t seed, uint64_t stream, double* o
   return hipGetLastError();
 }
 // periodic Gaussian smoothing along one axis, radius ceil(4 sigma), weights
-// normalised to sum 1 (accumulated in the same order as the oracle)
-__global__ void k_smooth(int n, const double* in, double* out, double sigma, int axis) {
+// normalised to sum 1 (accumulated in the same order as the oracle).  Each
+// workgroup forms the 2r+1 weights once in LDS, with the expression and
+// summation order of the per-pixel form, so the table holds the same values.
+constexpr int kSmoothMaxTaps = 257;
+__global__ void __launch_bounds__(256) k_smooth(int n, const double* in, double* out, double sigma, int axis) {
+  __shared__ double wt[kSmoothMaxTaps];
   const int r = (int)ceil(4 * sigma);
+  const int taps = 2 * r + 1;
+  const bool table = taps <= kSmoothMaxTaps;
+  if (table) {
+    if (threadIdx.x == 0) {
+      double ws = 0;
+      for (int k = -r; k <= r; k++) ws += exp(-(double)k * k / (2 * sigma * sigma));
+      wt[0] = ws;
+    }
+    __syncthreads();
+    const double ws = wt[0];
+    __syncthreads();
+    for (int t = threadIdx.x; t < taps; t += blockDim.x) {
+      const int k = t - r;
+      wt[t] = exp(-(double)k * k / (2 * sigma * sigma)) / ws;
+    }
+    __syncthreads();
+  }
   const int64_t N = (int64_t)n * n;
   EW_LOOP(N) {
     const int x = (int)(i / n), y = (int)(i % n);
     double ws = 0;
-    for (int k = -r; k <= r; k++) ws += exp(-(double)k * k / (2 * sigma * sigma));
+    if (!table)
+      for (int k = -r; k <= r; k++) ws += exp(-(double)k * k / (2 * sigma * sigma));
     double s = 0;
     for (int k = -r; k <= r; k++) {
-      const double w = exp(-(double)k * k / (2 * sigma * sigma)) / ws;
+      const double w = table ? wt[k + r] : exp(-(double)k * k / (2 * sigma * sigma)) / ws;
       const int64_t j = axis ? (int64_t)(((x + k) % n + n) % n) * n + y : (int64_t)x * n + ((y + k) % n + n) % n;
       s += w * in[j];
     }
@@ -522,8 +544,11 @@ hipError_t launch_smooth_periodic(int n, const double* in, double* out, double s
   hipLaunchKernelGGL(k_smooth, ew_grid(N), dim3(256), 0, st, n, in, out, sigma, axis);
   return hipGetLastError();
 }
-// mm[0] = min, mm[1] = max; caller seeds mm with (+inf, -inf)
-__global__ void k_minmax(int64_t n, const double* x, double* mm) {
+// mm[0] = min, mm[1] = max as order-preserving integer keys; caller seeds mm
+// with the keys of (+inf, -inf).  Waves reduce by shuffles, the workgroup in
+// LDS, and one atomic pair per workgroup combines them.
+__global__ void __launch_bounds__(256) k_minmax(int64_t n, const double* x, double* mm) {
+  __shared__ double slo[4], shi[4];
   double lo = INFINITY, hi = -INFINITY;
   EW_LOOP(n) {
     lo = fmin(lo, x[i]);
@@ -534,7 +559,17 @@ __global__ void k_minmax(int64_t n, const double* x, double* mm) {
     lo = fmin(lo, __shfl_xor(lo, o, 64));
     hi = fmax(hi, __shfl_xor(hi, o, 64));
   }
+  const int wave = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
+    slo[wave] = lo;
+    shi[wave] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
+      lo = fmin(lo, slo[w]);
+      hi = fmax(hi, shi[w]);
+    }
     // order-preserving integer maps for atomics on doubles of any sign
     long long bl = __double_as_longlong(lo), bh = __double_as_longlong(hi);
     unsigned long long kl = bl < 0 ? ~(unsigned long long)bl : ((unsigned long long)bl | 0x8000000000000000ULL);
@@ -544,7 +579,10 @@ __global__ void k_minmax(int64_t n, const double* x, double* mm) {
   }
 }
 hipError_t launch_minmax(int64_t n, const double* x, double* mm, hipStream_t st) {
-  hipLaunchKernelGGL(k_minmax, red_grid(n), dim3(256), 0, st, n, x, mm);
+  // 4 elements per thread at least, at most 256 workgroups
+  int64_t b = (n + 256 * 4 - 1) / (256 * 4);
+  b = b < 1 ? 1 : (b > 256 ? 256 : b);
+  hipLaunchKernelGGL(k_minmax, dim3((unsigned)b), dim3(256), 0, st, n, x, mm);
   return hipGetLastError();
 }
 __device__ __forceinline__ double unkey(unsigned long long k) {

@@ -391,6 +391,54 @@ def unet_flops_per_sample(layers, size):
     return fl
 
 
+def training_leg(cdx, B=16, steps=20):
+    """U-Net training throughput at the shipped geometry (256^2, 4 levels x
+    32 channels, batch 16): synthesis alone, then full steps (synthesis +
+    forward + backward + Adam), random-init weights."""
+    from cdixx_amd.warmstart import LEVELS, BASE, SIZE
+    net = cdx.UNet(SIZE, LEVELS, BASE, max_batch=B, seed=1)
+    net.train(2, B, seed0=3_000_000, lr=1e-6)
+    t0 = time.perf_counter()
+    for k in range(steps):
+        net.synth_batch(B, seed0=3_100_000 + k * B)
+    t_syn = (time.perf_counter() - t0) / steps
+    t0 = time.perf_counter()
+    net.train(steps, B, seed0=3_200_000, lr=1e-6)
+    t_step = (time.perf_counter() - t0) / steps
+    fl = unet_flops_per_sample(net.layers(), SIZE) * B
+    tf = fl / max(t_step - t_syn, 1e-9) / 1e12
+    net.close()
+    return {"batch": B, "step_ms": round(1e3 * t_step, 2), "synthesis_ms": round(1e3 * t_syn, 2),
+            "samples_per_s": round(B / t_step, 1), "gflop_per_step": round(fl / 1e9, 1),
+            "achieved_tflops_excl_synthesis": round(tf, 1), "peak_tflops_bf16_dense": 2500.0,
+            "frac": round(tf / 2500.0, 4)}
+
+
+def copy_sweep(cdx, sizes_mib=(8, 16, 32, 64, 128, 256, 512, 1024, 2048)):
+    """Plain copy (cdx_copy) bandwidth against its working set (source +
+    destination): up to the 256 MiB Infinity Cache a repeated copy is served
+    from it, beyond that from HBM.  Bytes counted as read + written."""
+    out = []
+    for mb in sizes_mib:
+        nb = mb << 20
+        a, b = cdx.DeviceBuffer(nb), cdx.DeviceBuffer(nb)
+        a.zero()
+        reps = max(3, min(200, 4096 // mb))
+        for _ in range(2):
+            cdx.check(cdx.lib.cdx_copy(None, b.ptr, a.ptr, nb))
+        cdx.check(cdx.lib.cdx_device_synchronize())
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            cdx.check(cdx.lib.cdx_copy(None, b.ptr, a.ptr, nb))
+        cdx.check(cdx.lib.cdx_device_synchronize())
+        el = time.perf_counter() - t0
+        out.append({"copy_mib": mb, "working_set_mib": 2 * mb, "gbps": round(2 * nb * reps / el / 1e9, 1)})
+        print(json.dumps(out[-1]), flush=True)
+        a.free()
+        b.free()
+    return out
+
+
 def learned_leg(cdx):
     """Learned warm start (U-Net, trained weights cdixx_amd/models/unet256.bin)
     against random starts on 64 held-out frames per photon range, same
@@ -406,24 +454,9 @@ def learned_leg(cdx):
     for ph in ((1e5, 1e8), (1e9, 1e10)):
         _, summ = evaluate(cdx, net, 64, photons=ph, blocks=100, arms=("random", "learned"))
         out[f"photons_{ph[0]:.0e}_{ph[1]:.0e}"] = {k: summ.get(k) for k in keep}
-    # training throughput: synthesis alone, then full steps (synthesis +
-    # forward + backward + Adam), batch 16 at 256^2
-    B, steps = 16, 20
-    net.train(2, B, seed0=3_000_000, lr=1e-6)
-    t0 = time.perf_counter()
-    for k in range(steps):
-        net.synth_batch(B, seed0=3_100_000 + k * B)
-    t_syn = (time.perf_counter() - t0) / steps
-    t0 = time.perf_counter()
-    net.train(steps, B, seed0=3_200_000, lr=1e-6)
-    t_step = (time.perf_counter() - t0) / steps
-    fl = unet_flops_per_sample(net.layers(), SIZE) * B
-    tf = fl / max(t_step - t_syn, 1e-9) / 1e12
-    out["training"] = {"batch": B, "step_ms": round(1e3 * t_step, 2), "synthesis_ms": round(1e3 * t_syn, 2),
-                       "samples_per_s": round(B / t_step, 1), "gflop_per_step": round(fl / 1e9, 1),
-                       "achieved_tflops_excl_synthesis": round(tf, 1), "peak_tflops_bf16_dense": 2500.0,
-                       "frac": round(tf / 2500.0, 4)}
     net.close()
+    # training throughput (random-init network of the same geometry)
+    out["training"] = training_leg(cdx)
     return out
 
 
@@ -613,6 +646,8 @@ def main():
     ap.add_argument("--batch-only", action="store_true", help="run only the batched leg (profiling)")
     ap.add_argument("--learned-large-only", action="store_true", help="run only the 2048^2 learned warm-start leg")
     ap.add_argument("--slab-only", action="store_true", help="run only the one-GPU slab-share leg")
+    ap.add_argument("--training-only", action="store_true", help="run only the U-Net training throughput (profiling)")
+    ap.add_argument("--copy-sweep", action="store_true", help="run only the copy bandwidth vs working-set sweep")
     ap.add_argument("--batch-iters", type=int, default=500)
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="run a plain copy kernel of this many MiB first (counter calibration)")
@@ -634,6 +669,12 @@ def main():
         return
     if args.slab_only:
         print(json.dumps({"slab_share_2048_over_8": slab_leg(cdx)}), flush=True)
+        return
+    if args.training_only:
+        print(json.dumps({"training": training_leg(cdx)}), flush=True)
+        return
+    if args.copy_sweep:
+        print(json.dumps({"copy_sweep": copy_sweep(cdx)}), flush=True)
         return
     if args.copy_calib_mb:
         nb = args.copy_calib_mb << 20
@@ -696,6 +737,8 @@ def main():
     dom = "row" if row_ms >= col_ms else "col"
     t_dom = max(row_ms, col_ms)
     achieved = (ROW_BYTES if dom == "row" else COL_BYTES) / (t_dom * 1e-3) / 1e9
+    t_ser = ser_row if dom == "row" else ser_col
+    achieved_ser = (ROW_BYTES if dom == "row" else COL_BYTES) / (t_ser * 1e-3) / 1e9
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic_1024.json")
     if os.path.exists(tfile):
@@ -716,7 +759,12 @@ def main():
                      "frac": round(achieved / PEAK_GBS, 4), "traffic": traffic,
                      "kernel": f"{dom} pass", "avg_ms": round(t_dom, 5),
                      "col_pass_ms": round(col_ms, 5), "row_pass_ms": round(row_ms, 5),
-                     "per_launch_events_ms": {"col_pass": round(ser_col, 5), "row_pass": round(ser_row, 5)}},
+                     "per_launch_events_ms": {"col_pass": round(ser_col, 5), "row_pass": round(ser_row, 5)},
+                     # each pass launched between its own pair of events, the
+                     # serialized dispatch a kernel-trace profiler imposes: the
+                     # figure its per-kernel average reproduces
+                     "serialized": {"avg_ms": round(t_ser, 5), "achieved": round(achieved_ser, 1),
+                                    "frac": round(achieved_ser / PEAK_GBS, 4)}},
         "wall_s_timed": round(wall, 4),
         "with_rfactor": {"iters_per_s": round(1000.0 * args.steps / rf_ms, 1), "col_pass_ms": round(rf_col, 5)},
         "end_to_end": {"schedule": "50*(20*RAAR+1*shrinkWrap)", "warmup": "2*(20*RAAR+1*shrinkWrap), untimed",

@@ -531,6 +531,8 @@ __global__ void __launch_bounds__(256) k_smooth(int n, const double* in, double*
     if (!table)
       for (int k = -r; k <= r; k++) ws += exp(-(double)k * k / (2 * sigma * sigma));
     double s = 0;
+    // unrolled so that several taps' loads are in flight (same summation order)
+#pragma unroll 8
     for (int k = -r; k <= r; k++) {
       const double w = table ? wt[k + r] : exp(-(double)k * k / (2 * sigma * sigma)) / ws;
       const int64_t j = axis ? (int64_t)(((x + k) % n + n) % n) * n + y : (int64_t)x * n + ((y + k) % n + n) % n;

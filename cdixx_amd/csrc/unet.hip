@@ -574,6 +574,8 @@ __global__ void k_pack_input(int n, const float* __restrict__ I, u16* __restrict
 // one workgroup: sum of a complex float image (fixed order)
 __global__ void k_csum(int64_t n, const cplx<float>* __restrict__ z, double* out) {
   double re = 0, im = 0;
+  // unrolled so that several loads are in flight (same summation order)
+#pragma unroll 8
   for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
     re += z[i].x;
     im += z[i].y;

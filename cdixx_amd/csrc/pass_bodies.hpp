@@ -21,8 +21,9 @@
 // column tiles of kT1Tile columns (t1_index), so the column pass reads
 // contiguous blocks; T1 and T2 are moved with streaming (non-temporal) loads
 // and stores, while g and S are loaded plainly (and g stored plainly up to
-// 1024^2): a workgroup id handles the same rows on the same XCD every
-// iteration, so they can stay in its L2.
+// 1024^2), the combination measured fastest.  (FETCH_SIZE shows g and S
+// fetched from beyond L2 every iteration all the same: the gain is the load
+// policy, not reuse.)
 #pragma once
 #include "device_ops.hpp"
 
@@ -139,8 +140,8 @@ __device__ __forceinline__ void col_body(const cplx<T>* __restrict__ in, cplx<T>
 #pragma unroll
     for (int i = 0; i < E; i++) v[i] = ok ? in[(size_t)L::dout_pos(i, t) * cols + col] : cplx<T>{(T)0, (T)0};
   } else if constexpr (TILED) {
-    // T1 is read once per iteration: streaming loads keep it from evicting
-    // what stays resident in L2 between iterations (g, S, the measurement)
+    // T1 is read once per iteration: streaming loads (measured faster than
+    // plain ones here)
 #pragma unroll
     for (int i = 0; i < E; i++)
       v[i] = ok ? load_stream(&in[t1_index(L::din_pos(i, t), col, N)]) : cplx<T>{(T)0, (T)0};
@@ -267,13 +268,11 @@ __device__ __forceinline__ void row_body(const cplx<T>* __restrict__ in, cplx<T>
     cplx<T> gv[upd ? E : 1];
     T sv[upd ? E : 1];
     cplx<T>* gamma0 = (cplx<T>*)ua.gamma0;
-    // A workgroup id handles the same rows on the same XCD every iteration,
-    // so plain loads of g and S can hit in L2 when the data read once per
-    // iteration streams past them.  Streaming loads of T2 pay off for
-    // 1024-point rows only (per iteration at 1024^2: 20.1 us with g, S and T2
-    // all plain, 19.0 with all streamed, 17.9 with only T2 streamed and T1
-    // streamed in the column pass; at 2048^2 streaming T2 cost 3.6 us, at
-    // smaller sizes it changed nothing).
+    // Plain loads of g and S with streaming loads of T2, the fastest
+    // combination for 1024-point rows (per iteration at 1024^2: 20.1 us with
+    // g, S and T2 all plain, 19.0 with all streamed, 17.9 with only T2
+    // streamed and T1 streamed in the column pass; at 2048^2 streaming T2
+    // cost 3.6 us, at smaller sizes it changed nothing).
     constexpr bool ld_nt = TILED && upd && N == 1024;
     if constexpr (ld_nt) {
 #pragma unroll
@@ -347,8 +346,8 @@ __device__ __forceinline__ void row_body(const cplx<T>* __restrict__ in, cplx<T>
         }
         v[i] = support_update(gv[i], p, in, ALGO >= 0 ? ALGO : ua.algo, ua.beta, ua.relax);
         if (ok) {
-          // GPLAIN: the single-frame loop up to 1024^2 reads g back from L2
-          // next iteration (g and S fit); larger frames and batches stream it
+          // GPLAIN: plain g stores measured faster in the single-frame loop
+          // up to 1024^2; larger frames and batches stream it
           if constexpr (GPLAIN) g[base + L::din_pos(i, t)] = v[i];
           else store_stream(&g[base + L::din_pos(i, t)], v[i]);
         }

@@ -112,53 +112,78 @@ __device__ __forceinline__ uint4 load_patch8(const u16* x, int H, int W, int Cin
 // row of accumulator element r in the 32x32 block (column = lane & 31)
 __device__ __forceinline__ int acc_row(int r, int lane) { return (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3); }
 
-// Implicit-GEMM convolution on a TM x TN output tile (TM = 32 WM pixels, TN =
-// 32 WN channels, four waves of 32 x 32): K steps of 64 through two LDS
-// buffers, the next step's global loads in flight during the current step's
-// MFMAs, one barrier per step.  WM = 4, WN = 1 (128 x 32) serves layers of
-// at most 32 output channels without idle MFMA columns; WM = WN = 2 (64 x 64)
-// the wider ones.
-template <int WM, int WN>
+// Implicit-GEMM convolution on a TM x TN output tile: four waves in a WM x WN
+// grid, each computing BM x BN blocks of 32 x 32 (TM = 32 WM BM pixels, TN =
+// 32 WN BN channels), so a wave reads BM + BN LDS operands per BM BN MFMAs.
+// K in steps of BK through two LDS buffers, the next step's global loads in
+// flight during the current step's MFMAs, one barrier per step.
+template <int WM, int WN, int BM, int BN, int BK>
 __global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
-  constexpr int TM = 32 * WM, TN = 32 * WN, BK = 64, LP = BK + 8;
-  constexpr int AV = TM * BK / 8 / 256;  // 16-byte A pieces per thread
-  constexpr int BV = TN * BK / 8 / 256;  // 16-byte B pieces per thread
-  static_assert(WM * WN == 4 && AV >= 1 && BV >= 1, "four waves");
+  constexpr int TM = 32 * WM * BM, TN = 32 * WN * BN, LP = BK + 8;
+  constexpr int KC = BK / 8;                       // 16-byte pieces per row and step
+  constexpr int AP = TM * KC, BP = TN * KC;        // pieces per step
+  constexpr int AV = (AP + 255) / 256, BV = (BP + 255) / 256;
+  static_assert(WM * WN == 4 && BK % 16 == 0, "four waves, 16-wide MFMA steps");
   __shared__ __align__(16) u16 As[2][TM][LP];
   __shared__ __align__(16) u16 Bs[2][TN][LP];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave / WN, wn = wave % WN;
   const int HW = a.H * a.W, M = a.B * HW, K = a.taps * a.Cin;
   const int m0 = blockIdx.x * TM, n0 = blockIdx.y * TN;
-  // this thread's A pieces: row (pixel) and k offset within the step
+  // this thread's A pieces: row (pixel) and k offset within the step; the
+  // pixel's element offset in x
   int arow[AV], akc[AV], ab[AV], ah[AV], aw[AV];
-  bool aok[AV];
+  size_t apix[AV];
+  bool aok[AV], aon[AV];
 #pragma unroll
   for (int j = 0; j < AV; j++) {
     const int idx = tid + 256 * j;
-    arow[j] = idx >> 3;
-    akc[j] = (idx & 7) * 8;
+    aon[j] = idx < AP;
+    arow[j] = aon[j] ? idx / KC : 0;
+    akc[j] = (idx % KC) * 8;
     const int m = m0 + arow[j];
-    aok[j] = m < M;
+    aok[j] = aon[j] && m < M;
     ab[j] = ah[j] = aw[j] = 0;
     if (aok[j]) pixel_of(m, HW, a.W, ab[j], ah[j], aw[j]);
+    apix[j] = ((size_t)(ab[j] * a.H + ah[j]) * a.W + aw[j]) * a.Cin;
   }
+  // When BK divides Cin every step lies in one tap: the tap and its pixel
+  // offset are the same for the whole workgroup (no per-piece division)
+  const bool one_tap = a.Cin % BK == 0;
   int brow[BV], bkc[BV];
-  bool bok[BV];
+  bool bok[BV], bon[BV];
 #pragma unroll
   for (int j = 0; j < BV; j++) {
     const int idx = tid + 256 * j;
-    brow[j] = idx >> 3;
-    bkc[j] = (idx & 7) * 8;
-    bok[j] = n0 + brow[j] < a.Cout;
+    bon[j] = idx < BP;
+    brow[j] = bon[j] ? idx / KC : 0;
+    bkc[j] = (idx % KC) * 8;
+    bok[j] = bon[j] && n0 + brow[j] < a.Cout;
   }
   uint4 ra[AV], rb[BV];
   auto load = [&](int k0) {
+    if (one_tap) {
+      const int tap = k0 / a.Cin, ci0 = k0 - tap * a.Cin;
+      int dy = 0, dx = 0;
+      if (a.taps == 9) {
+        dy = tap / 3 - 1;
+        dx = tap - (tap / 3) * 3 - 1;
+      }
+      const long long off = ((long long)dy * a.W + dx) * a.Cin + ci0;
+      const bool kin = k0 < K;
 #pragma unroll
-    for (int j = 0; j < AV; j++) {
-      const int k = k0 + akc[j];
-      ra[j] = (aok[j] && k < K) ? load_patch8(a.x, a.H, a.W, a.Cin, a.taps, ab[j], ah[j], aw[j], k)
-                                : make_uint4(0, 0, 0, 0);
+      for (int j = 0; j < AV; j++) {
+        const int hh = ah[j] + dy, ww = aw[j] + dx;
+        const bool in = aok[j] && kin && hh >= 0 && hh < a.H && ww >= 0 && ww < a.W;
+        ra[j] = in ? *(const uint4*)(a.x + (long long)apix[j] + off + akc[j]) : make_uint4(0, 0, 0, 0);
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < AV; j++) {
+        const int k = k0 + akc[j];
+        ra[j] = (aok[j] && k < K) ? load_patch8(a.x, a.H, a.W, a.Cin, a.taps, ab[j], ah[j], aw[j], k)
+                                  : make_uint4(0, 0, 0, 0);
+      }
     }
 #pragma unroll
     for (int j = 0; j < BV; j++) {
@@ -168,11 +193,17 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
   };
   auto store = [&](int buf) {
 #pragma unroll
-    for (int j = 0; j < AV; j++) *(uint4*)&As[buf][arow[j]][akc[j]] = ra[j];
+    for (int j = 0; j < AV; j++)
+      if (aon[j]) *(uint4*)&As[buf][arow[j]][akc[j]] = ra[j];
 #pragma unroll
-    for (int j = 0; j < BV; j++) *(uint4*)&Bs[buf][brow[j]][bkc[j]] = rb[j];
+    for (int j = 0; j < BV; j++)
+      if (bon[j]) *(uint4*)&Bs[buf][brow[j]][bkc[j]] = rb[j];
   };
-  f32x16 acc = {};
+  f32x16 acc[BM][BN];
+#pragma unroll
+  for (int i = 0; i < BM; i++)
+#pragma unroll
+    for (int j = 0; j < BN; j++) acc[i][j] = f32x16{};
   load(0);
   store(0);
   __syncthreads();
@@ -183,37 +214,61 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
 #pragma unroll
     for (int s4 = 0; s4 < BK / 16; s4++) {
       const int kk = s4 * 16 + (lane >> 5) * 8;
-      const bf16x8 av = *(const bf16x8*)&As[buf][wm * 32 + (lane & 31)][kk];
-      const bf16x8 bv = *(const bf16x8*)&Bs[buf][wn * 32 + (lane & 31)][kk];
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc, 0, 0, 0);
+      bf16x8 av[BM], bv[BN];
+#pragma unroll
+      for (int i = 0; i < BM; i++) av[i] = *(const bf16x8*)&As[buf][(wm * BM + i) * 32 + (lane & 31)][kk];
+#pragma unroll
+      for (int j = 0; j < BN; j++) bv[j] = *(const bf16x8*)&Bs[buf][(wn * BN + j) * 32 + (lane & 31)][kk];
+#pragma unroll
+      for (int i = 0; i < BM; i++)
+#pragma unroll
+        for (int j = 0; j < BN; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[i], bv[j], acc[i][j], 0, 0, 0);
     }
     // the other buffer was last read in the previous step, before its barrier
     if (more) store(buf ^ 1);
     __syncthreads();
     buf ^= 1;
   }
-  const int coo = n0 + wn * 32 + (lane & 31);
-  if (coo >= a.Cout) return;
-  const float bias = a.bias ? a.bias[coo] : 0.f;
 #pragma unroll
-  for (int r = 0; r < 16; r++) {
-    const int mm = m0 + wm * 32 + acc_row(r, lane);
-    if (mm < M) {
-      float v = acc[r] + bias;
-      if (a.relu) v = fmaxf(v, 0.f);
-      if (a.mask && !(bf2f(a.mask[(size_t)mm * a.Cout + coo]) > 0.f)) v = 0.f;
-      if (a.y) a.y[(size_t)mm * a.Cout + coo] = f2bf(v);
-      if (a.yf) a.yf[(size_t)mm * a.Cout + coo] = v;
+  for (int j = 0; j < BN; j++) {
+    const int coo = n0 + (wn * BN + j) * 32 + (lane & 31);
+    if (coo >= a.Cout) continue;
+    const float bias = a.bias ? a.bias[coo] : 0.f;
+#pragma unroll
+    for (int i = 0; i < BM; i++) {
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const int mm = m0 + (wm * BM + i) * 32 + acc_row(r, lane);
+        if (mm < M) {
+          float v = acc[i][j][r] + bias;
+          if (a.relu) v = fmaxf(v, 0.f);
+          if (a.mask && !(bf2f(a.mask[(size_t)mm * a.Cout + coo]) > 0.f)) v = 0.f;
+          if (a.y) a.y[(size_t)mm * a.Cout + coo] = f2bf(v);
+          if (a.yf) a.yf[(size_t)mm * a.Cout + coo] = v;
+        }
+      }
     }
   }
 }
-// the conv tile for `cout` output channels
+// The conv tile for `cout` output channels: the largest of 256 x 32 / 128 x 32
+// (at most 32 channels: no idle MFMA columns), 128 x 128, 128 x 64 and 64 x 64
+// that still gives about 1024 workgroups (four per CU).
 void launch_conv(const ConvArgs& a, int cout, hipStream_t st) {
-  const int M = a.B * a.H * a.W;
-  if (cout <= 32)
-    hipLaunchKernelGGL((k_conv<4, 1>), dim3((M + 127) / 128, (cout + 31) / 32), dim3(256), 0, st, a);
-  else
-    hipLaunchKernelGGL((k_conv<2, 2>), dim3((M + 63) / 64, (cout + 63) / 64), dim3(256), 0, st, a);
+  const long long M = (long long)a.B * a.H * a.W;
+  auto wgs = [&](int tm, int tn) { return ((M + tm - 1) / tm) * ((cout + tn - 1) / tn); };
+  const dim3 blk(256);
+  if (cout <= 32) {
+    if (wgs(256, 32) >= 1024)
+      hipLaunchKernelGGL((k_conv<4, 1, 2, 1, 32>), dim3((unsigned)((M + 255) / 256), 1), blk, 0, st, a);
+    else
+      hipLaunchKernelGGL((k_conv<4, 1, 1, 1, 32>), dim3((unsigned)((M + 127) / 128), 1), blk, 0, st, a);
+  } else if (cout % 128 == 0 && wgs(128, 128) >= 1024) {
+    hipLaunchKernelGGL((k_conv<2, 2, 2, 2, 32>), dim3((unsigned)((M + 127) / 128), cout / 128), blk, 0, st, a);
+  } else if (wgs(128, 64) >= 1024) {
+    hipLaunchKernelGGL((k_conv<2, 2, 2, 1, 32>), dim3((unsigned)((M + 127) / 128), (cout + 63) / 64), blk, 0, st, a);
+  } else {
+    hipLaunchKernelGGL((k_conv<2, 2, 1, 1, 32>), dim3((unsigned)((M + 63) / 64), (cout + 63) / 64), blk, 0, st, a);
+  }
 }
 
 // ---------------------------------------------------------------------------

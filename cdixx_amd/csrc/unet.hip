@@ -280,21 +280,28 @@ void launch_conv(const ConvArgs& a, int cout, hipStream_t st) {
 struct WgradArgs {
   const u16* dz;  // [M][Cout]
   const u16* x;   // [B][H][W][Cin]
-  float* part;  // [nsplit][Cout*K + Cout]: each split's weight then bias partials
+  float* part;  // [nsplit][Cout*K + tn*Cout]: each split's weight then bias partials
   int B, H, W, Cin, Cout, taps, chunk;
 };
 
 // TC x TN tile (TC = 32 WM output channels, TN = 32 WN weight columns, four
-// waves of 32 x 32), 64 pixels per step through two LDS buffers, the next
+// waves of 32 x 32), BK pixels per step through two LDS buffers, the next
 // step's global loads in flight during the current step's MFMAs.  Both MFMA
 // operands run along the pixels, which are the rows of dz and of the input:
 // each thread loads a block of 4 pixels x 8 channels (four 16-byte rows),
-// transposes it in registers and stores 8 rows of 4 pixels.  Workgroups of
-// n-tile 0 also add up the dz blocks they hold for the bias gradient.
-// Requires W % 4 == 0 (a 4-pixel block lies in one image row).
-template <int WM, int WN>
+// transposes it in registers (byte permutes) and stores 8 rows of 4 pixels.
+// A thread's im2col column (tap, channel) is fixed for the whole kernel and
+// its pixel advances by the step, so no index division is left in the loop.
+// The bias gradient rides on the dz blocks: the workgroup of n-tile y adds
+// the blocks of the steps s with s % tn == y, so the tn workgroups of a
+// (channel tile, split) share that work and each stores its own partial.
+// BK is chosen so that the blocks of a step, (TC + TN) / 8 x BK / 4, are at
+// most one per thread: a second block on some threads would set the pace of
+// every step.  Requires W % 4 == 0 (a 4-pixel block lies in one image row).
+template <int WM, int WN, int BK>
 __global__ void __launch_bounds__(256) k_wgrad(WgradArgs a) {
-  constexpr int TC = 32 * WM, TN = 32 * WN, BK = 64, LP = BK + 8;
+  constexpr int TC = 32 * WM, TN = 32 * WN, LP = BK + 8;
+  static_assert(BK % 16 == 0, "16-wide MFMA steps");
   constexpr int NA = TC / 8 * (BK / 4), NB = TN / 8 * (BK / 4);  // 4x8 blocks per step
   constexpr int NJ = (NA + NB + 255) / 256;
   static_assert(WM * WN == 4, "four waves");
@@ -304,12 +311,13 @@ __global__ void __launch_bounds__(256) k_wgrad(WgradArgs a) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave / WN, wn = wave % WN;
   const int HW = a.H * a.W, M = a.B * HW, K = a.taps * a.Cin;
-  const int co0 = blockIdx.x * TC, n0 = blockIdx.y * TN;
+  const int co0 = blockIdx.x * TC, n0 = blockIdx.y * TN, tn = gridDim.y;
   const int p_lo = blockIdx.z * a.chunk, p_hi = min(M, p_lo + a.chunk);
-  const bool with_bias = blockIdx.y == 0;
   // this thread's blocks: A (dz) blocks first, then B (im2col) blocks
   bool isA[NJ], act[NJ], cok[NJ];
   int pg[NJ], col[NJ];
+  // B blocks: the column's tap offset and channel, the current pixel
+  int bdy[NJ], bdx[NJ], bci[NJ], pb[NJ], ph[NJ], pw[NJ];
 #pragma unroll
   for (int jj = 0; jj < NJ; jj++) {
     const int j = tid + 256 * jj;
@@ -319,6 +327,16 @@ __global__ void __launch_bounds__(256) k_wgrad(WgradArgs a) {
     pg[jj] = q % (BK / 4);
     col[jj] = (q / (BK / 4)) * 8;
     cok[jj] = act[jj] && (isA[jj] ? co0 + col[jj] < a.Cout : n0 + col[jj] < K);
+    bdy[jj] = bdx[jj] = bci[jj] = 0;
+    if (!isA[jj]) {
+      const int n = n0 + col[jj], tap = n / a.Cin;
+      bci[jj] = n - tap * a.Cin;
+      if (a.taps == 9) {
+        bdy[jj] = tap / 3 - 1;
+        bdx[jj] = tap - (tap / 3) * 3 - 1;
+      }
+    }
+    pixel_of(min(p_lo + 4 * pg[jj], M - 1), HW, a.W, pb[jj], ph[jj], pw[jj]);
   }
   uint4 r[NJ][4];
   float bs[NJ][8];
@@ -326,27 +344,45 @@ __global__ void __launch_bounds__(256) k_wgrad(WgradArgs a) {
   for (int jj = 0; jj < NJ; jj++)
 #pragma unroll
     for (int e = 0; e < 8; e++) bs[jj][e] = 0.f;
+  // loads of the step at p0; then every B block's pixel advances by BK
   auto load = [&](int p0) {
 #pragma unroll
     for (int jj = 0; jj < NJ; jj++) {
       const int m = p0 + 4 * pg[jj];
 #pragma unroll
       for (int k = 0; k < 4; k++) r[jj][k] = make_uint4(0, 0, 0, 0);
-      if (!cok[jj] || m >= p_hi) continue;
-      if (isA[jj]) {
+      if (cok[jj] && m < p_hi) {
+        if (isA[jj]) {
 #pragma unroll
-        for (int k = 0; k < 4; k++)
-          if (m + k < p_hi) r[jj][k] = *(const uint4*)(a.dz + (size_t)(m + k) * a.Cout + co0 + col[jj]);
-      } else {
-        int b, h, w;
-        pixel_of(m, HW, a.W, b, h, w);
+          for (int k = 0; k < 4; k++)
+            if (m + k < p_hi) r[jj][k] = *(const uint4*)(a.dz + (size_t)(m + k) * a.Cout + co0 + col[jj]);
+        } else {
+          const int hh = ph[jj] + bdy[jj];
+          if (hh >= 0 && hh < a.H) {
+            const u16* row = a.x + ((size_t)(pb[jj] * a.H + hh) * a.W) * a.Cin + bci[jj];
 #pragma unroll
-        for (int k = 0; k < 4; k++)
-          if (m + k < p_hi) r[jj][k] = load_patch8(a.x, a.H, a.W, a.Cin, a.taps, b, h, w + k, n0 + col[jj]);
+            for (int k = 0; k < 4; k++) {
+              const int ww = pw[jj] + k + bdx[jj];
+              if (m + k < p_hi && ww >= 0 && ww < a.W) r[jj][k] = *(const uint4*)(row + (size_t)ww * a.Cin);
+            }
+          }
+        }
+      }
+      if (!isA[jj]) {
+        pw[jj] += BK;
+        while (pw[jj] >= a.W) {
+          pw[jj] -= a.W;
+          if (++ph[jj] == a.H) {
+            ph[jj] = 0;
+            ++pb[jj];
+          }
+        }
       }
     }
   };
-  auto store = [&](int buf) {
+  // transpose to [channel][pixel] rows: channel e of the four pixels is
+  // half (e & 1) of dword e >> 1 of each 16-byte row
+  auto store = [&](int buf, bool bias) {
 #pragma unroll
     for (int jj = 0; jj < NJ; jj++) {
       if (!act[jj]) continue;
@@ -357,22 +393,24 @@ __global__ void __launch_bounds__(256) k_wgrad(WgradArgs a) {
       const unsigned* w3 = (const unsigned*)&r[jj][3];
 #pragma unroll
       for (int e = 0; e < 8; e++) {
-        const int d = e >> 1, sh = (e & 1) * 16;
-        const unsigned v0 = (w0[d] >> sh) & 0xFFFFu, v1 = (w1[d] >> sh) & 0xFFFFu;
-        const unsigned v2 = (w2[d] >> sh) & 0xFFFFu, v3 = (w3[d] >> sh) & 0xFFFFu;
-        *(uint2*)&T[col[jj] + e][4 * pg[jj]] = make_uint2(v0 | (v1 << 16), v2 | (v3 << 16));
-        if (with_bias && isA[jj])
-          bs[jj][e] += (bf2f((u16)v0) + bf2f((u16)v1)) + (bf2f((u16)v2) + bf2f((u16)v3));
+        const int d = e >> 1;
+        const unsigned sel = (e & 1) ? 0x07060302u : 0x05040100u;
+        const unsigned lo = __builtin_amdgcn_perm(w1[d], w0[d], sel);
+        const unsigned hi = __builtin_amdgcn_perm(w3[d], w2[d], sel);
+        *(uint2*)&T[col[jj] + e][4 * pg[jj]] = make_uint2(lo, hi);
+        if (bias && isA[jj])
+          bs[jj][e] += (bf2f((u16)(lo & 0xFFFFu)) + bf2f((u16)(lo >> 16))) +
+                       (bf2f((u16)(hi & 0xFFFFu)) + bf2f((u16)(hi >> 16)));
       }
     }
   };
   if (tid < TC) bsum[tid] = 0.f;
   f32x16 acc = {};
   load(p_lo);
-  store(0);
+  store(0, blockIdx.y == 0);
   __syncthreads();
-  int buf = 0;
-  for (int p0 = p_lo; p0 < p_hi; p0 += BK) {
+  int buf = 0, step = 0;
+  for (int p0 = p_lo; p0 < p_hi; p0 += BK, step++) {
     const bool more = p0 + BK < p_hi;
     if (more) load(p0 + BK);
 #pragma unroll
@@ -383,23 +421,21 @@ __global__ void __launch_bounds__(256) k_wgrad(WgradArgs a) {
       acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc, 0, 0, 0);
     }
     // the other buffer was last read in the previous step, before its barrier
-    if (more) store(buf ^ 1);
+    if (more) store(buf ^ 1, (step + 1) % tn == (int)blockIdx.y);
     __syncthreads();
     buf ^= 1;
   }
-  if (with_bias) {
 #pragma unroll
-    for (int jj = 0; jj < NJ; jj++)
-      if (cok[jj] && isA[jj])
+  for (int jj = 0; jj < NJ; jj++)
+    if (cok[jj] && isA[jj])
 #pragma unroll
-        for (int e = 0; e < 8; e++) atomicAdd(&bsum[col[jj] + e], bs[jj][e]);
-    __syncthreads();
-    if (tid < TC && co0 + tid < a.Cout)
-      a.part[(size_t)blockIdx.z * a.Cout * (K + 1) + (size_t)a.Cout * K + co0 + tid] = bsum[tid];
-  }
+      for (int e = 0; e < 8; e++) atomicAdd(&bsum[col[jj] + e], bs[jj][e]);
+  __syncthreads();
+  const size_t nout = (size_t)a.Cout * (K + tn);
+  float* part = a.part + (size_t)blockIdx.z * nout;
+  if (tid < TC && co0 + tid < a.Cout) part[(size_t)a.Cout * K + (size_t)blockIdx.y * a.Cout + co0 + tid] = bsum[tid];
   const int n = n0 + wn * 32 + (lane & 31);
   if (n < K) {
-    float* part = a.part + (size_t)blockIdx.z * a.Cout * (K + 1);
 #pragma unroll
     for (int r2 = 0; r2 < 16; r2++) {
       const int co = co0 + wm * 32 + acc_row(r2, lane);
@@ -411,8 +447,8 @@ __global__ void __launch_bounds__(256) k_wgrad(WgradArgs a) {
 // The splits' partials are added in two fixed-order stages.  Stage 1:
 // workgroup (x, g) adds splits [g*zc, (g+1)*zc) of 64 consecutive partial
 // values (4 lanes of 64 threads, each lane every 4th split, then the lanes in
-// order) into part2[g][.].  nout = Cout*K + Cout values: the weight partials
-// [Cout][K] followed by the bias partials [Cout] of each split.
+// order) into part2[g][.].  nout = Cout*K + tn*Cout values: the weight
+// partials [Cout][K] followed by the bias partials [tn][Cout] of each split.
 __global__ void __launch_bounds__(256) k_wgrad_reduce1(int nsplit, int zc, int64_t nout, const float* __restrict__ part,
                                                        float* __restrict__ part2) {
   __shared__ float red[4][64];
@@ -426,20 +462,32 @@ __global__ void __launch_bounds__(256) k_wgrad_reduce1(int nsplit, int zc, int64
   __syncthreads();
   if (zl == 0 && i < nout) part2[blockIdx.y * nout + i] = (red[0][il] + red[1][il]) + (red[2][il] + red[3][il]);
 }
-// Stage 2: dw[co][n] += sum_g part2[g][co*K + n], db[co] += sum_g part2[g][Cout*K + co]
-__global__ void k_wgrad_reduce2(int ng, int Cout, int K, int Kp, const float* __restrict__ part2,
-                                float* __restrict__ dw, float* __restrict__ db) {
-  const int64_t nw = (int64_t)Cout * K, nout = nw + Cout;
-  EW_LOOP(nout) {
+// Stage 2: dw[co][n] += sum_g part2[g][co*K + n]
+__global__ void k_wgrad_reduce2(int ng, int Cout, int K, int Kp, int tn, const float* __restrict__ part2,
+                                float* __restrict__ dw) {
+  const int64_t nw = (int64_t)Cout * K, nout = nw + (int64_t)tn * Cout;
+  EW_LOOP(nw) {
     float s = 0.f;
     for (int g = 0; g < ng; g++) s += part2[g * nout + i];
-    if (i < nw) {
-      const int co = (int)(i / K), n = (int)(i - (int64_t)co * K);
-      dw[(size_t)co * Kp + n] += s;
-    } else {
-      db[i - nw] += s;
-    }
+    const int co = (int)(i / K), n = (int)(i - (int64_t)co * K);
+    dw[(size_t)co * Kp + n] += s;
   }
+}
+// db[c] += sum over g < ng, y < tn of part2[g][Cout*K + y*Cout + c]: one wave
+// per channel, lane l adding the entries e = l, l + 64, ... (e = g*tn + y),
+// then the lanes in a fixed butterfly order
+__global__ void __launch_bounds__(64) k_wgrad_bias_reduce(int ng, int Cout, int K, int tn,
+                                                          const float* __restrict__ part2, float* __restrict__ db) {
+  const int c = blockIdx.x, l = threadIdx.x;
+  const int64_t nw = (int64_t)Cout * K, nout = nw + (int64_t)tn * Cout;
+  float s = 0.f;
+  for (int e = l; e < ng * tn; e += 64) {
+    const int g = e / tn, y = e - g * tn;
+    s += part2[g * nout + nw + (int64_t)y * Cout + c];
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (l == 0) db[c] += s;
 }
 
 // The layout kernels below move 8 channels (16 bytes) per thread: every
@@ -779,32 +827,35 @@ int conv_wgrad(cdx_unet* u, const UConv& L, int B, int h, const u16* dz, const u
   if (h % 4 || L.cin % 8 || L.cout % 8)
     return ufail(CDX_ERR_ARG, "unet: weight gradient needs sides divisible by 4 and channels by 8");
   const int M = B * h * h, K = L.taps * L.cin;
-  const bool narrow = L.cout <= 32;  // 32 x 128 tiles, else 64 x 64
-  const int TC = narrow ? 32 : 64, TN = narrow ? 128 : 64;
+  const bool narrow = L.cout <= 32;  // 32 x 128 tiles with 48-pixel steps, else 64 x 64 with 64
+  const int TC = narrow ? 32 : 64, TN = narrow ? 128 : 64, BK = narrow ? 48 : 64;
   const int tm = (L.cout + TC - 1) / TC, tn = (K + TN - 1) / TN;
   // about 1024 workgroups, at least 256 pixels each, within the workspace
   // (the splits' partials and the reduction's first stage, 1/16 of them)
-  const int64_t nout = (int64_t)L.cout * (K + 1);
+  const int64_t nout = (int64_t)L.cout * (K + tn);
   int nsplit = 1024 / (tm * tn);
   nsplit = std::max(1, std::min(nsplit, M / 256));
   nsplit = (int)std::min<int64_t>(nsplit, kWgradPartFloats / (nout + nout / 16 + 1) - 1);
   if (nsplit < 1) return ufail(CDX_ERR_ARG, "unet: layer too large for the weight-gradient workspace");
   int chunk = (M + nsplit - 1) / nsplit;
-  chunk = (chunk + 63) / 64 * 64;
+  chunk = (chunk + BK - 1) / BK * BK;
   nsplit = (M + chunk - 1) / chunk;
   WgradArgs a{dz, x, u->wg_part, B, h, h, L.cin, L.cout, L.taps, chunk};
   if (narrow)
-    hipLaunchKernelGGL((k_wgrad<1, 4>), dim3(tm, tn, nsplit), dim3(256), 0, u->st, a);
+    hipLaunchKernelGGL((k_wgrad<1, 4, 48>), dim3(tm, tn, nsplit), dim3(256), 0, u->st, a);
   else
-    hipLaunchKernelGGL((k_wgrad<2, 2>), dim3(tm, tn, nsplit), dim3(256), 0, u->st, a);
+    hipLaunchKernelGGL((k_wgrad<2, 2, 64>), dim3(tm, tn, nsplit), dim3(256), 0, u->st, a);
   UCHK(hipGetLastError());
   const int zc = 16, ng = (nsplit + zc - 1) / zc;
   float* part2 = u->wg_part + (int64_t)nsplit * nout;
   hipLaunchKernelGGL(k_wgrad_reduce1, dim3((unsigned)((nout + 63) / 64), ng), dim3(256), 0, u->st, nsplit, zc, nout,
                      (const float*)u->wg_part, part2);
   UCHK(hipGetLastError());
-  hipLaunchKernelGGL(k_wgrad_reduce2, dim3(ew_blocks(nout)), dim3(256), 0, u->st, ng, L.cout, K, L.kp,
-                     (const float*)part2, u->g + L.w_off, u->g + L.b_off);
+  hipLaunchKernelGGL(k_wgrad_reduce2, dim3(ew_blocks((int64_t)L.cout * K)), dim3(256), 0, u->st, ng, L.cout, K,
+                     L.kp, tn, (const float*)part2, u->g + L.w_off);
+  UCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_wgrad_bias_reduce, dim3(L.cout), dim3(64), 0, u->st, ng, L.cout, K, tn, (const float*)part2,
+                     u->g + L.b_off);
   UCHK(hipGetLastError());
   return CDX_OK;
 }

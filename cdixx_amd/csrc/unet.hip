@@ -109,6 +109,27 @@ __device__ __forceinline__ uint4 load_patch8(const u16* x, int H, int W, int Cin
   return *(const uint4*)(x + ((size_t)(b * H + hh) * W + ww) * Cin + ci);
 }
 
+// 8 BF16 values of a 16-byte piece <-> floats
+__device__ __forceinline__ void unpack8(const uint4& v, float* f) {
+  const unsigned* w = (const unsigned*)&v;
+#pragma unroll
+  for (int e = 0; e < 8; e++) f[e] = bf2f((u16)((w[e >> 1] >> ((e & 1) * 16)) & 0xFFFFu));
+}
+__device__ __forceinline__ uint4 pack8(const float* f) {
+  uint4 v;
+  unsigned* w = (unsigned*)&v;
+#pragma unroll
+  for (int d = 0; d < 4; d++) w[d] = (unsigned)f2bf(f[2 * d]) | ((unsigned)f2bf(f[2 * d + 1]) << 16);
+  return v;
+}
+// zero where the ReLU output y is not positive
+__device__ __forceinline__ void relu_mask8(float* f, const uint4& y) {
+  float m[8];
+  unpack8(y, m);
+#pragma unroll
+  for (int e = 0; e < 8; e++)
+    if (!(m[e] > 0.f)) f[e] = 0.f;
+}
 // row of accumulator element r in the 32x32 block (column = lane & 31)
 __device__ __forceinline__ int acc_row(int r, int lane) { return (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3); }
 
@@ -124,8 +145,12 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
   constexpr int AP = TM * KC, BP = TN * KC;        // pieces per step
   constexpr int AV = (AP + 255) / 256, BV = (BP + 255) / 256;
   static_assert(WM * WN == 4 && BK % 16 == 0, "four waves, 16-wide MFMA steps");
-  __shared__ __align__(16) u16 As[2][TM][LP];
-  __shared__ __align__(16) u16 Bs[2][TN][LP];
+  // the two operand buffers, and after the K loop the epilogue's staging
+  // area (a 32 x 33 float block per wave) in the same LDS
+  constexpr size_t kOps = (size_t)2 * (TM + TN) * LP * sizeof(u16), kStage = (size_t)4 * 32 * 33 * sizeof(float);
+  __shared__ __align__(16) unsigned char smem[kOps > kStage ? kOps : kStage];
+  u16(*As)[TM][LP] = reinterpret_cast<u16(*)[TM][LP]>(smem);
+  u16(*Bs)[TN][LP] = reinterpret_cast<u16(*)[TN][LP]>(smem + (size_t)2 * TM * LP * sizeof(u16));
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave / WN, wn = wave % WN;
   const int HW = a.H * a.W, M = a.B * HW, K = a.taps * a.Cin;
@@ -229,22 +254,38 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
     __syncthreads();
     buf ^= 1;
   }
+  // Epilogue, one 32 x 32 block at a time: the wave stages its accumulators
+  // in LDS (element (row, col) at row * 33 + col), then lane l takes pixel
+  // row l / 2 and channels 16 (l % 2) .. +15 as two 8-channel segments, so
+  // bias, ReLU, mask and rounding run per 8 channels and y moves in 16-byte
+  // stores (channel counts are multiples of 8: a segment is all in or out).
+  float* stg = reinterpret_cast<float*>(smem) + wave * 32 * 33;
+  const int srow = lane >> 1, sc0 = (lane & 1) * 16;
 #pragma unroll
-  for (int j = 0; j < BN; j++) {
-    const int coo = n0 + (wn * BN + j) * 32 + (lane & 31);
-    if (coo >= a.Cout) continue;
-    const float bias = a.bias ? a.bias[coo] : 0.f;
+  for (int i = 0; i < BM; i++) {
 #pragma unroll
-    for (int i = 0; i < BM; i++) {
+    for (int j = 0; j < BN; j++) {
+      __syncthreads();  // the K loop's or the previous block's LDS reads are done
 #pragma unroll
-      for (int r = 0; r < 16; r++) {
-        const int mm = m0 + (wm * BM + i) * 32 + acc_row(r, lane);
-        if (mm < M) {
-          float v = acc[i][j][r] + bias;
-          if (a.relu) v = fmaxf(v, 0.f);
-          if (a.mask && !(bf2f(a.mask[(size_t)mm * a.Cout + coo]) > 0.f)) v = 0.f;
-          if (a.y) a.y[(size_t)mm * a.Cout + coo] = f2bf(v);
-          if (a.yf) a.yf[(size_t)mm * a.Cout + coo] = v;
+      for (int r = 0; r < 16; r++) stg[acc_row(r, lane) * 33 + (lane & 31)] = acc[i][j][r];
+      __syncthreads();
+      const int mm = m0 + (wm * BM + i) * 32 + srow;
+#pragma unroll
+      for (int sg = 0; sg < 2; sg++) {
+        const int c = sc0 + 8 * sg, coo = n0 + (wn * BN + j) * 32 + c;
+        if (mm >= M || coo >= a.Cout) continue;
+        float v[8];
+#pragma unroll
+        for (int e = 0; e < 8; e++) v[e] = stg[srow * 33 + c + e] + (a.bias ? a.bias[coo + e] : 0.f);
+        if (a.relu)
+#pragma unroll
+          for (int e = 0; e < 8; e++) v[e] = fmaxf(v[e], 0.f);
+        const size_t o = (size_t)mm * a.Cout + coo;
+        if (a.mask) relu_mask8(v, *(const uint4*)(a.mask + o));
+        if (a.y) *(uint4*)(a.y + o) = pack8(v);
+        if (a.yf) {
+          *(float4*)(a.yf + o) = make_float4(v[0], v[1], v[2], v[3]);
+          *(float4*)(a.yf + o + 4) = make_float4(v[4], v[5], v[6], v[7]);
         }
       }
     }
@@ -493,26 +534,6 @@ __global__ void __launch_bounds__(64) k_wgrad_bias_reduce(int ng, int Cout, int 
 // The layout kernels below move 8 channels (16 bytes) per thread: every
 // channel count of the network is a multiple of 8, and the element counts
 // they take (pixels x channels / 8) fit 32 bits (checked by the launchers).
-__device__ __forceinline__ void unpack8(const uint4& v, float* f) {
-  const unsigned* w = (const unsigned*)&v;
-#pragma unroll
-  for (int e = 0; e < 8; e++) f[e] = bf2f((u16)((w[e >> 1] >> ((e & 1) * 16)) & 0xFFFFu));
-}
-__device__ __forceinline__ uint4 pack8(const float* f) {
-  uint4 v;
-  unsigned* w = (unsigned*)&v;
-#pragma unroll
-  for (int d = 0; d < 4; d++) w[d] = (unsigned)f2bf(f[2 * d]) | ((unsigned)f2bf(f[2 * d + 1]) << 16);
-  return v;
-}
-// zero where the ReLU output y is not positive
-__device__ __forceinline__ void relu_mask8(float* f, const uint4& y) {
-  float m[8];
-  unpack8(y, m);
-#pragma unroll
-  for (int e = 0; e < 8; e++)
-    if (!(m[e] > 0.f)) f[e] = 0.f;
-}
 #define EW_LOOP32(n) for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += gridDim.x * blockDim.x)
 
 // 2x2 average pool, NHWC; n8 = B * H/2 * W/2 * C/8

@@ -966,8 +966,7 @@ static int sim_object_sp(cdx_ctx* c, const ShapeParams& sp, uint64_t seed, doubl
   HIPCHK(hipFreeAsync(a, c->st));
   HIPCHK(hipFreeAsync(ph, c->st));
   HIPCHK(hipFreeAsync(t, c->st));
-  HIPCHK(hipStreamSynchronize(c->st));
-  return CDX_OK;
+  return CDX_OK;  // ordered on the context's stream like the other context calls
 }
 
 extern "C" int cdx_sim_object(cdx_ctx* c, double radius, uint64_t seed, double sigma, void* obj) {
@@ -1018,8 +1017,7 @@ extern "C" int cdx_sim_intensity(cdx_ctx* c, const void* obj, double photons, ui
     HIPCHK(launch_sum_mod2<float>(c->n, c->T2, c->scal + S_TOT, c->st));
     HIPCHK(launch_poisson_intensity<float>(c->n, c->T2, c->scal + S_TOT, photons, seed, poisson, (float*)I, c->st));
   }
-  HIPCHK(hipStreamSynchronize(c->st));
-  return CDX_OK;
+  return CDX_OK;  // ordered on the context's stream
 }
 
 // ---------------------------------------------------------------------------

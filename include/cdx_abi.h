@@ -288,7 +288,8 @@ int cdx_get_tiling(cdx_ctx* ctx, int* col_lines, int* row_lines);
  * (pass 0=column/1=row, lines, microseconds per launch), up to cap doubles. */
 int cdx_autotune(cdx_ctx* ctx, int reps, double* report, int cap);
 
-/* ---- synthetic forward model (Philox4x64-10 streams) ---- */
+/* ---- synthetic forward model (Philox4x64-10 streams) ----
+ * Asynchronous on the context's stream, like the other context calls. */
 /* n x n object: smoothed-noise amplitude in [0.2,1] and phase in [-pi,pi]
  * inside a centred disk of the given radius.  Context must be n x n. */
 int cdx_sim_object(cdx_ctx* ctx, double radius, uint64_t seed, double smooth_sigma, void* obj);

@@ -541,9 +541,51 @@ __global__ void __launch_bounds__(256) k_smooth(int n, const double* in, double*
     out[i] = s;
   }
 }
+// The same smoothing with one workgroup per line: the line (row for axis 0,
+// column for axis 1) and the weights are read into LDS once, then every
+// output sums its taps from LDS in the order of k_smooth, so the results are
+// identical.  Lines of up to kSmoothMaxLine points, radius below n.
+constexpr int kSmoothMaxLine = 2048;
+__global__ void __launch_bounds__(256) k_smooth_line(int n, const double* in, double* out, double sigma, int axis) {
+  __shared__ double wt[kSmoothMaxTaps];
+  __shared__ double line[kSmoothMaxLine];
+  const int r = (int)ceil(4 * sigma);
+  const int taps = 2 * r + 1;
+  const int l = blockIdx.x;
+  if (threadIdx.x == 0) {
+    double ws = 0;
+    for (int k = -r; k <= r; k++) ws += exp(-(double)k * k / (2 * sigma * sigma));
+    wt[0] = ws;
+  }
+  __syncthreads();
+  const double ws = wt[0];
+  __syncthreads();
+  for (int t = threadIdx.x; t < taps; t += blockDim.x) {
+    const int k = t - r;
+    wt[t] = exp(-(double)k * k / (2 * sigma * sigma)) / ws;
+  }
+  // element p of line l
+  auto at = [&](int p) { return axis ? (int64_t)p * n + l : (int64_t)l * n + p; };
+  for (int p = threadIdx.x; p < n; p += blockDim.x) line[p] = in[at(p)];
+  __syncthreads();
+  for (int p = threadIdx.x; p < n; p += blockDim.x) {
+    double s = 0;
+    for (int k = -r; k <= r; k++) {
+      int q = p + k;
+      q += q < 0 ? n : (q >= n ? -n : 0);
+      s += wt[k + r] * line[q];
+    }
+    out[at(p)] = s;
+  }
+}
 hipError_t launch_smooth_periodic(int n, const double* in, double* out, double sigma, int axis, hipStream_t st) {
-  const int64_t N = (int64_t)n * n;
-  hipLaunchKernelGGL(k_smooth, ew_grid(N), dim3(256), 0, st, n, in, out, sigma, axis);
+  const int r = (int)ceil(4 * sigma);
+  if (n <= kSmoothMaxLine && 2 * r + 1 <= kSmoothMaxTaps && r < n) {
+    hipLaunchKernelGGL(k_smooth_line, dim3(n), dim3(256), 0, st, n, in, out, sigma, axis);
+  } else {
+    const int64_t N = (int64_t)n * n;
+    hipLaunchKernelGGL(k_smooth, ew_grid(N), dim3(256), 0, st, n, in, out, sigma, axis);
+  }
   return hipGetLastError();
 }
 // mm[0] = min, mm[1] = max as order-preserving integer keys; caller seeds mm

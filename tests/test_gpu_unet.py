@@ -33,14 +33,19 @@ def _rand_net(cdx, size, levels, base, B, seed=0):
     return net, ref, flat.astype(np.float64)
 
 
-@pytest.mark.parametrize("size,levels,base,B", [(32, 3, 8, 2), (64, 4, 16, 1), (128, 4, 32, 1)])
-def test_forward_and_gradients_match_oracle(cdx, size, levels, base, B):
+# (size, levels, base, batch, forward bar, gradient bar): the bars are about
+# twice the measured relative L2 of these seeded cases (the GPU side is
+# deterministic: fixed-order reductions): 32^2 8e-8 / 8e-8, 64^2 5.9e-3 /
+# 1.4e-2, the shipped 4 x 32 geometry at 128^2 2.5e-3 / 2.7e-3
+@pytest.mark.parametrize("size,levels,base,B,fbar,gbar", [(32, 3, 8, 2, 1e-6, 1e-6), (64, 4, 16, 1, 1e-2, 3e-2),
+                                                         (128, 4, 32, 1, 5e-3, 6e-3)])
+def test_forward_and_gradients_match_oracle(cdx, size, levels, base, B, fbar, gbar):
     """Forward pass, loss and every layer's weight and bias gradients against
     the numpy oracle (same BF16 rounding points, float64 arithmetic), up to
     the shipped geometry (4 levels x 32 base channels, here at 128^2).  The
     residual differences come from FP32 accumulation order flipping BF16
-    roundings of activations: relative L2 bounds 1e-2 (forward) and 3e-2
-    (gradients); the measured values are printed."""
+    roundings of activations, which grows with the channel count and the
+    depth; the measured values are printed."""
     print(f"unet parity size={size} levels={levels} base={base} B={B}", flush=True)
     net, ref, flat = _rand_net(cdx, size, levels, base, B)
     rs = np.random.default_rng(5)
@@ -51,7 +56,7 @@ def test_forward_and_gradients_match_oracle(cdx, size, levels, base, B):
     y = net.forward(B)
     yr, cache = ref.forward(x, ref.unpack(flat))
     print(f"  forward rel-L2 {rel_l2(y, yr[..., :3]):.2e}", flush=True)
-    assert rel_l2(y, yr[..., :3]) < 1e-2, rel_l2(y, yr[..., :3])
+    assert rel_l2(y, yr[..., :3]) < fbar, rel_l2(y, yr[..., :3])
     net.zero_grads()
     loss = net.backward(B)
     lr_, dy = ref.loss(yr, tgt)
@@ -62,7 +67,7 @@ def test_forward_and_gradients_match_oracle(cdx, size, levels, base, B):
         ew = rel_l2(g[wo:wo + cout * kp], gr[wo:wo + cout * kp])
         eb = rel_l2(g[bo:bo + cout], gr[bo:bo + cout])
         print(f"  layer {i} ({cin}->{cout}, {taps} taps): dW {ew:.2e} db {eb:.2e}")
-        assert ew < 3e-2 and eb < 3e-2, (i, ew, eb)
+        assert ew < gbar and eb < gbar, (i, ew, eb)
     net.close()
 
 

@@ -24,8 +24,18 @@ sys.path.insert(0, ROOT)
 METRIC = "HIO iters/s + HBM GB/s at 1024² c64; iters-to-R≤0.05 learned vs random start"
 N = 1024
 BYTES_PER_ITER = 80 * N * N          # algorithmic bytes of one iteration (c64/FP32)
-COL_BYTES = 20 * N * N               # column pass: read T1 8 + read I 4 + write T2 8 B/px
-ROW_BYTES = 36 * N * N               # row pass: read T2 8, g 8, S 4; write g 8, T1 8 B/px
+COL_BPP = 20.0                       # column pass: read T1 8 + read I 4 + write T2 8 B/px
+# row pass: read T2 8, g 8 and the support bits (one 4-byte word per lane,
+# i.e. 4/E B/px for E points per lane); write g 8, T1 8 B/px
+ROW_LANES = {1024: 128, 2048: 128}   # lanes per FP32 row line of these lengths
+
+
+def row_bpp(cols):
+    return 32.0 + 4.0 * ROW_LANES[cols] / cols
+
+
+COL_BYTES = int(COL_BPP * N * N)
+ROW_BYTES = int(row_bpp(N) * N * N)  # 32.5 B/px at 1024
 PEAK_GBS = 8000.0
 
 
@@ -583,7 +593,7 @@ def slab_leg(cdx, iters=200):
         p = cdx.Params.default(beta=0.85)
         _, col_ms, row_ms = ctx.time_iterations(p, cdx.RAAR, F, g, I, spt, iters, per_kernel=True)
         ms = row_ms if pick == 2 else col_ms
-        nbytes = (ROW_BYTES // (N * N) if pick == 2 else COL_BYTES // (N * N)) * r * c
+        nbytes = (row_bpp(c) if pick == 2 else COL_BPP) * r * c
         out[key] = {"us": round(1000 * ms, 3), "GBps": round(nbytes / (ms * 1e-3) / 1e9, 1)}
         ctx.close()
     out["local_us_per_iteration"] = round(out["row_pass_256x2048"]["us"] + out["col_pass_2048x256"]["us"], 3)

@@ -69,6 +69,7 @@ struct cdx_ctx {
   double* partials;  // kMaxPartials per-workgroup sums
   double* ring;
   int ring_cap;
+  unsigned* smask = nullptr;  // support bits of the update pass (launch_support_mask)
   int rcolor;
   double scale, beta, relax;
   int col_lines = 0, row_lines = 0;  // lines per workgroup of the passes (0: default)
@@ -183,7 +184,8 @@ extern "C" int cdx_create(cdx_ctx** out, int rows, int cols, int precision, void
       hipMalloc(&c->Ipk, c->n * c->rsz) != hipSuccess || hipMalloc(&c->tmpr, c->n * c->rsz) != hipSuccess ||
       hipMalloc((void**)&c->scal, S_COUNT * sizeof(double)) != hipSuccess ||
       hipMalloc((void**)&c->partials, kMaxPartials * sizeof(double)) != hipSuccess ||
-      hipMalloc((void**)&c->ring, 2 * kRfSlots * (size_t)c->ring_cap * sizeof(double)) != hipSuccess) {
+      hipMalloc((void**)&c->ring, 2 * kRfSlots * (size_t)c->ring_cap * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&c->smask, support_mask_bytes(rows, cols, c->csz)) != hipSuccess) {
     cdx_destroy(c);
     return fail(CDX_ERR_HIP, "cdx_create: out of device memory for %dx%d", rows, cols);
   }
@@ -209,7 +211,7 @@ extern "C" int cdx_destroy(cdx_ctx* c) {
   if (!c) return CDX_OK;
   if (c->st) (void)hipStreamSynchronize(c->st);
   for (void* p : {c->tw_row, c->tw_col, c->T1, c->T2, c->Ipk, c->tmpr, (void*)c->scal, (void*)c->partials,
-                  (void*)c->ring, c->prtf_map, c->gamma0, (void*)c->prtf_dev})
+                  (void*)c->ring, (void*)c->smask, c->prtf_map, c->gamma0, (void*)c->prtf_dev})
     (void)hipFree(p);
   if (c->own_stream && c->st) (void)hipStreamDestroy(c->st);
   delete c;
@@ -323,6 +325,9 @@ static UpdArgs<T> upd_args(cdx_ctx* c, int algo, double beta, double relax) {
   ua.relax = (T)relax;
   ua.invN = (T)(1.0 / (double)c->n);
   ua.scal = c->scal;
+  ua.gamma0 = nullptr;
+  ua.prtf = 0;
+  ua.smask = c->smask;
   return ua;
 }
 extern "C" int cdx_apply_support(cdx_ctx* c, void* g, const void* gp, int algo, const void* spt) {
@@ -542,6 +547,8 @@ struct Loop {
     pa.rcolor = (T)p->rcolor;
     pa.rf = nullptr;
     pa.rfs = 2 * c->ring_cap;
+    // the update pass's support bits from the caller's support and threshold
+    HIPCHK(launch_support_mask<T>(c->rows, c->cols, spt, c->scal, c->smask, c->st));
     return CDX_OK;
   }
   int iterate(int algo, double* rf) {
@@ -599,6 +606,8 @@ struct Loop {
       HIPCHK(launch_blur_real<T>(c->rows, c->cols, g, 1, (T*)c->tmpr, spt, (T)sigma, slots(c), c->st));
       HIPCHK(launch_threshold<T>(c->scal, slots(c), p->shrinkThreshold, c->rows, c->cols, 1, c->st));
     }
+    // new support and threshold: new support bits
+    HIPCHK(launch_support_mask<T>(c->rows, c->cols, spt, c->scal, c->smask, c->st));
     return CDX_OK;
   }
   int finish() {
@@ -1102,6 +1111,7 @@ static int autotune_t(cdx_ctx* c, int reps, double* report, int cap) {
   pa.scale = 1;
   pa.rcolor = 65535;
   UpdArgs<T> ua = upd_args<T>(c, CDX_HIO, 1.0, 1.9);
+  HIPCHK(launch_support_mask<T>(c->rows, c->cols, (const T*)spt, c->scal, c->smask, c->st));
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
@@ -1155,7 +1165,7 @@ extern "C" int cdx_autotune(cdx_ctx* c, int reps, double* report, int cap) {
 // ---------------------------------------------------------------------------
 // cdx_phase_retrieve_static: the loop for callers that cannot allocate
 // device memory.  Its workspace (T1, T2, the packed measurement, the blur
-// buffer, twiddle tables, scalars, partial sums) is the file-scope array
+// buffer, twiddle tables, scalars, partial sums, support bits) is the file-scope array
 // `scratch`, sized for frames of up to kStaticMaxSide points per side in
 // FP32; the kernels are those of cdx_phase_retrieve.
 // ---------------------------------------------------------------------------
@@ -1171,7 +1181,10 @@ constexpr size_t kOffTwR = kOffTmp + al256(kStaticMaxN * 4);
 constexpr size_t kOffTwC = kOffTwR + al256(kStaticMaxSide * 8);
 constexpr size_t kOffScal = kOffTwC + al256(kStaticMaxSide * 8);
 constexpr size_t kOffPart = kOffScal + al256(S_COUNT * sizeof(double));
-constexpr size_t kScratchBytes = kOffPart + al256(kMaxPartials * sizeof(double));  // 6 MiB + 38 KiB
+constexpr size_t kOffMask = kOffPart + al256(kMaxPartials * sizeof(double));
+// support bits: rows x lanes words, at most 512 x 32 for these line lengths
+constexpr size_t kMaskBytes = (size_t)kStaticMaxSide * 32 * sizeof(unsigned);
+constexpr size_t kScratchBytes = kOffMask + al256(kMaskBytes);  // 6 MiB + 102 KiB
 constexpr int kMaxDevices = 64;
 }  // namespace
 
@@ -1254,6 +1267,9 @@ extern "C" int cdx_phase_retrieve_static(int rows, int cols, const char* algorit
   c.tw_col = base + kOffTwC;
   c.scal = (double*)(base + kOffScal);
   c.partials = (double*)(base + kOffPart);
+  c.smask = (unsigned*)(base + kOffMask);
+  if (support_mask_bytes(rows, cols, c.csz) > kMaskBytes)
+    return fail(CDX_ERR_ARG, "cdx_phase_retrieve_static: %dx%d support mask exceeds the static workspace", rows, cols);
   c.ring = nullptr;
   c.ring_cap = 0;
   c.rcolor = 65535;

@@ -70,6 +70,9 @@ struct UpdArgs {
   const double* scal;  // scal[0] = support threshold
   void* gamma0;        // PRTF reference object P_S(g') (ROW_INV_UPDATE_FWD_PRTF)
   int prtf;            // 1: store gamma0, sum gamma0^2; 2: sum P_S(g') conj(gamma0)
+  // support test S > threshold of the single-frame update pass, one word per
+  // (row, lane): bit i is the lane's i-th pixel (launch_support_mask)
+  const unsigned* smask;
 };
 
 // Deterministic reductions: every workgroup of the producing kernel stores
@@ -99,6 +102,15 @@ int tiling_default(int n, size_t csz, int column_pass);
 template <class T>
 hipError_t launch_col(int op, int rows, int cols, const void* in, void* out, const void* tw,
                       const T* I, const ProjArgs<T>& pa, double gauss_sigma, hipStream_t st, int lines = 0);
+// The support mask of the single-frame update pass (ROW_INV_UPDATE_FWD):
+// word row * lanes + t holds, in bit i, spt[row][x_i] > (T)scal[0] for the
+// i-th pixel x_i the row pass gives lane t (its E <= 32 pixels of the row).
+// Rebuilt whenever the support or the threshold changes; the test is the
+// one the pass made on the float support, so the iterates are unchanged.
+template <class T>
+hipError_t launch_support_mask(int rows, int cols, const T* spt, const double* scal, unsigned* mask, hipStream_t st);
+// bytes of that mask for a rows x cols frame of element size csz (0: unsupported length)
+size_t support_mask_bytes(int rows, int cols, size_t csz);
 template <class T>
 hipError_t launch_row(int op, int rows, int cols, const void* in, void* out, const void* tw,
                       void* g, const T* spt, const UpdArgs<T>& ua, T scale, T* sout,

@@ -231,7 +231,8 @@ __device__ __forceinline__ void col_body(const cplx<T>* __restrict__ in, cplx<T>
 // rule of ROW_INV_UPDATE_FWD (enum Algorithm).  TILED: the update ops store
 // T1 in its column-tile layout.
 // ---------------------------------------------------------------------------
-template <class T, int N, int OP, int ALGO, bool TILED = false, int V = 0, bool T2L = false, bool GPLAIN = false>
+template <class T, int N, int OP, int ALGO, bool TILED = false, int V = 0, bool T2L = false, bool GPLAIN = false,
+          bool SMASK = false>
 __device__ __forceinline__ void row_body(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out, int rows,
                                          const cplx<T>* __restrict__ tw, cplx<T>* __restrict__ g,
                                          const T* __restrict__ spt, const UpdArgs<T>& ua, T scale,
@@ -266,7 +267,11 @@ __device__ __forceinline__ void row_body(const cplx<T>* __restrict__ in, cplx<T>
     return;
   } else {
     cplx<T> gv[upd ? E : 1];
-    T sv[upd ? E : 1];
+    // SMASK: the support test comes as one word of bits per lane instead of
+    // E floats (ua.smask, built by launch_support_mask)
+    static_assert(!SMASK || E <= 32, "one mask word per lane");
+    T sv[upd && !SMASK ? E : 1];
+    unsigned mw = 0;
     cplx<T>* gamma0 = (cplx<T>*)ua.gamma0;
     // Plain loads of g and S with streaming loads of T2, the fastest
     // combination for 1024-point rows (per iteration at 1024^2: 20.1 us with
@@ -280,9 +285,12 @@ __device__ __forceinline__ void row_body(const cplx<T>* __restrict__ in, cplx<T>
         v[i] = ok ? load_stream(&in[T2L ? t2_index(row, L::dout_pos(i, t), N) : base + L::dout_pos(i, t)])
                   : cplx<T>{(T)0, (T)0};
 #pragma unroll
-      for (int i = 0; i < E; i++) {
-        gv[i] = ok ? g[base + L::din_pos(i, t)] : cplx<T>{(T)0, (T)0};
-        sv[i] = ok ? spt[base + L::din_pos(i, t)] : (T)0;
+      for (int i = 0; i < E; i++) gv[i] = ok ? g[base + L::din_pos(i, t)] : cplx<T>{(T)0, (T)0};
+      if constexpr (SMASK) {
+        mw = ok ? ua.smask[(size_t)row * TL + t] : 0u;
+      } else {
+#pragma unroll
+        for (int i = 0; i < E; i++) sv[i] = ok ? spt[base + L::din_pos(i, t)] : (T)0;
       }
     } else {
 #pragma unroll
@@ -290,9 +298,12 @@ __device__ __forceinline__ void row_body(const cplx<T>* __restrict__ in, cplx<T>
         v[i] = ok ? in[T2L ? t2_index(row, L::dout_pos(i, t), N) : base + L::dout_pos(i, t)] : cplx<T>{(T)0, (T)0};
       if constexpr (upd) {
 #pragma unroll
-        for (int i = 0; i < E; i++) {
-          gv[i] = ok ? g[base + L::din_pos(i, t)] : cplx<T>{(T)0, (T)0};
-          sv[i] = ok ? spt[base + L::din_pos(i, t)] : (T)0;
+        for (int i = 0; i < E; i++) gv[i] = ok ? g[base + L::din_pos(i, t)] : cplx<T>{(T)0, (T)0};
+        if constexpr (SMASK) {
+          mw = ok ? ua.smask[(size_t)row * TL + t] : 0u;
+        } else {
+#pragma unroll
+          for (int i = 0; i < E; i++) sv[i] = ok ? spt[base + L::din_pos(i, t)] : (T)0;
         }
       }
     }
@@ -330,7 +341,9 @@ __device__ __forceinline__ void row_body(const cplx<T>* __restrict__ in, cplx<T>
 #pragma unroll
       for (int i = 0; i < E; i++) {
         const cplx<T> p = {v[i].x * ua.invN, v[i].y * ua.invN};
-        const bool in = sv[i] > thr;
+        bool in;
+        if constexpr (SMASK) in = (mw >> i) & 1u;
+        else in = sv[i] > thr;
         if constexpr (prtf) {
           if (ua.prtf == 1) {  // gamma0 = P_S(g'), sum gamma0^2
             const cplx<T> q = in ? p : cplx<T>{(T)0, (T)0};

@@ -67,6 +67,10 @@ static size_t wg_lds_bytes(int n, size_t csz, int lines) {
 #ifdef CDX_PASS_HOST
 int line_supported(int n) { return line_lanes(n, 8) != 0; }
 
+size_t support_mask_bytes(int rows, int cols, size_t csz) {
+  return (size_t)rows * (size_t)line_lanes(cols, csz) * sizeof(unsigned);
+}
+
 int tiling_valid(int n, size_t csz, int lines) {
   const int tl = line_lanes(n, csz);
   if (tl == 0 || lines <= 0 || (lines & (lines - 1))) return 0;
@@ -114,8 +118,29 @@ k_row(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out, int rows, const
   // two 64-byte halves of a 128-byte T1 tile line leave from the same L2
   // (1024^2: 54.8k -> 55.3k iterations/s; at 2048^2 it cost 3.5 us, so not there)
   constexpr bool xcd_rows = tiled && N == 1024;
-  row_body<T, N, OP, ALGO, tiled, 0, t2l, tiled && N <= 1024>(in, out, rows, tw, g, spt, ua, scale, sout, rs,
-                                          xcd_rows ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x, lsh, smem);
+  // the loop's update pass tests the support from its bit mask
+  constexpr bool smask = OP == ROW_INV_UPDATE_FWD;
+  row_body<T, N, OP, ALGO, tiled, 0, t2l, tiled && N <= 1024, smask>(
+      in, out, rows, tw, g, spt, ua, scale, sout, rs, xcd_rows ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x,
+      lsh, smem);
+}
+// support mask of the update pass: word w = row * TL + t, bit i =
+// spt[row][din_pos(i, t)] > (T)scal[0], the comparison row_body makes
+template <class T, int N>
+__global__ void __launch_bounds__(256) k_support_mask(int rows, const T* __restrict__ spt,
+                                                      const double* __restrict__ scal, unsigned* __restrict__ mask) {
+  using L = LineOf<T, N, 0>;
+  constexpr int TL = LineCfgT<T, N, 0>::TL;
+  static_assert(L::E <= 32, "one mask word per lane");
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= rows * TL) return;
+  const int row = w / TL, t = w % TL;
+  const T thr = (T)scal[0];
+  const T* s = spt + (size_t)row * N;
+  unsigned m = 0;
+#pragma unroll
+  for (int i = 0; i < L::E; i++) m |= (s[L::din_pos(i, t)] > thr ? 1u : 0u) << i;
+  mask[w] = m;
 }
 
 // ---------------------------------------------------------------------------
@@ -312,6 +337,22 @@ hipError_t prepare_row(int cols) {
     }
   return hipSuccess;
 }
+template <class T>
+hipError_t launch_support_mask(int rows, int cols, const T* spt, const double* scal, unsigned* mask, hipStream_t st) {
+  switch (cols) {
+#define F(n_)                                                                                                    \
+  case n_: {                                                                                                     \
+    const int words = rows * LineCfgT<T, n_>::TL;                                                               \
+    hipLaunchKernelGGL((k_support_mask<T, n_>), dim3((words + 255) / 256), dim3(256), 0, st, rows, spt, scal, mask); \
+    return hipGetLastError();                                                                                    \
+  }
+    CDX_SIZES(F)
+#undef F
+    default: return hipErrorInvalidValue;
+  }
+}
+template hipError_t launch_support_mask<CDX_PASS_T>(int, int, const CDX_PASS_T*, const double*, unsigned*,
+                                                    hipStream_t);
 template hipError_t launch_row<CDX_PASS_T>(int, int, int, const void*, void*, const void*, void*,
                                            const CDX_PASS_T*, const UpdArgs<CDX_PASS_T>&, CDX_PASS_T, CDX_PASS_T*,
                                            const ReduceSlots&, hipStream_t, int);

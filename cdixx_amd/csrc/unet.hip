@@ -291,10 +291,194 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
     }
   }
 }
+// 3x3 convolution on a block of TH x TW pixels of one image (TM = TH TW =
+// 32 WM BM pixels) x TN = 32 WN BN output channels, for Cin a multiple of 32
+// and sides divisible by the block.  The input channels go in chunks of 32:
+// per chunk the (TH + 2) x (TW + 2) halo of the block (zero outside the
+// image) and the chunk's 9 x 32 weight columns of the TN channels are staged
+// in LDS once, and the 9 taps x 2 MFMA k-steps read shifted windows of the
+// halo at compile-time offsets (k_conv instead gathers every tap's im2col
+// rows from global memory, one K step of 32 at a time).  The next chunk's
+// pieces are loaded into registers during the current chunk's MFMAs.  LDS
+// pitches: 40 bf16 per halo pixel and 9 x 32 + 8 per weight row, so the 32
+// rows of an MFMA operand read spread over all banks.  Same epilogue and
+// arguments as k_conv (taps = 9, Kp = 9 Cin).
+template <int WM, int WN, int BM, int BN, int TW>
+__global__ void __launch_bounds__(256) k_conv3_halo(ConvArgs a) {
+  constexpr int TM = 32 * WM * BM, TN = 32 * WN * BN, TH = TM / TW;
+  constexpr int HW2 = TW + 2, HPIX = (TH + 2) * HW2;  // halo pixels
+  constexpr int HP = 40, WP = 9 * 32 + 8;            // LDS pitches in bf16
+  constexpr int HPC = HPIX * 4, WPC = TN * 36;        // 16-byte pieces per chunk
+  constexpr int HV = (HPC + 255) / 256, WV = (WPC + 255) / 256;
+  static_assert(WM * WN == 4 && TM % TW == 0, "four waves, whole pixel rows");
+  constexpr size_t kOps = ((size_t)HPIX * HP + (size_t)TN * WP) * sizeof(u16),
+                   kStage = (size_t)4 * 32 * 33 * sizeof(float);
+  __shared__ __align__(16) unsigned char smem[kOps > kStage ? kOps : kStage];
+  u16(*Hs)[HP] = reinterpret_cast<u16(*)[HP]>(smem);
+  u16(*Ws)[WP] = reinterpret_cast<u16(*)[WP]>(smem + (size_t)HPIX * HP * sizeof(u16));
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave / WN, wn = wave % WN;
+  const int nwb = a.W / TW, nhb = a.H / TH;
+  const int bx = blockIdx.x, wb = bx % nwb, hb = (bx / nwb) % nhb, b = bx / (nwb * nhb);
+  const int h0 = hb * TH, w0 = wb * TW, n0 = blockIdx.y * TN;
+
+  uint4 rh[HV], rw[WV];
+  auto load = [&](int ci0) {
+#pragma unroll
+    for (int j = 0; j < HV; j++) {
+      const int q = tid + 256 * j;
+      rh[j] = make_uint4(0, 0, 0, 0);
+      if (q < HPC) {
+        const int hp = q >> 2, part = q & 3;
+        const int hh = h0 + hp / HW2 - 1, ww = w0 + hp % HW2 - 1;
+        if (hh >= 0 && hh < a.H && ww >= 0 && ww < a.W)
+          rh[j] = *(const uint4*)(a.x + ((size_t)(b * a.H + hh) * a.W + ww) * a.Cin + ci0 + part * 8);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < WV; j++) {
+      const int q = tid + 256 * j;
+      rw[j] = make_uint4(0, 0, 0, 0);
+      if (q < WPC) {
+        const int co = q / 36, rem = q - co * 36, tap = rem >> 2, part = rem & 3;
+        if (n0 + co < a.Cout)
+          rw[j] = *(const uint4*)(a.w + (size_t)(n0 + co) * a.Kp + tap * a.Cin + ci0 + part * 8);
+      }
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int j = 0; j < HV; j++) {
+      const int q = tid + 256 * j;
+      if (q < HPC) *(uint4*)&Hs[q >> 2][(q & 3) * 8] = rh[j];
+    }
+#pragma unroll
+    for (int j = 0; j < WV; j++) {
+      const int q = tid + 256 * j;
+      if (q < WPC) {
+        const int co = q / 36, rem = q - co * 36;
+        *(uint4*)&Ws[co][(rem >> 2) * 32 + (rem & 3) * 8] = rw[j];
+      }
+    }
+  };
+  // halo pixel of this lane's pixel row in block i (tap (0, 0))
+  int hbase[BM];
+#pragma unroll
+  for (int i = 0; i < BM; i++) {
+    const int q = (wm * BM + i) * 32 + (lane & 31);
+    hbase[i] = (q / TW) * HW2 + q % TW;
+  }
+  const int kl = (lane >> 5) * 8;
+  f32x16 acc[BM][BN];
+#pragma unroll
+  for (int i = 0; i < BM; i++)
+#pragma unroll
+    for (int j = 0; j < BN; j++) acc[i][j] = f32x16{};
+  load(0);
+  store();
+  __syncthreads();
+  for (int ci0 = 0; ci0 < a.Cin; ci0 += 32) {
+    const bool more = ci0 + 32 < a.Cin;
+    if (more) load(ci0 + 32);
+#pragma unroll
+    for (int tap = 0; tap < 9; tap++) {
+      const int toff = (tap / 3) * HW2 + tap % 3;
+#pragma unroll
+      for (int k16 = 0; k16 < 2; k16++) {
+        const int kk = k16 * 16 + kl;
+        bf16x8 av[BM], bv[BN];
+#pragma unroll
+        for (int i = 0; i < BM; i++) av[i] = *(const bf16x8*)&Hs[hbase[i] + toff][kk];
+#pragma unroll
+        for (int j = 0; j < BN; j++) bv[j] = *(const bf16x8*)&Ws[(wn * BN + j) * 32 + (lane & 31)][tap * 32 + kk];
+#pragma unroll
+        for (int i = 0; i < BM; i++)
+#pragma unroll
+          for (int j = 0; j < BN; j++)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[i], bv[j], acc[i][j], 0, 0, 0);
+      }
+    }
+    if (more) {
+      __syncthreads();  // this chunk's operands are read
+      store();
+      __syncthreads();
+    }
+  }
+  // epilogue as k_conv's, with the block's pixel rows mapped to the image
+  float* stg = reinterpret_cast<float*>(smem) + wave * 32 * 33;
+  const int srow = lane >> 1, sc0 = (lane & 1) * 16;
+#pragma unroll
+  for (int i = 0; i < BM; i++) {
+#pragma unroll
+    for (int j = 0; j < BN; j++) {
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < 16; r++) stg[acc_row(r, lane) * 33 + (lane & 31)] = acc[i][j][r];
+      __syncthreads();
+      const int q = (wm * BM + i) * 32 + srow;
+      const int mm = (b * a.H + h0 + q / TW) * a.W + w0 + q % TW;
+#pragma unroll
+      for (int sg = 0; sg < 2; sg++) {
+        const int c = sc0 + 8 * sg, coo = n0 + (wn * BN + j) * 32 + c;
+        if (coo >= a.Cout) continue;
+        float v[8];
+#pragma unroll
+        for (int e = 0; e < 8; e++) v[e] = stg[srow * 33 + c + e] + (a.bias ? a.bias[coo + e] : 0.f);
+        if (a.relu)
+#pragma unroll
+          for (int e = 0; e < 8; e++) v[e] = fmaxf(v[e], 0.f);
+        const size_t o = (size_t)mm * a.Cout + coo;
+        if (a.mask) relu_mask8(v, *(const uint4*)(a.mask + o));
+        if (a.y) *(uint4*)(a.y + o) = pack8(v);
+        if (a.yf) {
+          *(float4*)(a.yf + o) = make_float4(v[0], v[1], v[2], v[3]);
+          *(float4*)(a.yf + o + 4) = make_float4(v[4], v[5], v[6], v[7]);
+        }
+      }
+    }
+  }
+}
+
+// The halo kernel's block for a layer, or false when it does not apply
+// (1x1 taps, Cin not a multiple of 32, sides not divisible by the block):
+// TN = 64 when the channels divide by 64, else 32; the largest block of
+// TM = 256 / 128 / 64 pixels (16 wide) that still gives about 1024
+// workgroups.
+bool launch_conv_halo(const ConvArgs& a, int cout, hipStream_t st) {
+  if (a.taps != 9 || a.Cin % 32 || a.Kp != 9 * a.Cin || a.W % 16 || cout % 32) return false;
+  const long long px = (long long)a.B * a.H * a.W;
+  const dim3 blk(256);
+  auto blocks = [&](int tm) { return (unsigned)(px / tm); };
+  if (cout % 64 == 0) {
+    const unsigned ny = (unsigned)(cout / 64);
+    if (a.H % 8 == 0 && blocks(128) * ny >= 1024) {
+      hipLaunchKernelGGL((k_conv3_halo<2, 2, 2, 1, 16>), dim3(blocks(128), ny), blk, 0, st, a);
+      return true;
+    }
+    if (a.H % 4 == 0) {
+      hipLaunchKernelGGL((k_conv3_halo<2, 2, 1, 1, 16>), dim3(blocks(64), ny), blk, 0, st, a);
+      return true;
+    }
+    return false;
+  }
+  const unsigned ny = (unsigned)(cout / 32);
+  if (a.H % 16 == 0 && blocks(256) * ny >= 1024) {
+    hipLaunchKernelGGL((k_conv3_halo<4, 1, 2, 1, 16>), dim3(blocks(256), ny), blk, 0, st, a);
+    return true;
+  }
+  if (a.H % 8 == 0) {
+    hipLaunchKernelGGL((k_conv3_halo<4, 1, 1, 1, 16>), dim3(blocks(128), ny), blk, 0, st, a);
+    return true;
+  }
+  return false;
+}
+
 // The conv tile for `cout` output channels: the largest of 256 x 32 / 128 x 32
 // (at most 32 channels: no idle MFMA columns), 128 x 128, 128 x 64 and 64 x 64
-// that still gives about 1024 workgroups (four per CU).
+// that still gives about 1024 workgroups (four per CU).  3x3 layers with
+// 32-channel chunks run the halo kernel.
 void launch_conv(const ConvArgs& a, int cout, hipStream_t st) {
+  if (launch_conv_halo(a, cout, st)) return;
   const long long M = (long long)a.B * a.H * a.W;
   auto wgs = [&](int tm, int tn) { return ((M + tm - 1) / tm) * ((cout + tn - 1) / tn); };
   const dim3 blk(256);

@@ -669,6 +669,172 @@ __global__ void __launch_bounds__(256) k_wgrad(WgradArgs a) {
   }
 }
 
+// Weight gradient of a 3x3 layer by pixel blocks with the input halo:
+// workgroup (x, y, z) computes dw for output channels [32x, 32x + 32), all 9
+// taps and input channels [32y, 32y + 32) over split z of the pixel blocks
+// (TH = 4 RW rows x 16 columns of one image).  Per block, dz [pixels][32
+// channels] and the (TH + 2) x 18 halo of the input [pixels][32 channels]
+// go to LDS as loaded (16-byte pieces), and wave w takes rows RW w ..
+// RW w + RW - 1 of the block, 16 pixels per MFMA k-step: its dz operand (32
+// channels x 16 pixels) is read once and each tap's input operand (16
+// pixels x 32 channels) is the halo window shifted by the tap, both with LDS
+// transpose reads (ds_read_b64_tr_b16: within each group of 16 lanes, lane i
+// receives element i % 4 of the 8 bytes addressed by lane 4e + i / 4 as its
+// element e, so lanes 4e' + q addressing pixel k0 + e', channels c + 4q ..
+// c + 4q + 3 give lane i pixels k0 .. k0 + 3 of channel c + i; two such
+// reads make one operand of v_mfma_f32_32x32x16_bf16).  So dz is read once
+// per input-channel chunk and the input once per output-channel tile (plus
+// the halo), where k_wgrad re-reads dz for every n-tile and the input for
+// every tap.  Nine 32 x 32 accumulators per wave (the waves' sums over their
+// rows added in wave order at the end); the next block's pieces load during
+// the current block's MFMAs.  Bias gradient (chunk 0 only):
+// per-thread sums of the dz pieces, then a fixed-order sum over the threads
+// of each 8-channel group.  Partials as k_wgrad with tn = 1.  Requires Cin
+// and Cout multiples of 32, H % TH == 0, W % 16 == 0.
+template <int RW>
+__global__ void __launch_bounds__(256) k_wgrad_halo(WgradArgs a) {
+  constexpr int TH = 4 * RW, HPIX = (TH + 2) * 18, PIX = TH * 16;
+  constexpr int ND = PIX * 4, NH = HPIX * 4;  // 16-byte pieces per block
+  constexpr int JD = (ND + 255) / 256, JH = (NH + 255) / 256;
+  // dz [pixel][channel] and the input halo [pixel][channel], two buffers
+  // each; after the pixel loop the same LDS adds the waves' accumulators
+  constexpr size_t kOps = (size_t)2 * (PIX + HPIX) * 32 * sizeof(u16), kRed = (size_t)4 * 1024 * sizeof(float);
+  __shared__ __align__(16) unsigned char smem[kOps > kRed ? kOps : kRed];
+  u16(*Ds)[PIX][32] = reinterpret_cast<u16(*)[PIX][32]>(smem);
+  u16(*Xs)[HPIX][32] = reinterpret_cast<u16(*)[HPIX][32]>(smem + (size_t)2 * PIX * 32 * sizeof(u16));
+  __shared__ float bpart[256][9];
+  typedef short v4s __attribute__((ext_vector_type(4)));
+  typedef v4s __attribute__((address_space(3))) * lds4;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int co0 = blockIdx.x * 32, ci0 = blockIdx.y * 32;
+  const int K = 9 * a.Cin;
+  const int nwb = a.W / 16, nhb = a.H / TH, nblk = a.B * nhb * nwb;
+  const int b_lo = blockIdx.z * a.chunk, b_hi = min(nblk, b_lo + a.chunk);
+  const bool bias = blockIdx.y == 0;
+  uint4 rd[JD], rx[JH];
+  float bs[8];
+#pragma unroll
+  for (int e = 0; e < 8; e++) bs[e] = 0.f;
+  auto load = [&](int blk) {
+    const int wb = blk % nwb, hb = (blk / nwb) % nhb, bb = blk / (nwb * nhb);
+    const int h0 = hb * TH, w0 = wb * 16;
+#pragma unroll
+    for (int j = 0; j < JD; j++) {
+      const int q = tid + 256 * j, px = q >> 2;
+      rd[j] = make_uint4(0, 0, 0, 0);
+      if (q < ND) {
+        const size_t m = (size_t)(bb * a.H + h0 + px / 16) * a.W + w0 + px % 16;
+        rd[j] = *(const uint4*)(a.dz + m * a.Cout + co0 + 8 * (q & 3));
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < JH; j++) {
+      const int q = tid + 256 * j, hp = q >> 2;
+      rx[j] = make_uint4(0, 0, 0, 0);
+      if (q < NH) {
+        const int hh = h0 + hp / 18 - 1, ww = w0 + hp % 18 - 1;
+        if (hh >= 0 && hh < a.H && ww >= 0 && ww < a.W)
+          rx[j] = *(const uint4*)(a.x + ((size_t)(bb * a.H + hh) * a.W + ww) * a.Cin + ci0 + 8 * (q & 3));
+      }
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int j = 0; j < JD; j++) {
+      const int q = tid + 256 * j;
+      if (q < ND) {
+        *(uint4*)&Ds[buf][q >> 2][8 * (q & 3)] = rd[j];
+        if (bias) {
+          float f[8];
+          unpack8(rd[j], f);
+#pragma unroll
+          for (int e = 0; e < 8; e++) bs[e] += f[e];
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < JH; j++) {
+      const int q = tid + 256 * j;
+      if (q < NH) *(uint4*)&Xs[buf][q >> 2][8 * (q & 3)] = rx[j];
+    }
+  };
+  // transpose-read rows and columns of this lane: pixel offset within a
+  // 16-pixel k-step and channel, for operand half h = 0 (h = 1: +4 pixels)
+  const int g = lane >> 4, s = lane & 15;
+  const int tk = 8 * (g >> 1) + (s >> 2), tc = 16 * (g & 1) + 4 * (s & 3);
+  f32x16 acc[9];
+#pragma unroll
+  for (int t = 0; t < 9; t++) acc[t] = f32x16{};
+  if (b_lo < b_hi) {
+    load(b_lo);
+    store(0);
+  }
+  __syncthreads();
+  int buf = 0;
+  for (int blk = b_lo; blk < b_hi; blk++) {
+    const bool more = blk + 1 < b_hi;
+    if (more) load(blk + 1);
+#pragma unroll
+    for (int rr = 0; rr < RW; rr++) {
+      const int row = RW * wave + rr;  // block row of this k-step
+      bf16x8 av;
+      {
+        const v4s lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds4)&Ds[buf][16 * row + tk][tc]);
+        const v4s hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds4)&Ds[buf][16 * row + tk + 4][tc]);
+        const short tmp[8] = {lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+        __builtin_memcpy(&av, tmp, 16);
+      }
+#pragma unroll
+      for (int t = 0; t < 9; t++) {
+        const int hp = (row + t / 3) * 18 + t % 3 + tk;  // halo pixel of k = tk, tap t
+        const v4s lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds4)&Xs[buf][hp][tc]);
+        const v4s hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds4)&Xs[buf][hp + 4][tc]);
+        const short tmp[8] = {lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+        bf16x8 bv;
+        __builtin_memcpy(&bv, tmp, 16);
+        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc[t], 0, 0, 0);
+      }
+    }
+    if (more) {
+      __syncthreads();  // this block's operands are read
+      store(buf ^ 1);
+    }
+    __syncthreads();
+    buf ^= 1;
+  }
+  const size_t nout = (size_t)a.Cout * (K + 1);
+  float* part = a.part + (size_t)blockIdx.z * nout;
+  if (bias) {
+#pragma unroll
+    for (int e = 0; e < 8; e++) bpart[tid][e] = bs[e];
+    __syncthreads();
+    if (tid < 32) {
+      // threads of channel group tid / 8 hold pieces q = tid' with q % 4 == tid / 8
+      float sum = 0.f;
+      for (int t2 = tid >> 3; t2 < 256; t2 += 4) sum += bpart[t2][tid & 7];
+      part[(size_t)a.Cout * K + co0 + tid] = sum;
+    }
+  }
+  // The four waves hold the sums over their own block rows: per tap, their
+  // 32 x 32 blocks (acc[t] element r: output channel acc_row(r, lane), input
+  // channel lane % 32) are added in wave order through LDS, then stored
+  // channel-row by channel-row
+  float* red = reinterpret_cast<float*>(smem);
+#pragma unroll
+  for (int t = 0; t < 9; t++) {
+    __syncthreads();  // the pixel loop's (or the previous tap's) LDS reads are done
+#pragma unroll
+    for (int r2 = 0; r2 < 16; r2++) red[wave * 1024 + acc_row(r2, lane) * 32 + (lane & 31)] = acc[t][r2];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int e = tid + 256 * k, row = e >> 5, col = e & 31;
+      const float v = ((red[e] + red[1024 + e]) + red[2048 + e]) + red[3072 + e];
+      part[(size_t)(co0 + row) * K + t * a.Cin + ci0 + col] = v;
+    }
+  }
+}
+
 // The splits' partials are added in two fixed-order stages.  Stage 1:
 // workgroup (x, g) adds splits [g*zc, (g+1)*zc) of 64 consecutive partial
 // values (4 lanes of 64 threads, each lane every 4th split, then the lanes in
@@ -1032,21 +1198,30 @@ int conv_wgrad(cdx_unet* u, const UConv& L, int B, int h, const u16* dz, const u
   if (h % 4 || L.cin % 8 || L.cout % 8)
     return ufail(CDX_ERR_ARG, "unet: weight gradient needs sides divisible by 4 and channels by 8");
   const int M = B * h * h, K = L.taps * L.cin;
-  const bool narrow = L.cout <= 32;  // 32 x 128 tiles with 48-pixel steps, else 64 x 64 with 64
-  const int TC = narrow ? 32 : 64, TN = narrow ? 128 : 64, BK = narrow ? 48 : 64;
-  const int tm = (L.cout + TC - 1) / TC, tn = (K + TN - 1) / TN;
+  // 3x3 layers with 32-channel chunks: the halo kernel (blocks of 4 x 16
+  // pixels), each workgroup all 9 taps of a 32 x 32 channel pair
+  const bool halo = L.taps == 9 && L.cin % 32 == 0 && L.cout % 32 == 0 && h % 16 == 0;
+  // otherwise tiles of 32 x 128 with 48-pixel steps for at most 32 channels,
+  // else 64 x 64 with 64
+  const bool narrow = L.cout <= 32;
+  const int TC = halo ? 32 : narrow ? 32 : 64, TN = halo ? 0 : narrow ? 128 : 64, BK = narrow ? 48 : 64;
+  const int tm = (L.cout + TC - 1) / TC, tn = halo ? 1 : (K + TN - 1) / TN;
+  const int ty = halo ? L.cin / 32 : tn;              // grid y: input-channel chunks or n-tiles
+  const int units = halo ? B * (h / 4) * (h / 16) : M;  // what the splits share: pixel blocks or pixels
+  const int64_t nout = (int64_t)L.cout * (K + tn);
   // about 1024 workgroups, at least 256 pixels each, within the workspace
   // (the splits' partials and the reduction's first stage, 1/16 of them)
-  const int64_t nout = (int64_t)L.cout * (K + tn);
-  int nsplit = 1024 / (tm * tn);
-  nsplit = std::max(1, std::min(nsplit, M / 256));
+  int nsplit = 1024 / (tm * ty);
+  nsplit = std::max(1, std::min(nsplit, halo ? units / 2 : M / 256));
   nsplit = (int)std::min<int64_t>(nsplit, kWgradPartFloats / (nout + nout / 16 + 1) - 1);
   if (nsplit < 1) return ufail(CDX_ERR_ARG, "unet: layer too large for the weight-gradient workspace");
-  int chunk = (M + nsplit - 1) / nsplit;
-  chunk = (chunk + BK - 1) / BK * BK;
-  nsplit = (M + chunk - 1) / chunk;
+  int chunk = (units + nsplit - 1) / nsplit;
+  if (!halo) chunk = (chunk + BK - 1) / BK * BK;
+  nsplit = (units + chunk - 1) / chunk;
   WgradArgs a{dz, x, u->wg_part, B, h, h, L.cin, L.cout, L.taps, chunk};
-  if (narrow)
+  if (halo)
+    hipLaunchKernelGGL((k_wgrad_halo<1>), dim3(tm, ty, nsplit), dim3(256), 0, u->st, a);
+  else if (narrow)
     hipLaunchKernelGGL((k_wgrad<1, 4, 48>), dim3(tm, tn, nsplit), dim3(256), 0, u->st, a);
   else
     hipLaunchKernelGGL((k_wgrad<2, 2, 64>), dim3(tm, tn, nsplit), dim3(256), 0, u->st, a);

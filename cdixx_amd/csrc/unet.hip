@@ -692,7 +692,7 @@ __global__ void __launch_bounds__(256) k_wgrad(WgradArgs a) {
 // of each 8-channel group.  Partials as k_wgrad with tn = 1.  Requires Cin
 // and Cout multiples of 32, H % TH == 0, W % 16 == 0.
 template <int RW>
-__global__ void __launch_bounds__(256) k_wgrad_halo(WgradArgs a) {
+__global__ void __launch_bounds__(256, 2) k_wgrad_halo(WgradArgs a) {
   constexpr int TH = 4 * RW, HPIX = (TH + 2) * 18, PIX = TH * 16;
   constexpr int ND = PIX * 4, NH = HPIX * 4;  // 16-byte pieces per block
   constexpr int JD = (ND + 255) / 256, JH = (NH + 255) / 256;
@@ -1209,9 +1209,11 @@ int conv_wgrad(cdx_unet* u, const UConv& L, int B, int h, const u16* dz, const u
   const int ty = halo ? L.cin / 32 : tn;              // grid y: input-channel chunks or n-tiles
   const int units = halo ? B * (h / 4) * (h / 16) : M;  // what the splits share: pixel blocks or pixels
   const int64_t nout = (int64_t)L.cout * (K + tn);
-  // about 1024 workgroups, at least 256 pixels each, within the workspace
-  // (the splits' partials and the reduction's first stage, 1/16 of them)
-  int nsplit = 1024 / (tm * ty);
+  // about 1024 workgroups (512 for the halo kernel: two per CU, each with
+  // nine partial tiles to write), at least 256 pixels each, within the
+  // workspace (the splits' partials and the reduction's first stage, 1/16 of
+  // them)
+  int nsplit = (halo ? 512 : 1024) / (tm * ty);
   nsplit = std::max(1, std::min(nsplit, halo ? units / 2 : M / 256));
   nsplit = (int)std::min<int64_t>(nsplit, kWgradPartFloats / (nout + nout / 16 + 1) - 1);
   if (nsplit < 1) return ufail(CDX_ERR_ARG, "unet: layer too large for the weight-gradient workspace");

@@ -845,14 +845,15 @@ static int time_t_(cdx_ctx* c, const cdx_params* p, int algo, void* F, void* g, 
   if (ms_total) *ms_total = ms;
   res.drop_graph();
   if (col_ms || row_ms) {
-    // average launch duration of each pass: up to 200 launches of that pass
-    // alone, back to back (one captured graph when use_graph, as the loop
-    // runs), between two events; the median of three such runs.  Every
+    // average launch duration of each pass: 200 launches of that pass
+    // alone (whatever the number of timed iterations: the first launches of
+    // a graph run slower), back to back (one captured graph when use_graph,
+    // as the loop runs), between two events; the median of three such runs.  Every
     // column-pass launch reads the same T1 and every row-pass launch the same
     // T2, so each moves the bytes of its pass in the loop.  The row-pass
     // launches re-apply the update to g, so the state afterwards is no
     // iterate of the loop.
-    n = n_pass > 0 ? n_pass : (n < 200 ? n : 200);
+    n = n_pass > 0 ? n_pass : 200;
     UpdArgs<T> ua = upd_args<T>(c, algo, p->beta, p->relax);
     L.pa.rf = rf;
     for (int pass = 0; pass < 2; pass++) {

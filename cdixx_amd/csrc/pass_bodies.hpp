@@ -136,23 +136,27 @@ __device__ __forceinline__ void col_body(const cplx<T>* __restrict__ in, cplx<T>
   cplx<T> v[E];
   T iv[proj ? E : 1];
 
+  // Lanes past the last column load the last column (their results are
+  // dropped), so the loads carry no branches: behind a branch the compiler
+  // waits for every load issued so far (I included) before the first
+  // transform.
+  const int lc = ok ? col : cols - 1;
   if constexpr (from_f) {
 #pragma unroll
-    for (int i = 0; i < E; i++) v[i] = ok ? in[(size_t)L::dout_pos(i, t) * cols + col] : cplx<T>{(T)0, (T)0};
+    for (int i = 0; i < E; i++) v[i] = in[(size_t)L::dout_pos(i, t) * cols + lc];
   } else if constexpr (TILED) {
     // T1 is read once per iteration: streaming loads (measured faster than
     // plain ones here)
 #pragma unroll
-    for (int i = 0; i < E; i++)
-      v[i] = ok ? load_stream(&in[t1_index(L::din_pos(i, t), col, N)]) : cplx<T>{(T)0, (T)0};
+    for (int i = 0; i < E; i++) v[i] = load_stream(&in[t1_index(L::din_pos(i, t), lc, N)]);
   } else {
 #pragma unroll
-    for (int i = 0; i < E; i++) v[i] = ok ? in[(size_t)L::din_pos(i, t) * cols + col] : cplx<T>{(T)0, (T)0};
+    for (int i = 0; i < E; i++) v[i] = in[(size_t)L::din_pos(i, t) * cols + lc];
   }
   if constexpr (proj) {
 #pragma unroll
     for (int i = 0; i < E; i++)
-      iv[i] = ok ? I[TILED_I ? t1_index(L::dout_pos(i, t), col, N) : (size_t)L::dout_pos(i, t) * cols + col] : (T)NAN;
+      iv[i] = I[TILED_I ? t1_index(L::dout_pos(i, t), lc, N) : (size_t)L::dout_pos(i, t) * cols + lc];
   }
   // table twiddles of both transforms, loaded behind the data
   typename L::TwF twf;
@@ -177,7 +181,7 @@ __device__ __forceinline__ void col_body(const cplx<T>* __restrict__ in, cplx<T>
     const bool amp = packed_amplitude<MODE>(pa.mode);
 #pragma unroll
     for (int i = 0; i < E; i++) {
-      if (rf && !isnan(iv[i])) {
+      if (rf && ok && !isnan(iv[i])) {
         const T s = amp ? iv[i] : (iv[i] > (T)0 ? dsqrt(iv[i]) : (T)0);
         rnum += fabs(dsqrt(v[i].x * v[i].x + v[i].y * v[i].y) - s);
         rden += s;
@@ -251,13 +255,17 @@ __device__ __forceinline__ void row_body(const cplx<T>* __restrict__ in, cplx<T>
   const bool ok = row < rows;
   cplx<T>* my = lds + rr * pitch;
   const size_t base = (size_t)row * N;
+  // loads of lanes past the last row read the last row (results dropped):
+  // branch-free loads, as in the column pass
+  const int lrow = ok ? row : rows - 1;
+  const size_t lbase = (size_t)lrow * N;
   constexpr bool prtf = OP == ROW_INV_UPDATE_FWD_PRTF;
   constexpr bool upd = OP == ROW_INV_UPDATE_FWD || prtf;
   cplx<T> v[E];
 
   if constexpr (OP == ROW_FWD) {
 #pragma unroll
-    for (int i = 0; i < E; i++) v[i] = ok ? in[base + L::din_pos(i, t)] : cplx<T>{(T)0, (T)0};
+    for (int i = 0; i < E; i++) v[i] = in[lbase + L::din_pos(i, t)];
     const typename L::TwF twf = L::template tw_load<-1>(tw, t);
     L::forward(v, my, alt, ph, twf, t);
     if (ok) {
@@ -282,28 +290,27 @@ __device__ __forceinline__ void row_body(const cplx<T>* __restrict__ in, cplx<T>
     if constexpr (ld_nt) {
 #pragma unroll
       for (int i = 0; i < E; i++)
-        v[i] = ok ? load_stream(&in[T2L ? t2_index(row, L::dout_pos(i, t), N) : base + L::dout_pos(i, t)])
-                  : cplx<T>{(T)0, (T)0};
+        v[i] = load_stream(&in[T2L ? t2_index(lrow, L::dout_pos(i, t), N) : lbase + L::dout_pos(i, t)]);
 #pragma unroll
-      for (int i = 0; i < E; i++) gv[i] = ok ? g[base + L::din_pos(i, t)] : cplx<T>{(T)0, (T)0};
+      for (int i = 0; i < E; i++) gv[i] = g[lbase + L::din_pos(i, t)];
       if constexpr (SMASK) {
-        mw = ok ? ua.smask[(size_t)row * TL + t] : 0u;
+        mw = ua.smask[(size_t)lrow * TL + t];
       } else {
 #pragma unroll
-        for (int i = 0; i < E; i++) sv[i] = ok ? spt[base + L::din_pos(i, t)] : (T)0;
+        for (int i = 0; i < E; i++) sv[i] = spt[lbase + L::din_pos(i, t)];
       }
     } else {
 #pragma unroll
       for (int i = 0; i < E; i++)
-        v[i] = ok ? in[T2L ? t2_index(row, L::dout_pos(i, t), N) : base + L::dout_pos(i, t)] : cplx<T>{(T)0, (T)0};
+        v[i] = in[T2L ? t2_index(lrow, L::dout_pos(i, t), N) : lbase + L::dout_pos(i, t)];
       if constexpr (upd) {
 #pragma unroll
-        for (int i = 0; i < E; i++) gv[i] = ok ? g[base + L::din_pos(i, t)] : cplx<T>{(T)0, (T)0};
+        for (int i = 0; i < E; i++) gv[i] = g[lbase + L::din_pos(i, t)];
         if constexpr (SMASK) {
-          mw = ok ? ua.smask[(size_t)row * TL + t] : 0u;
+          mw = ua.smask[(size_t)lrow * TL + t];
         } else {
 #pragma unroll
-          for (int i = 0; i < E; i++) sv[i] = ok ? spt[base + L::din_pos(i, t)] : (T)0;
+          for (int i = 0; i < E; i++) sv[i] = spt[lbase + L::din_pos(i, t)];
         }
       }
     }
@@ -346,7 +353,7 @@ __device__ __forceinline__ void row_body(const cplx<T>* __restrict__ in, cplx<T>
         else in = sv[i] > thr;
         if constexpr (prtf) {
           if (ua.prtf == 1) {  // gamma0 = P_S(g'), sum gamma0^2
-            const cplx<T> q = in ? p : cplx<T>{(T)0, (T)0};
+            const cplx<T> q = in && ok ? p : cplx<T>{(T)0, (T)0};
             if (ok) gamma0[base + L::din_pos(i, t)] = q;
             are += (double)q.x * q.x - (double)q.y * q.y;
             aim += 2.0 * q.x * q.y;

@@ -82,7 +82,9 @@ struct cdx_ctx {
 };
 
 // twiddle table of a line length n: tw[m] = W_n^m = exp(-2 pi i m/n), m < n,
-// evaluated in double with exact integer reduction of the angle
+// evaluated in double with exact integer reduction of the angle.  An FP32
+// table (n float pairs) is followed by the same n values as double pairs,
+// from which the FP32 passes form the powers of a twiddle before rounding.
 int make_twiddles(int n, int prec, void** dptr) {
   std::vector<double> re((size_t)n), im((size_t)n);
   for (int m = 0; m < n; m++) {
@@ -99,15 +101,14 @@ int make_twiddles(int n, int prec, void** dptr) {
       default: re[m] = -s; im[m] = c; break;
     }
   }
-  size_t es = prec == CDX_F64 ? 16 : 8;
-  std::vector<unsigned char> host(es * (size_t)n);
+  const size_t dbl_off = prec == CDX_F64 ? 0 : 8 * (size_t)n;
+  std::vector<unsigned char> host(dbl_off + 16 * (size_t)n);
   for (size_t i = 0; i < (size_t)n; i++) {
-    if (prec == CDX_F64) {
-      double v[2] = {re[i], im[i]};
-      memcpy(&host[i * es], v, 16);
-    } else {
-      float v[2] = {(float)re[i], (float)im[i]};
-      memcpy(&host[i * es], v, 8);
+    const double v[2] = {re[i], im[i]};
+    memcpy(&host[dbl_off + i * 16], v, 16);
+    if (prec != CDX_F64) {
+      const float f[2] = {(float)re[i], (float)im[i]};
+      memcpy(&host[i * 8], f, 8);
     }
   }
   HIPCHK(hipMalloc(dptr, host.size()));
@@ -1178,14 +1179,15 @@ constexpr size_t kOffT1 = 0;
 constexpr size_t kOffT2 = kOffT1 + al256(kStaticMaxN * 8);
 constexpr size_t kOffIpk = kOffT2 + al256(kStaticMaxN * 8);
 constexpr size_t kOffTmp = kOffIpk + al256(kStaticMaxN * 4);
+// FP32 twiddle tables: float pairs, then the same values as double pairs
 constexpr size_t kOffTwR = kOffTmp + al256(kStaticMaxN * 4);
-constexpr size_t kOffTwC = kOffTwR + al256(kStaticMaxSide * 8);
-constexpr size_t kOffScal = kOffTwC + al256(kStaticMaxSide * 8);
+constexpr size_t kOffTwC = kOffTwR + al256(kStaticMaxSide * 24);
+constexpr size_t kOffScal = kOffTwC + al256(kStaticMaxSide * 24);
 constexpr size_t kOffPart = kOffScal + al256(S_COUNT * sizeof(double));
 constexpr size_t kOffMask = kOffPart + al256(kMaxPartials * sizeof(double));
 // support bits: rows x lanes words, at most 512 x 32 for these line lengths
 constexpr size_t kMaskBytes = (size_t)kStaticMaxSide * 32 * sizeof(unsigned);
-constexpr size_t kScratchBytes = kOffMask + al256(kMaskBytes);  // 6 MiB + 102 KiB
+constexpr size_t kScratchBytes = kOffMask + al256(kMaskBytes);  // 6 MiB + 120 KiB
 constexpr int kMaxDevices = 64;
 }  // namespace
 
@@ -1214,8 +1216,9 @@ static int scratch_address(void** out) {
   return CDX_OK;
 }
 
-// twiddle table tw[m] = W_n^m = exp(-2 pi i m/n) in float pairs, the values
-// make_twiddles uploads (double evaluation after exact octant reduction)
+// twiddle table tw[m] = W_n^m = exp(-2 pi i m/n) in float pairs followed by
+// double pairs, the values make_twiddles uploads (double evaluation after
+// exact octant reduction)
 __global__ void k_twiddles_f32(int n, float* tw) {
   for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < n; m += gridDim.x * blockDim.x) {
     const long e8 = 8L * m;
@@ -1231,6 +1234,9 @@ __global__ void k_twiddles_f32(int n, float* tw) {
     }
     tw[2 * m] = (float)re;
     tw[2 * m + 1] = (float)im;
+    double* twd = reinterpret_cast<double*>(tw + 2 * n);
+    twd[2 * m] = re;
+    twd[2 * m + 1] = im;
   }
 }
 

@@ -194,8 +194,8 @@ struct Line {
 
   // Table twiddles of a transform.  Pass p (radix R, Ns > 1) multiplies
   // butterfly j = t + TL*s by W_N^(r * b), b = (j mod Ns) * N/(Ns*R): for a
-  // power-of-two R the lane loads W^(2^k b) and forms the other powers by
-  // products, otherwise it loads W^(r b), r = 1..R-1.  All of a transform's
+  // power-of-two R the lane holds W^(2^k b) and forms the other powers by
+  // products, otherwise it holds W^(r b), r = 1..R-1.  All of a transform's
   // loads are issued together (tw_load) so their latency overlaps the data
   // loads instead of stalling every exchange.
   static constexpr bool pow2(int r) { return (r & (r - 1)) == 0; }
@@ -219,6 +219,13 @@ struct Line {
     static constexpr int K = tw_off<DIR>(M);
     cplx<T> w[K > 0 ? K : 1];
   };
+  // The lane's table twiddles.  FP64 lines load each value.  FP32 lines
+  // read the table's double copy (the n double pairs after the n float
+  // pairs) and form in double, rounding each value once: for a power-of-two
+  // R, W^(2^k b) by squaring W^b; for butterflies t + TL*s whose index mod Ns
+  // equals that of s = 0, s = 0's values; where no lane's index wraps mod Ns
+  // (TL*(s+1) <= Ns), s = 0's values times the compile-time root
+  // W^(rr*TL*s*stride).  A pass then issues one load per distinct angle.
   template <int DIR>
   __device__ __forceinline__ static Tw<DIR> tw_load(const cplx<T>* __restrict__ tw, int t) {
     Tw<DIR> out;
@@ -229,13 +236,43 @@ struct Line {
       if constexpr (Ns > 1) {
         constexpr int stride = N / (Ns * R);
         constexpr int per = tw_per<DIR>(p);
-        static_for<E / R>([&](auto s) {
-          const int b = ((t + TL * s.value) % Ns) * stride;
-          static_for<per>([&](auto k) {
-            constexpr int rr = pow2(R) ? (1 << k.value) : k.value + 1;
-            out.w[tw_off<DIR>(p) + s.value * per + k.value] = tw[rr * b];
+        constexpr int o0 = tw_off<DIR>(p);
+        if constexpr (sizeof(T) == 8) {
+          static_for<E / R>([&](auto s) {
+            const int b = ((t + TL * s.value) % Ns) * stride;
+            static_for<per>([&](auto k) {
+              constexpr int rr = pow2(R) ? (1 << k.value) : k.value + 1;
+              out.w[o0 + s.value * per + k.value] = tw[rr * b];
+            });
           });
-        });
+        } else {
+          const cplx<double>* __restrict__ twd = reinterpret_cast<const cplx<double>*>(tw + N);
+          cplx<double> w0[per];  // s = 0's values in double
+          static_for<E / R>([&](auto s) {
+            constexpr int sv = s.value;
+            constexpr int o = o0 + sv * per;
+            constexpr bool same = sv > 0 && (TL * sv) % Ns == 0;
+            constexpr bool shifted = sv > 0 && !same && TL * (sv + 1) <= Ns;
+            const int b = ((t + TL * sv) % Ns) * stride;
+            cplx<double> w = {0.0, 0.0};
+            static_for<per>([&](auto k) {
+              constexpr int rr = pow2(R) ? (1 << k.value) : k.value + 1;
+              if constexpr (same) {
+                out.w[o + k.value] = out.w[o0 + k.value];
+              } else {
+                if constexpr (shifted) {
+                  w = twiddle_const<rr * TL * sv * stride, N, -1>(w0[k.value]);
+                } else if constexpr (pow2(R) && k.value > 0) {
+                  w = {w.x * w.x - w.y * w.y, 2.0 * w.x * w.y};
+                } else {
+                  w = twd[rr * b];
+                }
+                if constexpr (sv == 0) w0[k.value] = w;
+                out.w[o + k.value] = {(T)w.x, (T)w.y};
+              }
+            });
+          });
+        }
       }
     });
     return out;

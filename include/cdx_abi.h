@@ -178,7 +178,7 @@ int cdx_phase_retrieve(cdx_ctx* ctx, const char* algorithm, const cdx_params* p,
  * serving loops, harnesses that forbid it).  FP32 frames of rows x cols <=
  * 512 x 512 (FFT lengths); F, g, I, bs, spt as in cdx_phase_retrieve; the
  * support threshold comes in and goes out through *threshold (host); stream
- * may be NULL.  Its workspace is one static device array of 6 MiB + 102 KiB
+ * may be NULL.  Its workspace is one static device array of 6 MiB + 120 KiB
  * per GPU, held until the process ends: no device memory is allocated or
  * freed (the schedule string is parsed on the host).  Launches go straight
  * to the stream (no graph); PRTF accumulation is not available (prtfIter

@@ -299,6 +299,51 @@ struct Line {
     });
   }
 
+  // Exchange indices (before the padding swizzle).  After pass p, butterfly
+  // j = t + TL*s writes its output r to (j/Ns)*Ns*R + (j mod Ns) + r*Ns; pass
+  // p+1 reads its input r of butterfly j from j + r*N/R2.
+  static constexpr int swz_c(int p) { return PAD > 0 ? p + (p >> PAD) : p; }
+  template <int DIR>
+  static constexpr int w_index(int p, int t, int s, int r) {
+    const int R = radix<DIR>(p), Ns = ns<DIR>(p), j = t + TL * s;
+    return (j / Ns) * Ns * R + (j % Ns) + r * Ns;
+  }
+  template <int DIR>
+  static constexpr int r_index(int p, int t, int s, int r) {
+    return t + TL * s + r * (N / radix<DIR>(p + 1));
+  }
+  // Offsets from the lane's first element, swizzled, and whether they are
+  // the same for every lane (checked over all t at compile time): then one
+  // base address and immediate offsets serve the whole exchange.
+  template <int DIR>
+  static constexpr int w_offset(int p, int s, int r) {
+    return swz_c(w_index<DIR>(p, 0, s, r)) - swz_c(w_index<DIR>(p, 0, 0, 0));
+  }
+  template <int DIR>
+  static constexpr int r_offset(int p, int s, int r) {
+    return swz_c(r_index<DIR>(p, 0, s, r)) - swz_c(r_index<DIR>(p, 0, 0, 0));
+  }
+  template <int DIR>
+  static constexpr bool w_linear(int p) {
+    const int R = radix<DIR>(p);
+    for (int t = 0; t < TL; t++)
+      for (int s = 0; s < E / R; s++)
+        for (int r = 0; r < R; r++)
+          if (swz_c(w_index<DIR>(p, t, s, r)) - swz_c(w_index<DIR>(p, t, 0, 0)) != w_offset<DIR>(p, s, r))
+            return false;
+    return true;
+  }
+  template <int DIR>
+  static constexpr bool r_linear(int p) {
+    const int R2 = radix<DIR>(p + 1);
+    for (int t = 0; t < TL; t++)
+      for (int s = 0; s < E / R2; s++)
+        for (int r = 0; r < R2; r++)
+          if (swz_c(r_index<DIR>(p, t, s, r)) - swz_c(r_index<DIR>(p, t, 0, 0)) != r_offset<DIR>(p, s, r))
+            return false;
+    return true;
+  }
+
   // DIR = -1: Din -> Dout (forward); DIR = +1: Dout -> Din (inverse).
   // lds: this line's first exchange buffer (LDS_ELEMS), lds + alt its second;
   // ph selects the buffer of the next exchange and flips after each one.  One
@@ -322,17 +367,31 @@ struct Line {
       if constexpr (p < M - 1) {
         cplx<T>* lds = lds0 + ph * alt;
         ph ^= 1;
-        static_for<S>([&](auto s) {
-          const int j = t + TL * s.value;
-          const int base = (j / Ns) * Ns * R + (j % Ns);
-          static_for<R>([&](auto r) { lds[swz(base + r.value * Ns)] = v[s.value * R + r.value]; });
-        });
+        if constexpr (w_linear<DIR>(p)) {
+          // every write at the lane's base plus a compile-time offset
+          cplx<T>* wb = lds + swz(w_index<DIR>(p, t, 0, 0));
+          static_for<S>([&](auto s) {
+            static_for<R>([&](auto r) {
+              wb[w_offset<DIR>(p, s.value, r.value)] = v[s.value * R + r.value];
+            });
+          });
+        } else {
+          static_for<S>([&](auto s) {
+            static_for<R>([&](auto r) { lds[swz(w_index<DIR>(p, t, s.value, r.value))] = v[s.value * R + r.value]; });
+          });
+        }
         __syncthreads();
         constexpr int R2 = radix<DIR>(p + 1);
-        static_for<E / R2>([&](auto s) {
-          const int j = t + TL * s.value;
-          static_for<R2>([&](auto r) { v[s.value * R2 + r.value] = lds[swz(j + r.value * (N / R2))]; });
-        });
+        if constexpr (r_linear<DIR>(p)) {
+          const cplx<T>* rb = lds + swz(r_index<DIR>(p, t, 0, 0));
+          static_for<E / R2>([&](auto s) {
+            static_for<R2>([&](auto r) { v[s.value * R2 + r.value] = rb[r_offset<DIR>(p, s.value, r.value)]; });
+          });
+        } else {
+          static_for<E / R2>([&](auto s) {
+            static_for<R2>([&](auto r) { v[s.value * R2 + r.value] = lds[swz(r_index<DIR>(p, t, s.value, r.value))]; });
+          });
+        }
       }
     });
   }

@@ -712,6 +712,13 @@ def main():
     wall = time.perf_counter() - t0
     ms_per_step = ev_ms / args.steps
     iters_s = 1000.0 / ms_per_step
+    # two more back-to-back samples of each pass (the same measurement as the
+    # timed call's): the median of three keeps one slow sample from making
+    # the shorter pass look dominant
+    more = [ctx.time_passes(p, cdx.RAAR, d["F"], d["g"], d["I"], d["spt"], 200, bs=d["bs"], mode=0)
+            for _ in range(2)]
+    col_ms = float(np.median([col_ms] + [c for c, _ in more]))
+    row_ms = float(np.median([row_ms] + [r for _, r in more]))
     # each pass launched between its own pair of events (launches serialized
     # as under a kernel-trace profiler): the duration comparable with a trace
     ser_col, ser_row = ctx.time_passes(p, cdx.RAAR, d["F"], d["g"], d["I"], d["spt"], 200, bs=d["bs"], mode=1)

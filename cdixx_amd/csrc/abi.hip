@@ -980,6 +980,28 @@ static int sim_object_sp(cdx_ctx* c, const ShapeParams& sp, uint64_t seed, doubl
   return CDX_OK;  // ordered on the context's stream like the other context calls
 }
 
+// The forward model for the frames of f side by side (launch_synth_fields):
+// objects into obj, their 2D transforms into F (the same row and column
+// passes as cdx_fft, over B*n rows of n points and n rows of B*n columns:
+// each frame's transform is the one cdx_fft makes of it), Poisson
+// intensities into I.  Square FP32 contexts; ordered on the context's stream.
+int sim_stack(cdx_ctx* c, const SynthFrames& f, double sigma, double* a, double* ph, double* ta, double* tph,
+              double* mm, double* tot, void* obj, void* F, float* I) {
+  CTX_CHECK(c);
+  if (c->prec != CDX_F32 || c->rows != c->cols) return fail(CDX_ERR_ARG, "sim_stack: square FP32 contexts only");
+  if (f.B < 1 || f.B > kSynthMax || !synth_stack_supported(c->rows, sigma))
+    return fail(CDX_ERR_ARG, "sim_stack: %d frames of %d points, smoothing %g not supported", f.B, c->rows, sigma);
+  const int n = c->rows;
+  HIPCHK(launch_synth_fields(n, f, sigma, a, ph, ta, tph, mm, obj, c->st));
+  ProjArgs<float> pa{};
+  UpdArgs<float> ua{};
+  ReduceSlots rs{};
+  HIPCHK(launch_row<float>(ROW_FWD, f.B * n, n, obj, F, c->tw_row, nullptr, nullptr, ua, 1.f, nullptr, rs, c->st));
+  HIPCHK(launch_col<float>(COL_FWD, n, f.B * n, F, F, c->tw_col, nullptr, pa, 0, c->st));
+  HIPCHK(launch_synth_intensity(n, f, F, tot, I, c->st));
+  return CDX_OK;
+}
+
 extern "C" int cdx_sim_object(cdx_ctx* c, double radius, uint64_t seed, double sigma, void* obj) {
   CTX_CHECK(c);
   ShapeParams sp{};

@@ -755,6 +755,184 @@ hipError_t launch_poisson_intensity(int64_t n, const void* F, const double* tot,
 }
 
 // ---------------------------------------------------------------------------
+// the forward model for a stack of B frames of n x n side by side: element
+// (r, c) of frame b at r * W + b * n + c, W = B * n.  Every kernel computes
+// what its one-frame counterpart above computes for each frame (frame-local
+// pixel indices for the Philox counters, the same summation orders), so a
+// frame of the stack equals the frame made alone.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void stack_pos(int64_t k, int n, int64_t W, int& r, int& b, int& c) {
+  r = (int)(k / W);
+  const int rem = (int)(k - (int64_t)r * W);
+  b = rem / n;
+  c = rem - b * n;
+}
+__global__ void k_uniform2_stack(int n, SynthFrames f, double* __restrict__ a, double* __restrict__ ph) {
+  const int64_t W = (int64_t)f.B * n;
+  EW_LOOP(W * n) {
+    int r, b, c;
+    stack_pos(i, n, W, r, b, c);
+    const uint64_t il = (uint64_t)r * n + c;
+    a[i] = philox_uniform(f.seed[b], 0, il);
+    ph[i] = philox_uniform(f.seed[b], 1, il);
+  }
+}
+// k_smooth_line for every line of the stack and two fields (blockIdx.y)
+__global__ void __launch_bounds__(256) k_smooth_line_stack(int n, int B, const double* in0, double* out0,
+                                                           const double* in1, double* out1, double sigma, int axis) {
+  __shared__ double wt[kSmoothMaxTaps];
+  __shared__ double line[kSmoothMaxLine];
+  const double* in = blockIdx.y ? in1 : in0;
+  double* out = blockIdx.y ? out1 : out0;
+  const int r = (int)ceil(4 * sigma);
+  const int taps = 2 * r + 1;
+  const int b = blockIdx.x / n, l = blockIdx.x % n;
+  const int64_t W = (int64_t)B * n;
+  if (threadIdx.x == 0) {
+    double ws = 0;
+    for (int k = -r; k <= r; k++) ws += exp(-(double)k * k / (2 * sigma * sigma));
+    wt[0] = ws;
+  }
+  __syncthreads();
+  const double ws = wt[0];
+  __syncthreads();
+  for (int t = threadIdx.x; t < taps; t += blockDim.x) {
+    const int k = t - r;
+    wt[t] = exp(-(double)k * k / (2 * sigma * sigma)) / ws;
+  }
+  auto at = [&](int p) { return axis ? (int64_t)p * W + (int64_t)b * n + l : (int64_t)l * W + (int64_t)b * n + p; };
+  for (int p = threadIdx.x; p < n; p += blockDim.x) line[p] = in[at(p)];
+  __syncthreads();
+  for (int p = threadIdx.x; p < n; p += blockDim.x) {
+    double s = 0;
+    for (int k = -r; k <= r; k++) {
+      int q = p + k;
+      q += q < 0 ? n : (q >= n ? -n : 0);
+      s += wt[k + r] * line[q];
+    }
+    out[at(p)] = s;
+  }
+}
+// min/max keys of each frame of two fields: mm[2 * (field * B + b) + {0, 1}]
+__global__ void k_seed_minmax_stack(int count, unsigned long long* mm) {
+  for (int k = threadIdx.x; k < count; k += blockDim.x) {
+    mm[2 * k] = ~0ull;
+    mm[2 * k + 1] = 0ull;
+  }
+}
+__global__ void __launch_bounds__(256) k_minmax_stack(int n, int B, const double* x0, const double* x1, double* mm) {
+  __shared__ double slo[4], shi[4];
+  const double* x = blockIdx.z ? x1 : x0;
+  const int b = blockIdx.y;
+  const int64_t W = (int64_t)B * n, N = (int64_t)n * n;
+  double lo = INFINITY, hi = -INFINITY;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+    const double v = x[(i / n) * W + (int64_t)b * n + i % n];
+    lo = fmin(lo, v);
+    hi = fmax(hi, v);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    lo = fmin(lo, __shfl_xor(lo, o, 64));
+    hi = fmax(hi, __shfl_xor(hi, o, 64));
+  }
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    slo[wave] = lo;
+    shi[wave] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
+      lo = fmin(lo, slo[w]);
+      hi = fmax(hi, shi[w]);
+    }
+    long long bl = __double_as_longlong(lo), bh = __double_as_longlong(hi);
+    unsigned long long kl = bl < 0 ? ~(unsigned long long)bl : ((unsigned long long)bl | 0x8000000000000000ULL);
+    unsigned long long kh = bh < 0 ? ~(unsigned long long)bh : ((unsigned long long)bh | 0x8000000000000000ULL);
+    unsigned long long* dst = (unsigned long long*)mm + 2 * ((int)blockIdx.z * B + b);
+    atomicMin(dst, kl);
+    atomicMax(dst + 1, kh);
+  }
+}
+__global__ void k_compose_stack(int n, SynthFrames f, const double* __restrict__ a, const double* __restrict__ ph,
+                                const double* __restrict__ mm, cplx<float>* __restrict__ obj) {
+  const int64_t W = (int64_t)f.B * n;
+  const int c0 = n >> 1;
+  EW_LOOP(W * n) {
+    int x, b, y;
+    stack_pos(i, n, W, x, b, y);
+    const unsigned long long* ka = (const unsigned long long*)mm + 2 * b;
+    const unsigned long long* kp = (const unsigned long long*)mm + 2 * (f.B + b);
+    const double amin = unkey(ka[0]), amax = unkey(ka[1]);
+    const double pmin = unkey(kp[0]), pmax = unkey(kp[1]);
+    if (!shape_inside(f.sp[b], (float)(x - c0), (float)(y - c0))) {
+      obj[i] = {0.f, 0.f};
+      continue;
+    }
+    const double amp = 0.2 + 0.8 * (a[i] - amin) / (amax - amin);
+    const double phs = -kPi + 2 * kPi * (ph[i] - pmin) / (pmax - pmin);
+    obj[i] = {(float)(amp * cos(phs)), (float)(amp * sin(phs))};
+  }
+}
+__global__ void k_sum_mod2_stack(int n, int B, const cplx<float>* __restrict__ F, double* acc) {
+  const int b = blockIdx.y;
+  const int64_t W = (int64_t)B * n, N = (int64_t)n * n;
+  double s = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+    const cplx<float> z = F[(i / n) * W + (int64_t)b * n + i % n];
+    s += (double)z.x * z.x + (double)z.y * z.y;
+  }
+  __shared__ double sh[16];
+  block_atomic_sum(s, acc + b, sh);
+}
+__global__ void k_poisson_stack(int n, SynthFrames f, const cplx<float>* __restrict__ F, const double* __restrict__ tot,
+                                float* __restrict__ I) {
+  const int64_t W = (int64_t)f.B * n;
+  EW_LOOP(W * n) {
+    int r, b, c;
+    stack_pos(i, n, W, r, b, c);
+    const double sc = tot[b] > 0 ? f.photons[b] / tot[b] : 0;
+    const double lam = ((double)F[i].x * F[i].x + (double)F[i].y * F[i].y) * sc;
+    I[i] = (float)poisson_draw(lam, f.seed[b], (uint64_t)r * n + c);
+  }
+}
+bool synth_stack_supported(int n, double sigma) {
+  const int r = (int)ceil(4 * sigma);
+  return n <= kSmoothMaxLine && 2 * r + 1 <= kSmoothMaxTaps && r < n;
+}
+hipError_t launch_synth_fields(int n, const SynthFrames& f, double sigma, double* a, double* ph, double* ta,
+                               double* tph, double* mm, void* obj, hipStream_t st) {
+  if (f.B < 1 || f.B > kSynthMax || !synth_stack_supported(n, sigma)) return hipErrorInvalidValue;
+  const int64_t tot = (int64_t)f.B * n * n;
+  hipLaunchKernelGGL(k_uniform2_stack, ew_grid(tot), dim3(256), 0, st, n, f, a, ph);
+  hipLaunchKernelGGL(k_smooth_line_stack, dim3(f.B * n, 2), dim3(256), 0, st, n, f.B, (const double*)a, ta,
+                     (const double*)ph, tph, sigma, 0);
+  hipLaunchKernelGGL(k_smooth_line_stack, dim3(f.B * n, 2), dim3(256), 0, st, n, f.B, (const double*)ta, a,
+                     (const double*)tph, ph, sigma, 1);
+  hipLaunchKernelGGL(k_seed_minmax_stack, dim3(1), dim3(64), 0, st, 2 * f.B, (unsigned long long*)mm);
+  // blocks per frame as launch_minmax for one frame: 4 elements per thread, at most 256
+  int64_t nb = ((int64_t)n * n + 256 * 4 - 1) / (256 * 4);
+  nb = nb < 1 ? 1 : (nb > 256 ? 256 : nb);
+  hipLaunchKernelGGL(k_minmax_stack, dim3((unsigned)nb, f.B, 2), dim3(256), 0, st, n, f.B, (const double*)a,
+                     (const double*)ph, mm);
+  hipLaunchKernelGGL(k_compose_stack, ew_grid(tot), dim3(256), 0, st, n, f, (const double*)a, (const double*)ph,
+                     (const double*)mm, (cplx<float>*)obj);
+  return hipGetLastError();
+}
+hipError_t launch_synth_intensity(int n, const SynthFrames& f, const void* F, double* tot, float* I,
+                                  hipStream_t st) {
+  if (f.B < 1 || f.B > kSynthMax) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(256), 0, st, (int64_t)f.B, tot, 0.0);
+  // blocks per frame as launch_sum_mod2 for one frame
+  const dim3 g1 = red_grid((int64_t)n * n);
+  hipLaunchKernelGGL(k_sum_mod2_stack, dim3(g1.x, f.B), dim3(256), 0, st, n, f.B, (const cplx<float>*)F, tot);
+  hipLaunchKernelGGL(k_poisson_stack, ew_grid((int64_t)f.B * n * n), dim3(256), 0, st, n, f, (const cplx<float>*)F,
+                     (const double*)tot, I);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // setup kernels of CDI::prepareIter / readObjectWave / checkAutoCorrelation
 // ---------------------------------------------------------------------------
 // convertFOPhase (cudaConfig.cu:861-870): multiply by (-1)^(x+y) * norm

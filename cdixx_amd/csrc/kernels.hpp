@@ -208,4 +208,28 @@ template <class T> hipError_t launch_poisson_intensity(int64_t n, const void* F,
                                                        uint64_t seed, int poisson, T* I, hipStream_t st);
 template <class T> hipError_t launch_sum_mod2(int64_t n, const void* F, double* acc, hipStream_t st);
 
+// The forward model for B <= kSynthMax frames of n x n side by side (element
+// (r, c) of frame b at r * B * n + b * n + c): per frame the seed, support
+// shape and photon count.  launch_synth_fields makes the FP32 objects
+// (buffers a, ph, ta, tph: B * n * n doubles, mm: 4 * B doubles);
+// launch_synth_intensity the FP32 Poisson intensities of their transforms F
+// (tot: B doubles).  Each frame equals the frame the one-frame kernels make
+// from the same seed, shape and photons.
+constexpr int kSynthMax = 16;
+struct SynthFrames {
+  int B;
+  uint64_t seed[kSynthMax];
+  double photons[kSynthMax];
+  ShapeParams sp[kSynthMax];
+};
+bool synth_stack_supported(int n, double sigma);
+hipError_t launch_synth_fields(int n, const SynthFrames& f, double sigma, double* a, double* ph, double* ta,
+                               double* tph, double* mm, void* obj, hipStream_t st);
+hipError_t launch_synth_intensity(int n, const SynthFrames& f, const void* F, double* tot, float* I, hipStream_t st);
+
 }  // namespace cdx
+
+// the forward model for a stack of frames on a context's stream (abi.hip)
+struct cdx_ctx;
+int sim_stack(cdx_ctx* c, const cdx::SynthFrames& f, double sigma, double* a, double* ph, double* ta, double* tph,
+              double* mm, double* tot, void* obj, void* F, float* I);

@@ -1074,6 +1074,55 @@ __global__ void k_pack_target(int64_t n, const cplx<float>* __restrict__ obj, co
     o[3] = 0.f;
   }
 }
+// The same packing for the frames of a forward-model stack (frame b's
+// element (r, c) at r * B * n + b * n + c) into batch entries b0 + b.
+__global__ void k_pack_input_stack(int n, int B, int b0, const float* __restrict__ I, u16* __restrict__ x) {
+  const int64_t N = (int64_t)n * n, W = (int64_t)B * n;
+  EW_LOOP((int64_t)B * N) {
+    const int b = (int)(i / N);
+    const int64_t k = i - (int64_t)b * N;
+    const int h = (int)(k / n), w = (int)(k % n);
+    const int hs = h + n / 2 < n ? h + n / 2 : h - n / 2, ws = w + n / 2 < n ? w + n / 2 : w - n / 2;
+    const float v = I[(int64_t)hs * W + (int64_t)b * n + ws];
+    u16* o = x + ((size_t)(b0 + b) * N + k) * 8;
+    o[0] = f2bf(log1pf(fmaxf(v, 0.f)) * (1.f / 16.f));
+#pragma unroll
+    for (int c = 1; c < 8; c++) o[c] = 0;
+  }
+}
+// k_csum per frame of a stack: one workgroup per frame, the same order
+__global__ void k_csum_stack(int n, int B, const cplx<float>* __restrict__ z, double* out) {
+  const int b = blockIdx.x;
+  const int64_t N = (int64_t)n * n, W = (int64_t)B * n;
+  double re = 0, im = 0;
+#pragma unroll 8
+  for (int64_t i = threadIdx.x; i < N; i += blockDim.x) {
+    const cplx<float> v = z[(i / n) * W + (int64_t)b * n + i % n];
+    re += v.x;
+    im += v.y;
+  }
+  __shared__ double sh[16];
+  block_store_sum(re, out + 2 * b, sh);
+  block_store_sum(im, out + 2 * b + 1, sh);
+}
+__global__ void k_pack_target_stack(int n, int B, int b0, const cplx<float>* __restrict__ obj,
+                                    const double* __restrict__ csum, float* __restrict__ tg) {
+  const int64_t N = (int64_t)n * n, W = (int64_t)B * n;
+  EW_LOOP((int64_t)B * N) {
+    const int b = (int)(i / N);
+    const int64_t k = i - (int64_t)b * N;
+    const float th = (float)atan2(csum[2 * b + 1], csum[2 * b]);
+    const float c = cosf(th), s = sinf(th);
+    const cplx<float> z = obj[(k / n) * W + (int64_t)b * n + k % n];
+    const float re = z.x * c + z.y * s, im = z.y * c - z.x * s;  // z e^{-i th}
+    const float a = sqrtf(z.x * z.x + z.y * z.y);
+    float* o = tg + ((size_t)(b0 + b) * N + k) * 4;
+    o[0] = a;
+    o[1] = a > 0.f ? atan2f(im, re) : 0.f;
+    o[2] = a > 0.f ? 1.f : 0.f;
+    o[3] = 0.f;
+  }
+}
 // warm start from the head output of sample 0: support = sigmoid(logit) > thr,
 // g0 = amplitude e^{i phase} inside the support
 __global__ void k_ws_compose(int64_t n, const float* __restrict__ yf, float thr, cplx<float>* __restrict__ g0,
@@ -1154,6 +1203,13 @@ struct cdx_unet {
   double* dscal = nullptr;  // reductions (loss partials, sums)
   float* wg_part = nullptr;  // weight-gradient partials of the pixel splits
   void *obj = nullptr, *I = nullptr;  // forward-model scratch (one frame)
+  // forward model for up to synth_chunk frames side by side: fields a, ph
+  // and their smoothing temporaries, min/max keys, |F|^2 totals and
+  // complex sums (syn); objects, transforms and intensities
+  int synth_chunk = 0;
+  double* syn = nullptr;
+  void *sobj = nullptr, *sF = nullptr;
+  float* sI = nullptr;
   int64_t adam_t = 0;
 };
 
@@ -1374,6 +1430,8 @@ int adam(cdx_unet* u, double lr) {
   return CDX_OK;
 }
 
+// smoothing of the synthetic objects' fields, in pixels
+constexpr double kSynthSigma = 3.0;
 // forward-model sample: shape, photons and seeds drawn from `seed`
 struct SampleSpec {
   int kind, np;
@@ -1403,14 +1461,64 @@ SampleSpec draw_sample(uint64_t seed, double lo, double hi) {
 
 int synth_frame(cdx_unet* u, uint64_t seed, double lo, double hi, void* obj, void* I) {
   const SampleSpec s = draw_sample(seed, lo, hi);
-  RCHK(cdx_sim_object_shape(u->ctx, s.kind, s.prm, s.np, seed, 3.0, obj));
+  RCHK(cdx_sim_object_shape(u->ctx, s.kind, s.prm, s.np, seed, kSynthSigma, obj));
   RCHK(cdx_sim_intensity(u->ctx, obj, s.photons, seed, 1, I));
   return CDX_OK;
 }
 
+// the support shape of a drawn sample (the parameters cdx_sim_object_shape takes)
+ShapeParams sample_shape(const SampleSpec& s) {
+  ShapeParams sp{};
+  sp.kind = s.kind;
+  if (s.kind == 1) {
+    sp.a = (float)s.prm[0];
+    sp.b = (float)s.prm[1];
+    sp.rot = (float)s.prm[2];
+  } else {
+    sp.nv = s.np / 2;
+    for (int i = 0; i < sp.nv; i++) {
+      sp.ang[i] = (float)s.prm[i];
+      sp.rad[i] = (float)s.prm[sp.nv + i];
+    }
+  }
+  return sp;
+}
+
+// Batch entries b of seeds seed0 + b.  Frames go through the forward model
+// synth_chunk at a time side by side (the same values as one frame at a
+// time, in a few launches per chunk instead of ~17 per frame).
 int synth_batch(cdx_unet* u, int B, uint64_t seed0, double lo, double hi) {
   const int n = u->size;
   const int64_t N = (int64_t)n * n;
+  if (u->synth_chunk > 0) {
+    for (int b0 = 0; b0 < B; b0 += u->synth_chunk) {
+      SynthFrames f{};
+      f.B = B - b0 < u->synth_chunk ? B - b0 : u->synth_chunk;
+      for (int j = 0; j < f.B; j++) {
+        const uint64_t seed = seed0 + (uint64_t)(b0 + j);
+        const SampleSpec spec = draw_sample(seed, lo, hi);
+        f.seed[j] = seed;
+        f.photons[j] = spec.photons;
+        f.sp[j] = sample_shape(spec);
+      }
+      const size_t fN = (size_t)u->synth_chunk * N;
+      double* a = u->syn;
+      double* mm = u->syn + 4 * fN;
+      double* tot = mm + 4 * (size_t)u->synth_chunk;
+      double* cs = tot + u->synth_chunk;
+      RCHK(sim_stack(u->ctx, f, kSynthSigma, a, a + fN, a + 2 * fN, a + 3 * fN, mm, tot, u->sobj, u->sF, u->sI));
+      const int64_t tot_px = (int64_t)f.B * N;
+      hipLaunchKernelGGL(k_pack_input_stack, dim3(ew_blocks(tot_px)), dim3(256), 0, u->st, n, f.B, b0,
+                         (const float*)u->sI, u->in0.y);
+      UCHK(hipGetLastError());
+      hipLaunchKernelGGL(k_csum_stack, dim3(f.B), dim3(1024), 0, u->st, n, f.B, (const cplx<float>*)u->sobj, cs);
+      UCHK(hipGetLastError());
+      hipLaunchKernelGGL(k_pack_target_stack, dim3(ew_blocks(tot_px)), dim3(256), 0, u->st, n, f.B, b0,
+                         (const cplx<float>*)u->sobj, (const double*)cs, u->tg);
+      UCHK(hipGetLastError());
+    }
+    return CDX_OK;
+  }
   for (int b = 0; b < B; b++) {
     RCHK(synth_frame(u, seed0 + (uint64_t)b, lo, hi, u->obj, u->I));
     hipLaunchKernelGGL(k_pack_input, dim3(ew_blocks(N)), dim3(256), 0, u->st, n, (const float*)u->I,
@@ -1430,7 +1538,8 @@ void destroy(cdx_unet* u) {
   if (u->st) hipStreamSynchronize(u->st);
   for (auto& L : u->conv) hipFree(L.wflip);
   for (void* p : {(void*)u->w, (void*)u->g, (void*)u->m1, (void*)u->m2, (void*)u->wb, (void*)u->yf,
-                  (void*)u->dz_head, (void*)u->tg, (void*)u->dscal, (void*)u->wg_part, u->obj, u->I})
+                  (void*)u->dz_head, (void*)u->tg, (void*)u->dscal, (void*)u->wg_part, u->obj, u->I, (void*)u->syn,
+                  u->sobj, u->sF, (void*)u->sI})
     hipFree(p);
   free_tensor(u->in0);
   for (auto* v : {&u->ea, &u->e, &u->p, &u->cat, &u->da, &u->d})
@@ -1501,6 +1610,14 @@ extern "C" int cdx_unet_create(cdx_unet** out, int size, int levels, int base, i
             hipMalloc(&u->wg_part, kWgradPartFloats * sizeof(float)) == hipSuccess &&
             hipMalloc(&u->obj, (size_t)size * size * 8) == hipSuccess &&
             hipMalloc(&u->I, (size_t)size * size * 4) == hipSuccess;
+  if (ok && synth_stack_supported(size, kSynthSigma)) {
+    const int ch = max_batch < kSynthMax ? max_batch : kSynthMax;
+    const size_t fN = (size_t)ch * size * size;
+    ok = hipMalloc(&u->syn, (4 * fN + 7 * (size_t)ch) * sizeof(double)) == hipSuccess &&
+         hipMalloc(&u->sobj, fN * 8) == hipSuccess && hipMalloc(&u->sF, fN * 8) == hipSuccess &&
+         hipMalloc((void**)&u->sI, fN * 4) == hipSuccess;
+    u->synth_chunk = ch;
+  }
   for (auto& L : u->conv) {
     ok = ok && hipMalloc(&L.wflip, (size_t)L.cin * L.kp2 * sizeof(u16)) == hipSuccess;
     if (ok) {  // the K padding of the flipped weights stays zero

@@ -268,3 +268,41 @@ def test_guarded_warm_start_never_ends_above_random_at_2048(cdx):
     ctx.close()
     cs.close()
     net.close()
+
+
+def test_batched_synthesis_matches_single_frames(cdx):
+    """synth_batch makes its frames side by side (16 at a time); each batch
+    entry must be the packing of the frame synth_frame makes from the same
+    seed.  Packing restated here: input bf16(log1p(max(I, 0)) / 16) of the
+    centred intensity, targets |obj|, arg(obj e^-i theta) with theta =
+    arg(sum obj), mask |obj| > 0.  20 frames: two chunks (16 + 4).
+    Tolerances: input within one bf16 step (log1p rounding), amplitude 1e-6
+    relative, phase 2e-5 on the unit circle, mask exact; the |F|^2 totals
+    are summed with atomics in both paths, so a Poisson draw may differ at a
+    boundary: at most 1e-4 of the input pixels outside the bound."""
+    n, B, seed0 = 128, 20, 4321
+    net = cdx.UNet(n, 3, 8, max_batch=B, seed=1)
+    net.synth_batch(B, seed0=seed0)
+    x, t = net.get_batch(B)
+    ctx = cdx.Context(n)
+    obj, I = ctx.cplx(), ctx.real()
+    bad_x = 0
+    for b in range(B):
+        net.synth_frame(seed0 + b, obj, I)
+        o = ctx.get_cplx(obj).astype(np.complex64)
+        Ih = np.fft.fftshift(ctx.get_real(I).astype(np.float32))
+        xin = np.log1p(np.maximum(Ih, 0)).astype(np.float32) / np.float32(16)
+        bad_x += int((np.abs(x[b] - xin) > 2.0 ** -7 * np.abs(xin) + 1e-30).sum())
+        a = np.abs(o)
+        m = a > 0
+        assert np.array_equal(t[b, ..., 2] > 0, m), b
+        assert np.allclose(t[b, ..., 0], a, rtol=1e-6, atol=0), b
+        s = o.astype(np.complex128).sum()
+        ph = np.angle(o * np.exp(-1j * np.angle(s)))
+        d = np.abs(np.exp(1j * t[b, ..., 1][m]) - np.exp(1j * ph[m]))
+        assert d.max() < 2e-5, (b, d.max())
+    print(f"input pixels outside one bf16 step: {bad_x} of {B * n * n}", flush=True)
+    assert bad_x <= 1e-4 * B * n * n
+    net.close()
+    ctx.close()
+

@@ -249,13 +249,8 @@ __device__ __forceinline__ void row_body(const cplx<T>* __restrict__ in, cplx<T>
   int ph = 0;
   constexpr int E = L::E;
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem);
-  // lane -> (row rr, butterfly t) with the workgroup's rows interleaved in
-  // groups of four lanes: a wave holds four neighbouring columns of each of
-  // its rows, so the pair-of-rows T2 is read and the 4-column T1 pieces of
-  // neighbouring rows are written in contiguous runs per instruction
-  static_assert(TL % 4 == 0, "rows interleave in groups of four lanes");
-  const int rr = (threadIdx.x >> 2) & ((1 << lsh) - 1);
-  const int t = (threadIdx.x & 3) | ((threadIdx.x >> (2 + lsh)) << 2);
+  const int t = threadIdx.x % TL;
+  const int rr = threadIdx.x / TL;
   const int row = (tile << lsh) + rr;
   const bool ok = row < rows;
   cplx<T>* my = lds + rr * pitch;

@@ -118,6 +118,35 @@ def _injected(o64, n, rs_seed=0):
 ALGOS = {"RAAR": 0, "ER": 1, "POSER": 2, "HIO": 3, "POSHIO": 4, "FHIO": 5}
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("n", [16, 32])
+def test_partial_workgroups_match_full_ones(cdx, o64, n, prec):
+    """Workgroups of 64 lines on frames of 16 or 32 lines: the lanes past the
+    last line load the last line and drop their results (the passes' loads
+    carry no branches).  The same schedule with workgroups of 16 lines (no
+    partial workgroup) must give the same bits, and both match the FP64
+    oracle (1e-5 FP32, 1e-12 FP64)."""
+    print(f"partial workgroups n={n} prec={prec}", flush=True)
+    F, g, I, spt = _injected(o64, n)
+    alg = "3*RAAR+2*HIO+2*ER"
+    p = cdx.Params.default(beta=0.9)
+    out = []
+    for tiling in ((16, 16), (64, 64)):
+        ctx = cdx.Context(n, precision=prec)
+        ctx.set_tiling(*tiling)
+        assert ctx.tiling() == tiling
+        ctx.set_threshold(0.5)
+        dF, dg, dI, ds = ctx.cplx(F), ctx.cplx(g), ctx.real(I), ctx.real(spt)
+        ctx.phase_retrieve(alg, p, dF, dg, dI, ds)
+        out.append((ctx.get_cplx(dF), ctx.get_cplx(dg)))
+        ctx.close()
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    ref = o64.phase_retrieve(alg, F, g, I, spt, 0.5, beta=0.9)
+    tol = 1e-5 if prec == 0 else 1e-12
+    assert rel_l2(out[1][0], ref["F"]) < tol and rel_l2(out[1][1], ref["g"]) < tol, (
+        rel_l2(out[1][0], ref["F"]), rel_l2(out[1][1], ref["g"]))
+
+
 @pytest.mark.parametrize("n", [256, 384, 1024])
 @pytest.mark.parametrize("algo", list(ALGOS))
 @pytest.mark.parametrize("prec", [0, 1])

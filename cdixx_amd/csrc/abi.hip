@@ -944,39 +944,23 @@ extern "C" int cdx_time_iterations(cdx_ctx* c, const cdx_params* p, int algo, vo
 // ---------------------------------------------------------------------------
 // synthetic forward model
 // ---------------------------------------------------------------------------
-__global__ void k_seed_minmax(unsigned long long* mm) {
-  mm[0] = ~0ull;
-  mm[1] = 0ull;
-  mm[2] = ~0ull;
-  mm[3] = 0ull;
-}
 
 static int sim_object_sp(cdx_ctx* c, const ShapeParams& sp, uint64_t seed, double sigma, void* obj) {
   if (!obj) return fail(CDX_ERR_ARG, "cdx_sim_object: null buffer");
   if (c->rows != c->cols) return fail(CDX_ERR_ARG, "cdx_sim_object: square contexts only");
   const int n = c->rows;
-  double *a = nullptr, *ph = nullptr, *t = nullptr;
-  size_t bytes = (size_t)c->n * sizeof(double);
-  HIPCHK(hipMallocAsync((void**)&a, bytes, c->st));
-  HIPCHK(hipMallocAsync((void**)&ph, bytes, c->st));
-  HIPCHK(hipMallocAsync((void**)&t, bytes, c->st));
-  HIPCHK(launch_uniform_field(n, seed, 0, a, c->st));
-  HIPCHK(launch_uniform_field(n, seed, 1, ph, c->st));
-  HIPCHK(launch_smooth_periodic(n, a, t, sigma, 0, c->st));
-  HIPCHK(launch_smooth_periodic(n, t, a, sigma, 1, c->st));
-  HIPCHK(launch_smooth_periodic(n, ph, t, sigma, 0, c->st));
-  HIPCHK(launch_smooth_periodic(n, t, ph, sigma, 1, c->st));
-  hipLaunchKernelGGL(k_seed_minmax, dim3(1), dim3(1), 0, c->st, (unsigned long long*)(c->scal + S_MMA));
-  HIPCHK(hipGetLastError());
-  HIPCHK(launch_minmax(c->n, a, c->scal + S_MMA, c->st));
-  HIPCHK(launch_minmax(c->n, ph, c->scal + S_MMP, c->st));
-  if (c->prec == CDX_F64)
-    HIPCHK(launch_compose_object<double>(n, a, ph, c->scal + S_MMA, c->scal + S_MMP, sp, obj, c->st));
-  else
-    HIPCHK(launch_compose_object<float>(n, a, ph, c->scal + S_MMA, c->scal + S_MMP, sp, obj, c->st));
-  HIPCHK(hipFreeAsync(a, c->st));
-  HIPCHK(hipFreeAsync(ph, c->st));
-  HIPCHK(hipFreeAsync(t, c->st));
+  SynthFrames f{};
+  f.B = 1;
+  f.seed[0] = seed;
+  f.sp[0] = sp;
+  double* fld = nullptr;  // a, ph and their smoothing temporaries
+  HIPCHK(hipMallocAsync((void**)&fld, 4 * (size_t)c->n * sizeof(double), c->st));
+  double *a = fld, *ph = fld + c->n, *ta = fld + 2 * c->n, *tph = fld + 3 * c->n;
+  // min/max keys of a and ph in scal[S_MMA .. S_MMA + 3]
+  static_assert(S_MMP == S_MMA + 2, "min/max keys of both fields are contiguous");
+  if (c->prec == CDX_F64) HIPCHK(launch_synth_fields<double>(n, f, sigma, a, ph, ta, tph, c->scal + S_MMA, obj, c->st));
+  else HIPCHK(launch_synth_fields<float>(n, f, sigma, a, ph, ta, tph, c->scal + S_MMA, obj, c->st));
+  HIPCHK(hipFreeAsync(fld, c->st));
   return CDX_OK;  // ordered on the context's stream like the other context calls
 }
 
@@ -992,13 +976,13 @@ int sim_stack(cdx_ctx* c, const SynthFrames& f, double sigma, double* a, double*
   if (f.B < 1 || f.B > kSynthMax || !synth_stack_supported(c->rows, sigma))
     return fail(CDX_ERR_ARG, "sim_stack: %d frames of %d points, smoothing %g not supported", f.B, c->rows, sigma);
   const int n = c->rows;
-  HIPCHK(launch_synth_fields(n, f, sigma, a, ph, ta, tph, mm, obj, c->st));
+  HIPCHK(launch_synth_fields<float>(n, f, sigma, a, ph, ta, tph, mm, obj, c->st));
   ProjArgs<float> pa{};
   UpdArgs<float> ua{};
   ReduceSlots rs{};
   HIPCHK(launch_row<float>(ROW_FWD, f.B * n, n, obj, F, c->tw_row, nullptr, nullptr, ua, 1.f, nullptr, rs, c->st));
   HIPCHK(launch_col<float>(COL_FWD, n, f.B * n, F, F, c->tw_col, nullptr, pa, 0, c->st));
-  HIPCHK(launch_synth_intensity(n, f, F, tot, I, c->st));
+  HIPCHK(launch_synth_intensity<float>(n, n, f, F, tot, 1, I, c->st));
   return CDX_OK;
 }
 
@@ -1042,14 +1026,14 @@ extern "C" int cdx_sim_intensity(cdx_ctx* c, const void* obj, double photons, ui
   if (!obj || !I) return fail(CDX_ERR_ARG, "cdx_sim_intensity: null buffer");
   int rc = cdx_fft(c, obj, c->T2, CDX_FORWARD);
   if (rc) return rc;
-  HIPCHK(launch_fill_f64(1, c->scal + S_TOT, 0.0, c->st));
-  if (c->prec == CDX_F64) {
-    HIPCHK(launch_sum_mod2<double>(c->n, c->T2, c->scal + S_TOT, c->st));
-    HIPCHK(launch_poisson_intensity<double>(c->n, c->T2, c->scal + S_TOT, photons, seed, poisson, (double*)I, c->st));
-  } else {
-    HIPCHK(launch_sum_mod2<float>(c->n, c->T2, c->scal + S_TOT, c->st));
-    HIPCHK(launch_poisson_intensity<float>(c->n, c->T2, c->scal + S_TOT, photons, seed, poisson, (float*)I, c->st));
-  }
+  SynthFrames f{};
+  f.B = 1;
+  f.seed[0] = seed;
+  f.photons[0] = photons;
+  if (c->prec == CDX_F64)
+    HIPCHK(launch_synth_intensity<double>(c->rows, c->cols, f, c->T2, c->scal + S_TOT, poisson, (double*)I, c->st));
+  else
+    HIPCHK(launch_synth_intensity<float>(c->rows, c->cols, f, c->T2, c->scal + S_TOT, poisson, (float*)I, c->st));
   return CDX_OK;  // ordered on the context's stream
 }
 

@@ -490,14 +490,29 @@ hipError_t launch_cast_c(int64_t n, const void* in, int in_f64, void* out, hipSt
 
 // ---------------------------------------------------------------------------
 // synthetic forward model (same streams and arithmetic as the CPU oracle)
+//
+// The kernels take a stack of B frames of rows x cols side by side: element
+// (r, c) of frame b at r * B * cols + b * cols + c (B = 1: one frame in its
+// usual layout).  Per frame they use frame-local pixel indices for the
+// Philox counters and the one-frame summation orders, so a frame of a stack
+// equals the frame made alone.
 // ---------------------------------------------------------------------------
-__global__ void k_uniform_field(int64_t n, uint64_t seed, uint64_t stream, double* out) {
-  EW_LOOP(n) out[i] = philox_uniform(seed, stream, (uint64_t)i);
+__device__ __forceinline__ void stack_pos(int64_t k, int cols, int64_t W, int& r, int& b, int& c) {
+  r = (int)(k / W);
+  const int rem = (int)(k - (int64_t)r * W);
+  b = rem / cols;
+  c = rem - b * cols;
 }
-hipError_t launch_uniform_field(int n, uint64_t seed, uint64_t stream, double* out, hipStream_t st) {
-  const int64_t N = (int64_t)n * n;
-  hipLaunchKernelGGL(k_uniform_field, ew_grid(N), dim3(256), 0, st, N, seed, stream, out);
-  return hipGetLastError();
+// both fields of the object: U(0,1) from Philox streams 0 (amplitude) and 1 (phase)
+__global__ void k_uniform2_stack(int n, SynthFrames f, double* __restrict__ a, double* __restrict__ ph) {
+  const int64_t W = (int64_t)f.B * n;
+  EW_LOOP(W * n) {
+    int r, b, c;
+    stack_pos(i, n, W, r, b, c);
+    const uint64_t il = (uint64_t)r * n + c;
+    a[i] = philox_uniform(f.seed[b], 0, il);
+    ph[i] = philox_uniform(f.seed[b], 1, il);
+  }
 }
 // periodic Gaussian smoothing along one axis, radius ceil(4 sigma), weights
 // normalised to sum 1 (accumulated in the same order as the oracle).  Each
@@ -546,12 +561,17 @@ __global__ void __launch_bounds__(256) k_smooth(int n, const double* in, double*
 // output sums its taps from LDS in the order of k_smooth, so the results are
 // identical.  Lines of up to kSmoothMaxLine points, radius below n.
 constexpr int kSmoothMaxLine = 2048;
-__global__ void __launch_bounds__(256) k_smooth_line(int n, const double* in, double* out, double sigma, int axis) {
+// Every line of a stack of B frames, two fields (blockIdx.y) per launch.
+__global__ void __launch_bounds__(256) k_smooth_line_stack(int n, int B, const double* in0, double* out0,
+                                                           const double* in1, double* out1, double sigma, int axis) {
   __shared__ double wt[kSmoothMaxTaps];
   __shared__ double line[kSmoothMaxLine];
+  const double* in = blockIdx.y ? in1 : in0;
+  double* out = blockIdx.y ? out1 : out0;
   const int r = (int)ceil(4 * sigma);
   const int taps = 2 * r + 1;
-  const int l = blockIdx.x;
+  const int b = blockIdx.x / n, l = blockIdx.x % n;
+  const int64_t W = (int64_t)B * n;
   if (threadIdx.x == 0) {
     double ws = 0;
     for (int k = -r; k <= r; k++) ws += exp(-(double)k * k / (2 * sigma * sigma));
@@ -564,8 +584,8 @@ __global__ void __launch_bounds__(256) k_smooth_line(int n, const double* in, do
     const int k = t - r;
     wt[t] = exp(-(double)k * k / (2 * sigma * sigma)) / ws;
   }
-  // element p of line l
-  auto at = [&](int p) { return axis ? (int64_t)p * n + l : (int64_t)l * n + p; };
+  // element p of line l of frame b
+  auto at = [&](int p) { return axis ? (int64_t)p * W + (int64_t)b * n + l : (int64_t)l * W + (int64_t)b * n + p; };
   for (int p = threadIdx.x; p < n; p += blockDim.x) line[p] = in[at(p)];
   __syncthreads();
   for (int p = threadIdx.x; p < n; p += blockDim.x) {
@@ -578,25 +598,26 @@ __global__ void __launch_bounds__(256) k_smooth_line(int n, const double* in, do
     out[at(p)] = s;
   }
 }
-hipError_t launch_smooth_periodic(int n, const double* in, double* out, double sigma, int axis, hipStream_t st) {
-  const int r = (int)ceil(4 * sigma);
-  if (n <= kSmoothMaxLine && 2 * r + 1 <= kSmoothMaxTaps && r < n) {
-    hipLaunchKernelGGL(k_smooth_line, dim3(n), dim3(256), 0, st, n, in, out, sigma, axis);
-  } else {
-    const int64_t N = (int64_t)n * n;
-    hipLaunchKernelGGL(k_smooth, ew_grid(N), dim3(256), 0, st, n, in, out, sigma, axis);
+// Min and max of each frame of two fields as order-preserving integer keys
+// (doubles of any sign): mm[2 * (field * B + b) + {0, 1}], seeded with the
+// keys of (+inf, -inf).  Waves reduce by shuffles, the workgroup in LDS, and
+// one atomic pair per workgroup combines them.
+__global__ void k_seed_minmax_stack(int count, unsigned long long* mm) {
+  for (int k = threadIdx.x; k < count; k += blockDim.x) {
+    mm[2 * k] = ~0ull;
+    mm[2 * k + 1] = 0ull;
   }
-  return hipGetLastError();
 }
-// mm[0] = min, mm[1] = max as order-preserving integer keys; caller seeds mm
-// with the keys of (+inf, -inf).  Waves reduce by shuffles, the workgroup in
-// LDS, and one atomic pair per workgroup combines them.
-__global__ void __launch_bounds__(256) k_minmax(int64_t n, const double* x, double* mm) {
+__global__ void __launch_bounds__(256) k_minmax_stack(int n, int B, const double* x0, const double* x1, double* mm) {
   __shared__ double slo[4], shi[4];
+  const double* x = blockIdx.z ? x1 : x0;
+  const int b = blockIdx.y;
+  const int64_t W = (int64_t)B * n, N = (int64_t)n * n;
   double lo = INFINITY, hi = -INFINITY;
-  EW_LOOP(n) {
-    lo = fmin(lo, x[i]);
-    hi = fmax(hi, x[i]);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+    const double v = x[(i / n) * W + (int64_t)b * n + i % n];
+    lo = fmin(lo, v);
+    hi = fmax(hi, v);
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
@@ -614,20 +635,13 @@ __global__ void __launch_bounds__(256) k_minmax(int64_t n, const double* x, doub
       lo = fmin(lo, slo[w]);
       hi = fmax(hi, shi[w]);
     }
-    // order-preserving integer maps for atomics on doubles of any sign
     long long bl = __double_as_longlong(lo), bh = __double_as_longlong(hi);
     unsigned long long kl = bl < 0 ? ~(unsigned long long)bl : ((unsigned long long)bl | 0x8000000000000000ULL);
     unsigned long long kh = bh < 0 ? ~(unsigned long long)bh : ((unsigned long long)bh | 0x8000000000000000ULL);
-    atomicMin((unsigned long long*)mm, kl);
-    atomicMax((unsigned long long*)mm + 1, kh);
+    unsigned long long* dst = (unsigned long long*)mm + 2 * ((int)blockIdx.z * B + b);
+    atomicMin(dst, kl);
+    atomicMax(dst + 1, kh);
   }
-}
-hipError_t launch_minmax(int64_t n, const double* x, double* mm, hipStream_t st) {
-  // 4 elements per thread at least, at most 256 workgroups
-  int64_t b = (n + 256 * 4 - 1) / (256 * 4);
-  b = b < 1 ? 1 : (b > 256 ? 256 : b);
-  hipLaunchKernelGGL(k_minmax, dim3((unsigned)b), dim3(256), 0, st, n, x, mm);
-  return hipGetLastError();
 }
 __device__ __forceinline__ double unkey(unsigned long long k) {
   const unsigned long long b = (k & 0x8000000000000000ULL) ? (k & 0x7FFFFFFFFFFFFFFFULL) : ~k;
@@ -654,15 +668,18 @@ __device__ __forceinline__ bool shape_inside(const ShapeParams& sp, float dx, fl
   return (x1 - x0) * (dy - y0) - (y1 - y0) * (dx - x0) >= 0.f;
 }
 template <class T>
-__global__ void k_compose(int n, const double* a, const double* ph, const double* mma, const double* mmp,
-                          ShapeParams sp, cplx<T>* obj) {
-  const int64_t N = (int64_t)n * n;
-  const double amin = unkey(((const unsigned long long*)mma)[0]), amax = unkey(((const unsigned long long*)mma)[1]);
-  const double pmin = unkey(((const unsigned long long*)mmp)[0]), pmax = unkey(((const unsigned long long*)mmp)[1]);
-  const int c = n >> 1;
-  EW_LOOP(N) {
-    const int x = (int)(i / n), y = (int)(i % n);
-    if (!shape_inside(sp, (float)(x - c), (float)(y - c))) {
+__global__ void k_compose_stack(int n, SynthFrames f, const double* __restrict__ a, const double* __restrict__ ph,
+                                const double* __restrict__ mm, cplx<T>* __restrict__ obj) {
+  const int64_t W = (int64_t)f.B * n;
+  const int c0 = n >> 1;
+  EW_LOOP(W * n) {
+    int x, b, y;
+    stack_pos(i, n, W, x, b, y);
+    const unsigned long long* ka = (const unsigned long long*)mm + 2 * b;
+    const unsigned long long* kp = (const unsigned long long*)mm + 2 * (f.B + b);
+    const double amin = unkey(ka[0]), amax = unkey(ka[1]);
+    const double pmin = unkey(kp[0]), pmax = unkey(kp[1]);
+    if (!shape_inside(f.sp[b], (float)(x - c0), (float)(y - c0))) {
       obj[i] = {(T)0, (T)0};
       continue;
     }
@@ -671,27 +688,19 @@ __global__ void k_compose(int n, const double* a, const double* ph, const double
     obj[i] = {(T)(amp * cos(phs)), (T)(amp * sin(phs))};
   }
 }
+// sum |F|^2 per frame of rows x cols (one atomic per workgroup into acc[b])
 template <class T>
-hipError_t launch_compose_object(int n, const double* a, const double* ph, const double* mma, const double* mmp,
-                                 const ShapeParams& sp, void* obj, hipStream_t st) {
-  hipLaunchKernelGGL(k_compose<T>, ew_grid((int64_t)n * n), dim3(256), 0, st, n, a, ph, mma, mmp, sp,
-                     (cplx<T>*)obj);
-  return hipGetLastError();
-}
-
-template <class T>
-__global__ void k_sum_mod2(int64_t n, const cplx<T>* F, double* acc) {
+__global__ void k_sum_mod2_stack(int rows, int cols, int B, const cplx<T>* __restrict__ F, double* acc) {
+  const int b = blockIdx.y;
+  const int64_t W = (int64_t)B * cols, N = (int64_t)rows * cols;
   double s = 0;
-  EW_LOOP(n) s += (double)F[i].x * F[i].x + (double)F[i].y * F[i].y;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+    const cplx<T> z = F[(i / cols) * W + (int64_t)b * cols + i % cols];
+    s += (double)z.x * z.x + (double)z.y * z.y;
+  }
   __shared__ double sh[16];
-  block_atomic_sum(s, acc, sh);
+  block_atomic_sum(s, acc + b, sh);
 }
-template <class T>
-hipError_t launch_sum_mod2(int64_t n, const void* F, double* acc, hipStream_t st) {
-  hipLaunchKernelGGL(k_sum_mod2<T>, red_grid(n), dim3(256), 0, st, n, (const cplx<T>*)F, acc);
-  return hipGetLastError();
-}
-
 // Poisson(lam): multiplication method below 10, PTRS (Hoermann 1993) above;
 // uniforms from Philox stream 2, block counter (pixel, j).
 struct UStream {
@@ -737,198 +746,62 @@ __device__ double poisson_draw(double lam, uint64_t seed, uint64_t idx) {
   }
   return floor(lam + 0.5);
 }
+// I = |F|^2 photons / sum |F|^2 per frame, drawn Poisson (counter: the
+// frame-local pixel index) or kept as the mean
 template <class T>
-__global__ void k_poisson(int64_t n, const cplx<T>* F, const double* tot, double photons, uint64_t seed, int poisson,
-                          T* I) {
-  const double sc = *tot > 0 ? photons / *tot : 0;
-  EW_LOOP(n) {
-    const double lam = ((double)F[i].x * F[i].x + (double)F[i].y * F[i].y) * sc;
-    I[i] = poisson ? (T)poisson_draw(lam, seed, (uint64_t)i) : (T)lam;
-  }
-}
-template <class T>
-hipError_t launch_poisson_intensity(int64_t n, const void* F, const double* tot, double photons, uint64_t seed,
-                                    int poisson, T* I, hipStream_t st) {
-  hipLaunchKernelGGL(k_poisson<T>, ew_grid(n), dim3(256), 0, st, n, (const cplx<T>*)F, tot, photons, seed, poisson,
-                     I);
-  return hipGetLastError();
-}
-
-// ---------------------------------------------------------------------------
-// the forward model for a stack of B frames of n x n side by side: element
-// (r, c) of frame b at r * W + b * n + c, W = B * n.  Every kernel computes
-// what its one-frame counterpart above computes for each frame (frame-local
-// pixel indices for the Philox counters, the same summation orders), so a
-// frame of the stack equals the frame made alone.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void stack_pos(int64_t k, int n, int64_t W, int& r, int& b, int& c) {
-  r = (int)(k / W);
-  const int rem = (int)(k - (int64_t)r * W);
-  b = rem / n;
-  c = rem - b * n;
-}
-__global__ void k_uniform2_stack(int n, SynthFrames f, double* __restrict__ a, double* __restrict__ ph) {
-  const int64_t W = (int64_t)f.B * n;
-  EW_LOOP(W * n) {
+__global__ void k_poisson_stack(int rows, int cols, SynthFrames f, const cplx<T>* __restrict__ F,
+                                const double* __restrict__ tot, int poisson, T* __restrict__ I) {
+  const int64_t W = (int64_t)f.B * cols;
+  EW_LOOP(W * rows) {
     int r, b, c;
-    stack_pos(i, n, W, r, b, c);
-    const uint64_t il = (uint64_t)r * n + c;
-    a[i] = philox_uniform(f.seed[b], 0, il);
-    ph[i] = philox_uniform(f.seed[b], 1, il);
-  }
-}
-// k_smooth_line for every line of the stack and two fields (blockIdx.y)
-__global__ void __launch_bounds__(256) k_smooth_line_stack(int n, int B, const double* in0, double* out0,
-                                                           const double* in1, double* out1, double sigma, int axis) {
-  __shared__ double wt[kSmoothMaxTaps];
-  __shared__ double line[kSmoothMaxLine];
-  const double* in = blockIdx.y ? in1 : in0;
-  double* out = blockIdx.y ? out1 : out0;
-  const int r = (int)ceil(4 * sigma);
-  const int taps = 2 * r + 1;
-  const int b = blockIdx.x / n, l = blockIdx.x % n;
-  const int64_t W = (int64_t)B * n;
-  if (threadIdx.x == 0) {
-    double ws = 0;
-    for (int k = -r; k <= r; k++) ws += exp(-(double)k * k / (2 * sigma * sigma));
-    wt[0] = ws;
-  }
-  __syncthreads();
-  const double ws = wt[0];
-  __syncthreads();
-  for (int t = threadIdx.x; t < taps; t += blockDim.x) {
-    const int k = t - r;
-    wt[t] = exp(-(double)k * k / (2 * sigma * sigma)) / ws;
-  }
-  auto at = [&](int p) { return axis ? (int64_t)p * W + (int64_t)b * n + l : (int64_t)l * W + (int64_t)b * n + p; };
-  for (int p = threadIdx.x; p < n; p += blockDim.x) line[p] = in[at(p)];
-  __syncthreads();
-  for (int p = threadIdx.x; p < n; p += blockDim.x) {
-    double s = 0;
-    for (int k = -r; k <= r; k++) {
-      int q = p + k;
-      q += q < 0 ? n : (q >= n ? -n : 0);
-      s += wt[k + r] * line[q];
-    }
-    out[at(p)] = s;
-  }
-}
-// min/max keys of each frame of two fields: mm[2 * (field * B + b) + {0, 1}]
-__global__ void k_seed_minmax_stack(int count, unsigned long long* mm) {
-  for (int k = threadIdx.x; k < count; k += blockDim.x) {
-    mm[2 * k] = ~0ull;
-    mm[2 * k + 1] = 0ull;
-  }
-}
-__global__ void __launch_bounds__(256) k_minmax_stack(int n, int B, const double* x0, const double* x1, double* mm) {
-  __shared__ double slo[4], shi[4];
-  const double* x = blockIdx.z ? x1 : x0;
-  const int b = blockIdx.y;
-  const int64_t W = (int64_t)B * n, N = (int64_t)n * n;
-  double lo = INFINITY, hi = -INFINITY;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
-    const double v = x[(i / n) * W + (int64_t)b * n + i % n];
-    lo = fmin(lo, v);
-    hi = fmax(hi, v);
-  }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    lo = fmin(lo, __shfl_xor(lo, o, 64));
-    hi = fmax(hi, __shfl_xor(hi, o, 64));
-  }
-  const int wave = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) {
-    slo[wave] = lo;
-    shi[wave] = hi;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
-      lo = fmin(lo, slo[w]);
-      hi = fmax(hi, shi[w]);
-    }
-    long long bl = __double_as_longlong(lo), bh = __double_as_longlong(hi);
-    unsigned long long kl = bl < 0 ? ~(unsigned long long)bl : ((unsigned long long)bl | 0x8000000000000000ULL);
-    unsigned long long kh = bh < 0 ? ~(unsigned long long)bh : ((unsigned long long)bh | 0x8000000000000000ULL);
-    unsigned long long* dst = (unsigned long long*)mm + 2 * ((int)blockIdx.z * B + b);
-    atomicMin(dst, kl);
-    atomicMax(dst + 1, kh);
-  }
-}
-__global__ void k_compose_stack(int n, SynthFrames f, const double* __restrict__ a, const double* __restrict__ ph,
-                                const double* __restrict__ mm, cplx<float>* __restrict__ obj) {
-  const int64_t W = (int64_t)f.B * n;
-  const int c0 = n >> 1;
-  EW_LOOP(W * n) {
-    int x, b, y;
-    stack_pos(i, n, W, x, b, y);
-    const unsigned long long* ka = (const unsigned long long*)mm + 2 * b;
-    const unsigned long long* kp = (const unsigned long long*)mm + 2 * (f.B + b);
-    const double amin = unkey(ka[0]), amax = unkey(ka[1]);
-    const double pmin = unkey(kp[0]), pmax = unkey(kp[1]);
-    if (!shape_inside(f.sp[b], (float)(x - c0), (float)(y - c0))) {
-      obj[i] = {0.f, 0.f};
-      continue;
-    }
-    const double amp = 0.2 + 0.8 * (a[i] - amin) / (amax - amin);
-    const double phs = -kPi + 2 * kPi * (ph[i] - pmin) / (pmax - pmin);
-    obj[i] = {(float)(amp * cos(phs)), (float)(amp * sin(phs))};
-  }
-}
-__global__ void k_sum_mod2_stack(int n, int B, const cplx<float>* __restrict__ F, double* acc) {
-  const int b = blockIdx.y;
-  const int64_t W = (int64_t)B * n, N = (int64_t)n * n;
-  double s = 0;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
-    const cplx<float> z = F[(i / n) * W + (int64_t)b * n + i % n];
-    s += (double)z.x * z.x + (double)z.y * z.y;
-  }
-  __shared__ double sh[16];
-  block_atomic_sum(s, acc + b, sh);
-}
-__global__ void k_poisson_stack(int n, SynthFrames f, const cplx<float>* __restrict__ F, const double* __restrict__ tot,
-                                float* __restrict__ I) {
-  const int64_t W = (int64_t)f.B * n;
-  EW_LOOP(W * n) {
-    int r, b, c;
-    stack_pos(i, n, W, r, b, c);
+    stack_pos(i, cols, W, r, b, c);
     const double sc = tot[b] > 0 ? f.photons[b] / tot[b] : 0;
     const double lam = ((double)F[i].x * F[i].x + (double)F[i].y * F[i].y) * sc;
-    I[i] = (float)poisson_draw(lam, f.seed[b], (uint64_t)r * n + c);
+    I[i] = poisson ? (T)poisson_draw(lam, f.seed[b], (uint64_t)r * cols + c) : (T)lam;
   }
 }
 bool synth_stack_supported(int n, double sigma) {
   const int r = (int)ceil(4 * sigma);
   return n <= kSmoothMaxLine && 2 * r + 1 <= kSmoothMaxTaps && r < n;
 }
+template <class T>
 hipError_t launch_synth_fields(int n, const SynthFrames& f, double sigma, double* a, double* ph, double* ta,
                                double* tph, double* mm, void* obj, hipStream_t st) {
-  if (f.B < 1 || f.B > kSynthMax || !synth_stack_supported(n, sigma)) return hipErrorInvalidValue;
+  const bool lines = synth_stack_supported(n, sigma);
+  if (f.B < 1 || f.B > kSynthMax || (!lines && f.B != 1)) return hipErrorInvalidValue;
   const int64_t tot = (int64_t)f.B * n * n;
   hipLaunchKernelGGL(k_uniform2_stack, ew_grid(tot), dim3(256), 0, st, n, f, a, ph);
-  hipLaunchKernelGGL(k_smooth_line_stack, dim3(f.B * n, 2), dim3(256), 0, st, n, f.B, (const double*)a, ta,
-                     (const double*)ph, tph, sigma, 0);
-  hipLaunchKernelGGL(k_smooth_line_stack, dim3(f.B * n, 2), dim3(256), 0, st, n, f.B, (const double*)ta, a,
-                     (const double*)tph, ph, sigma, 1);
+  if (lines) {
+    hipLaunchKernelGGL(k_smooth_line_stack, dim3(f.B * n, 2), dim3(256), 0, st, n, f.B, (const double*)a, ta,
+                       (const double*)ph, tph, sigma, 0);
+    hipLaunchKernelGGL(k_smooth_line_stack, dim3(f.B * n, 2), dim3(256), 0, st, n, f.B, (const double*)ta, a,
+                       (const double*)tph, ph, sigma, 1);
+  } else {  // one frame, lines beyond the line kernel's limits
+    const dim3 g = ew_grid((int64_t)n * n);
+    hipLaunchKernelGGL(k_smooth, g, dim3(256), 0, st, n, (const double*)a, ta, sigma, 0);
+    hipLaunchKernelGGL(k_smooth, g, dim3(256), 0, st, n, (const double*)ta, a, sigma, 1);
+    hipLaunchKernelGGL(k_smooth, g, dim3(256), 0, st, n, (const double*)ph, tph, sigma, 0);
+    hipLaunchKernelGGL(k_smooth, g, dim3(256), 0, st, n, (const double*)tph, ph, sigma, 1);
+  }
   hipLaunchKernelGGL(k_seed_minmax_stack, dim3(1), dim3(64), 0, st, 2 * f.B, (unsigned long long*)mm);
-  // blocks per frame as launch_minmax for one frame: 4 elements per thread, at most 256
+  // 4 elements per thread at least, at most 256 workgroups per frame
   int64_t nb = ((int64_t)n * n + 256 * 4 - 1) / (256 * 4);
   nb = nb < 1 ? 1 : (nb > 256 ? 256 : nb);
   hipLaunchKernelGGL(k_minmax_stack, dim3((unsigned)nb, f.B, 2), dim3(256), 0, st, n, f.B, (const double*)a,
                      (const double*)ph, mm);
-  hipLaunchKernelGGL(k_compose_stack, ew_grid(tot), dim3(256), 0, st, n, f, (const double*)a, (const double*)ph,
-                     (const double*)mm, (cplx<float>*)obj);
+  hipLaunchKernelGGL(k_compose_stack<T>, ew_grid(tot), dim3(256), 0, st, n, f, (const double*)a, (const double*)ph,
+                     (const double*)mm, (cplx<T>*)obj);
   return hipGetLastError();
 }
-hipError_t launch_synth_intensity(int n, const SynthFrames& f, const void* F, double* tot, float* I,
-                                  hipStream_t st) {
+template <class T>
+hipError_t launch_synth_intensity(int rows, int cols, const SynthFrames& f, const void* F, double* tot, int poisson,
+                                  T* I, hipStream_t st) {
   if (f.B < 1 || f.B > kSynthMax) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(256), 0, st, (int64_t)f.B, tot, 0.0);
-  // blocks per frame as launch_sum_mod2 for one frame
-  const dim3 g1 = red_grid((int64_t)n * n);
-  hipLaunchKernelGGL(k_sum_mod2_stack, dim3(g1.x, f.B), dim3(256), 0, st, n, f.B, (const cplx<float>*)F, tot);
-  hipLaunchKernelGGL(k_poisson_stack, ew_grid((int64_t)f.B * n * n), dim3(256), 0, st, n, f, (const cplx<float>*)F,
-                     (const double*)tot, I);
+  const dim3 g1 = red_grid((int64_t)rows * cols);  // per frame
+  hipLaunchKernelGGL(k_sum_mod2_stack<T>, dim3(g1.x, f.B), dim3(256), 0, st, rows, cols, f.B, (const cplx<T>*)F, tot);
+  hipLaunchKernelGGL(k_poisson_stack<T>, ew_grid((int64_t)f.B * rows * cols), dim3(256), 0, st, rows, cols, f,
+                     (const cplx<T>*)F, (const double*)tot, poisson, I);
   return hipGetLastError();
 }
 
@@ -1133,11 +1006,10 @@ hipError_t launch_radial(const void* src, int rows, int cols, int kind, T norm, 
   template hipError_t launch_residual<T>(int64_t, const void*, const T*, const T*, double*, hipStream_t);        \
   template hipError_t launch_rfactor<T>(int64_t, const void*, const T*, double*, hipStream_t);                   \
   template hipError_t launch_cast_c<T>(int64_t, const void*, int, void*, hipStream_t);                           \
-  template hipError_t launch_compose_object<T>(int, const double*, const double*, const double*, const double*, \
-                                               const ShapeParams&, void*, hipStream_t);                          \
-  template hipError_t launch_poisson_intensity<T>(int64_t, const void*, const double*, double, uint64_t, int, T*, \
-                                                  hipStream_t);                                                  \
-  template hipError_t launch_sum_mod2<T>(int64_t, const void*, double*, hipStream_t);                          \
+  template hipError_t launch_synth_fields<T>(int, const SynthFrames&, double, double*, double*, double*, double*, \
+                                             double*, void*, hipStream_t);                                         \
+  template hipError_t launch_synth_intensity<T>(int, int, const SynthFrames&, const void*, double*, int, T*,       \
+                                                hipStream_t);                                                      \
   template hipError_t launch_prtf_add<T>(int64_t, void*, const void*, const double*, hipStream_t);               \
   template hipError_t launch_radial<T>(const void*, int, int, int, T, double*, hipStream_t);
 INST(float)

@@ -189,10 +189,6 @@ template <class T> hipError_t launch_wavefront(int rows, int cols, const T* I, c
                                                void* out, hipStream_t st);
 
 // synthetic forward model
-hipError_t launch_uniform_field(int n, uint64_t seed, uint64_t stream, double* out, hipStream_t st);
-hipError_t launch_smooth_periodic(int n, const double* in, double* out, double sigma, int axis,
-                                  hipStream_t st);
-hipError_t launch_minmax(int64_t n, const double* x, double* mm, hipStream_t st);
 // support shape of a synthetic object: kind 0 disk (radius a), 1 ellipse
 // (semi-axes a, b, rotation rot), 2 polygon (nv <= 8 vertices at increasing
 // angles ang[] in [0, 2 pi) and radii rad[], star-shaped about the centre)
@@ -201,20 +197,14 @@ struct ShapeParams {
   float a, b, rot;
   float ang[8], rad[8];
 };
-template <class T> hipError_t launch_compose_object(int n, const double* a, const double* ph, const double* mma,
-                                                    const double* mmp, const ShapeParams& sp, void* obj,
-                                                    hipStream_t st);
-template <class T> hipError_t launch_poisson_intensity(int64_t n, const void* F, const double* tot, double photons,
-                                                       uint64_t seed, int poisson, T* I, hipStream_t st);
-template <class T> hipError_t launch_sum_mod2(int64_t n, const void* F, double* acc, hipStream_t st);
-
-// The forward model for B <= kSynthMax frames of n x n side by side (element
-// (r, c) of frame b at r * B * n + b * n + c): per frame the seed, support
-// shape and photon count.  launch_synth_fields makes the FP32 objects
-// (buffers a, ph, ta, tph: B * n * n doubles, mm: 4 * B doubles);
-// launch_synth_intensity the FP32 Poisson intensities of their transforms F
-// (tot: B doubles).  Each frame equals the frame the one-frame kernels make
-// from the same seed, shape and photons.
+// B <= kSynthMax frames side by side (element (r, c) of frame b at
+// r * B * cols + b * cols + c; B = 1: one frame): per frame the seed, support
+// shape and photon count.  launch_synth_fields makes the objects of n x n
+// frames (field buffers a, ph, ta, tph: B * n * n doubles; mm: 4 * B
+// doubles, the min/max keys); launch_synth_intensity the intensities of
+// their transforms F (tot: B doubles), Poisson draws or (poisson = 0) the
+// means.  Each frame of a stack equals the frame made alone from the same
+// seed, shape and photons.
 constexpr int kSynthMax = 16;
 struct SynthFrames {
   int B;
@@ -222,10 +212,14 @@ struct SynthFrames {
   double photons[kSynthMax];
   ShapeParams sp[kSynthMax];
 };
+// whether the line smoothing serves n x n frames (otherwise one frame at a time)
 bool synth_stack_supported(int n, double sigma);
+template <class T>
 hipError_t launch_synth_fields(int n, const SynthFrames& f, double sigma, double* a, double* ph, double* ta,
                                double* tph, double* mm, void* obj, hipStream_t st);
-hipError_t launch_synth_intensity(int n, const SynthFrames& f, const void* F, double* tot, float* I, hipStream_t st);
+template <class T>
+hipError_t launch_synth_intensity(int rows, int cols, const SynthFrames& f, const void* F, double* tot, int poisson,
+                                  T* I, hipStream_t st);
 
 }  // namespace cdx
 

@@ -5,7 +5,11 @@ amplitude in [0.2,1] and phase in [-pi,pi], support radius 160 px), 1e8
 photons with Poisson noise, central beamstop of radius 8 px left
 unconstrained, RAAR at beta = 0.85.  A step is one iteration of the loop
 (column pass + row pass) with every input resident in HBM; shrink-wrap
-steps are timed separately (end-to-end field).
+steps are timed separately (end-to-end field).  Right before the timed
+steps the captured graph runs untimed until at least WARM_ITERS iterations
+have run (besides the --warmup steps): after a fraction of a millisecond of
+work the GPU runs the loop ~3% slower than after tens of milliseconds, which
+would otherwise bias short runs (--steps 20).
 
 Prints one JSON line.  --gpus above 1 is refused: this build runs on one GPU.
 """
@@ -23,6 +27,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "HIO iters/s + HBM GB/s at 1024² c64; iters-to-R≤0.05 learned vs random start"
 N = 1024
+# untimed loop iterations right before the timed ones (steady clocks)
+WARM_ITERS = 2000
 BYTES_PER_ITER = 80 * N * N          # algorithmic bytes of one iteration (c64/FP32)
 COL_BPP = 20.0                       # column pass: read T1 8 + read I 4 + write T2 8 B/px
 # row pass: read T2 8, g 8 and the support bits (one 4-byte word per lane,
@@ -700,6 +706,7 @@ def main():
     p = cdx.Params.default(beta=0.85, useBS=1, freeBeamstop=1)
     tuning = ctx.autotune(reps=40)  # picks the lines-per-workgroup of both passes
     tiling = ctx.tiling()
+    ctx.set_timing_warmup(WARM_ITERS)
 
     # warmup (untimed), then exactly `steps` iterations bracketed by syncs
     ctx.time_iterations(p, cdx.RAAR, d["F"], d["g"], d["I"], d["spt"], max(args.warmup, 1), bs=d["bs"],
@@ -769,6 +776,7 @@ def main():
         "config": {"workload": "single-frame RAAR beta=0.85, 1024x1024 complex64, object radius 160 px, "
                                "disk support radius 204.8 px (0.2 N), beamstop r=8 (left unconstrained)",
                    "frame": [N, N], "algorithm": "RAAR", "graph": not args.no_graph,
+                   "untimed_iterations_before_timed": WARM_ITERS,
                    "tiling": {"col_lines": tiling[0], "row_lines": tiling[1]},
                    "tuner_us": {f"{'col' if p_ == 0 else 'row'}{l_}": round(us, 2) for p_, l_, us in tuning}},
         "hbm_gbps_effective": round(iters_s * BYTES_PER_ITER / 1e9, 1),

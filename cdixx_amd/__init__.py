@@ -109,6 +109,7 @@ _SIGS = {
     "cdx_batch_set_schedule": ([_vp, _i, _ip], _i),
     "cdx_batch_get_schedule": ([_vp, _ip, _ip, ctypes.c_char_p, _i], _i),
     "cdx_set_tiling": ([_vp, _i, _i], _i), "cdx_get_tiling": ([_vp, _ip, _ip], _i),
+    "cdx_set_timing_warmup": ([_vp, _i], _i),
     "cdx_autotune": ([_vp, _i, _dp, _i], _i),
     "cdx_convert_fo_phase": ([_vp, _vp, _d], _i), "cdx_mask_multiply": ([_vp, _vp, _vp, _i], _i),
     "cdx_poisson_noise_wo": ([_vp, _vp, _d, _u64], _i), "cdx_extend_to_complex": ([_vp, _vp, _vp], _i),
@@ -317,6 +318,11 @@ class Context:
     # tiling of the FFT passes
     def set_tiling(self, col_lines=0, row_lines=0):
         check(lib.cdx_set_tiling(self.h, col_lines, row_lines))
+
+    def set_timing_warmup(self, iters):
+        """untimed iterations time_iterations runs (at least) right before its
+        timed ones (cdx_set_timing_warmup)"""
+        check(lib.cdx_set_timing_warmup(self.h, int(iters)))
 
     def tiling(self):
         a, b = ctypes.c_int(), ctypes.c_int()

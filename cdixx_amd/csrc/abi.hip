@@ -73,6 +73,7 @@ struct cdx_ctx {
   int rcolor;
   double scale, beta, relax;
   int col_lines = 0, row_lines = 0;  // lines per workgroup of the passes (0: default)
+  int timing_warm = 0;  // cdx_time_iterations: untimed iterations before the timed ones (at least)
   // PRTF of the last cdx_phase_retrieve (allocated on first use)
   void* prtf_map = nullptr;   // summed unit phasors, complex N
   void* gamma0 = nullptr;     // reference object P_S(g'), complex N
@@ -829,7 +830,12 @@ static int time_t_(cdx_ctx* c, const cdx_params* p, int algo, void* F, void* g, 
       return r;
     });
     if (rc) return rc;
-    HIPCHK(hipGraphLaunch(res.exec, c->st));  // warm the graph once
+    // warm the graph: once, or until at least timing_warm iterations have run
+    const int reps = c->timing_warm > n ? (c->timing_warm + n - 1) / n : 1;
+    for (int w = 0; w < reps; w++) HIPCHK(hipGraphLaunch(res.exec, c->st));
+  } else {
+    for (int w = 0; w < c->timing_warm && !rc; w++) rc = L.iterate(algo, rf);
+    if (rc) return rc;
   }
   HIPCHK(hipStreamSynchronize(c->st));
   HIPCHK(hipEventRecord(res.e0, c->st));
@@ -1086,6 +1092,13 @@ extern "C" int cdx_create_wavefront(cdx_ctx* c, const void* I, const void* P, in
 // ---------------------------------------------------------------------------
 // tiling of the passes and its tuner
 // ---------------------------------------------------------------------------
+extern "C" int cdx_set_timing_warmup(cdx_ctx* c, int iters) {
+  CTX_CHECK(c);
+  if (iters < 0) return fail(CDX_ERR_ARG, "cdx_set_timing_warmup: negative iteration count %d", iters);
+  c->timing_warm = iters;
+  return CDX_OK;
+}
+
 extern "C" int cdx_set_tiling(cdx_ctx* c, int col_lines, int row_lines) {
   CTX_CHECK(c);
   if (col_lines && !tiling_valid(c->rows, c->csz, col_lines))

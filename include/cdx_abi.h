@@ -253,19 +253,27 @@ int cdx_read_config_json(const char* path, char* buf, int cap);
 /* ---- measurement helpers (timing with HIP events on the context stream) ---- */
 /* Runs iterations of `algo` from the state F, g (g as cdx_phase_retrieve
  * takes it, IFFT(F)/sqrt(N)) without shrink-wrap: one untimed iteration, with
- * use_graph one untimed launch of the captured n_iter iterations, then n_iter
- * timed iterations (so 1 + n_iter, or 1 + 2 n_iter with use_graph, in all).
- * ms_total: wall time of the n_iter timed iterations between two events.
- * F and g are left as cdx_phase_retrieve leaves them.  If col_ms/row_ms are
- * given, each pass is then launched min(n_iter, 200) times on its own, back
- * to back (one graph when use_graph), between two events, three times, and
- * the median mean time per launch of the column pass and of the row pass is
- * returned; those launches re-apply the update to the same input, so F and g
- * are then no iterate of the loop.  With p->recordR the column pass also
- * forms the R-factor sums (the cost of recording R every iteration). */
+ * use_graph one untimed launch of the captured n_iter iterations (repeated
+ * until at least the cdx_set_timing_warmup count has run), without it that
+ * many untimed iterations, then n_iter timed iterations.  ms_total: wall time
+ * of the n_iter timed iterations between two events.  F and g are left as
+ * cdx_phase_retrieve leaves them.  If col_ms/row_ms are given, each pass is
+ * then launched 200 times on its own, back to back (one graph when
+ * use_graph), between two events, three times, and the median mean time per
+ * launch of the column pass and of the row pass is returned; those launches
+ * re-apply the update to the same input, so F and g are then no iterate of
+ * the loop.  With p->recordR the column pass also forms the R-factor sums
+ * (the cost of recording R every iteration). */
 int cdx_time_iterations(cdx_ctx* ctx, const cdx_params* p, int algo, void* F, void* g, const void* I,
                         const void* bs, const void* spt, int n_iter, int use_graph, double* ms_total,
                         double* col_ms, double* row_ms);
+
+/* Untimed iterations cdx_time_iterations runs right before its timed ones
+ * (at least; default 0: one untimed launch of the captured iterations).  No
+ * counterpart in the reference; for callers timing short runs: after a
+ * fraction of a millisecond of work this GPU runs the loop ~3% slower than
+ * after tens of milliseconds. */
+int cdx_set_timing_warmup(cdx_ctx* ctx, int iters);
 
 /* Per-pass launch durations of the loop (ms per launch) from the state F, g
  * after one untimed iteration: each pass launched n times on its own.

@@ -91,17 +91,20 @@ def test_benched_column_pass_three_steps(cdx, o64, o32, n, record):
     ctx.close()
 
 
-@pytest.mark.parametrize("n", [1024, 2048])
-def test_benched_timing_entry_point_iterates(cdx, o64, n):
+@pytest.mark.parametrize("n,warm", [(1024, 0), (2048, 0), (1024, 4)])
+def test_benched_timing_entry_point_iterates(cdx, o64, n, warm):
     """cdx_time_iterations, the benchmark's timed entry point, advances the
     state: with one graph iteration and no per-kernel timing it runs the
-    untimed first step, the graph's warm-up launch and the timed launch, i.e.
-    three RAAR steps, and leaves F = FFT(g) of the last one."""
-    print(f"timing entry point n={n}", flush=True)
+    untimed first step, the graph's warm-up launches (one, or as many as
+    cdx_set_timing_warmup asks: 4) and the timed launch, i.e. three (six)
+    RAAR steps, and leaves F = FFT(g) of the last one."""
+    print(f"timing entry point n={n} warm={warm}", flush=True)
     d = bench_inputs(cdx, n, seed=1)
-    ref = o64.phase_retrieve("3*RAAR", d["F"], d["g"], d["I"], d["spt"], 0.5, bs=d["bs"], **BENCH)
+    steps = 2 + max(warm, 1)
+    ref = o64.phase_retrieve(f"{steps}*RAAR", d["F"], d["g"], d["I"], d["spt"], 0.5, bs=d["bs"], **BENCH)
     ctx = cdx.Context(n)
     ctx.autotune(reps=10)
+    ctx.set_timing_warmup(warm)
     ctx.set_threshold(0.5)
     dF, dg, dI, ds, dbs = _upload(ctx, d)
     p = cdx.Params.default(**BENCH)

@@ -271,7 +271,7 @@ int cdx_time_iterations(cdx_ctx* ctx, const cdx_params* p, int algo, void* F, vo
 /* Untimed iterations cdx_time_iterations runs right before its timed ones
  * (at least; default 0: one untimed launch of the captured iterations).  No
  * counterpart in the reference; for callers timing short runs: after a
- * fraction of a millisecond of work this GPU runs the loop ~3% slower than
+ * fraction of a millisecond of work an MI355X runs the loop ~3% slower than
  * after tens of milliseconds. */
 int cdx_set_timing_warmup(cdx_ctx* ctx, int iters);
 

@@ -853,32 +853,25 @@ __global__ void __launch_bounds__(256) k_wgrad_reduce1(int nsplit, int zc, int64
   __syncthreads();
   if (zl == 0 && i < nout) part2[blockIdx.y * nout + i] = (red[0][il] + red[1][il]) + (red[2][il] + red[3][il]);
 }
-// Stage 2: dw[co][n] += sum_g part2[g][co*K + n]
+// Stage 2: dw[co][n] += sum_g part2[g][co*K + n], and (threads past the
+// weights) db[c] += sum over g < ng, y < tn of part2[g][Cout*K + y*Cout + c],
+// both in that fixed order
 __global__ void k_wgrad_reduce2(int ng, int Cout, int K, int Kp, int tn, const float* __restrict__ part2,
-                                float* __restrict__ dw) {
+                                float* __restrict__ dw, float* __restrict__ db) {
   const int64_t nw = (int64_t)Cout * K, nout = nw + (int64_t)tn * Cout;
-  EW_LOOP(nw) {
+  EW_LOOP(nw + Cout) {
     float s = 0.f;
-    for (int g = 0; g < ng; g++) s += part2[g * nout + i];
-    const int co = (int)(i / K), n = (int)(i - (int64_t)co * K);
-    dw[(size_t)co * Kp + n] += s;
+    if (i < nw) {
+      for (int g = 0; g < ng; g++) s += part2[g * nout + i];
+      const int co = (int)(i / K), n = (int)(i - (int64_t)co * K);
+      dw[(size_t)co * Kp + n] += s;
+    } else {
+      const int c = (int)(i - nw);
+      for (int g = 0; g < ng; g++)
+        for (int y = 0; y < tn; y++) s += part2[g * nout + nw + (int64_t)y * Cout + c];
+      db[c] += s;
+    }
   }
-}
-// db[c] += sum over g < ng, y < tn of part2[g][Cout*K + y*Cout + c]: one wave
-// per channel, lane l adding the entries e = l, l + 64, ... (e = g*tn + y),
-// then the lanes in a fixed butterfly order
-__global__ void __launch_bounds__(64) k_wgrad_bias_reduce(int ng, int Cout, int K, int tn,
-                                                          const float* __restrict__ part2, float* __restrict__ db) {
-  const int c = blockIdx.x, l = threadIdx.x;
-  const int64_t nw = (int64_t)Cout * K, nout = nw + (int64_t)tn * Cout;
-  float s = 0.f;
-  for (int e = l; e < ng * tn; e += 64) {
-    const int g = e / tn, y = e - g * tn;
-    s += part2[g * nout + nw + (int64_t)y * Cout + c];
-  }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-  if (l == 0) db[c] += s;
 }
 
 // The layout kernels below move 8 channels (16 bytes) per thread: every
@@ -968,14 +961,26 @@ __global__ void k_upcat_bwd_low(int n8, int Hl, int Wl, int Cu, int Cs, const u1
 }
 
 // wf[ci][t' * Cout + co] = w[co][t * Cin + ci], t' = taps - 1 - t
-__global__ void k_flip(int Cout, int Cin, int taps, int Kp, const u16* __restrict__ w, int Kp2,
-                       u16* __restrict__ wf) {
-  const int64_t n = (int64_t)Cin * taps * Cout;
-  EW_LOOP(n) {
-    const int co = (int)(i % Cout);
-    const int t2 = (int)((i / Cout) % taps);
-    const int ci = (int)(i / ((int64_t)Cout * taps));
-    wf[(size_t)ci * Kp2 + t2 * Cout + co] = w[(size_t)co * Kp + (taps - 1 - t2) * Cin + ci];
+// Every layer in one launch: layer l covers elements [start[l], start[l+1])
+// of the concatenated flips (at most kMaxLayers = 4 * 6 - 1 convolutions).
+constexpr int kMaxLayers = 23;
+struct FlipTable {
+  int n;
+  int64_t start[kMaxLayers + 1];
+  int cout[kMaxLayers], cin[kMaxLayers], taps[kMaxLayers], kp[kMaxLayers], kp2[kMaxLayers];
+  const u16* w[kMaxLayers];
+  u16* wf[kMaxLayers];
+};
+__global__ void k_flip_all(FlipTable f) {
+  EW_LOOP(f.start[f.n]) {
+    int l = 0;
+    while (i >= f.start[l + 1]) l++;
+    const int64_t j = i - f.start[l];
+    const int Cout = f.cout[l], Cin = f.cin[l], taps = f.taps[l];
+    const int co = (int)(j % Cout);
+    const int t2 = (int)((j / Cout) % taps);
+    const int ci = (int)(j / ((int64_t)Cout * taps));
+    f.wf[l][(size_t)ci * f.kp2[l] + t2 * Cout + co] = f.w[l][(size_t)co * f.kp[l] + (taps - 1 - t2) * Cin + ci];
   }
 }
 
@@ -1289,11 +1294,8 @@ int conv_wgrad(cdx_unet* u, const UConv& L, int B, int h, const u16* dz, const u
   hipLaunchKernelGGL(k_wgrad_reduce1, dim3((unsigned)((nout + 63) / 64), ng), dim3(256), 0, u->st, nsplit, zc, nout,
                      (const float*)u->wg_part, part2);
   UCHK(hipGetLastError());
-  hipLaunchKernelGGL(k_wgrad_reduce2, dim3(ew_blocks((int64_t)L.cout * K)), dim3(256), 0, u->st, ng, L.cout, K,
-                     L.kp, tn, (const float*)part2, u->g + L.w_off);
-  UCHK(hipGetLastError());
-  hipLaunchKernelGGL(k_wgrad_bias_reduce, dim3(L.cout), dim3(64), 0, u->st, ng, L.cout, K, tn, (const float*)part2,
-                     u->g + L.b_off);
+  hipLaunchKernelGGL(k_wgrad_reduce2, dim3(ew_blocks((int64_t)L.cout * K + L.cout)), dim3(256), 0, u->st, ng,
+                     L.cout, K, L.kp, tn, (const float*)part2, u->g + L.w_off, u->g + L.b_off);
   UCHK(hipGetLastError());
   return CDX_OK;
 }
@@ -1303,16 +1305,30 @@ int ew8_grid(int64_t n8, dim3& grid) {
   grid = dim3((unsigned)ew_blocks(n8));
   return CDX_OK;
 }
+// the flipped, transposed BF16 weights of the input gradients, all layers
+int flip_weights(cdx_unet* u) {
+  FlipTable f{};
+  if ((int)u->conv.size() > kMaxLayers) return ufail(CDX_ERR_ARG, "unet: more layers than the flip table holds");
+  f.n = (int)u->conv.size();
+  for (int l = 0; l < f.n; l++) {
+    const UConv& L = u->conv[l];
+    f.start[l + 1] = f.start[l] + (int64_t)L.cin * L.taps * L.cout;
+    f.cout[l] = L.cout;
+    f.cin[l] = L.cin;
+    f.taps[l] = L.taps;
+    f.kp[l] = L.kp;
+    f.kp2[l] = L.kp2;
+    f.w[l] = (const u16*)(u->wb + L.w_off);
+    f.wf[l] = L.wflip;
+  }
+  hipLaunchKernelGGL(k_flip_all, dim3(ew_blocks(f.start[f.n])), dim3(256), 0, u->st, f);
+  UCHK(hipGetLastError());
+  return CDX_OK;
+}
 int refresh_weights(cdx_unet* u) {
   hipLaunchKernelGGL(k_to_bf16, dim3(ew_blocks(u->nparam)), dim3(256), 0, u->st, u->nparam, u->w, u->wb);
   UCHK(hipGetLastError());
-  for (auto& L : u->conv) {
-    const int64_t n = (int64_t)L.cin * L.taps * L.cout;
-    hipLaunchKernelGGL(k_flip, dim3(ew_blocks(n)), dim3(256), 0, u->st, L.cout, L.cin, L.taps, L.kp,
-                       (const u16*)(u->wb + L.w_off), L.kp2, L.wflip);
-    UCHK(hipGetLastError());
-  }
-  return CDX_OK;
+  return flip_weights(u);
 }
 
 int forward(cdx_unet* u, int B) {
@@ -1421,13 +1437,7 @@ int adam(cdx_unet* u, double lr) {
   hipLaunchKernelGGL(k_adam, dim3(ew_blocks(u->nparam)), dim3(256), 0, u->st, u->nparam, u->w, u->g, u->m1, u->m2,
                      u->wb, (float)lr, (float)b1, (float)b2, 1e-8f, c1, c2);
   UCHK(hipGetLastError());
-  for (auto& L : u->conv) {
-    const int64_t n = (int64_t)L.cin * L.taps * L.cout;
-    hipLaunchKernelGGL(k_flip, dim3(ew_blocks(n)), dim3(256), 0, u->st, L.cout, L.cin, L.taps, L.kp,
-                       (const u16*)(u->wb + L.w_off), L.kp2, L.wflip);
-    UCHK(hipGetLastError());
-  }
-  return CDX_OK;
+  return flip_weights(u);
 }
 
 // smoothing of the synthetic objects' fields, in pixels

@@ -122,11 +122,14 @@ __host__ __device__ constexpr size_t lds_bytes(int lsh) {
   return (size_t)2 * ((size_t)1 << lsh) * lds_pitch<T, N, V>(lsh) * sizeof(cplx<T>);
 }
 
-// index of element (r, c) of a rows-row image in the column-tile layout of
-// the loop state T1 and the packed measurement
-__device__ __forceinline__ size_t t1_index(int r, int c, int rows) {
-  return (size_t)(c / kT1Tile) * ((size_t)rows * kT1Tile) + (size_t)r * kT1Tile + (c % kT1Tile);
+// index of element (r, c) of a rows-row image in a layout of column tiles W
+// wide: the loop state T1 (t1_index) and the packed measurement (i_index)
+template <int W>
+__device__ __forceinline__ size_t tile_index(int r, int c, int rows) {
+  return (size_t)(c / W) * ((size_t)rows * W) + (size_t)r * W + (c % W);
 }
+__device__ __forceinline__ size_t t1_index(int r, int c, int rows) { return tile_index<kT1Tile>(r, c, rows); }
+__device__ __forceinline__ size_t i_index(int r, int c, int rows) { return tile_index<kITile>(r, c, rows); }
 
 // index of element (r, c) of a cols-column image in the pair-of-rows layout
 // of the single-frame loop's T2: rows 2k and 2k+1 interleaved in groups of

@@ -40,7 +40,7 @@ hipError_t launch_pack_intensity(int64_t n, const T* I, const T* bs, T* out, int
   hipLaunchKernelGGL(k_pack<T>, ew_grid(n), dim3(256), 0, st, n, I, bs, out, amp);
   return hipGetLastError();
 }
-// the same, written in the column-tile layout of a single-frame context
+// the same, written in the column-tile layout of the measurement (i_index)
 template <class T>
 __global__ void k_pack_tiled(int rows, int cols, const T* __restrict__ I, const T* __restrict__ bs,
                              T* __restrict__ out, int amp) {
@@ -48,7 +48,7 @@ __global__ void k_pack_tiled(int rows, int cols, const T* __restrict__ I, const 
   EW_LOOP(n) {
     const T v = I[i];
     const int r = (int)(i / cols), c = (int)(i % cols);
-    out[t1_index(r, c, rows)] = (bs && bs[i] > (T)0.5) ? (T)NAN : amp ? (v > (T)0 ? dsqrt(v) : (T)0) : v;
+    out[i_index(r, c, rows)] = (bs && bs[i] > (T)0.5) ? (T)NAN : amp ? (v > (T)0 ? dsqrt(v) : (T)0) : v;
   }
 }
 template <class T>

@@ -18,8 +18,8 @@
 // latency overlaps the first transform.
 //
 // In the single-frame loop (TILED) T1 and the packed measurement are kept in
-// column tiles of kT1Tile columns (t1_index), so the column pass reads
-// contiguous blocks; T1 and T2 are moved with streaming (non-temporal) loads
+// column tiles (t1_index, i_index), so the column pass reads whole or half
+// 64-byte pieces; T1 and T2 are moved with streaming (non-temporal) loads
 // and stores, while g and S are loaded plainly (and g stored plainly up to
 // 1024^2), the combination measured fastest.  (FETCH_SIZE shows g and S
 // fetched from beyond L2 every iteration all the same: the gain is the load
@@ -100,9 +100,9 @@ __device__ __forceinline__ T load_stream(const T* p) {
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
-// L2.  Column tiles are narrower than a cache line (8 complex64 = 64 B), so
-// the tiles sharing a line must run on the same XCD: give each XCD a
-// contiguous run of tiles.
+// L2.  Neighbouring column workgroups read pieces of the same cache lines
+// (halves of one T1 piece; the two 32-byte measurement pieces of a line), so
+// they must run on the same XCD: give each XCD a contiguous run of tiles.
 __device__ __forceinline__ int xcd_tile(int bid, int nb) {
   if (nb & 7) return bid;
   return (bid & 7) * (nb >> 3) + (bid >> 3);
@@ -112,7 +112,7 @@ __device__ __forceinline__ int xcd_tile(int bid, int nb) {
 // column pass body: columns [tile << lsh, (tile+1) << lsh).  MODE/RF = -1:
 // projection mode and R-factor chosen at run time.  TILED: the transformed
 // input (ops that start with the forward transform) is T1 in its column-tile
-// layout.  TILED_I: the packed measurement is in that layout too.
+// layout.  TILED_I: the packed measurement is in its column-tile layout.
 // ---------------------------------------------------------------------------
 template <class T, int N, int OP, int MODE, int RF, bool TILED = false, bool TILED_I = false, int V = 0,
           bool T2L = false>
@@ -156,7 +156,7 @@ __device__ __forceinline__ void col_body(const cplx<T>* __restrict__ in, cplx<T>
   if constexpr (proj) {
 #pragma unroll
     for (int i = 0; i < E; i++)
-      iv[i] = I[TILED_I ? t1_index(L::dout_pos(i, t), lc, N) : (size_t)L::dout_pos(i, t) * cols + lc];
+      iv[i] = I[TILED_I ? i_index(L::dout_pos(i, t), lc, N) : (size_t)L::dout_pos(i, t) * cols + lc];
   }
   // table twiddles of both transforms, loaded behind the data
   typename L::TwF twf;

@@ -114,9 +114,11 @@ k_row(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out, int rows, const
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr bool tiled = OP == ROW_INV_UPDATE_FWD || OP == ROW_INV_UPDATE_FWD_PRTF;
   constexpr bool t2l = OP == ROW_INV_UPDATE_FWD || OP == ROW_INV_UPDATE_FWD_PRTF || OP == ROW_INV_MOD2;
-  // 1024-point update passes: neighbouring row pairs run on one XCD, so the
-  // two 64-byte halves of a 128-byte T1 tile line leave from the same L2
-  // (1024^2: 54.8k -> 55.3k iterations/s; at 2048^2 it cost 3.5 us, so not there)
+  // 1024-point update passes: neighbouring row workgroups run on one XCD, so
+  // the two rows of a 128-byte T1 tile line leave from the same L2 also when
+  // a workgroup holds one row (with 4-column T1 tiles, whose lines spanned
+  // two workgroups, 1024^2: 54.8k -> 55.3k iterations/s; at 2048^2 it cost
+  // 3.5 us, so not there)
   constexpr bool xcd_rows = tiled && N == 1024;
   // the loop's update pass tests the support from its bit mask
   constexpr bool smask = OP == ROW_INV_UPDATE_FWD;

@@ -722,8 +722,13 @@ def main():
     # two more back-to-back samples of each pass (the same measurement as the
     # timed call's): the median of three keeps one slow sample from making
     # the shorter pass look dominant
-    more = [ctx.time_passes(p, cdx.RAAR, d["F"], d["g"], d["I"], d["spt"], 200, bs=d["bs"], mode=0)
-            for _ in range(2)]
+    # (with --no-graph these launches are stream launches too)
+    if args.no_graph:
+        more = [ctx.time_iterations(p, cdx.RAAR, d["F"], d["g"], d["I"], d["spt"], 2, bs=d["bs"], use_graph=False,
+                                    per_kernel=True)[1:] for _ in range(2)]
+    else:
+        more = [ctx.time_passes(p, cdx.RAAR, d["F"], d["g"], d["I"], d["spt"], 200, bs=d["bs"], mode=0)
+                for _ in range(2)]
     col_ms = float(np.median([col_ms] + [c for c, _ in more]))
     row_ms = float(np.median([row_ms] + [r for _, r in more]))
     # each pass launched between its own pair of events (launches serialized

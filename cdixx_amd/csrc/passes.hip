@@ -114,12 +114,12 @@ k_row(const cplx<T>* __restrict__ in, cplx<T>* __restrict__ out, int rows, const
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr bool tiled = OP == ROW_INV_UPDATE_FWD || OP == ROW_INV_UPDATE_FWD_PRTF;
   constexpr bool t2l = OP == ROW_INV_UPDATE_FWD || OP == ROW_INV_UPDATE_FWD_PRTF || OP == ROW_INV_MOD2;
-  // 1024-point update passes: neighbouring row workgroups run on one XCD, so
-  // the two rows of a 128-byte T1 tile line leave from the same L2 also when
-  // a workgroup holds one row (with 4-column T1 tiles, whose lines spanned
-  // two workgroups, 1024^2: 54.8k -> 55.3k iterations/s; at 2048^2 it cost
-  // 3.5 us, so not there)
-  constexpr bool xcd_rows = tiled && N == 1024;
+  // Update passes of 1024 and 2048 points with one row per workgroup:
+  // neighbouring row workgroups run on one XCD, so the two rows of each
+  // 128-byte T1 tile line leave from the same L2 (2048^2: 18.4k -> 18.7k
+  // iterations/s, the tuner then taking one row per workgroup).  A two-row
+  // workgroup fills whole lines itself; remapping those cost 0.3% at 1024^2.
+  const bool xcd_rows = tiled && N >= 1024 && lsh == 0;
   // the loop's update pass tests the support from its bit mask
   constexpr bool smask = OP == ROW_INV_UPDATE_FWD;
   row_body<T, N, OP, ALGO, tiled, 0, t2l, tiled && N <= 1024, smask>(
